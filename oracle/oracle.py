@@ -1,0 +1,345 @@
+"""ctypes wrapper of the fwi2d oracle -- TEST INFRASTRUCTURE ONLY.
+
+Loaded by tests/, by __graft_entry__.smoke() and by the cpu_baseline leg of
+bench.py, as the checker.  The product package (fwi2d_amd) never imports it.
+
+`Oracle(64)` wraps liboracle64.so (REAL=double), `Oracle(32)` liboracle32.so
+(REAL=float).  `RefSBP()` wraps _ref/libref_sbp.so, the reference's own SBP
+templates compiled from source (present only where the reference was).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+class OrcGrid(C.Structure):
+    _fields_ = [("nx", C.c_int), ("nz", C.c_int), ("dx", C.c_double), ("dz", C.c_double),
+                ("ox", C.c_double), ("oz", C.c_double), ("nt", C.c_int), ("dt", C.c_double),
+                ("sub", C.c_int), ("bc_top", C.c_int), ("bc_bottom", C.c_int), ("bc_left", C.c_int),
+                ("bc_right", C.c_int), ("taper_top", C.c_int), ("taper_bottom", C.c_int),
+                ("taper_left", C.c_int), ("taper_right", C.c_int), ("taper_strength", C.c_double),
+                ("vti", C.c_int), ("gdamp_tmin", C.c_double), ("gdamp_tmax", C.c_double),
+                ("gdamp_power", C.c_double)]
+
+
+class OrcModelPar(C.Structure):
+    _fields_ = [("vpmin", C.c_double), ("vpmax", C.c_double), ("vsmin", C.c_double), ("vsmax", C.c_double),
+                ("rhomin", C.c_double), ("rhomax", C.c_double), ("deltamin", C.c_double),
+                ("deltamax", C.c_double), ("epsmin", C.c_double), ("epsmax", C.c_double),
+                ("soft_clip", C.c_int), ("courant", C.c_double), ("dt", C.c_double), ("sub", C.c_int),
+                ("nt_data", C.c_int), ("dt_data", C.c_double), ("vmax", C.c_double), ("vmin", C.c_double),
+                ("dt_prop", C.c_double), ("courant_out", C.c_double), ("nt_prop", C.c_int),
+                ("sub_out", C.c_int)]
+
+
+class OrcOp(C.Structure):
+    _fields_ = [("g", OrcGrid), ("mp", OrcModelPar), ("ns", C.c_int), ("sxz", C.POINTER(C.c_double)),
+                ("nrec", C.POINTER(C.c_int)), ("rxz", C.POINTER(C.c_double)), ("seismotype", C.c_int),
+                ("resampling", C.c_int), ("sinc_hl", C.c_int)]
+
+
+def grid_struct(p):
+    """Build an OrcGrid from a dict-like problem description."""
+    g = OrcGrid()
+    for k, _ in OrcGrid._fields_:
+        if k in p:
+            setattr(g, k, p[k])
+    return g
+
+
+def _ptr(a, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype)) if a is not None else None
+
+
+class Oracle:
+    """Numpy front end of the oracle restatement."""
+
+    def __init__(self, bits=64):
+        self.bits = bits
+        path = os.path.join(HERE, "liboracle%d.so" % bits)
+        if not os.path.exists(path):
+            raise FileNotFoundError("%s missing: run `make -C oracle`" % path)
+        self.lib = C.CDLL(path)
+        self.dtype = np.float64 if bits == 64 else np.float32
+        self.rt = C.c_double if bits == 64 else C.c_float
+        assert self.lib.orc_real_size() == (8 if bits == 64 else 4)
+
+    def arr(self, a):
+        return np.ascontiguousarray(a, dtype=self.dtype)
+
+    def p(self, a):
+        return _ptr(a, self.rt)
+
+    # ---- operators --------------------------------------------------------
+    def d2_table(self):
+        out = np.zeros(384)
+        self.lib.orc_d2_table(_ptr(out, C.c_double))
+        return out.reshape(6, 8, 8)
+
+    def Dx(self, u, d, p=None, add=None, a=1.0):
+        u = self.arr(u)
+        out = np.zeros_like(u) if add is None else self.arr(add).copy()
+        pp = self.arr(p) if p is not None else None
+        self.lib.orc_Dx(int(add is not None), self.p(u), self.p(out), self.p(pp), u.shape[0], u.shape[1],
+                        C.c_double(d), C.c_double(a))
+        return out
+
+    def Dz(self, u, d, p=None, add=None, a=1.0):
+        u = self.arr(u)
+        out = np.zeros_like(u) if add is None else self.arr(add).copy()
+        pp = self.arr(p) if p is not None else None
+        self.lib.orc_Dz(int(add is not None), self.p(u), self.p(out), self.p(pp), u.shape[0], u.shape[1],
+                        C.c_double(d), C.c_double(a))
+        return out
+
+    def Dxx_var(self, u, c, d):
+        u, c = self.arr(u), self.arr(c)
+        out = np.zeros_like(u)
+        self.lib.orc_Dxx_var(0, self.p(u), self.p(out), self.p(c), u.shape[0], u.shape[1], C.c_double(d),
+                             C.c_double(1.0))
+        return out
+
+    def Dzz_var(self, u, c, d):
+        u, c = self.arr(u), self.arr(c)
+        out = np.zeros_like(u)
+        self.lib.orc_Dzz_var(0, self.p(u), self.p(out), self.p(c), u.shape[0], u.shape[1], C.c_double(d),
+                             C.c_double(1.0))
+        return out
+
+    def applyH(self, u, d, axis_x, inv=False):
+        u = self.arr(u).copy()
+        self.lib.orc_applyH(int(axis_x), int(inv), self.p(u), u.shape[0], u.shape[1], C.c_double(d))
+        return u
+
+    def esat(self, side, in0, in1, in2, out, f1, f2, f3, dx, dz, dt):
+        out = self.arr(out).copy()
+        in0, in1, f1, f2 = self.arr(in0), self.arr(in1), self.arr(f1), self.arr(f2)
+        in2 = self.arr(in2) if in2 is not None else None
+        f3 = self.arr(f3) if f3 is not None else None
+        self.lib.orc_esat(side, self.p(in0), self.p(in1), self.p(in2), self.p(out), self.p(f1), self.p(f2),
+                          self.p(f3), out.shape[0], out.shape[1], C.c_double(dx), C.c_double(dz),
+                          C.c_double(dt))
+        return out
+
+    def esat_dn(self, side, u, out, f, dx, dz):
+        out = self.arr(out).copy()
+        u, f = self.arr(u), self.arr(f)
+        self.lib.orc_esat_dn(side, self.p(u), self.p(out), self.p(f), out.shape[0], out.shape[1],
+                             C.c_double(dx), C.c_double(dz))
+        return out
+
+    def scale_boundaries(self, ux, uz, model, vti, dx, dz, dt, top, bottom, left, right):
+        ux, uz, model = self.arr(ux).copy(), self.arr(uz).copy(), self.arr(model)
+        self.lib.orc_scale_boundaries(self.p(ux), self.p(uz), self.p(model), int(vti), ux.shape[0],
+                                      ux.shape[1], C.c_double(dx), C.c_double(dz), C.c_double(dt),
+                                      int(top), int(bottom), int(left), int(right))
+        return ux, uz
+
+    def taper(self, u, ttop, tbot, tleft, tright, a):
+        u = self.arr(u).copy()
+        self.lib.orc_taper(self.p(u), u.shape[0], u.shape[1], ttop, tbot, tleft, tright, C.c_double(a))
+        return u
+
+    def delta_m3(self, nx, nz, dx, dz, ox, oz, locx, locz):
+        locx = np.ascontiguousarray(locx, dtype=np.float64)
+        locz = np.ascontiguousarray(locz, dtype=np.float64)
+        n = len(locx)
+        xi = np.zeros((n, 3), np.int32)
+        zi = np.zeros((n, 3), np.int32)
+        xw = np.zeros((n, 2, 3), self.dtype)
+        zw = np.zeros((n, 2, 3), self.dtype)
+        self.lib.orc_delta_m3(nx, nz, C.c_double(dx), C.c_double(dz), C.c_double(ox), C.c_double(oz), n,
+                              _ptr(locx, C.c_double), _ptr(locz, C.c_double), _ptr(xi, C.c_int),
+                              _ptr(zi, C.c_int), self.p(xw), self.p(zw))
+        return xi, zi, xw, zw
+
+    def rhs(self, g, model, curr, prev=None):
+        model, curr = self.arr(model), self.arr(curr)
+        prev = self.arr(prev) if prev is not None else None
+        out = np.zeros_like(curr)
+        rc = self.lib.orc_rhs(C.byref(g), self.p(model), self.p(curr), self.p(prev), self.p(out))
+        assert rc == 0, rc
+        return out
+
+    def propagate(self, g, model, inj, inj_traces, ext=None, full=None, grad=None):
+        """inj = (xi, zi, xw, zw) tables; inj_traces = [2][ninj][nt].  Returns receiver traces
+        [2][next][nt] in forward mode (grad is None)."""
+        model = self.arr(model)
+        xi, zi, xw, zw = inj
+        tr = self.arr(inj_traces)
+        nt = g.nt
+        assert tr.shape[-1] == nt
+        if ext is not None:
+            exi, ezi, exw, ezw = ext
+            nr = exi.shape[0]
+            out = np.zeros((2, nr, nt), self.dtype)
+            ep = (_ptr(exi, C.c_int), _ptr(ezi, C.c_int), self.p(self.arr(exw)), self.p(self.arr(ezw)))
+            outx, outz = self.p(out[0]), self.p(out[1])
+        else:
+            nr, out, ep, outx, outz = 0, None, (None, None, None, None), None, None
+        rc = self.lib.orc_propagate(C.byref(g), self.p(model), xi.shape[0], _ptr(xi, C.c_int), _ptr(zi, C.c_int),
+                                    self.p(self.arr(xw)), self.p(self.arr(zw)), self.p(tr[0]), self.p(tr[1]),
+                                    nr, ep[0], ep[1], ep[2], ep[3], outx, outz,
+                                    self.p(full), self.p(grad))
+        assert rc == 0, "orc_propagate rc=%d" % rc
+        return out
+
+    # ---- model and traces ------------------------------------------------
+    def analyze_model(self, mp, model, dx, dz):
+        model = self.arr(model).copy()
+        self.lib.orc_analyze_model(C.byref(mp), self.p(model), model.shape[0], model.shape[1],
+                                   model.shape[2], C.c_double(dx), C.c_double(dz))
+        return model
+
+    def convert_model(self, model):
+        m = self.arr(model).copy()
+        self.lib.orc_convert_model(self.p(m), m.shape[0], C.c_long(m[0].size))
+        return m
+
+    def sinc(self, adjoint, ntm, dtm, ntd, dtd, hl, mod=None, dat=None):
+        if not adjoint:
+            mod = self.arr(mod)
+            ntr = mod.size // ntm
+            dat = np.zeros((ntr, ntd), self.dtype)
+        else:
+            dat = self.arr(dat)
+            ntr = dat.size // ntd
+            mod = np.zeros((ntr, ntm), self.dtype)
+        self.lib.orc_sinc(int(adjoint), 0, ntr, ntm, C.c_double(dtm), ntd, C.c_double(dtd), hl, self.p(mod),
+                          self.p(dat))
+        return mod if adjoint else dat
+
+    def Dt(self, adj, traces, dt):
+        traces = self.arr(traces)
+        nt = traces.shape[-1]
+        out = np.zeros_like(traces)
+        self.lib.orc_Dt(int(adj), traces.size // nt, nt, C.c_double(dt), self.p(traces), self.p(out))
+        return out
+
+    def Ht(self, inv, traces, dt):
+        traces = self.arr(traces).copy()
+        nt = traces.shape[-1]
+        self.lib.orc_Ht(int(inv), traces.size // nt, nt, C.c_double(dt), self.p(traces))
+        return traces
+
+    # ---- operator level -------------------------------------------------
+    def make_op(self, prob):
+        """prob: dict with grid/bc/taper/vti keys, 'mp' dict, shots (sxz [ns,2]),
+        receivers per shot (list of [n,2] arrays), seismotype, resampling, sinc_hl."""
+        op = OrcOp()
+        op.g = grid_struct(prob)
+        mp = OrcModelPar()
+        for k, v in prob["mp"].items():
+            setattr(mp, k, v)
+        op.mp = mp
+        sxz = np.ascontiguousarray(prob["sxz"], dtype=np.float64)
+        nrec = np.ascontiguousarray([len(r) for r in prob["rxz"]], dtype=np.int32)
+        rxz = np.ascontiguousarray(np.concatenate(prob["rxz"], axis=0), dtype=np.float64)
+        op.ns = len(sxz)
+        op.sxz, op.nrec, op.rxz = _ptr(sxz, C.c_double), _ptr(nrec, C.c_int), _ptr(rxz, C.c_double)
+        op.seismotype = prob.get("seismotype", 0)
+        op.resampling = prob.get("resampling", 0)
+        op.sinc_hl = prob.get("sinc_hl", 11)
+        op._keep = (sxz, nrec, rxz)
+        return op
+
+    def apply_forward(self, op, model, allsrc, keep_full=False):
+        model = self.arr(model)
+        allsrc = self.arr(allsrc)
+        nr = int(op._keep[1].sum())
+        data = np.zeros((2, nr, op.mp.nt_data), self.dtype)
+        full = None
+        if keep_full:
+            # size from a dry analysis of the model
+            tmp = OrcModelPar.from_buffer_copy(op.mp)
+            self.analyze_model(tmp, model, op.g.dx, op.g.dz)
+            sub = tmp.sub_out
+            nsnap = 1 + tmp.nt_prop // sub
+            full = np.zeros((op.ns, nsnap, 2, op.g.nx, op.g.nz), self.dtype)
+        rc = self.lib.orc_apply_forward(C.byref(op), 0, self.p(model), self.p(allsrc), self.p(data), self.p(full))
+        assert rc == 0, rc
+        return data, full
+
+    def apply_jacobianT(self, op, model0, data, full):
+        model0 = self.arr(model0)
+        data = self.arr(data)
+        np_ = 5 if op.g.vti else 3
+        grad = np.zeros((np_, op.g.nx, op.g.nz), self.dtype)
+        rc = self.lib.orc_apply_jacobianT(C.byref(op), 0, self.p(grad), self.p(model0), self.p(data), self.p(full))
+        assert rc == 0, rc
+        return grad
+
+
+class RefSBP:
+    """The reference's own SBP templates and spatial_operators.cpp, compiled from source
+    into _ref/libref_sbp.so (data_t = double)."""
+
+    def __init__(self):
+        path = os.path.join(HERE, "_ref", "libref_sbp.so")
+        if not os.path.exists(path):
+            raise FileNotFoundError(path)
+        self.lib = C.CDLL(path)
+        assert self.lib.ref_data_t_size() == 8
+
+    @staticmethod
+    def p(a):
+        return _ptr(a, C.c_double) if a is not None else None
+
+    @staticmethod
+    def arr(a):
+        return np.ascontiguousarray(a, dtype=np.float64)
+
+    def Dzz_var(self, u, c, d):
+        u, c = self.arr(u), self.arr(c)
+        out = np.zeros_like(u)
+        self.lib.ref_Dzz_var(0, self.p(u), self.p(out), self.p(c), u.shape[0], u.shape[1], C.c_double(d),
+                             C.c_double(1.0))
+        return out
+
+    def Dxx_var(self, u, c, d):
+        u, c = self.arr(u), self.arr(c)
+        out = np.zeros_like(u)
+        self.lib.ref_Dxx_var(0, self.p(u), self.p(out), self.p(c), u.shape[0], u.shape[1], C.c_double(d),
+                             C.c_double(1.0))
+        return out
+
+    def Dzz(self, u, d):
+        u = self.arr(u)
+        out = np.zeros_like(u)
+        self.lib.ref_Dzz(0, self.p(u), self.p(out), u.shape[0], u.shape[1], C.c_double(d))
+        return out
+
+    def esat(self, side, in0, in1, in2, out, f1, f2, f3, dx, dz, dt):
+        out = self.arr(out).copy()
+        args = [self.arr(x) if x is not None else None for x in (in0, in1, in2, f1, f2, f3)]
+        self.lib.ref_esat(side, self.p(args[0]), self.p(args[1]), self.p(args[2]), self.p(out), self.p(args[3]),
+                          self.p(args[4]), self.p(args[5]), out.shape[0], out.shape[1], C.c_double(dx),
+                          C.c_double(dz), C.c_double(dt))
+        return out
+
+    def esat_dn(self, side, u, out, f, dx, dz):
+        out = self.arr(out).copy()
+        u, f = self.arr(u), self.arr(f)
+        self.lib.ref_esat_dn(side, self.p(u), self.p(out), self.p(f), out.shape[0], out.shape[1], C.c_double(dx),
+                             C.c_double(dz))
+        return out
+
+    def scale_boundaries(self, ux, uz, model, vti, dx, dz, dt, top, bottom, left, right):
+        ux, uz, model = self.arr(ux).copy(), self.arr(uz).copy(), self.arr(model)
+        self.lib.ref_scale_boundaries(self.p(ux), self.p(uz), self.p(model), int(vti), ux.shape[0], ux.shape[1],
+                                      C.c_double(dx), C.c_double(dz), C.c_double(dt), int(top), int(bottom),
+                                      int(left), int(right))
+        return ux, uz
+
+    def applyH(self, u, d, axis_x, inv=False):
+        u = self.arr(u)
+        out = np.zeros_like(u)
+        self.lib.ref_applyH(int(axis_x), int(inv), self.p(u), self.p(out), u.shape[0], u.shape[1], C.c_double(d))
+        return out
+
+
+def ref_available():
+    return os.path.exists(os.path.join(HERE, "_ref", "libref_sbp.so"))

@@ -1,0 +1,282 @@
+"""Mathematical pins of the oracle restatement (no reference code involved).
+
+Each check is chosen so that a plausible transcription mistake (a wrong closure
+coefficient, a dropped term, a sign or index error) fails it:
+  * D1: H.D + D^T.H = diag(-1, 0..0, 1) and polynomial exactness (deg 2 at the
+    boundary, deg 4 inside).
+  * D2(c): H.D2(c) = -M(c) - c_0 e_0 S_0^T + c_{n-1} e_{n-1} S_{n-1}^T with M symmetric
+    positive semi-definite and S the one-sided boundary derivative of the SAT terms.
+  * The full 2-D elastic operator (D2var + D1 products + free-surface SAT), iso and VTI,
+    is self-adjoint and non-positive in the H inner product.
+  * Leapfrog with free surfaces conserves the discrete energy.
+  * delta_m3 injection satisfies the H-weighted moment conditions, extraction is exact
+    quadratic interpolation.
+  * cos^2 taper values at the band ends.
+  * Imaging condition: along rho (lambda, mu fixed) the gradient equals the finite-
+    difference derivative of the L2 misfit; along lambda it does too when the
+    reference's one-step lag of the adjoint derivatives is switched off, and with the
+    lag the discrepancy is first order in dt.
+"""
+import numpy as np
+import pytest
+
+from fwi2d_amd import workloads as W
+from oracle.oracle import grid_struct
+
+
+def hvec(n, h=1.0):
+    return h * np.array([17 / 48, 59 / 48, 43 / 48, 49 / 48] + [1] * (n - 8) + [49 / 48, 43 / 48, 59 / 48, 17 / 48])
+
+
+def d1_matrix(orc, n):
+    return orc.Dz(np.eye(n), 1.0).T
+
+
+def test_d1_sbp_identity(orc64):
+    n = 20
+    D = d1_matrix(orc64, n)
+    H = np.diag(hvec(n))
+    B = np.zeros((n, n))
+    B[0, 0], B[-1, -1] = -1, 1
+    assert np.abs(H @ D + D.T @ H - B).max() < 1e-14
+
+
+def test_d1_accuracy(orc64):
+    n = 24
+    D = d1_matrix(orc64, n)
+    x = np.arange(n, dtype=float)
+    for k in range(3):
+        err = D @ x ** k - (k * x ** (k - 1) if k else 0)
+        assert np.abs(err).max() < 1e-11
+    for k in (3, 4):
+        err = D @ x ** k - k * x ** (k - 1)
+        assert np.abs(err[4:-4]).max() < 1e-9 * n ** k
+        assert np.abs(err[:4]).max() > 1e-3  # closure is only second-order: table really differs
+
+
+def test_d2var_summation_by_parts(orc64):
+    n = 22
+    rng = np.random.default_rng(1)
+    c = 1 + rng.random(n)
+    D2 = orc64.Dzz_var(np.eye(n), np.tile(c, (n, 1)), 1.0).T
+    A = np.diag(hvec(n)) @ D2
+    s0 = np.zeros(n)
+    s0[:4] = [-11 / 6, 3, -1.5, 1 / 3]
+    sn = np.zeros(n)
+    sn[-4:] = [-1 / 3, 1.5, -3, 11 / 6]
+    e0, en = np.eye(n)[0], np.eye(n)[-1]
+    M = -(A + c[0] * np.outer(e0, s0) - c[-1] * np.outer(en, sn))
+    assert np.abs(M - M.T).max() < 1e-13
+    assert np.linalg.eigvalsh(M).min() > -1e-12
+
+
+def test_d2var_accuracy_linear_coefficient(orc64):
+    n = 30
+    x = np.arange(n, dtype=float) / n
+    c = 1 + x
+    u = x ** 2
+    out = orc64.Dzz_var(np.tile(u, (2, 1)), np.tile(c, (2, 1)), 1.0 / n)[0]
+    exact = 2 * (1 + x) + 2 * x  # d/dx((1+x) 2x)
+    assert np.abs(out - exact).max() < 1e-9
+
+
+def _model(nx, nz, vti, seed=3):
+    rng = np.random.default_rng(seed)
+    vp = 2 + 0.3 * rng.random((nx, nz))
+    vs = vp / 1.8
+    rho = 2 + 0.2 * rng.random((nx, nz))
+    lam, mu = rho * (vp ** 2 - 2 * vs ** 2), rho * vs ** 2
+    if not vti:
+        return np.stack([lam, mu, rho])
+    delta = 0.05 * rng.random((nx, nz))
+    eps = 0.1 + 0.1 * rng.random((nx, nz))
+    c13 = np.sqrt(2 * (lam + 2 * mu) * (lam + mu) * delta + (lam + mu) ** 2) - mu
+    return np.stack([lam, mu, rho, c13, eps])
+
+
+@pytest.mark.parametrize("vti", [0, 1])
+def test_elastic_operator_self_adjoint(orc64, vti):
+    nx, nz, dx, dz = 14, 13, 0.011, 0.009
+    model = _model(nx, nz, vti)
+    g = grid_struct(dict(nx=nx, nz=nz, dx=dx, dz=dz, nt=2, dt=0.001, sub=0, bc_top=1, bc_bottom=1,
+                         bc_left=1, bc_right=1, vti=vti))
+    N = 2 * nx * nz
+    A = np.zeros((N, N))
+    for k in range(N):
+        e = np.zeros(N)
+        e[k] = 1
+        A[:, k] = orc64.rhs(g, model, e.reshape(2, nx, nz)).ravel()
+    w = np.outer(hvec(nx, dx), hvec(nz, dz)).ravel()
+    K = np.concatenate([w, w])[:, None] * A
+    scale = np.abs(K).max()
+    assert np.abs(K - K.T).max() < 1e-14 * scale
+    assert np.linalg.eigvalsh((K + K.T) / 2).max() < 1e-14 * scale
+
+
+def test_energy_conservation_free_surface(orc64):
+    nx, nz, h = 24, 20, 0.01
+    dt, nt = 0.001, 400
+    model = _model(nx, nz, 0)
+    g = grid_struct(dict(nx=nx, nz=nz, dx=h, dz=h, nt=nt, dt=dt, sub=1, bc_top=1, bc_bottom=1,
+                         bc_left=1, bc_right=1, vti=0))
+    xi, zi, xw, zw = orc64.delta_m3(nx, nz, h, h, 0, 0, [0.113], [0.087])
+    w = W.ricker(nt, dt, 25.0)
+    w[80:] = 0.0
+    full = np.zeros((1 + nt, 2, nx, nz))
+    orc64.propagate(g, model, (xi, zi, xw, zw), np.stack([w[None], w[None]]), full=full)
+    wq = np.outer(hvec(nx, h), hvec(nz, h))
+    rho = model[2]
+    E = []
+    for n in range(100, nt - 1):
+        u0, u1 = full[n], full[n + 1]
+        kin = np.sum(wq * rho * (u1 - u0) ** 2) / dt ** 2
+        pot = -np.sum(wq * (u1 * orc64.rhs(g, model, u0)))
+        E.append(kin + pot)
+    E = np.array(E)
+    assert E.min() > 0
+    assert (E.max() - E.min()) / E.mean() < 1e-10
+
+
+def test_delta_m3_moments(orc64):
+    nx, nz, h = 40, 30, 0.01
+    rng = np.random.default_rng(2)
+    locx = rng.uniform(0, (nx - 1) * h, 20)
+    locz = rng.uniform(0, (nz - 1) * h, 20)
+    locx[:2] = [0.0, (nx - 1) * h]  # pivots shifted off the boundary rows
+    locz[2:4] = [0.0, (nz - 1) * h]
+    xi, zi, xw, zw = orc64.delta_m3(nx, nz, h, h, 0, 0, locx, locz)
+    Hx, Hz = hvec(nx, h), hvec(nz, h)
+    for t in range(20):
+        xs, zs = locx[t], locz[t]
+        X = xi[t] * h
+        Z = zi[t] * h
+        for k in range(3):
+            # injection: sum_i H_i w_i x_i^k = x_s^k
+            assert abs(np.sum(Hx[xi[t]] * xw[t, 0] * X ** k) - xs ** k) < 1e-12
+            assert abs(np.sum(Hz[zi[t]] * zw[t, 0] * Z ** k) - zs ** k) < 1e-12
+            # extraction: quadratic interpolation
+            assert abs(np.sum(xw[t, 1] * X ** k) - xs ** k) < 1e-12
+            assert abs(np.sum(zw[t, 1] * Z ** k) - zs ** k) < 1e-12
+
+
+def test_taper_closed_form(orc64):
+    nx, nz, T, a = 40, 35, 7, 0.08
+    u = orc64.taper(np.ones((nx, nz)), T, T, T, T, a)
+    edge = np.cos(a * np.pi / 2 * (T - 1) / T) ** 2
+    assert abs(u[20, 0] - edge) < 1e-15 and abs(u[20, nz - 1] - edge) < 1e-15
+    assert abs(u[0, 17] - edge) < 1e-15 and abs(u[nx - 1, 17] - edge) < 1e-15
+    assert u[20, T - 1] == 1.0 and u[20, nz - T] == 1.0 and u[20, T] == 1.0
+    assert abs(u[0, 0] - edge ** 2) < 1e-15
+    assert np.all(u[T:nx - T, T:nz - T] == 1.0)
+
+
+def test_sinc_identity_and_adjoint(orc64):
+    rng = np.random.default_rng(4)
+    x = rng.standard_normal((3, 200))
+    # same sampling: interpolation is the identity up to the tiny sinc(pi k) leakage
+    y = orc64.sinc(False, 200, 0.002, 200, 0.002, 11, mod=x)
+    assert np.abs(y - x).max() < 1e-12
+    # adjoint identity <S x, d> = <x, S^T d> for 2 ms -> 0.5 ms
+    d = rng.standard_normal((3, 797))
+    Sx = orc64.sinc(False, 200, 0.002, 797, 0.0005, 11, mod=x)
+    Std = orc64.sinc(True, 200, 0.002, 797, 0.0005, 11, dat=d)
+    assert abs(np.sum(Sx * d) - np.sum(x * Std)) < 1e-10 * abs(np.sum(Sx * d))
+    # band-limited signal is interpolated accurately
+    t = np.arange(200) * 0.002
+    sig = np.sin(2 * np.pi * 20 * t)[None]
+    yi = orc64.sinc(False, 200, 0.002, 797, 0.0005, 11, mod=sig)[0]
+    ti = np.arange(797) * 0.0005
+    assert np.abs(yi[100:-100] - np.sin(2 * np.pi * 20 * ti[100:-100])).max() < 2e-2
+
+
+def test_time_derivative_and_quadrature(orc64):
+    dt = 0.001
+    t = np.arange(100) * dt
+    x = (3 * t ** 2)[None]
+    d = orc64.Dt(False, x, dt)[0]
+    assert np.abs(d[1:-1] - 6 * t[1:-1]).max() < 1e-12
+    # reference adjoint form: Ht^-1 Dt^T Ht (misc.hpp:20-22), in reverse time
+    rng = np.random.default_rng(0)
+    a, b = rng.standard_normal((1, 50)), rng.standard_normal((1, 50))
+    Hta = orc64.Ht(False, a, dt)
+    lhs = np.sum(orc64.Dt(False, a, dt) * orc64.Ht(False, b, dt))
+    brev = b[:, ::-1]
+    adj = orc64.Dt(True, brev, dt)[:, ::-1]
+    rhs = np.sum(Hta * adj)
+    assert abs(lhs - rhs) < 1e-10 * max(abs(lhs), 1)
+
+
+# ---------------------------------------------------------------------------
+# Imaging condition against finite differences of the L2 misfit
+# ---------------------------------------------------------------------------
+def _fd_problem(dt, T=0.45):
+    nx, nz, h = 50, 40, 0.01
+    true = W.add_ellipses(W.gradient_model(nx, nz, h, h, seed=0, pert=0.0), h, h, seed=1)
+    start = W.smooth_model(true, h, h, 0.05)
+    rx = np.stack([np.arange(5, 45) * h, np.full(40, 0.12)], 1)
+    nt = int(round(T / dt)) + 1
+    prob = W.base_problem(nx, nz, h, h, nt, dt, [[0.25, 0.2]], [rx], taper=0, bc=(1, 1, 1, 1), dt=dt, sub=-1)
+    src = W.vertical_force(W.ricker(nt, dt, 10.0), 1)
+    return prob, true, start, src
+
+
+def _misfit_and_gradient(orc, prob, true, start, src):
+    dt = prob["mp"]["dt_data"]
+    op = orc.make_op(prob)
+    dobs, _ = orc.apply_forward(op, true, src)
+    d0, full = orc.apply_forward(op, start, src, keep_full=True)
+
+    def J(m):
+        d, _ = orc.apply_forward(orc.make_op(prob), m, src)
+        r = d - dobs
+        return 0.5 * dt * (np.sum(r ** 2) - 0.5 * np.sum(r[..., 0] ** 2 + r[..., -1] ** 2))
+
+    r = d0 - dobs
+    rw = r * dt
+    rw[..., 0] *= 0.5
+    rw[..., -1] *= 0.5
+    return J, orc.apply_jacobianT(op, start, rw, full)
+
+
+def _directions(start, s):
+    lam = start[2] * (start[0] ** 2 - 2 * start[1] ** 2)
+    mu = start[2] * start[1] ** 2
+    rho0 = start[2]
+
+    def m(l, u, r):
+        return np.stack([np.sqrt((l + 2 * u) / r), np.sqrt(u / r), r])
+
+    return {"rho": lambda e: m(lam, mu, rho0 * (1 + e * s)),
+            "lambda": lambda e: m(lam * (1 + e * s), mu, rho0),
+            "mu": lambda e: m(lam, mu * (1 + e * s), rho0)}
+
+
+def _fd_errors(orc, dt, lag):
+    from scipy.ndimage import gaussian_filter
+    orc.lib.orc_set_imaging_lag(int(lag))
+    try:
+        prob, true, start, src = _fd_problem(dt)
+        J, g = _misfit_and_gradient(orc, prob, true, start, src)
+    finally:
+        orc.lib.orc_set_imaging_lag(1)
+    s = gaussian_filter(np.random.default_rng(5).standard_normal(start[0].shape), 3) * 0.01
+    out = {}
+    for name, path in _directions(start, s).items():
+        fd = (J(path(0.1)) - J(path(-0.1))) / 0.2
+        dm = (path(1e-6) - path(-1e-6)) / 2e-6
+        out[name] = abs(fd - np.sum(g * dm)) / abs(fd)
+    return out
+
+
+@pytest.mark.slow
+def test_imaging_condition_finite_difference(orc64):
+    nolag = _fd_errors(orc64, 0.001, lag=False)
+    assert nolag["rho"] < 1e-5      # mass term: exact discrete gradient
+    assert nolag["lambda"] < 1e-5   # wide-stencil lambda term: exact without the lag
+    assert nolag["mu"] < 0.04       # narrow D2(2mu) vs wide product: reference hazard
+    lag1 = _fd_errors(orc64, 0.001, lag=True)
+    lag2 = _fd_errors(orc64, 0.0005, lag=True)
+    assert lag1["rho"] < 1e-5 and lag2["rho"] < 1e-5
+    # the lag makes the lambda gradient first-order accurate in dt
+    assert 0.005 < lag1["lambda"] < 0.05
+    assert 1.6 < lag1["lambda"] / lag2["lambda"] < 2.6
