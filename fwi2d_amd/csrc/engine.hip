@@ -1,0 +1,846 @@
+// Batched elastic propagation engine: device state, shot tables, checkpointed adjoint.
+//
+// The engine replaces the reference's per-shot device hook propagate_gpu
+// (src/cuda/cudaWeOp.cpp:10-876): instead of one host-driven shot with ~60 launches
+// and ~20 synchronisations per step, every time step of all shots is one fused
+// launch, and the adjoint recomputes forward snapshots from device checkpoints
+// (segments of K snapshots) instead of streaming a host-stored full wavefield.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "fwi2d_hip.h"
+#include "kernels.h"
+#include "sbp_tables.h"
+
+namespace fwi2d {
+
+thread_local std::string g_last_error;
+
+#define HIPCHECK(x)                                                                                   \
+    do {                                                                                              \
+        hipError_t err_ = (x);                                                                        \
+        if (err_ != hipSuccess)                                                                       \
+            throw std::runtime_error(std::string(#x) + ": " + hipGetErrorString(err_));               \
+    } while (0)
+
+struct DevMem {
+    void* p = nullptr;
+    size_t bytes = 0;
+    void reserve(size_t b)
+    {
+        if (b <= bytes) return;
+        release();
+        HIPCHECK(hipMalloc(&p, b));
+        bytes = b;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <class T> T* as() const { return static_cast<T*>(p); }
+    ~DevMem() { release(); }
+};
+
+// CSR of point entries per (shot, tile) -- host build, device copy
+struct TileLists {
+    std::vector<int> ptr, idx;
+    DevMem d_ptr, d_idx;
+    void upload(hipStream_t s)
+    {
+        d_ptr.reserve(std::max<size_t>(ptr.size(), 1) * sizeof(int));
+        d_idx.reserve(std::max<size_t>(idx.size(), 1) * sizeof(int));
+        if (!ptr.empty()) HIPCHECK(hipMemcpyAsync(d_ptr.p, ptr.data(), ptr.size() * sizeof(int), hipMemcpyHostToDevice, s));
+        if (!idx.empty()) HIPCHECK(hipMemcpyAsync(d_idx.p, idx.data(), idx.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    }
+};
+
+static void tile_range_host(int k, int ntile, int T, int n, int& s, int& e)
+{
+    if (ntile == 1) { s = 0; e = n; return; }
+    if (k == ntile - 1) { s = n - T; e = n; return; }
+    s = k * T;
+    e = std::min(s + T, n - T);
+}
+
+struct Engine {
+    fwi2d_grid g{};
+    int device = 0;
+    hipStream_t st = nullptr;
+    int np = 3;
+    long long nxz = 0;
+    int tile_x = 16, ntx = 0, ntz = 0;
+    int opt_tile_x = 16, opt_segment = 0, opt_lag = 1, opt_graph = 0;
+    bool vti = false;
+
+    DevMem d_model, d_div_tb, d_div_lr, d_tx, d_tz;
+    std::vector<float> h_model;
+    long long model_epoch = 0, ckpt_epoch = -1;
+
+    // acquisition
+    int ns = 0, nr = 0;
+    std::vector<PointEntry> src_ents, rcv_ents;  // flat, ordered by shot
+    std::vector<int> src_shot_ptr, rcv_shot_ptr;
+    DevMem d_src_ents, d_rcv_ents, d_src_shot_ptr, d_rcv_shot_ptr;
+    TileLists fwd_inj, fwd_ext, adj_inj;
+
+    // state and checkpoints
+    DevMem d_fwd, d_adj, d_ckpt, d_seg, d_gshot, d_gout, d_src, d_rcv, d_adjsrc, d_partial, d_scalar;
+    int seg_K = 0, nseg = 0, nsnap = 0, seg_loaded = -1;
+    const float* fwd_src = nullptr;  // source traces of the checkpointed forward run (device)
+    long long stat_steps = 0, stat_recompute_steps = 0, stat_image_calls = 0;
+
+    // ------------------------------------------------------------------
+    explicit Engine(const fwi2d_grid& grid, int dev) : g(grid), device(dev)
+    {
+        if (g.nx < 12 || g.nz < 12) throw std::runtime_error("grid must have nx >= 12 and nz >= 12");
+        if (g.nt < 2) throw std::runtime_error("nt must be >= 2");
+        if (!(g.dx > 0 && g.dz > 0 && g.dt > 0)) throw std::runtime_error("dx, dz and dt must be positive");
+        if (g.sub < 0) throw std::runtime_error("sub must be >= 0");
+        if (g.sub > 1 && g.nt % g.sub == 0)
+            throw std::runtime_error("sub > 1 with nt a multiple of sub leaves the last imaging snapshot undefined "
+                                     "(the reference reads an unwritten snapshot); use nt = k*sub + 1");
+        for (int b : {g.bc_top, g.bc_bottom, g.bc_left, g.bc_right})
+            if (b != 1 && b != 2) throw std::runtime_error("boundary conditions must be 1 (free surface) or 2 (absorbing)");
+        vti = g.vti != 0;
+        np = vti ? 5 : 3;
+        nxz = (long long)g.nx * g.nz;
+        HIPCHECK(hipSetDevice(device));
+        HIPCHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        upload_constants(st);
+        set_tile(opt_tile_x);
+        build_tapers();
+    }
+    ~Engine()
+    {
+        if (st) {
+            (void)hipStreamSynchronize(st);
+            (void)hipStreamDestroy(st);
+        }
+    }
+
+    void set_tile(int tx)
+    {
+        if (tx != 8 && tx != 16 && tx != 32) throw std::runtime_error("tile_x must be 8, 16 or 32");
+        tile_x = tx;
+        ntx = tiles_along(g.nx, tile_x);
+        ntz = tiles_along(g.nz, kTileZ);
+        if (ns > 0) build_tile_lists();
+    }
+    int ntiles() const { return ntx * ntz; }
+
+    // cos^2 taper factors per axis (taperz/taperx, spatial_operators.cpp:523-549)
+    void build_tapers()
+    {
+        auto axis = [&](int n, int lo, int hi) {
+            std::vector<float> f(n, 1.f);
+            for (int i = 0; i < n; i++) {
+                if (lo > 0 && i < lo) {
+                    float v = (float)std::cos(g.taper_strength * 0.5 * M_PI * (double)(i + 1 - lo) / (double)(0 - lo));
+                    f[i] *= v * v;
+                }
+                if (hi > 0 && i >= n - hi) {
+                    float v = (float)std::cos(g.taper_strength * 0.5 * M_PI * (double)(i - (n - hi)) / (double)hi);
+                    f[i] *= v * v;
+                }
+            }
+            return f;
+        };
+        std::vector<float> tz = axis(g.nz, g.taper_top, g.taper_bottom);
+        std::vector<float> tx = axis(g.nx, g.taper_left, g.taper_right);
+        d_tz.reserve(tz.size() * sizeof(float));
+        d_tx.reserve(tx.size() * sizeof(float));
+        HIPCHECK(hipMemcpyAsync(d_tz.p, tz.data(), tz.size() * sizeof(float), hipMemcpyHostToDevice, st));
+        HIPCHECK(hipMemcpyAsync(d_tx.p, tx.data(), tx.size() * sizeof(float), hipMemcpyHostToDevice, st));
+        HIPCHECK(hipStreamSynchronize(st));
+    }
+
+    // Divisors 1 + c dt/(2 h h0) of the absorbing boundaries, with the corner
+    // accumulation of esat_scale_boundaries / vtisat_scale_boundaries
+    // (spatial_operators.cpp:307-521) for the full grid.
+    void build_divisors()
+    {
+        const int nx = g.nx, nz = g.nz;
+        const float* la = h_model.data();
+        const float* mu = la + nxz;
+        const float* rho = la + 2 * nxz;
+        const float* eps = vti ? la + 4 * nxz : nullptr;
+        std::vector<double> dvx(nxz, 1.0), dvz(nxz, 1.0);
+        const double h0 = kH0, dt = g.dt, dx = g.dx, dz = g.dz;
+        auto vpx2 = [&](long long i) { return (vti ? (1 + 2 * (double)eps[i]) : 1.0) * ((double)la[i] + 2 * (double)mu[i]); };
+        auto scale = [&](long long i, double sx, double sz) { dvx[i] = 1 + sx; dvz[i] = 1 + sz; };
+        const bool top = g.bc_top == 2, bottom = g.bc_bottom == 2, left = g.bc_left == 2, right = g.bc_right == 2;
+        double sc1x = 0, sc1z = 0, sc2x = 0, sc2z = 0;
+        if (top) {
+            sc1x = std::sqrt((double)mu[0] / rho[0]) * dt / (2 * dz * h0);
+            sc1z = sc1x * std::sqrt(((double)la[0] + 2 * (double)mu[0]) / mu[0]);
+            long long tr = (long long)(nx - 1) * nz;
+            sc2x = std::sqrt((double)mu[tr] / rho[tr]) * dt / (2 * dz * h0);
+            sc2z = sc1x * std::sqrt(((double)la[tr] + 2 * (double)mu[tr]) / mu[tr]);  // reference uses sc1x here
+            for (int ix = 1; ix < nx - 1; ix++) {
+                long long i = (long long)ix * nz;
+                double sx = std::sqrt((double)mu[i] / rho[i]) * dt / (2 * dz * h0);
+                scale(i, sx, sx * std::sqrt(((double)la[i] + 2 * (double)mu[i]) / mu[i]));
+            }
+        }
+        if (left) {
+            sc1x += std::sqrt(vpx2(0) / rho[0]) * dt / (2 * dx * h0);
+            sc1z += std::sqrt((double)mu[0] / rho[0]) * dt / (2 * dx * h0);
+            scale(0, sc1x, sc1z);
+            long long bl = nz - 1;
+            sc1x = std::sqrt(vpx2(bl) / rho[bl]) * dt / (2 * dx * h0);
+            sc1z = sc1x * std::sqrt((double)mu[bl] / vpx2(bl));
+            for (int iz = 1; iz < nz - 1; iz++) {
+                double sx = std::sqrt(vpx2(iz) / rho[iz]) * dt / (2 * dx * h0);
+                scale(iz, sx, sx * std::sqrt((double)mu[iz] / vpx2(iz)));
+            }
+        } else {
+            scale(0, sc1x, sc1z);
+            sc1x = sc1z = 0;
+        }
+        if (bottom) {
+            long long bl = nz - 1;
+            sc1x += std::sqrt((double)mu[bl] / rho[bl]) * dt / (2 * dz * h0);
+            sc1z += std::sqrt(((double)la[bl] + 2 * (double)mu[bl]) / rho[bl]) * dt / (2 * dz * h0);
+            scale(bl, sc1x, sc1z);
+            long long br = (long long)(nx - 1) * nz + nz - 1;
+            sc1x = std::sqrt((double)mu[br] / rho[br]) * dt / (2 * dz * h0);
+            sc1z = sc1x * std::sqrt(((double)la[br] + 2 * (double)mu[br]) / mu[br]);
+            for (int ix = 1; ix < nx - 1; ix++) {
+                long long i = (long long)ix * nz + nz - 1;
+                double sx = std::sqrt((double)mu[i] / rho[i]) * dt / (2 * dz * h0);
+                scale(i, sx, sx * std::sqrt(((double)la[i] + 2 * (double)mu[i]) / mu[i]));
+            }
+        } else {
+            scale(nz - 1, sc1x, sc1z);
+            sc1x = sc1z = 0;
+        }
+        if (right) {
+            long long br = (long long)(nx - 1) * nz + nz - 1;
+            sc1x += std::sqrt(vpx2(br) / rho[br]) * dt / (2 * dx * h0);
+            sc1z += std::sqrt((double)mu[br] / rho[br]) * dt / (2 * dx * h0);
+            scale(br, sc1x, sc1z);
+            for (int iz = 1; iz < nz - 1; iz++) {
+                long long i = (long long)(nx - 1) * nz + iz;
+                double sx = std::sqrt(vpx2(i) / rho[i]) * dt / (2 * dx * h0);
+                scale(i, sx, sx * std::sqrt((double)mu[i] / vpx2(i)));
+            }
+            long long tr = (long long)(nx - 1) * nz;
+            sc2x += std::sqrt(vpx2(tr) / rho[tr]) * dt / (2 * dx * h0);
+            sc2z += std::sqrt((double)mu[tr] / rho[tr]) * dt / (2 * dx * h0);
+            scale(tr, sc2x, sc2z);
+        } else {
+            scale((long long)(nx - 1) * nz + nz - 1, sc1x, sc1z);
+            scale((long long)(nx - 1) * nz, sc2x, sc2z);
+        }
+        // perimeter arrays: top row and bottom row (corners included), left and right columns
+        std::vector<float> tb(4 * (size_t)nx), lr(4 * (size_t)nz);
+        for (int ix = 0; ix < nx; ix++) {
+            tb[ix] = (float)dvx[(long long)ix * nz];
+            tb[nx + ix] = (float)dvz[(long long)ix * nz];
+            tb[2 * nx + ix] = (float)dvx[(long long)ix * nz + nz - 1];
+            tb[3 * nx + ix] = (float)dvz[(long long)ix * nz + nz - 1];
+        }
+        for (int iz = 0; iz < nz; iz++) {
+            lr[iz] = (float)dvx[iz];
+            lr[nz + iz] = (float)dvz[iz];
+            lr[2 * nz + iz] = (float)dvx[(long long)(nx - 1) * nz + iz];
+            lr[3 * nz + iz] = (float)dvz[(long long)(nx - 1) * nz + iz];
+        }
+        d_div_tb.reserve(tb.size() * sizeof(float));
+        d_div_lr.reserve(lr.size() * sizeof(float));
+        HIPCHECK(hipMemcpyAsync(d_div_tb.p, tb.data(), tb.size() * sizeof(float), hipMemcpyHostToDevice, st));
+        HIPCHECK(hipMemcpyAsync(d_div_lr.p, lr.data(), lr.size() * sizeof(float), hipMemcpyHostToDevice, st));
+        HIPCHECK(hipStreamSynchronize(st));
+    }
+
+    void set_model(const float* m)
+    {
+        const size_t n = (size_t)np * nxz;
+        for (size_t i = 0; i < n; i++)
+            if (!std::isfinite(m[i])) throw std::runtime_error("model contains a non-finite value");
+        h_model.assign(m, m + n);
+        const float* rho = m + 2 * nxz;
+        const float* mu = m + nxz;
+        for (long long i = 0; i < nxz; i++)
+            if (!(rho[i] > 0) || !(mu[i] > 0)) throw std::runtime_error("model must have rho > 0 and mu > 0");
+        d_model.reserve(n * sizeof(float));
+        HIPCHECK(hipMemcpyAsync(d_model.p, m, n * sizeof(float), hipMemcpyHostToDevice, st));
+        build_divisors();
+        model_epoch++;
+    }
+
+    // ------------------------------------------------------------------
+    // acquisition: delta_m3 weights (injector.cpp:3-64), tile lists
+    // ------------------------------------------------------------------
+    PointEntry make_point(double x, double z, double ox, double oz, int shot, long long trace) const
+    {
+        const int nx = g.nx, nz = g.nz;
+        const double dx = g.dx, dz = g.dz;
+        int px = (int)std::lround((x - ox) / dx);
+        int pz = (int)std::lround((z - oz) / dz);
+        if (px == 0) px = 1;
+        if (px == nx - 1) px = nx - 2;
+        if (pz == 0) pz = 1;
+        if (pz == nz - 1) pz = nz - 2;
+        if (px < 1 || px > nx - 2 || pz < 1 || pz > nz - 2)
+            throw std::runtime_error("source or receiver falls outside the model");
+        const double ax = (x - dx * px - ox) / dx, az = (z - dz * pz - oz) / dz;
+        PointEntry e{};
+        e.xi0 = px - 1;
+        e.zi0 = pz - 1;
+        e.wx[0] = (float)((ax * ax - ax) / (2 * dx * h_weight(nx, px - 1)));
+        e.wx[1] = (float)((1 - ax * ax) / (dx * h_weight(nx, px)));
+        e.wx[2] = (float)((ax * ax + ax) / (2 * dx * h_weight(nx, px + 1)));
+        e.wz[0] = (float)((az * az - az) / (2 * dz * h_weight(nz, pz - 1)));
+        e.wz[1] = (float)((1 - az * az) / (dz * h_weight(nz, pz)));
+        e.wz[2] = (float)((az * az + az) / (2 * dz * h_weight(nz, pz + 1)));
+        e.trace = trace;
+        e.shot = shot;
+        return e;
+    }
+
+    // extraction weights (Lagrange, injector.cpp:55-61) of the same point
+    static PointEntry extraction_weights(PointEntry e, double x, double z, double ox, double oz, double dx, double dz)
+    {
+        const double ax = (x - dx * (e.xi0 + 1) - ox) / dx, az = (z - dz * (e.zi0 + 1) - oz) / dz;
+        e.wx[0] = (float)((ax * ax - ax) / 2);
+        e.wx[1] = (float)(1 - ax * ax);
+        e.wx[2] = (float)((ax * ax + ax) / 2);
+        e.wz[0] = (float)((az * az - az) / 2);
+        e.wz[1] = (float)(1 - az * az);
+        e.wz[2] = (float)((az * az + az) / 2);
+        return e;
+    }
+
+    std::vector<PointEntry> rcv_ext_ents;  // receivers with extraction weights
+    DevMem d_rcv_ext_ents;
+
+    void set_shots(int nshots, const double* sxz, const int* nrec, const double* rxz, double ox, double oz)
+    {
+        if (nshots < 1) throw std::runtime_error("need at least one shot");
+        ns = nshots;
+        nr = 0;
+        for (int s = 0; s < ns; s++) {
+            if (nrec[s] < 0) throw std::runtime_error("negative receiver count");
+            nr += nrec[s];
+        }
+        const int nt = g.nt;
+        src_ents.clear();
+        rcv_ents.clear();
+        rcv_ext_ents.clear();
+        src_shot_ptr.assign(ns + 1, 0);
+        rcv_shot_ptr.assign(ns + 1, 0);
+        int r = 0;
+        for (int s = 0; s < ns; s++) {
+            src_ents.push_back(make_point(sxz[2 * s], sxz[2 * s + 1], ox, oz, s, (long long)s * nt));
+            src_shot_ptr[s + 1] = s + 1;
+            for (int k = 0; k < nrec[s]; k++, r++) {
+                PointEntry e = make_point(rxz[2 * r], rxz[2 * r + 1], ox, oz, s, (long long)r * nt);
+                rcv_ents.push_back(e);
+                rcv_ext_ents.push_back(extraction_weights(e, rxz[2 * r], rxz[2 * r + 1], ox, oz, g.dx, g.dz));
+            }
+            rcv_shot_ptr[s + 1] = r;
+        }
+        auto up = [&](DevMem& d, const void* h, size_t b) {
+            d.reserve(std::max<size_t>(b, 16));
+            if (b) HIPCHECK(hipMemcpyAsync(d.p, h, b, hipMemcpyHostToDevice, st));
+        };
+        up(d_src_ents, src_ents.data(), src_ents.size() * sizeof(PointEntry));
+        up(d_rcv_ents, rcv_ents.data(), rcv_ents.size() * sizeof(PointEntry));
+        up(d_rcv_ext_ents, rcv_ext_ents.data(), rcv_ext_ents.size() * sizeof(PointEntry));
+        up(d_src_shot_ptr, src_shot_ptr.data(), src_shot_ptr.size() * sizeof(int));
+        up(d_rcv_shot_ptr, rcv_shot_ptr.data(), rcv_shot_ptr.size() * sizeof(int));
+        build_tile_lists();
+        ckpt_epoch = -1;
+        HIPCHECK(hipStreamSynchronize(st));
+    }
+
+    // per-(shot, tile) lists: injectors whose 3x3 footprint meets the tile's write range,
+    // and extractors whose pivot lies in it (the owner reads the footprint from LDS)
+    void build_tile_lists()
+    {
+        const int nt_ = ntiles();
+        auto build = [&](TileLists& L, const std::vector<PointEntry>& ents, bool owner) {
+            std::vector<std::vector<int>> per((size_t)ns * nt_);
+            for (int k = 0; k < (int)ents.size(); k++) {
+                const PointEntry& e = ents[k];
+                for (int tx = 0; tx < ntx; tx++) {
+                    int sx, ex;
+                    tile_range_host(tx, ntx, tile_x, g.nx, sx, ex);
+                    for (int tz = 0; tz < ntz; tz++) {
+                        int sz, ez;
+                        tile_range_host(tz, ntz, kTileZ, g.nz, sz, ez);
+                        bool hit;
+                        if (owner) hit = (e.xi0 + 1 >= sx && e.xi0 + 1 < ex && e.zi0 + 1 >= sz && e.zi0 + 1 < ez);
+                        else hit = (e.xi0 + 2 >= sx && e.xi0 < ex && e.zi0 + 2 >= sz && e.zi0 < ez);
+                        if (hit) per[(size_t)e.shot * nt_ + tx * ntz + tz].push_back(k);
+                    }
+                }
+            }
+            L.ptr.assign(per.size() + 1, 0);
+            L.idx.clear();
+            for (size_t i = 0; i < per.size(); i++) {
+                L.idx.insert(L.idx.end(), per[i].begin(), per[i].end());
+                L.ptr[i + 1] = (int)L.idx.size();
+            }
+            L.upload(st);
+        };
+        build(fwd_inj, src_ents, false);
+        build(fwd_ext, rcv_ext_ents, true);
+        build(adj_inj, rcv_ents, false);
+        HIPCHECK(hipStreamSynchronize(st));
+    }
+
+    // ------------------------------------------------------------------
+    // stepping
+    // ------------------------------------------------------------------
+    long long state_stride() const { return (long long)ns * 2 * nxz; }  // one time level of all shots
+
+    StepArgs base_args() const
+    {
+        StepArgs a{};
+        a.nx = g.nx;
+        a.nz = g.nz;
+        a.ntx = ntx;
+        a.ntz = ntz;
+        a.nshots = ns;
+        a.nt = g.nt;
+        a.rdx = (float)(1.0 / g.dx);
+        a.rdz = (float)(1.0 / g.dz);
+        a.rdx2 = (float)(1.0 / (g.dx * g.dx));
+        a.rdz2 = (float)(1.0 / (g.dz * g.dz));
+        a.dt = (float)g.dt;
+        a.dt2 = (float)g.dt * (float)g.dt;
+        a.sat_x = (float)(1.0 / (g.dx * kH0));
+        a.sat_z = (float)(1.0 / (g.dz * kH0));
+        a.bc_top = g.bc_top;
+        a.bc_bottom = g.bc_bottom;
+        a.bc_left = g.bc_left;
+        a.bc_right = g.bc_right;
+        const float* m = d_model.as<float>();
+        a.lam = m;
+        a.mu = m + nxz;
+        a.rho = m + 2 * nxz;
+        a.c13 = vti ? m + 3 * nxz : nullptr;
+        a.eps = vti ? m + 4 * nxz : nullptr;
+        a.div_tb = d_div_tb.as<float>();
+        a.div_lr = d_div_lr.as<float>();
+        a.taper_x = d_tx.as<float>();
+        a.taper_z = d_tz.as<float>();
+        a.shot_stride = 2 * nxz;
+        return a;
+    }
+
+    // prev/curr/next of one propagation (forward or adjoint)
+    struct Levels {
+        float* b[3];
+        int prev = 0, curr = 1, next = 2;
+        void rotate()
+        {
+            int t = prev;
+            prev = curr;
+            curr = next;
+            next = t;
+        }
+        float* P() const { return b[prev]; }
+        float* C() const { return b[curr]; }
+        float* N() const { return b[next]; }
+    };
+
+    Levels levels_in(DevMem& m)
+    {
+        const long long ss = state_stride();
+        m.reserve((size_t)3 * ss * sizeof(float));
+        Levels L;
+        for (int i = 0; i < 3; i++) L.b[i] = m.as<float>() + i * ss;
+        return L;
+    }
+
+    // zero state, then prev = 1/2 dt^2/rho * inject(trace sample 0)
+    void init_levels(Levels& L, bool adjoint, const float* traces)
+    {
+        const long long ss = state_stride();
+        launch_fill(L.b[0], 3 * ss, 0.f, st);
+        (void)ss;
+        L.prev = 0;
+        L.curr = 1;
+        L.next = 2;
+        if (adjoint)
+            launch_init_prev(L.P(), 2 * nxz, g.nz, ns, d_rcv_shot_ptr.as<int>(), d_rcv_ents.as<PointEntry>(), traces,
+                             (long long)nr * g.nt, st);
+        else
+            launch_init_prev(L.P(), 2 * nxz, g.nz, ns, d_src_shot_ptr.as<int>(), d_src_ents.as<PointEntry>(), traces,
+                             (long long)ns * g.nt, st);
+        launch_scale_prev(L.P(), d_model.as<float>() + 2 * nxz, (float)(0.5 * g.dt * g.dt), nxz, ns, st);
+    }
+
+    void step(Levels& L, int it, bool adjoint, const float* traces, float* rcv)
+    {
+        StepArgs a = base_args();
+        a.it = it;
+        a.taper_prev = it > 0;
+        a.prev = L.P();
+        a.curr = L.C();
+        a.next = L.N();
+        const TileLists& inj = adjoint ? adj_inj : fwd_inj;
+        a.inj_ptr = inj.d_ptr.as<int>();
+        a.inj_idx = inj.d_idx.as<int>();
+        a.inj = adjoint ? d_rcv_ents.as<PointEntry>() : d_src_ents.as<PointEntry>();
+        a.inj_trace = traces;
+        a.inj_comp_stride = (long long)(adjoint ? nr : ns) * g.nt;
+        if (rcv) {
+            a.ext_ptr = fwd_ext.d_ptr.as<int>();
+            a.ext_idx = fwd_ext.d_idx.as<int>();
+            a.ext = d_rcv_ext_ents.as<PointEntry>();
+            a.ext_trace = rcv;
+            a.ext_comp_stride = (long long)nr * g.nt;
+        }
+        launch_step(a, vti, tile_x, st);
+        L.rotate();
+        stat_steps++;
+    }
+
+    // ------------------------------------------------------------------
+    // checkpoint plan: segment j holds snapshots idx in [max(jK-1,0), min((j+1)K, nsnap-1)]
+    // and restarts from the state at level (jK-1)*sub (j >= 1) or from the start (j = 0)
+    // ------------------------------------------------------------------
+    void plan_segments()
+    {
+        nsnap = (g.nt - 1) / g.sub + 1;
+        int K = opt_segment > 0 ? opt_segment : (int)std::ceil(std::sqrt(2.0 * nsnap));
+        K = std::max(1, std::min(K, nsnap));
+        seg_K = K;
+        nseg = (nsnap + K - 1) / K;
+    }
+    int seg_base(int j) const { return std::max(j * seg_K - 1, 0); }
+    int seg_last_idx(int j) const { return std::min((j + 1) * seg_K, nsnap - 1); }
+    float* seg_slot(int j, int idx) { return d_seg.as<float>() + (long long)(idx - seg_base(j)) * state_stride(); }
+    // checkpoints exist for segments j = 1 .. nseg-1
+    float* ckpt_curr(int j) { return d_ckpt.as<float>() + (long long)(2 * (j - 1)) * state_stride(); }
+    float* ckpt_prev(int j) { return d_ckpt.as<float>() + (long long)(2 * (j - 1) + 1) * state_stride(); }
+
+    const float* upload_traces(DevMem& d, const float* h, long long n)
+    {
+        d.reserve((size_t)n * sizeof(float));
+        HIPCHECK(hipMemcpyAsync(d.p, h, (size_t)n * sizeof(float), hipMemcpyHostToDevice, st));
+        return d.as<float>();
+    }
+
+    void require_ready() const
+    {
+        if (ns == 0) throw std::runtime_error("no shots: call fwi2d_engine_set_shots first");
+        if (model_epoch == 0) throw std::runtime_error("no model: call fwi2d_engine_set_model first");
+    }
+
+    void forward(const float* src, float* rcv, bool keep, bool on_device)
+    {
+        require_ready();
+        if (keep && g.sub <= 0) throw std::runtime_error("checkpoints need sub > 0");
+        const long long ntr_s = (long long)2 * ns * g.nt, ntr_r = (long long)2 * nr * g.nt;
+        const float* dsrc = on_device ? src : upload_traces(d_src, src, ntr_s);
+        float* drcv = rcv;
+        if (!on_device) {
+            d_rcv.reserve((size_t)std::max<long long>(ntr_r, 1) * sizeof(float));
+            drcv = d_rcv.as<float>();
+        }
+        if (nr > 0) launch_fill(drcv, ntr_r, 0.f, st);
+        Levels L = levels_in(d_fwd);
+        init_levels(L, false, dsrc);
+        int jlast = -1;
+        if (keep) {
+            plan_segments();
+            d_ckpt.reserve((size_t)std::max(2 * (nseg - 1), 1) * state_stride() * sizeof(float));
+            d_seg.reserve((size_t)(seg_K + 2) * state_stride() * sizeof(float));
+            jlast = nseg - 1;
+        }
+        const int nt = g.nt;
+        for (int it = 0; it < nt - 1; it++) {
+            if (keep) capture(L, it, jlast);
+            step(L, it, false, dsrc, nr > 0 ? drcv : nullptr);
+        }
+        if (keep) capture(L, nt - 1, jlast);
+        if (nr > 0)
+            launch_extract(L.C(), 2 * nxz, g.nz, nxz, d_rcv_ext_ents.as<PointEntry>(), nr, drcv, (long long)nr * nt,
+                           nt - 1, st);
+        if (!on_device && nr > 0)
+            HIPCHECK(hipMemcpyAsync(rcv, drcv, (size_t)ntr_r * sizeof(float), hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+        HIPCHECK(hipGetLastError());
+        if (keep) {
+            ckpt_epoch = model_epoch;
+            seg_loaded = jlast;
+            fwd_src = dsrc;
+        }
+    }
+
+    // during a forward run: store checkpoints and the last segment's snapshots
+    void capture(const Levels& L, int level, int jlast)
+    {
+        const long long ss = state_stride();
+        for (int j = 1; j < nseg; j++)
+            if (level == (j * seg_K - 1) * g.sub && level < g.nt - 1) {
+                launch_copy(ckpt_curr(j), L.C(), ss, st);
+                launch_copy(ckpt_prev(j), L.P(), ss, st);
+            }
+        if (level % g.sub == 0) {
+            int idx = level / g.sub;
+            if (idx >= seg_base(jlast) && idx <= seg_last_idx(jlast)) launch_copy(seg_slot(jlast, idx), L.C(), ss, st);
+        }
+    }
+
+    // recompute the snapshots of segment j from its checkpoint
+    void load_segment(int j, const float* dsrc)
+    {
+        if (seg_loaded == j) return;
+        const long long ss = state_stride();
+        Levels F = levels_in(d_fwd);
+        int start;
+        if (j == 0) {
+            init_levels(F, false, dsrc);
+            start = 0;
+        } else {
+            F.prev = 0;
+            F.curr = 1;
+            F.next = 2;
+            launch_copy(F.C(), ckpt_curr(j), ss, st);
+            launch_copy(F.P(), ckpt_prev(j), ss, st);
+            start = seg_base(j) * g.sub;
+        }
+        const int end = seg_last_idx(j) * g.sub;
+        for (int it = start;; it++) {
+            if (it % g.sub == 0) {
+                int idx = it / g.sub;
+                if (idx >= seg_base(j) && idx <= seg_last_idx(j)) launch_copy(seg_slot(j, idx), F.C(), ss, st);
+            }
+            if (it >= end) break;
+            step(F, it, false, dsrc, nullptr);
+            stat_recompute_steps++;
+        }
+        seg_loaded = j;
+    }
+
+    void image(const Levels& A, int idx, const float* dsrc)
+    {
+        const int j = idx / seg_K;
+        load_segment(j, dsrc);
+        ImageArgs a{};
+        a.nx = g.nx;
+        a.nz = g.nz;
+        a.ntx = ntx;
+        a.ntz = ntz;
+        a.nshots = ns;
+        a.mode = (idx > 0 && idx < g.nt / g.sub) ? 0 : (idx == 0 ? 1 : 2);
+        a.rdx = (float)(1.0 / g.dx);
+        a.rdz = (float)(1.0 / g.dz);
+        const double dtg = g.sub * g.dt;
+        const double t = idx * dtg;
+        double w = 1.0;
+        if (g.gdamp_tmax - t > 0 && t - g.gdamp_tmin >= 0)
+            w = std::cos(g.gdamp_power * M_PI * (t - g.gdamp_tmin) / (g.gdamp_tmax - g.gdamp_tmin));
+        w *= w;
+        a.wlm = (float)(idx == 0 ? w * 0.5 * dtg : w * dtg);
+        a.wrho = (float)(w * 1.0 / dtg);
+        const float* m = d_model.as<float>();
+        a.lam = m;
+        a.mu = m + nxz;
+        a.c13 = vti ? m + 3 * nxz : nullptr;
+        a.eps = vti ? m + 4 * nxz : nullptr;
+        a.adj = A.C();
+        a.adj_prev = A.P();
+        a.shot_stride = 2 * nxz;
+        a.f0 = seg_slot(j, idx);
+        a.fm = (a.mode != 1) ? seg_slot(j, idx - 1) : nullptr;
+        a.fp = (a.mode != 2) ? seg_slot(j, idx + 1) : nullptr;
+        a.fshot_stride = 2 * nxz;
+        a.grad = d_gshot.as<float>();
+        a.ncomp = np;
+        a.gshot_stride = (long long)np * nxz;
+        a.lag = opt_lag;
+        launch_image(a, vti, tile_x, st);
+        stat_image_calls++;
+    }
+
+    void adjoint(const float* adj, float* grad, bool on_device)
+    {
+        require_ready();
+        if (g.sub <= 0) throw std::runtime_error("the adjoint needs sub > 0");
+        if (ckpt_epoch != model_epoch)
+            throw std::runtime_error("no checkpoints of a forward run on the current model: run fwi2d_engine_forward "
+                                     "with keep_checkpoints first");
+        const long long ntr_r = (long long)2 * nr * g.nt;
+        const float* dadj = on_device ? adj : upload_traces(d_adjsrc, adj, std::max<long long>(ntr_r, 1));
+        const float* dsrc = fwd_src;
+        d_gshot.reserve((size_t)ns * np * nxz * sizeof(float));
+        launch_fill(d_gshot.as<float>(), (long long)ns * np * nxz, 0.f, st);
+        Levels A = levels_in(d_adj);
+        init_levels(A, true, dadj);
+        const int nt = g.nt, sub = g.sub;
+        for (int it = 0; it < nt - 1; it++) {
+            if (it != 0 && (nt - 1 - it) % sub == 0) image(A, (nt - 1 - it) / sub, dsrc);
+            step(A, it, true, dadj, nullptr);
+        }
+        image(A, 0, dsrc);
+        float* out = grad;
+        if (!on_device) {
+            d_gout.reserve((size_t)np * nxz * sizeof(float));
+            out = d_gout.as<float>();
+        }
+        launch_sum_shots(d_gshot.as<float>(), (long long)np * nxz, ns, (long long)np * nxz, out, 0, st);
+        if (!on_device) HIPCHECK(hipMemcpyAsync(grad, out, (size_t)np * nxz * sizeof(float), hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+        HIPCHECK(hipGetLastError());
+    }
+
+    double dot_f64(const float* x, const float* y, const float* w, long long n)
+    {
+        const int nb = 1024;
+        d_partial.reserve(nb * sizeof(double));
+        d_scalar.reserve(sizeof(double));
+        launch_dot_f64(x, y, w, n, d_partial.as<double>(), nb, st);
+        launch_reduce_f64(d_partial.as<double>(), nb, d_scalar.as<double>(), st);
+        double out = 0;
+        HIPCHECK(hipMemcpyAsync(&out, d_scalar.p, sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+        return out;
+    }
+};
+
+}  // namespace fwi2d
+
+using fwi2d::Engine;
+struct fwi2d_engine {
+    Engine* impl;
+};
+
+template <class F>
+static int guarded(F&& f)
+{
+    try {
+        f();
+        fwi2d::g_last_error.clear();
+        return 0;
+    } catch (const std::exception& ex) {
+        fwi2d::g_last_error = ex.what();
+        return 1;
+    } catch (...) {
+        fwi2d::g_last_error = "unknown error";
+        return 1;
+    }
+}
+
+extern "C" {
+
+const char* fwi2d_last_error(void) { return fwi2d::g_last_error.c_str(); }
+
+int fwi2d_engine_create(const fwi2d_grid* grid, int device, fwi2d_engine** out)
+{
+    return guarded([&] {
+        if (!grid || !out) throw std::runtime_error("null argument");
+        *out = nullptr;
+        Engine* e = new Engine(*grid, device);
+        *out = new fwi2d_engine{e};
+    });
+}
+
+int fwi2d_engine_set_model(fwi2d_engine* e, const float* model)
+{
+    return guarded([&] {
+        if (!e || !model) throw std::runtime_error("null argument");
+        HIPCHECK(hipSetDevice(e->impl->device));
+        e->impl->set_model(model);
+    });
+}
+
+int fwi2d_engine_set_shots(fwi2d_engine* e, int nshots, const double* sxz, const int* nrec, const double* rxz,
+                           double ox, double oz)
+{
+    return guarded([&] {
+        if (!e || !sxz || !nrec) throw std::runtime_error("null argument");
+        HIPCHECK(hipSetDevice(e->impl->device));
+        e->impl->set_shots(nshots, sxz, nrec, rxz, ox, oz);
+    });
+}
+
+int fwi2d_engine_forward(fwi2d_engine* e, const float* src, float* rcv, int keep_checkpoints, int on_device)
+{
+    return guarded([&] {
+        if (!e || !src) throw std::runtime_error("null argument");
+        HIPCHECK(hipSetDevice(e->impl->device));
+        e->impl->forward(src, rcv, keep_checkpoints != 0, on_device != 0);
+    });
+}
+
+int fwi2d_engine_adjoint(fwi2d_engine* e, const float* adj, float* grad, int on_device)
+{
+    return guarded([&] {
+        if (!e || !adj || !grad) throw std::runtime_error("null argument");
+        HIPCHECK(hipSetDevice(e->impl->device));
+        e->impl->adjoint(adj, grad, on_device != 0);
+    });
+}
+
+int fwi2d_engine_dot_f64(fwi2d_engine* e, const float* a, const float* b, const float* w, long long n, double* out)
+{
+    return guarded([&] {
+        if (!e || !a || !out) throw std::runtime_error("null argument");
+        HIPCHECK(hipSetDevice(e->impl->device));
+        *out = e->impl->dot_f64(a, b, w, n);
+    });
+}
+
+int fwi2d_engine_set_option(fwi2d_engine* e, const char* key, long long value)
+{
+    return guarded([&] {
+        if (!e || !key) throw std::runtime_error("null argument");
+        Engine* x = e->impl;
+        std::string k(key);
+        if (k == "tile_x") {
+            x->opt_tile_x = (int)value;
+            x->set_tile((int)value);
+        } else if (k == "segment") {
+            if (value < 0) throw std::runtime_error("segment must be >= 0");
+            x->opt_segment = (int)value;
+            x->ckpt_epoch = -1;
+        } else if (k == "lag") {
+            x->opt_lag = value ? 1 : 0;
+        } else if (k == "graph") {
+            x->opt_graph = value ? 1 : 0;
+        } else {
+            throw std::runtime_error("unknown option " + k);
+        }
+    });
+}
+
+long long fwi2d_engine_get_stat(fwi2d_engine* e, const char* key)
+{
+    if (!e || !key) return -1;
+    Engine* x = e->impl;
+    std::string k(key);
+    if (k == "steps") return x->stat_steps;
+    if (k == "recompute_steps") return x->stat_recompute_steps;
+    if (k == "image_calls") return x->stat_image_calls;
+    if (k == "segment") return x->seg_K;
+    if (k == "nseg") return x->nseg;
+    if (k == "tile_x") return x->tile_x;
+    if (k == "ntiles") return x->ntiles();
+    if (k == "nrec") return x->nr;
+    if (k == "nshots") return x->ns;
+    return -1;
+}
+
+void fwi2d_engine_destroy(fwi2d_engine* e)
+{
+    if (!e) return;
+    (void)hipSetDevice(e->impl->device);
+    delete e->impl;
+    delete e;
+}
+
+}  // extern "C"

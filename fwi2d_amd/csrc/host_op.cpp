@@ -1,0 +1,396 @@
+// Operator-level drop-in for the reference's nl_we_op_e / nl_we_op_vti
+// (src/include/we_op.hpp:54-86, 150-176): the host pre/post-processing of
+// apply_forward (src/cpp/we_op.cpp:904-1055) and apply_jacobianT
+// (src/cpp/we_op.cpp:1057-1239) around the batched device engine.
+//
+// Trace and gradient post-processing runs on the host in double precision; the
+// time stepping runs on the device in single precision.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "fwi2d_hip.h"
+
+namespace fwi2d {
+extern thread_local std::string g_last_error;
+}
+
+namespace {
+
+const double kPi = 3.14159265358979323846;
+
+void check(int rc)
+{
+    if (rc) throw std::runtime_error(fwi2d_last_error());
+}
+
+// analyzeModel (we_op.cpp:267-474) for 3 or 5 parameters: bounds, the Vs <= Vp/sqrt(2)
+// and Thomsen-delta admissibility clips, and the propagation time axis.
+struct TimeAxis {
+    int nt;
+    double dt;
+    int sub;
+};
+
+TimeAxis analyze_model(const fwi2d_op_desc& d, std::vector<double>& m)
+{
+    const long long n = (long long)d.nx * d.nz;
+    double* vp = m.data();
+    double* vs = vp + n;
+    double* rho = vp + 2 * n;
+    if (d.nparam == 5) {
+        double* del = vp + 3 * n;
+        for (long long i = 0; i < n; i++) {
+            const double v1 = -0.5 * (1 - vs[i] * vs[i] / (vp[i] * vp[i]));
+            const double v2 = 2.0 / (vp[i] * vp[i] / (vs[i] * vs[i]) - 1);
+            if (del[i] < v1) del[i] = 0.9999 * v1;
+            else if (del[i] > v2) del[i] = 0.9999 * v2;
+        }
+    }
+    auto clip = [&](double* a, double lo, double hi) {
+        for (long long i = 0; i < n; i++) a[i] = std::max(lo, std::min(hi, a[i]));
+    };
+    if (!d.soft_clip) {
+        clip(vp, d.vpmin, d.vpmax);
+        clip(vs, d.vsmin, d.vsmax);
+        clip(rho, d.rhomin, d.rhomax);
+        if (d.nparam == 5) {
+            clip(vp + 3 * n, d.deltamin, d.deltamax);
+            clip(vp + 4 * n, d.epsilonmin, d.epsilonmax);
+        }
+        for (long long i = 0; i < n; i++)
+            if (vp[i] < std::sqrt(2.0) * vs[i]) vs[i] = vp[i] / std::sqrt(2.0001);
+    }
+    double vmax = -1e300;
+    for (long long i = 0; i < n; i++) vmax = std::max(vmax, vp[i]);
+    if (d.nparam == 5)
+        for (long long i = 0; i < n; i++) vmax = std::max(vmax, vp[i] * std::sqrt(1 + 2 * vp[4 * n + i]));
+    const double hmin = std::min(d.dx, d.dz);
+    const double tmax = (d.nt_data - 1) * d.dt_data;
+    double dt = d.dt;
+    if (dt == 0) dt = d.courant * hmin / vmax;
+    else if (dt > 0) dt = std::min(dt, d.dt_data);
+    else {
+        dt = d.courant * hmin / vmax;
+        dt = (d.dt_data > dt) ? d.dt_data / std::ceil(d.dt_data / dt) : d.dt_data;
+    }
+    TimeAxis t;
+    t.dt = dt;
+    t.nt = (int)std::lround(tmax / dt) + 1;
+    t.sub = d.sub < 0 ? (int)std::lround(d.dt_data / dt) : d.sub;
+    return t;
+}
+
+// convert_model (we_op.cpp:476-494, 1441-1462): (vp, vs, rho[, delta, eps]) -> (lambda, mu, rho[, c13, eps])
+void convert_model(std::vector<double>& m, int np, long long n)
+{
+    for (long long i = 0; i < n; i++) {
+        const double vp = m[i], vs = m[n + i], rho = m[2 * n + i];
+        const double la = rho * (vp * vp - 2 * vs * vs), mu = rho * vs * vs;
+        m[i] = la;
+        m[n + i] = mu;
+        if (np == 5) m[3 * n + i] = std::sqrt(2 * (la + 2 * mu) * (la + mu) * m[3 * n + i] + (la + mu) * (la + mu)) - mu;
+    }
+}
+
+// sinc_resampler (operator.cpp:808-856, alpha = 0.5) and linear_resampler (760-806):
+// model axis (ntm, dtm) -> data axis (ntd, dtd), both with origin 0.
+void resample(bool adjoint, bool linear, int hl, int ntr, int ntm, double dtm, int ntd, double dtd, double* mod,
+              double* dat)
+{
+    if (adjoint) std::fill(mod, mod + (size_t)ntr * ntm, 0.0);
+    for (int x = 0; x < ntr; x++) {
+        double* pm = mod + (size_t)x * ntm;
+        double* pd = dat + (size_t)x * ntd;
+        for (int itd = 0; itd < ntd; itd++) {
+            const double t = itd * dtd;
+            if (linear) {
+                const int i = std::min(ntm - 1, (int)std::floor(t / dtm));
+                const int j = std::min(ntm - 1, (int)std::ceil(t / dtm));
+                const double wj = (t - i * dtm) / dtm, wi = 1 - wj;
+                if (!adjoint) pd[itd] = wi * pm[i] + wj * pm[j];
+                else { pm[i] += wi * pd[itd]; pm[j] += wj * pd[itd]; }
+                continue;
+            }
+            const int itm0 = (int)std::floor(t / dtm);
+            const int lo = std::max(0, itm0 - hl + 1), hi = std::min(ntm - 1, itm0 + hl);
+            double v = 0;
+            for (int itm = lo; itm <= hi; itm++) {
+                const double arg = kPi * (t - itm * dtm) / dtm;
+                const double s = arg == 0.0 ? 1.0 : std::sin(arg) / arg;
+                const double w = s * (0.5 + 0.5 * std::cos(kPi * (itm - t / dtm) / hl));
+                if (!adjoint) v += pm[itm] * w;
+                else pm[itm] += pd[itd] * w;
+            }
+            if (!adjoint) pd[itd] = v;
+        }
+    }
+}
+
+// Dt (misc.cpp:71-95): forward centred derivative with one-sided ends, or the
+// reference's reverse-time adjoint form
+void time_derivative(bool adj, int ntr, int nt, double dt, const double* in, double* out)
+{
+    for (int x = 0; x < ntr; x++) {
+        const double* a = in + (size_t)x * nt;
+        double* o = out + (size_t)x * nt;
+        if (!adj) {
+            o[0] = (a[1] - a[0]) / dt;
+            for (int t = 1; t < nt - 1; t++) o[t] = (a[t + 1] - a[t - 1]) / (2 * dt);
+            o[nt - 1] = (a[nt - 1] - a[nt - 2]) / dt;
+        } else {
+            o[0] = (a[1] + a[0]) / dt;
+            for (int t = 1; t < nt - 1; t++) o[t] = (a[t + 1] - a[t - 1]) / (2 * dt);
+            o[nt - 1] = (-a[nt - 1] - a[nt - 2]) / dt;
+        }
+    }
+}
+
+double h_weight(int n, int i)
+{
+    static const double w[4] = {17.0 / 48, 59.0 / 48, 43.0 / 48, 49.0 / 48};
+    if (i < 4) return w[i];
+    if (i >= n - 4) return w[n - 1 - i];
+    return 1.0;
+}
+
+}  // namespace
+
+struct fwi2d_op {
+    fwi2d_op_desc d{};
+    std::vector<double> sxz, rxz;
+    std::vector<int> nrec;
+    int nr = 0;
+    std::vector<double> allsrc;  // [2][ns][nt_data]
+    fwi2d_engine* eng = nullptr;
+    fwi2d_grid grid{};
+    TimeAxis axis{0, 0, 0};
+    std::vector<float> ckpt_model;  // converted model of the last checkpointed forward
+    std::vector<float> src_prop;    // source traces at the propagation step
+    bool shots_set = false;
+
+    int np() const { return d.nparam; }
+    long long nxz() const { return (long long)d.nx * d.nz; }
+
+    // model (physical) -> analysed + converted float model; sets the time axis
+    std::vector<float> prepare(const float* pmod)
+    {
+        const long long n = nxz();
+        std::vector<double> m(pmod, pmod + (size_t)np() * n);
+        for (double v : m)
+            if (!std::isfinite(v)) throw std::runtime_error("model contains a non-finite value");
+        TimeAxis t = analyze_model(d, m);
+        convert_model(m, np(), n);
+        ensure_engine(t);
+        return std::vector<float>(m.begin(), m.end());
+    }
+
+    void ensure_engine(const TimeAxis& t)
+    {
+        if (eng && t.nt == axis.nt && t.dt == axis.dt && t.sub == axis.sub) return;
+        if (eng) fwi2d_engine_destroy(eng);
+        eng = nullptr;
+        axis = t;
+        fwi2d_grid g{};
+        g.nx = d.nx;
+        g.nz = d.nz;
+        g.dx = d.dx;
+        g.dz = d.dz;
+        g.nt = t.nt;
+        g.dt = t.dt;
+        g.sub = t.sub;
+        g.bc_top = d.bc_top;
+        g.bc_bottom = d.bc_bottom;
+        g.bc_left = d.bc_left;
+        g.bc_right = d.bc_right;
+        g.taper_top = d.taper_top;
+        g.taper_bottom = d.taper_bottom;
+        g.taper_left = d.taper_left;
+        g.taper_right = d.taper_right;
+        g.taper_strength = d.taper_strength;
+        g.vti = d.nparam == 5;
+        g.gdamp_tmin = d.gdamp_tmin;
+        g.gdamp_tmax = d.gdamp_tmax;
+        g.gdamp_power = d.gdamp_power;
+        grid = g;
+        check(fwi2d_engine_create(&g, d.device, &eng));
+        check(fwi2d_engine_set_shots(eng, d.ns, sxz.data(), nrec.data(), rxz.data(), d.ox, d.oz));
+        ckpt_model.clear();
+        // sources at the propagation step (apply_forward, we_op.cpp:953-958)
+        std::vector<double> sp((size_t)2 * d.ns * t.nt);
+        resample(false, d.resampling == 1, d.sinc_half_length, 2 * d.ns, d.nt_data, d.dt_data, t.nt, t.dt,
+                 allsrc.data(), sp.data());
+        src_prop.assign(sp.begin(), sp.end());
+    }
+
+    void run_forward(const std::vector<float>& m, std::vector<float>& rcv)
+    {
+        check(fwi2d_engine_set_model(eng, m.data()));
+        rcv.assign((size_t)2 * nr * axis.nt, 0.f);
+        const int keep = axis.sub > 0;
+        check(fwi2d_engine_forward(eng, src_prop.data(), rcv.data(), keep, 0));
+        if (keep) ckpt_model = m;
+        else ckpt_model.clear();
+    }
+
+    void apply_forward(bool add, const float* pmod, float* pdat)
+    {
+        std::vector<float> m = prepare(pmod);
+        std::vector<float> rcv;
+        run_forward(m, rcv);
+        const int nt = axis.nt, ntr = 2 * nr;
+        std::vector<double> r(rcv.begin(), rcv.end());
+        if (d.seismotype == 1) {
+            std::vector<double> v(r.size());
+            time_derivative(false, ntr, nt, axis.dt, r.data(), v.data());
+            r.swap(v);
+        }
+        const double alpha = axis.dt / d.dt_data;
+        for (double& x : r) x *= alpha;
+        std::vector<double> out((size_t)ntr * d.nt_data);
+        resample(true, d.resampling == 1, d.sinc_half_length, ntr, d.nt_data, d.dt_data, nt, axis.dt, out.data(), r.data());
+        for (size_t i = 0; i < out.size(); i++) pdat[i] = (float)((add ? (double)pdat[i] : 0.0) + out[i]);
+    }
+
+    void apply_jacobianT(bool add, float* pmod, const float* pmod0, const float* pdat)
+    {
+        std::vector<float> m = prepare(pmod0);
+        if (axis.sub <= 0) throw std::runtime_error("apply_jacobianT needs sub != 0");
+        if (ckpt_model != m) {  // no checkpoints of this model: run the forward first
+            std::vector<float> rcv;
+            run_forward(m, rcv);
+        }
+        const int nt = axis.nt, ntr = 2 * nr;
+        const double dt = axis.dt, alpha = dt / d.dt_data;
+        // adjoint sources (we_op.cpp:1112-1127): resample, Ht^-1, reverse, -alpha, Dt^T
+        std::vector<double> data(pdat, pdat + (size_t)ntr * d.nt_data);
+        std::vector<double> a((size_t)ntr * nt);
+        resample(false, d.resampling == 1, d.sinc_half_length, ntr, d.nt_data, d.dt_data, nt, dt, data.data(), a.data());
+        for (int x = 0; x < ntr; x++) {
+            double* p = a.data() + (size_t)x * nt;
+            for (int t = 0; t < nt; t++) p[t] /= (t == 0 || t == nt - 1) ? 0.5 * dt : dt;
+            std::reverse(p, p + nt);
+            for (int t = 0; t < nt; t++) p[t] *= -alpha;
+        }
+        if (d.seismotype == 1) {
+            std::vector<double> v(a.size());
+            time_derivative(true, ntr, nt, dt, a.data(), v.data());
+            a.swap(v);
+        }
+        std::vector<float> af(a.begin(), a.end());
+        const long long n = nxz();
+        std::vector<float> graw((size_t)np() * n);
+        check(fwi2d_engine_adjoint(eng, af.data(), graw.data(), 0));
+        // H quadrature (we_op.cpp:1192-1199) and chain rule (we_op.cpp:1208-1219)
+        std::vector<double> gd(graw.begin(), graw.end());
+        for (int c = 0; c < 3; c++)
+            for (int ix = 0; ix < d.nx; ix++)
+                for (int iz = 0; iz < d.nz; iz++)
+                    gd[c * n + (long long)ix * d.nz + iz] *= h_weight(d.nz, iz) * d.dz * h_weight(d.nx, ix) * d.dx;
+        for (long long i = 0; i < n; i++) {
+            const double la = m[i], mu = m[n + i], rho = m[2 * n + i];
+            const double gl = gd[i], gm = gd[n + i], gr = gd[2 * n + i];
+            gd[2 * n + i] = (la / rho) * gl + (mu / rho) * gm + gr;
+            gd[n + i] = 2 * std::sqrt(mu * rho) * (gm - 2 * gl);
+            gd[i] = 2 * std::sqrt(rho * (la + 2 * mu)) * gl;
+        }
+        for (long long i = 3 * n; i < np() * n; i++) gd[i] = 0.0;
+        for (long long i = 0; i < np() * n; i++) pmod[i] = (float)((add ? (double)pmod[i] : 0.0) + gd[i]);
+    }
+
+    ~fwi2d_op()
+    {
+        if (eng) fwi2d_engine_destroy(eng);
+    }
+};
+
+template <class F>
+static int op_guard(F&& f)
+{
+    try {
+        f();
+        fwi2d::g_last_error.clear();
+        return 0;
+    } catch (const std::exception& ex) {
+        fwi2d::g_last_error = ex.what();
+        return 1;
+    }
+}
+
+extern "C" {
+
+int fwi2d_op_create(const fwi2d_op_desc* desc, const float* allsrc, fwi2d_op** out)
+{
+    return op_guard([&] {
+        if (!desc || !allsrc || !out) throw std::runtime_error("null argument");
+        const fwi2d_op_desc& d = *desc;
+        if (d.nparam != 3 && d.nparam != 5)
+            throw std::runtime_error("the model must contain 3 (vp, vs, rho) or 5 (vp, vs, rho, delta, eps) parameters");
+        if (d.ns < 1) throw std::runtime_error("need at least one shot");
+        if (d.nt_data < 2 || !(d.dt_data > 0)) throw std::runtime_error("invalid data time axis");
+        fwi2d_op* op = new fwi2d_op;
+        op->d = d;
+        op->sxz.assign(d.sxz, d.sxz + 2 * d.ns);
+        op->nrec.assign(d.nrec, d.nrec + d.ns);
+        op->nr = 0;
+        for (int r : op->nrec) op->nr += r;
+        op->rxz.assign(d.rxz, d.rxz + 2 * (size_t)op->nr);
+        op->d.sxz = nullptr;
+        op->d.nrec = nullptr;
+        op->d.rxz = nullptr;
+        if (op->d.sinc_half_length <= 0) op->d.sinc_half_length = 11;
+        // shot and receiver positions inside the model (analyzeGeometry, we_op.cpp:98-127)
+        const double prec = std::min(d.dx, d.dz) * 1e-3;
+        const double xmax = d.ox + (d.nx - 1) * d.dx, zmax = d.oz + (d.nz - 1) * d.dz;
+        auto inside = [&](double x, double z) {
+            return x - d.ox >= -prec && -prec <= xmax - x && z - d.oz >= -prec && -prec <= zmax - z;
+        };
+        for (int s = 0; s < d.ns; s++)
+            if (!inside(op->sxz[2 * s], op->sxz[2 * s + 1])) {
+                delete op;
+                throw std::runtime_error("Shot falls outside subsurface model boundaries");
+            }
+        for (int r = 0; r < op->nr; r++)
+            if (!inside(op->rxz[2 * r], op->rxz[2 * r + 1])) {
+                delete op;
+                throw std::runtime_error("Receiver " + std::to_string(r) + " falls outside subsurface model boundaries");
+            }
+        op->allsrc.assign(allsrc, allsrc + (size_t)2 * d.ns * d.nt_data);
+        *out = op;
+    });
+}
+
+int fwi2d_op_apply_forward(fwi2d_op* op, int add, const float* pmod, float* pdat)
+{
+    return op_guard([&] {
+        if (!op || !pmod || !pdat) throw std::runtime_error("null argument");
+        op->apply_forward(add != 0, pmod, pdat);
+    });
+}
+
+int fwi2d_op_apply_jacobianT(fwi2d_op* op, int add, float* pmod, const float* pmod0, const float* pdat)
+{
+    return op_guard([&] {
+        if (!op || !pmod || !pmod0 || !pdat) throw std::runtime_error("null argument");
+        op->apply_jacobianT(add != 0, pmod, pmod0, pdat);
+    });
+}
+
+fwi2d_engine* fwi2d_op_engine(fwi2d_op* op) { return op ? op->eng : nullptr; }
+
+int fwi2d_op_time_axis(fwi2d_op* op, int* nt, double* dt, int* sub)
+{
+    return op_guard([&] {
+        if (!op) throw std::runtime_error("null argument");
+        if (nt) *nt = op->axis.nt;
+        if (dt) *dt = op->axis.dt;
+        if (sub) *sub = op->axis.sub;
+    });
+}
+
+void fwi2d_op_destroy(fwi2d_op* op) { delete op; }
+
+}  // extern "C"

@@ -1,0 +1,82 @@
+// Kernel argument blocks and launchers of the elastic propagation engine (gfx950).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace fwi2d {
+
+constexpr int kTileZ = 64;     // one wavefront of lanes along z (the contiguous axis)
+constexpr int kThreads = 256;  // 4 wavefronts per workgroup
+
+// One point injector (delta_m3): 3x3 footprint starting at (xi0, zi0) with separable
+// weights, and the offset of its x-component trace (z component at + comp_stride).
+struct PointEntry {
+    int xi0, zi0;
+    float wx[3], wz[3];
+    long long trace;
+    int shot;
+    int pad;
+};
+
+struct StepArgs {
+    int nx, nz, ntx, ntz, nshots, nt, it, taper_prev;
+    float rdx, rdz, rdx2, rdz2, dt, dt2, sat_x, sat_z;
+    int bc_top, bc_bottom, bc_left, bc_right;
+    const float *lam, *mu, *rho, *c13, *eps;
+    const float *div_tb;   // [2 sides][2 comps][nx] absorbing divisors, top then bottom
+    const float *div_lr;   // [2 sides][2 comps][nz]
+    const float *taper_x, *taper_z;
+    const float* prev;     // shot 0 base of [2][nx][nz]; shot stride below
+    const float* curr;
+    float* next;
+    long long shot_stride;
+    // injection into the new level (sources forward, receivers adjoint)
+    const int* inj_ptr;    // [nshots * ntiles + 1] CSR over (shot, tile)
+    const int* inj_idx;    // indices into inj
+    const PointEntry* inj;
+    const float* inj_trace;
+    long long inj_comp_stride;
+    // extraction of the current level (forward only; ext_ptr may be null)
+    const int* ext_ptr;
+    const int* ext_idx;
+    const PointEntry* ext;
+    float* ext_trace;
+    long long ext_comp_stride;
+};
+
+struct ImageArgs {
+    int nx, nz, ntx, ntz, nshots, mode;   // mode 0 interior, 1 first (idx 0), 2 last
+    float rdx, rdz, wlm, wrho;            // weights: lambda/mu (w*dtg or w*dtg/2), rho (w/dtg)
+    const float *lam, *mu, *c13, *eps;
+    const float* adj;        // current adjoint level, shot 0 base
+    const float* adj_prev;   // previous adjoint level (its derivatives are imaged)
+    long long shot_stride;
+    const float* f0;         // forward snapshot idx   [shot][2][nxz] with stride fshot_stride
+    const float* fm;         // forward snapshot idx-1 (mode 0, 2)
+    const float* fp;         // forward snapshot idx+1 (mode 0, 1)
+    long long fshot_stride;
+    float* grad;             // per-shot raw gradients [shot][ncomp][nxz]
+    int ncomp;
+    long long gshot_stride;
+    int lag;                 // 1: derivatives of adj_prev (reference), 0: of adj
+};
+
+void upload_constants(hipStream_t s);
+
+int tiles_along(int n, int t);
+void launch_step(const StepArgs& a, bool vti, int tile_x, hipStream_t s);
+void launch_init_prev(float* prev, long long shot_stride, int nz, int nshots, const int* shot_ptr,
+                      const PointEntry* ents, const float* trace, long long comp_stride, hipStream_t s);
+void launch_scale_prev(float* prev, const float* rho, float half_dt2, long long nxz, int nshots, hipStream_t s);
+void launch_extract(const float* state, long long shot_stride, int nz, long long nxz, const PointEntry* ents,
+                    int n, float* trace, long long comp_stride, int it, hipStream_t s);
+void launch_image(const ImageArgs& a, bool vti, int tile_x, hipStream_t s);
+void launch_sum_shots(const float* per_shot, long long shot_stride, int nshots, long long n, float* out,
+                      int accumulate, hipStream_t s);
+void launch_fill(float* p, long long n, float v, hipStream_t s);
+void launch_copy(float* dst, const float* src, long long n, hipStream_t s);
+void launch_dot_f64(const float* a, const float* b, const float* w, long long n, double* partial,
+                    int nblocks, hipStream_t s);
+void launch_reduce_f64(const double* partial, int n, double* out, hipStream_t s);
+
+}  // namespace fwi2d

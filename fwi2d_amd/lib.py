@@ -1,0 +1,244 @@
+"""ctypes bindings of libfwi2d_hip.so (the C ABI declared in include/fwi2d_hip.h).
+
+The library is the product: every call below runs the HIP engine.  Loading fails
+loudly when the library has not been built; there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfwi2d_hip.so")
+
+_lib = None
+
+
+class Fwi2dGrid(C.Structure):
+    _fields_ = [("nx", C.c_int), ("nz", C.c_int), ("dx", C.c_double), ("dz", C.c_double),
+                ("nt", C.c_int), ("dt", C.c_double), ("sub", C.c_int),
+                ("bc_top", C.c_int), ("bc_bottom", C.c_int), ("bc_left", C.c_int), ("bc_right", C.c_int),
+                ("taper_top", C.c_int), ("taper_bottom", C.c_int), ("taper_left", C.c_int),
+                ("taper_right", C.c_int), ("taper_strength", C.c_double), ("vti", C.c_int),
+                ("gdamp_tmin", C.c_double), ("gdamp_tmax", C.c_double), ("gdamp_power", C.c_double)]
+
+
+class Fwi2dOpDesc(C.Structure):
+    _fields_ = [("nx", C.c_int), ("nz", C.c_int), ("dx", C.c_double), ("dz", C.c_double),
+                ("ox", C.c_double), ("oz", C.c_double), ("nt_data", C.c_int), ("dt_data", C.c_double),
+                ("nparam", C.c_int), ("ns", C.c_int), ("sxz", C.POINTER(C.c_double)),
+                ("nrec", C.POINTER(C.c_int)), ("rxz", C.POINTER(C.c_double)),
+                ("courant", C.c_double), ("dt", C.c_double), ("sub", C.c_int),
+                ("bc_top", C.c_int), ("bc_bottom", C.c_int), ("bc_left", C.c_int), ("bc_right", C.c_int),
+                ("taper_top", C.c_int), ("taper_bottom", C.c_int), ("taper_left", C.c_int),
+                ("taper_right", C.c_int), ("taper_strength", C.c_double), ("seismotype", C.c_int),
+                ("resampling", C.c_int), ("sinc_half_length", C.c_int),
+                ("vpmin", C.c_double), ("vpmax", C.c_double), ("vsmin", C.c_double), ("vsmax", C.c_double),
+                ("rhomin", C.c_double), ("rhomax", C.c_double), ("deltamin", C.c_double),
+                ("deltamax", C.c_double), ("epsilonmin", C.c_double), ("epsilonmax", C.c_double),
+                ("soft_clip", C.c_int), ("gdamp_tmin", C.c_double), ("gdamp_tmax", C.c_double),
+                ("gdamp_power", C.c_double), ("device", C.c_int)]
+
+
+EXPORTS = [
+    "fwi2d_last_error", "fwi2d_engine_create", "fwi2d_engine_set_model", "fwi2d_engine_set_shots",
+    "fwi2d_engine_forward", "fwi2d_engine_adjoint", "fwi2d_engine_dot_f64", "fwi2d_engine_set_option",
+    "fwi2d_engine_get_stat", "fwi2d_engine_destroy", "fwi2d_op_create", "fwi2d_op_apply_forward",
+    "fwi2d_op_apply_jacobianT", "fwi2d_op_engine", "fwi2d_op_time_axis", "fwi2d_op_destroy",
+]
+
+
+class Fwi2dError(RuntimeError):
+    pass
+
+
+def load():
+    """Load the HIP library (built by fwi2d_amd.build)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise Fwi2dError("%s is missing: build it with `python -m fwi2d_amd.build`" % LIB_PATH)
+    lib = C.CDLL(LIB_PATH)
+    P = C.c_void_p
+    lib.fwi2d_last_error.restype = C.c_char_p
+    lib.fwi2d_engine_create.argtypes = [C.POINTER(Fwi2dGrid), C.c_int, C.POINTER(P)]
+    lib.fwi2d_engine_set_model.argtypes = [P, P]
+    lib.fwi2d_engine_set_shots.argtypes = [P, C.c_int, P, P, P, C.c_double, C.c_double]
+    lib.fwi2d_engine_forward.argtypes = [P, P, P, C.c_int, C.c_int]
+    lib.fwi2d_engine_adjoint.argtypes = [P, P, P, C.c_int]
+    lib.fwi2d_engine_dot_f64.argtypes = [P, P, P, P, C.c_longlong, C.POINTER(C.c_double)]
+    lib.fwi2d_engine_set_option.argtypes = [P, C.c_char_p, C.c_longlong]
+    lib.fwi2d_engine_get_stat.argtypes = [P, C.c_char_p]
+    lib.fwi2d_engine_get_stat.restype = C.c_longlong
+    lib.fwi2d_engine_destroy.argtypes = [P]
+    lib.fwi2d_op_create.argtypes = [C.POINTER(Fwi2dOpDesc), P, C.POINTER(P)]
+    lib.fwi2d_op_apply_forward.argtypes = [P, C.c_int, P, P]
+    lib.fwi2d_op_apply_jacobianT.argtypes = [P, C.c_int, P, P, P]
+    lib.fwi2d_op_engine.argtypes = [P]
+    lib.fwi2d_op_engine.restype = P
+    lib.fwi2d_op_time_axis.argtypes = [P, C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_int)]
+    lib.fwi2d_op_destroy.argtypes = [P]
+    _lib = lib
+    return lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise Fwi2dError(load().fwi2d_last_error().decode())
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+def grid_from(prob, nt, dt, sub):
+    g = Fwi2dGrid()
+    for k in ("nx", "nz", "dx", "dz", "bc_top", "bc_bottom", "bc_left", "bc_right", "taper_top", "taper_bottom",
+              "taper_left", "taper_right", "taper_strength", "vti", "gdamp_tmin", "gdamp_tmax", "gdamp_power"):
+        if k in prob:
+            setattr(g, k, prob[k])
+    g.nt, g.dt, g.sub = int(nt), float(dt), int(sub)
+    return g
+
+
+class Engine:
+    """Batched device propagation (engine level of the C ABI)."""
+
+    def __init__(self, grid, device=0):
+        lib = load()
+        self._h = C.c_void_p()
+        self.grid = grid
+        _check(lib.fwi2d_engine_create(C.byref(grid), int(device), C.byref(self._h)))
+        self.np = 5 if grid.vti else 3
+
+    def close(self):
+        if self._h:
+            load().fwi2d_engine_destroy(self._h)
+            self._h = C.c_void_p()
+
+    __del__ = close
+
+    def set_model(self, model):
+        m = _f32(model)
+        assert m.shape == (self.np, self.grid.nx, self.grid.nz), m.shape
+        _check(load().fwi2d_engine_set_model(self._h, _ptr(m)))
+
+    def set_shots(self, sxz, rxz, ox=0.0, oz=0.0):
+        sxz = np.ascontiguousarray(sxz, dtype=np.float64).reshape(-1, 2)
+        nrec = np.ascontiguousarray([len(r) for r in rxz], dtype=np.int32)
+        allr = np.ascontiguousarray(np.concatenate([np.asarray(r, np.float64).reshape(-1, 2) for r in rxz], 0))
+        self.ns, self.nr = len(sxz), int(nrec.sum())
+        self._geom = (sxz, nrec, allr)
+        _check(load().fwi2d_engine_set_shots(self._h, self.ns, _ptr(sxz), _ptr(nrec), _ptr(allr), ox, oz))
+
+    def forward(self, src, keep_checkpoints=False):
+        """src [2][ns][nt] (host numpy) -> receivers [2][nr][nt]."""
+        s = _f32(src)
+        assert s.shape == (2, self.ns, self.grid.nt), s.shape
+        rcv = np.zeros((2, self.nr, self.grid.nt), np.float32)
+        _check(load().fwi2d_engine_forward(self._h, _ptr(s), _ptr(rcv), int(keep_checkpoints), 0))
+        return rcv
+
+    def forward_device(self, src_ptr, rcv_ptr, keep_checkpoints=False):
+        _check(load().fwi2d_engine_forward(self._h, C.c_void_p(src_ptr), C.c_void_p(rcv_ptr),
+                                           int(keep_checkpoints), 1))
+
+    def adjoint(self, adj):
+        a = _f32(adj)
+        assert a.shape == (2, self.nr, self.grid.nt), a.shape
+        g = np.zeros((self.np, self.grid.nx, self.grid.nz), np.float32)
+        _check(load().fwi2d_engine_adjoint(self._h, _ptr(a), _ptr(g), 0))
+        return g
+
+    def adjoint_device(self, adj_ptr, grad_ptr):
+        _check(load().fwi2d_engine_adjoint(self._h, C.c_void_p(adj_ptr), C.c_void_p(grad_ptr), 1))
+
+    def dot_f64(self, a_ptr, b_ptr, w_ptr, n):
+        out = C.c_double()
+        _check(load().fwi2d_engine_dot_f64(self._h, C.c_void_p(a_ptr), C.c_void_p(b_ptr) if b_ptr else None,
+                                           C.c_void_p(w_ptr) if w_ptr else None, int(n), C.byref(out)))
+        return out.value
+
+    def set_option(self, key, value):
+        _check(load().fwi2d_engine_set_option(self._h, key.encode(), int(value)))
+
+    def stat(self, key):
+        return load().fwi2d_engine_get_stat(self._h, key.encode())
+
+
+def op_desc_from(prob, device=0):
+    """Operator description from a problem dict (see fwi2d_amd.workloads)."""
+    d = Fwi2dOpDesc()
+    mp = prob["mp"]
+    for k in ("nx", "nz", "dx", "dz", "ox", "oz", "bc_top", "bc_bottom", "bc_left", "bc_right", "taper_top",
+              "taper_bottom", "taper_left", "taper_right", "taper_strength", "seismotype", "resampling",
+              "gdamp_tmin", "gdamp_tmax", "gdamp_power"):
+        if k in prob:
+            setattr(d, k, prob[k])
+    d.sinc_half_length = int(prob.get("sinc_hl", 11))
+    d.nparam = 5 if prob.get("vti") else 3
+    d.nt_data, d.dt_data = int(mp["nt_data"]), float(mp["dt_data"])
+    d.courant, d.dt, d.sub = float(mp["courant"]), float(mp["dt"]), int(mp["sub"])
+    d.vpmin, d.vpmax, d.vsmin, d.vsmax = mp["vpmin"], mp["vpmax"], mp["vsmin"], mp["vsmax"]
+    d.rhomin, d.rhomax = mp["rhomin"], mp["rhomax"]
+    d.deltamin, d.deltamax, d.epsilonmin, d.epsilonmax = mp["deltamin"], mp["deltamax"], mp["epsmin"], mp["epsmax"]
+    d.soft_clip = int(mp.get("soft_clip", 0))
+    d.device = int(device)
+    return d
+
+
+class WeOp:
+    """Drop-in for the reference's nl_we_op_e / nl_we_op_vti: apply_forward and
+    apply_jacobianT on host numpy arrays, computed by the HIP engine."""
+
+    def __init__(self, prob, allsrc, device=0):
+        lib = load()
+        self.prob = prob
+        self.np = 5 if prob.get("vti") else 3
+        d = op_desc_from(prob, device)
+        sxz = np.ascontiguousarray(prob["sxz"], dtype=np.float64)
+        nrec = np.ascontiguousarray([len(r) for r in prob["rxz"]], dtype=np.int32)
+        rxz = np.ascontiguousarray(np.concatenate(prob["rxz"], 0), dtype=np.float64)
+        d.ns = len(sxz)
+        d.sxz = sxz.ctypes.data_as(C.POINTER(C.c_double))
+        d.nrec = nrec.ctypes.data_as(C.POINTER(C.c_int))
+        d.rxz = rxz.ctypes.data_as(C.POINTER(C.c_double))
+        self.nr = int(nrec.sum())
+        src = _f32(allsrc)
+        assert src.shape == (2, d.ns, d.nt_data), src.shape
+        self._h = C.c_void_p()
+        _check(lib.fwi2d_op_create(C.byref(d), _ptr(src), C.byref(self._h)))
+        self.nt_data = d.nt_data
+
+    def close(self):
+        if self._h:
+            load().fwi2d_op_destroy(self._h)
+            self._h = C.c_void_p()
+
+    __del__ = close
+
+    def apply_forward(self, model, add_to=None):
+        m = _f32(model)
+        out = np.zeros((2, self.nr, self.nt_data), np.float32) if add_to is None else _f32(add_to).copy()
+        _check(load().fwi2d_op_apply_forward(self._h, int(add_to is not None), _ptr(m), _ptr(out)))
+        return out
+
+    def apply_jacobianT(self, model0, data, add_to=None):
+        m0 = _f32(model0)
+        d = _f32(data)
+        g = np.zeros((self.np,) + m0.shape[1:], np.float32) if add_to is None else _f32(add_to).copy()
+        _check(load().fwi2d_op_apply_jacobianT(self._h, int(add_to is not None), _ptr(g), _ptr(m0), _ptr(d)))
+        return g
+
+    def time_axis(self):
+        nt, dt, sub = C.c_int(), C.c_double(), C.c_int()
+        _check(load().fwi2d_op_time_axis(self._h, C.byref(nt), C.byref(dt), C.byref(sub)))
+        return nt.value, dt.value, sub.value
+
+    def engine_handle(self):
+        return load().fwi2d_op_engine(self._h)

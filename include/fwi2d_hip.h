@@ -1,0 +1,151 @@
+/*
+ * fwi2d_hip.h -- C ABI of the MI355X (gfx950) elastic propagation engine.
+ *
+ * Two levels, both plain C (pointers, sizes, status codes):
+ *
+ *  1. Operator level: a drop-in for the reference's non-linear elastic operators
+ *     nl_we_op_e / nl_we_op_vti (src/include/we_op.hpp:54-86, 150-176): the
+ *     apply_forward / apply_jacobianT pair that the reference's objectives
+ *     (nlls_fwi, nlls_fwi_eco: src/include/optimization.hpp:351, 452, 656, 890)
+ *     and WE_MODELING.x (src/main/we_modeling.cpp:73) call.
+ *
+ *  2. Engine level: batched device propagation that replaces the per-shot device
+ *     hook nl_we_op_e::propagate_gpu (src/include/we_op.hpp:82, 174; body in
+ *     src/cuda/cudaWeOp.cpp:10-876).  Many shots advance together; the adjoint
+ *     recomputes the forward wavefield from device-resident checkpoints instead of
+ *     reading a host-stored full wavefield.
+ *
+ * Conventions: arrays are [slowest]...[fastest]; models [np][nx][nz] (z fastest),
+ * traces [component][trace][time].  Status 0 is success; on failure the call
+ * returns non-zero and fwi2d_last_error() describes the error (the reference throws
+ * std::logic_error from successCheck, src/cpp/misc.cpp:4-9).  Calls are synchronous
+ * at return.  One engine or operator per host thread.
+ */
+#ifndef FWI2D_HIP_H
+#define FWI2D_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------- */
+/* Engine level                                                               */
+/* ------------------------------------------------------------------------- */
+
+/* Propagation grid and boundary set-up; the fields mean what the same-named param
+   fields mean in the reference (src/include/param.hpp:10-41). */
+typedef struct {
+    int nx, nz;            /* model axes 2 and 1 */
+    double dx, dz;
+    int nt;                /* number of propagation time steps (par.nt) */
+    double dt;             /* propagation time step (par.dt) */
+    int sub;               /* imaging/snapshot sub-sampling (par.sub); 0 = forward only */
+    int bc_top, bc_bottom, bc_left, bc_right;              /* 1 free surface, 2 absorbing */
+    int taper_top, taper_bottom, taper_left, taper_right;  /* cos^2 taper widths (cells) */
+    double taper_strength;
+    int vti;               /* 0: model (lambda, mu, rho); 1: (lambda, mu, rho, c13, eps) */
+    double gdamp_tmin, gdamp_tmax, gdamp_power;            /* gradient time window */
+} fwi2d_grid;
+
+typedef struct fwi2d_engine fwi2d_engine;
+
+/* Create an engine on HIP device `device`.  No reference counterpart: replaces the
+   per-call set-up of propagate_gpu (src/cuda/cudaWeOp.cpp:12-58). */
+int fwi2d_engine_create(const fwi2d_grid* grid, int device, fwi2d_engine** out);
+
+/* Upload the converted model [np][nx][nz] (np = 3 iso, 5 vti): lambda, mu, rho[, c13, eps],
+   as produced by convert_model (src/cpp/we_op.cpp:476-494, 1441-1462).  Invalidates
+   checkpoints of a previous forward run. */
+int fwi2d_engine_set_model(fwi2d_engine* e, const float* model);
+
+/* Acquisition: nshots sources at sxz[s] = (x, z) and nrec[s] point receivers per shot
+   at rxz (all shots concatenated, (x, z) pairs), grid origin (ox, oz).  The point
+   injector/extractor weights are those of delta_m3 (src/cpp/injector.cpp:3-64). */
+int fwi2d_engine_set_shots(fwi2d_engine* e, int nshots, const double* sxz, const int* nrec,
+                           const double* rxz, double ox, double oz);
+
+/* Forward propagation of all shots (propagate(adj=false), src/cpp/we_op.cpp:564-902).
+   src: [2][nshots][nt] source traces at the propagation step (x then z force);
+   rcv: [2][nrec_total][nt] displacement traces (output).  keep_checkpoints != 0 keeps
+   the device checkpoints needed by fwi2d_engine_adjoint.  on_device != 0 means src and
+   rcv are device pointers (inputs already resident in HBM). */
+int fwi2d_engine_forward(fwi2d_engine* e, const float* src, float* rcv, int keep_checkpoints, int on_device);
+
+/* Adjoint propagation with imaging (propagate(adj=true) + compute_gradients,
+   src/cpp/we_op.cpp:505-562, 1478-1557).  adj: [2][nrec_total][nt] adjoint source
+   traces in adjoint time order (already time-reversed and scaled, as prepared by
+   apply_jacobianT, src/cpp/we_op.cpp:1112-1127); grad: [np][nx][nz] raw gradient
+   (before H quadrature and chain rule) summed over shots, overwritten.  Needs the
+   checkpoints of a forward run on the same model. */
+int fwi2d_engine_adjoint(fwi2d_engine* e, const float* adj, float* grad, int on_device);
+
+/* FP64 reduction on the device: out = sum_i a_i * b_i * w_i (b, w may be NULL; b NULL
+   means b = a).  Misfit, norms and line-search dot products (the data_t reductions of
+   src/include/vecReg.hpp:257-325).  Pointers are device pointers. */
+int fwi2d_engine_dot_f64(fwi2d_engine* e, const float* a, const float* b, const float* w, long long n,
+                         double* out);
+
+/* Tuning and test options: "tile_x" (8|16|32, 0 = autotune), "segment" (snapshots per
+   checkpoint segment, 0 = auto), "lag" (1 reference imaging lag, 0 none), "graph"
+   (1 capture the step sequence in a hipGraph), "batch" (shots per launch, 0 = all). */
+int fwi2d_engine_set_option(fwi2d_engine* e, const char* key, long long value);
+long long fwi2d_engine_get_stat(fwi2d_engine* e, const char* key);
+
+void fwi2d_engine_destroy(fwi2d_engine* e);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* fwi2d_last_error(void);
+
+/* ------------------------------------------------------------------------- */
+/* Operator level (drop-in for nl_we_op_e / nl_we_op_vti)                      */
+/* ------------------------------------------------------------------------- */
+
+/* Operator description: the reference's param fields used on this path
+   (src/include/param.hpp:10-82) plus the model domain and source time axis.
+   Point forces (mt=0) and point receivers (gl=0) are supported. */
+typedef struct {
+    int nx, nz; double dx, dz, ox, oz;            /* model domain axes 2, 1 */
+    int nt_data; double dt_data;                  /* source/data time axis (origin 0) */
+    int nparam;                                   /* 3 iso (vp,vs,rho) or 5 vti (+delta,eps) */
+    int ns; const double* sxz;                    /* shots (x, z) */
+    const int* nrec; const double* rxz;           /* receivers per shot, (x, z) concatenated */
+    double courant, dt;                           /* par.courant, par.dt (<0 auto) */
+    int sub;                                      /* par.sub (<0 auto, 0 forward only) */
+    int bc_top, bc_bottom, bc_left, bc_right;
+    int taper_top, taper_bottom, taper_left, taper_right; double taper_strength;
+    int seismotype;                               /* 0 displacement, 1 particle velocity */
+    int resampling;                               /* 0 sinc, 1 linear */
+    int sinc_half_length;
+    double vpmin, vpmax, vsmin, vsmax, rhomin, rhomax, deltamin, deltamax, epsilonmin, epsilonmax;
+    int soft_clip;
+    double gdamp_tmin, gdamp_tmax, gdamp_power;
+    int device;
+} fwi2d_op_desc;
+
+typedef struct fwi2d_op fwi2d_op;
+
+/* Constructor of nl_we_op_e / nl_we_op_vti (we_op.hpp:58-66): allsrc is the source
+   after analyzeWavelet, [2][ns][nt_data]. */
+int fwi2d_op_create(const fwi2d_op_desc* desc, const float* allsrc, fwi2d_op** out);
+
+/* nl_we_op::apply_forward (we_op.hpp:48; we_op.cpp:904-1055):
+   pdat = add*pdat + F(pmod); pmod [nparam][nx][nz] physical model (km/s, g/cc),
+   pdat [2][nrec_total][nt_data].  Keeps device checkpoints for apply_jacobianT. */
+int fwi2d_op_apply_forward(fwi2d_op* op, int add, const float* pmod, float* pdat);
+
+/* nl_we_op::apply_jacobianT (we_op.hpp:49; we_op.cpp:1057-1239):
+   pmod = add*pmod + J(pmod0)^T pdat, gradient w.r.t. (vp, vs, rho[, 0, 0]). */
+int fwi2d_op_apply_jacobianT(fwi2d_op* op, int add, float* pmod, const float* pmod0, const float* pdat);
+
+/* Access to the engine of an operator (batch options, statistics). */
+fwi2d_engine* fwi2d_op_engine(fwi2d_op* op);
+
+/* Propagation time axis chosen by analyzeModel for the last call. */
+int fwi2d_op_time_axis(fwi2d_op* op, int* nt, double* dt, int* sub);
+
+void fwi2d_op_destroy(fwi2d_op* op);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FWI2D_HIP_H */

@@ -1,0 +1,213 @@
+"""GPU parity: the HIP engine against the FP64 oracle, through the C ABI.
+
+Tolerances (relative L2 per array):
+  * seismograms: <= max(1e-5, 2 x the FP32-vs-FP64 gap of the reference algorithm on the
+    same case).  The reference algorithm run in FP32 (oracle built with REAL=float) is
+    itself 1e-5..5e-5 away from FP64 after a few hundred to a few thousand steps, so
+    1e-5 alone is not attainable by any FP32 implementation of this scheme; the gap is
+    measured live and printed.
+  * gradients: <= 1e-4 per parameter.
+Bit-exact checks: results must not depend on the shot batch, the tile width, or the
+checkpoint segment length.
+"""
+import numpy as np
+import pytest
+
+from fwi2d_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(b))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    import fwi2d_amd
+    return fwi2d_amd
+
+
+def engine_case(nx=130, nz=90, nt=500, vti=False, bc=(1, 2, 2, 2), taper=20, ns=2, seed=0, f=15.0):
+    h, dt = 0.01, 0.001
+    m = W.gradient_model(nx, nz, h, h, seed=seed)
+    if vti:
+        m = np.concatenate([m, W.vti_fields(nx, nz, h, h)], 0)
+    rng = np.random.default_rng(seed)
+    sxz = np.stack([rng.uniform(0.05, (nx - 1) * h - 0.05, ns), rng.uniform(0.0, 0.3, ns)], 1)
+    sxz[0] = [0.0, 0.0]  # corner source: pivot shifted off the boundary rows
+    rxz = []
+    for s in range(ns):
+        n = 10 + 7 * s
+        x = np.linspace(0.0, (nx - 1) * h, n)
+        z = np.full(n, 0.02 if s % 2 == 0 else (nz - 1) * h)
+        r = np.stack([x, z], 1)
+        r[1] = r[2] + [0.004, 0.0]  # two receivers closer than the footprint
+        rxz.append(r)
+    prob = dict(nx=nx, nz=nz, dx=h, dz=h, bc_top=bc[0], bc_bottom=bc[1], bc_left=bc[2], bc_right=bc[3],
+                taper_top=taper if bc[0] == 2 else 0, taper_bottom=taper if bc[1] == 2 else 0,
+                taper_left=taper if bc[2] == 2 else 0, taper_right=taper if bc[3] == 2 else 0,
+                taper_strength=0.05, vti=int(vti), gdamp_tmin=0.0, gdamp_tmax=0.0, gdamp_power=1.0)
+    lam = m[2] * (m[0] ** 2 - 2 * m[1] ** 2)
+    mu = m[2] * m[1] ** 2
+    conv = [lam, mu, m[2]]
+    if vti:
+        conv += [np.sqrt(2 * (lam + 2 * mu) * (lam + mu) * m[3] + (lam + mu) ** 2) - mu, m[4]]
+    model = np.stack(conv)
+    wav = np.stack([W.ricker(nt, dt, f * (1 + 0.1 * s)) for s in range(ns)])
+    src = np.stack([wav * 0.3, wav])
+    return prob, model, sxz, rxz, src, nt, dt
+
+
+def oracle_forward(orc, prob, model, sxz, rxz, src, nt, dt, sub=0, full=None):
+    from oracle.oracle import grid_struct
+    p = dict(prob, nt=nt, dt=dt, sub=sub)
+    g = grid_struct(p)
+    out = []
+    for s in range(len(sxz)):
+        inj = orc.delta_m3(p["nx"], p["nz"], p["dx"], p["dz"], 0, 0, [sxz[s][0]], [sxz[s][1]])
+        ext = orc.delta_m3(p["nx"], p["nz"], p["dx"], p["dz"], 0, 0, rxz[s][:, 0], rxz[s][:, 1])
+        out.append(orc.propagate(g, model, inj, src[:, s:s + 1, :], ext=ext,
+                                 full=None if full is None else full[s]))
+    return np.concatenate(out, axis=1)
+
+
+def run_engine(lib, prob, model, sxz, rxz, src, nt, dt, sub=0, tile_x=16, keep=False):
+    eng = lib.Engine(lib.grid_from(prob, nt, dt, sub))
+    eng.set_option("tile_x", tile_x)
+    eng.set_model(model)
+    eng.set_shots(sxz, rxz)
+    return eng, eng.forward(src, keep_checkpoints=keep)
+
+
+@pytest.mark.parametrize("vti", [False, True])
+@pytest.mark.parametrize("bc", [(1, 2, 2, 2), (1, 1, 1, 1), (2, 2, 2, 2)])
+def test_forward_traces_match_oracle(lib, orc64, orc32, vti, bc):
+    case = engine_case(vti=vti, bc=bc)
+    prob, model, sxz, rxz, src, nt, dt = case
+    ref = oracle_forward(orc64, *case)
+    gap = rel(oracle_forward(orc32, *case), ref)
+    _, out = run_engine(lib, *case)
+    err = rel(out, ref)
+    tol = max(1e-5, 2 * gap)
+    print("vti=%s bc=%s engine-vs-fp64 %.3e  reference-fp32-vs-fp64 %.3e  tol %.1e" % (vti, bc, err, gap, tol))
+    assert err <= tol
+
+
+@pytest.mark.parametrize("nx,nz", [(12, 12), (13, 19), (21, 69), (37, 70), (69, 133)])
+def test_forward_small_and_ragged_grids(lib, orc64, orc32, nx, nz):
+    case = engine_case(nx=nx, nz=nz, nt=200, taper=4, ns=1)
+    ref = oracle_forward(orc64, *case)
+    gap = rel(oracle_forward(orc32, *case), ref)
+    for tx in (8, 16, 32):
+        _, out = run_engine(lib, *case, tile_x=tx)
+        err = rel(out, ref)
+        print("nx=%d nz=%d tile_x=%d err %.3e gap %.3e" % (nx, nz, tx, err, gap))
+        assert err <= max(1e-5, 2 * gap)
+
+
+def test_tile_width_and_batch_are_bit_exact(lib):
+    case = engine_case(ns=3)
+    prob, model, sxz, rxz, src, nt, dt = case
+    _, a = run_engine(lib, *case, tile_x=16)
+    _, b = run_engine(lib, *case, tile_x=8)
+    _, c = run_engine(lib, *case, tile_x=32)
+    assert np.array_equal(a, b) and np.array_equal(a, c)
+    # each shot alone gives the same traces as the batch
+    r0 = 0
+    for s in range(3):
+        _, one = run_engine(lib, prob, model, sxz[s:s + 1], rxz[s:s + 1], src[:, s:s + 1], nt, dt)
+        n = len(rxz[s])
+        assert np.array_equal(one, a[:, r0:r0 + n])
+        r0 += n
+
+
+def adjoint_case(vti=False, nt=401, sub=1):
+    case = list(engine_case(vti=vti, nt=nt, ns=2))
+    return case
+
+
+@pytest.mark.parametrize("vti", [False, True])
+def test_raw_gradient_matches_oracle(lib, orc64, vti):
+    prob, model, sxz, rxz, src, nt, dt = engine_case(vti=vti, nt=401, ns=2)
+    sub = 1
+    ns = len(sxz)
+    nsnap = 1 + nt // sub
+    full = np.zeros((ns, nsnap, 2, prob["nx"], prob["nz"]))
+    oracle_forward(orc64, prob, model, sxz, rxz, src, nt, dt, sub=sub, full=full)
+    rng = np.random.default_rng(7)
+    nr = sum(len(r) for r in rxz)
+    adj = rng.standard_normal((2, nr, nt)) * np.hanning(nt)
+    from oracle.oracle import grid_struct
+    g = grid_struct(dict(prob, nt=nt, dt=dt, sub=sub))
+    np_ = 5 if vti else 3
+    gref = np.zeros((np_, prob["nx"], prob["nz"]))
+    r0 = 0
+    for s in range(ns):
+        n = len(rxz[s])
+        inj = orc64.delta_m3(prob["nx"], prob["nz"], prob["dx"], prob["dz"], 0, 0, rxz[s][:, 0], rxz[s][:, 1])
+        orc64.propagate(g, model, inj, adj[:, r0:r0 + n], full=full[s], grad=gref)
+        r0 += n
+    eng, _ = run_engine(lib, prob, model, sxz, rxz, src, nt, dt, sub=sub, keep=True)
+    gk = eng.adjoint(adj)
+    for c in range(3):
+        err = rel(gk[c], gref[c])
+        print("vti=%s component %d raw gradient rel err %.3e" % (vti, c, err))
+        assert err <= 1e-4
+    if vti:
+        assert np.all(gk[3:] == 0)
+    # the checkpoint segment length changes nothing, bit for bit
+    for K in (1, 7, nt):
+        eng.set_option("segment", K)
+        eng.forward(src, keep_checkpoints=True)
+        assert np.array_equal(eng.adjoint(adj), gk), K
+
+
+def test_operator_level_forward_and_gradient(lib, orc64):
+    """apply_forward / apply_jacobianT with resampling (data 2 ms, propagation 1 ms),
+    particle-velocity data and the H-quadrature + chain rule post-processing."""
+    nx, nz, h = 110, 70, 0.01
+    nt_data, dt_data = 251, 0.002
+    true = W.add_ellipses(W.gradient_model(nx, nz, h, h, seed=3), h, h, seed=4)
+    start = W.smooth_model(true, h, h, 0.1)
+    rx = np.stack([np.arange(2, nx - 2, 3) * h, np.full(len(range(2, nx - 2, 3)), 0.02)], 1)
+    prob = W.base_problem(nx, nz, h, h, nt_data, dt_data, [[0.3, 0.02], [0.8, 0.05]], [rx, rx[::2]],
+                          taper=15, dt=0.001, sub=-1)
+    src = W.vertical_force(W.ricker(nt_data, dt_data, 12.0), 2)
+    op_ref = orc64.make_op(prob)
+    dobs, _ = orc64.apply_forward(op_ref, true, src)
+    d0, full = orc64.apply_forward(orc64.make_op(prob), start, src, keep_full=True)
+    op = lib.WeOp(prob, src)
+    d0k = op.apply_forward(start)
+    assert op.time_axis()[2] == 2
+    err_d = rel(d0k, d0)
+    print("operator forward rel err %.3e" % err_d)
+    assert err_d <= 1e-4
+    r = d0 - dobs
+    rw = r * dt_data
+    rw[..., 0] *= 0.5
+    rw[..., -1] *= 0.5
+    gref = orc64.apply_jacobianT(orc64.make_op(prob), start, rw, full)
+    gk = op.apply_jacobianT(start, rw)
+    for c in range(3):
+        err = rel(gk[c], gref[c])
+        print("operator gradient component %d rel err %.3e" % (c, err))
+        assert err <= 1e-4
+    # add semantics: pmod = pmod + J^T d
+    g2 = op.apply_jacobianT(start, rw, add_to=gk)
+    assert rel(g2, 2 * gk.astype(np.float64)) < 1e-6
+
+
+def test_errors_are_reported(lib):
+    prob, model, sxz, rxz, src, nt, dt = engine_case(nt=50)
+    eng = lib.Engine(lib.grid_from(prob, nt, dt, 1))
+    bad = model.copy()
+    bad[1, 3, 3] = np.nan
+    with pytest.raises(lib.Fwi2dError, match="non-finite"):
+        eng.set_model(bad)
+    eng.set_model(model)
+    eng.set_shots(sxz, rxz)
+    with pytest.raises(lib.Fwi2dError, match="checkpoints"):
+        eng.adjoint(np.zeros((2, sum(len(r) for r in rxz), nt), np.float32))
+    with pytest.raises(lib.Fwi2dError, match="outside"):
+        eng.set_shots(np.array([[100.0, 0.0]]), [rxz[0]])
