@@ -5,7 +5,7 @@ hard-clips models to vpmax=8 km/s, rhomax=8 g/cc in analyzeModel,
 we_op.cpp:334-338, so SI inputs must be converted first).
 
 A *problem* is a plain dict consumed by both the engine (fwi2d_amd.op) and the
-test oracle:
+CPU reference restatement used by the tests:
 
     nx, nz, dx, dz, ox, oz          grid (axis 1 = z fastest, axis 2 = x)
     bc_top/bottom/left/right        1 free surface, 2 absorbing
