@@ -543,6 +543,7 @@ struct Engine {
     {
         require_ready();
         if (keep && g.sub <= 0) throw std::runtime_error("checkpoints need sub > 0");
+        ckpt_epoch = -1;  // any forward run replaces the recomputation sources
         const long long ntr_s = (long long)2 * ns * g.nt, ntr_r = (long long)2 * nr * g.nt;
         const float* dsrc = on_device ? src : upload_traces(d_src, src, ntr_s);
         float* drcv = rcv;
