@@ -1,0 +1,253 @@
+"""Benchmark: batched FWI gradients (forward + checkpointed adjoint) on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--shots 64] [--nt 4000]
+
+Workload: batched isotropic elastic FWI gradients -- 64 shots on a 500 x 1700 grid
+(h = 10 m), nt = 4000 steps of 1 ms, 100-400 split-spread receivers per shot, 8-20 Hz
+Ricker sources, free surface on top, 30-cell tapers elsewhere.  A "step" is one
+gradient evaluation of the whole batch: forward modelling with device checkpoints, the
+data residual, and the adjoint propagation with the imaging condition.  Inputs
+(model, source traces, observed data) are resident in HBM before timing starts.
+
+Printed JSON line: shots/s (value) and Gcell-updates/s of the batched form, the plain
+one-shot-per-launch form on the same kernels, the roofline of the dominant kernel (the
+fused time step) from HIP events, the gradient error of a reduced case against the
+FP64 oracle, and a CPU baseline (the oracle's FP32 restatement on the host cores).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "FWI gradient shots/s & Gcell-updates/s, batched vs plain, grad err≤1e-4"
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_CELL_UPDATE_FWD = 36.0   # iso forward step: read curr 2 + prev 2 + model 3, write next 2 floats
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--shots", type=int, default=64)
+    ap.add_argument("--nt", type=int, default=4000)
+    ap.add_argument("--plain-shots", type=int, default=4)
+    ap.add_argument("--tile-x", type=int, default=16)
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def converted(m):
+    lam = m[2] * (m[0] ** 2 - 2 * m[1] ** 2)
+    mu = m[2] * m[1] ** 2
+    return np.stack([lam, mu, m[2]]).astype(np.float32)
+
+
+def workload(nshots, nt):
+    from fwi2d_amd import workloads as W
+    nz, nx, h, dt = 500, 1700, 0.01, 0.001
+    true = W.gradient_model(nx, nz, h, h, seed=0)
+    start = W.smooth_model(true, h, h, 0.3)
+    sxz, rxz, freqs = W.config2_geometry(nshots, nx, h)
+    wav = np.stack([W.ricker(nt, dt, f) for f in freqs])
+    src = np.stack([wav * np.cos(1.5708), wav * np.sin(1.5708)]).astype(np.float32)
+    prob = W.base_problem(nx, nz, h, h, nt, dt, sxz, rxz, dt=dt, sub=1)
+    return prob, converted(true), converted(start), sxz, rxz, src, dt
+
+
+class Batch:
+    """One engine holding a shot batch, with device-resident traces (torch tensors)."""
+
+    def __init__(self, fwi2d_amd, torch, prob, model_true, model, sxz, rxz, src, nt, dt, tile_x):
+        self.torch = torch
+        self.eng = fwi2d_amd.Engine(fwi2d_amd.grid_from(prob, nt, dt, 1))
+        self.eng.set_option("tile_x", tile_x)
+        self.eng.set_shots(sxz, rxz)
+        self.nt, self.dt = nt, dt
+        self.nr = self.eng.nr
+        dev = "cuda"
+        self.src = torch.from_numpy(np.ascontiguousarray(src)).to(dev)
+        self.rcv = torch.zeros((2, self.nr, nt), dtype=torch.float32, device=dev)
+        self.adj = torch.zeros_like(self.rcv)
+        self.grad = torch.zeros((3, prob["nx"], prob["nz"]), dtype=torch.float32, device=dev)
+        # observed data from the true model
+        self.eng.set_model(model_true)
+        self.eng.forward_device(self.src.data_ptr(), self.rcv.data_ptr(), keep_checkpoints=False)
+        self.dobs = self.rcv.clone()
+        self.eng.set_model(model)
+
+    def gradient(self):
+        t = self.torch
+        self.eng.forward_device(self.src.data_ptr(), self.rcv.data_ptr(), keep_checkpoints=True)
+        # adjoint source of the L2 misfit: residual, time-reversed, sign-flipped
+        # (apply_jacobianT's Ht^-1, revert and -alpha applied to Ht r, we_op.cpp:1112-1117)
+        self.adj.copy_(t.flip(self.dobs - self.rcv, dims=[2]))
+        t.cuda.synchronize()
+        self.eng.adjoint_device(self.adj.data_ptr(), self.grad.data_ptr())
+
+
+def grad_error_small():
+    """Gradient rel-L2 error vs the FP64 oracle on a reduced single-shot case."""
+    import fwi2d_amd
+    from fwi2d_amd import workloads as W
+    from oracle.oracle import Oracle, grid_struct
+    nx, nz, h, nt, dt = 160, 100, 0.01, 600, 0.001
+    true = W.add_ellipses(W.gradient_model(nx, nz, h, h, seed=0), h, h)
+    start = W.smooth_model(true, h, h, 0.1)
+    m0, m1 = converted(true).astype(np.float64), converted(start).astype(np.float64)
+    prob = dict(nx=nx, nz=nz, dx=h, dz=h, bc_top=1, bc_bottom=2, bc_left=2, bc_right=2, taper_top=0,
+                taper_bottom=20, taper_left=20, taper_right=20, taper_strength=0.05, vti=0)
+    sxz = np.array([[0.8, 0.02]])
+    rxz = [np.stack([np.arange(2, nx - 2, 4) * h, np.full(len(range(2, nx - 2, 4)), 0.02)], 1)]
+    w = W.ricker(nt, dt, 12.0)
+    src = np.stack([0 * w[None], w[None]])
+    orc = Oracle(64)
+    g = grid_struct(dict(prob, nt=nt, dt=dt, sub=1))
+    inj = orc.delta_m3(nx, nz, h, h, 0, 0, sxz[:, 0], sxz[:, 1])
+    ext = orc.delta_m3(nx, nz, h, h, 0, 0, rxz[0][:, 0], rxz[0][:, 1])
+    dobs = orc.propagate(g, m0, inj, src, ext=ext)
+    full = np.zeros((1 + nt, 2, nx, nz))
+    d = orc.propagate(g, m1, inj, src, ext=ext, full=full)
+    adj = -(d - dobs)[:, :, ::-1].copy()
+    gref = np.zeros((3, nx, nz))
+    orc.propagate(g, m1, ext, adj, full=full, grad=gref)
+    eng = fwi2d_amd.Engine(fwi2d_amd.grid_from(prob, nt, dt, 1))
+    eng.set_model(m1)
+    eng.set_shots(sxz, rxz)
+    eng.forward(src, keep_checkpoints=True)
+    gk = eng.adjoint(adj)
+    return max(float(np.linalg.norm(gk[c] - gref[c]) / np.linalg.norm(gref[c])) for c in range(3))
+
+
+def cpu_baseline(nt_sample=120):
+    """Oracle FP32 restatement (OpenMP, all host cores) on one shot of the same grid for
+    nt_sample steps, forward with snapshots + adjoint with imaging; scaled to shots/s."""
+    from fwi2d_amd import workloads as W
+    from oracle.oracle import Oracle, grid_struct
+    orc = Oracle(32)
+    nz, nx, h, dt = 500, 1700, 0.01, 0.001
+    m = converted(W.gradient_model(nx, nz, h, h, seed=0))
+    prob = dict(nx=nx, nz=nz, dx=h, dz=h, bc_top=1, bc_bottom=2, bc_left=2, bc_right=2, taper_top=0,
+                taper_bottom=30, taper_left=30, taper_right=30, taper_strength=0.05, vti=0)
+    sxz, rxz, freqs = W.config2_geometry(1, nx, h)
+    g = grid_struct(dict(prob, nt=nt_sample, dt=dt, sub=1))
+    w = W.ricker(nt_sample, dt, float(freqs[0]))
+    src = np.stack([0 * w[None], w[None]]).astype(np.float32)
+    inj = orc.delta_m3(nx, nz, h, h, 0, 0, sxz[:, 0], sxz[:, 1])
+    ext = orc.delta_m3(nx, nz, h, h, 0, 0, rxz[0][:, 0], rxz[0][:, 1])
+    full = np.zeros((1 + nt_sample, 2, nx, nz), np.float32)
+    grad = np.zeros((3, nx, nz), np.float32)
+    t0 = time.perf_counter()
+    d = orc.propagate(g, m, inj, src, ext=ext, full=full)
+    orc.propagate(g, m, ext, d[:, :, ::-1].copy(), full=full, grad=grad)
+    el = time.perf_counter() - t0
+    cells = 2.0 * nt_sample * nx * nz
+    return dict(cell_updates_per_s=cells / el, seconds=el, threads=int(orc.lib.orc_num_threads()),
+                sample="1 shot, 500x1700 grid, %d forward + %d adjoint steps (sub=1, imaging every step)"
+                % (nt_sample, nt_sample))
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build runs on a single MI355X; see DESIGN.md for "
+              "how the batched engine shards shots over GPUs.", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import fwi2d_amd
+    fwi2d_amd.load()
+    nt, ns = args.nt, args.shots
+    prob, m_true, m0, sxz, rxz, src, dt = workload(ns, nt)
+    nx, nz = prob["nx"], prob["nz"]
+    cells_per_grad = 2.0 * nt * nx * nz  # forward + adjoint, per shot
+
+    # ---- batched form -------------------------------------------------------
+    B = Batch(fwi2d_amd, torch, prob, m_true, m0, sxz, rxz, src, nt, dt, args.tile_x)
+    for _ in range(args.warmup):
+        B.gradient()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        B.gradient()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    ms_per_step = 1000.0 * el / args.steps
+    shots_per_s = ns * args.steps / el
+    gcells = shots_per_s * cells_per_grad / 1e9
+    grad_norm = float(torch.linalg.vector_norm(B.grad).item())
+
+    # ---- dominant kernel: fused step, timed with HIP events on its stream ----
+    B.eng.set_option("timing", 1)
+    B.eng.forward_device(B.src.data_ptr(), B.rcv.data_ptr(), keep_checkpoints=False)
+    loop_ns = B.eng.stat("fwd_loop_ns")
+    nsteps = B.eng.stat("fwd_loop_steps")
+    B.eng.set_option("timing", 0)
+    step_s = loop_ns * 1e-9 / nsteps
+    cells_per_launch = ns * nx * nz
+    achieved = BYTES_PER_CELL_UPDATE_FWD * cells_per_launch / step_s / 1e9
+    recompute = B.eng.stat("recompute_steps")
+    del B
+    torch.cuda.empty_cache()
+
+    # ---- plain form: one shot per launch, host-driven steps -----------------
+    P = args.plain_shots
+    t_plain = 0.0
+    for s in range(P):
+        one = Batch(fwi2d_amd, torch, prob, m_true, m0, sxz[s:s + 1], rxz[s:s + 1], src[:, s:s + 1], nt, dt,
+                    args.tile_x)
+        one.gradient()  # warm
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        one.gradient()
+        torch.cuda.synchronize()
+        t_plain += time.perf_counter() - t1
+        del one
+    plain_shots_per_s = P / t_plain
+
+    grad_err = grad_error_small()
+    cpu = None if args.no_cpu else cpu_baseline()
+
+    line = {
+        "metric": METRIC,
+        "value": round(shots_per_s, 4),
+        "unit": "shots/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "batched isotropic FWI gradients (forward + checkpointed adjoint): %d shots, "
+                               "500x1700 grid h=10 m, nt=%d dt=1 ms, 100-400 receivers/shot, 8-20 Hz Ricker"
+                               % (ns, nt),
+                   "shots_per_step": ns, "nx": nx, "nz": nz, "nt": nt, "tile_x": args.tile_x},
+        "gcell_updates_per_s": round(gcells, 2),
+        "plain_one_shot_per_launch_shots_per_s": round(plain_shots_per_s, 4),
+        "batched_vs_plain": round(shots_per_s / plain_shots_per_s, 3),
+        "recompute_steps_per_gradient": int(recompute // max(args.steps + args.warmup, 1)),
+        "grad_rel_err_vs_fp64": grad_err,
+        "grad_err_sample": "1 shot, 160x100 grid, nt=600, raw lambda/mu/rho gradients vs FP64 oracle",
+        "grad_norm": grad_norm,
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_step (fused time step, all shots)", "avg_launch_us": round(step_s * 1e6, 2),
+                     "bytes_per_cell_update": BYTES_PER_CELL_UPDATE_FWD, "cells_per_launch": cells_per_launch},
+    }
+    if cpu:
+        line["cpu_baseline"] = {"value": round(cpu["cell_updates_per_s"] / cells_per_grad, 6), "unit": "shots/s",
+                                "cores": cpu["threads"], "kind": "port", "sample": cpu["sample"],
+                                "gcell_updates_per_s": round(cpu["cell_updates_per_s"] / 1e9, 4)}
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -61,22 +61,18 @@ struct TileLists {
     }
 };
 
-static void tile_range_host(int k, int ntile, int T, int n, int& s, int& e)
-{
-    if (ntile == 1) { s = 0; e = n; return; }
-    if (k == ntile - 1) { s = n - T; e = n; return; }
-    s = k * T;
-    e = std::min(s + T, n - T);
-}
 
 struct Engine {
     fwi2d_grid g{};
     int device = 0;
     hipStream_t st = nullptr;
     int np = 3;
-    long long nxz = 0;
+    int ldz = 0;          // device z pitch: nz rounded up to a multiple of 4
+    long long nxz = 0;    // nx * ldz, the device size of one field
     int tile_x = 16, ntx = 0, ntz = 0;
-    int opt_tile_x = 16, opt_segment = 0, opt_lag = 1, opt_graph = 0;
+    int opt_tile_x = 16, opt_segment = 0, opt_lag = 1, opt_graph = 0, opt_timing = 0;
+    double last_fwd_loop_ms = 0.0;  // HIP-event time of the last forward step loop (timing option)
+    int last_fwd_loop_steps = 0;
     bool vti = false;
 
     DevMem d_model, d_div_tb, d_div_lr, d_tx, d_tz;
@@ -110,7 +106,9 @@ struct Engine {
             if (b != 1 && b != 2) throw std::runtime_error("boundary conditions must be 1 (free surface) or 2 (absorbing)");
         vti = g.vti != 0;
         np = vti ? 5 : 3;
-        nxz = (long long)g.nx * g.nz;
+        ldz = (g.nz + 3) & ~3;
+        nxz = (long long)g.nx * ldz;
+        if (nxz >= (1LL << 31)) throw std::runtime_error("grid too large: nx * nz must stay below 2^31 cells");
         HIPCHECK(hipSetDevice(device));
         HIPCHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
         upload_constants(st);
@@ -127,10 +125,10 @@ struct Engine {
 
     void set_tile(int tx)
     {
-        if (tx != 8 && tx != 16 && tx != 32) throw std::runtime_error("tile_x must be 8, 16 or 32");
+        if (tx != 16 && tx != 32) throw std::runtime_error("tile_x must be 16 or 32");
         tile_x = tx;
-        ntx = tiles_along(g.nx, tile_x);
-        ntz = tiles_along(g.nz, kTileZ);
+        ntx = tiles_x(g.nx, tile_x);
+        ntz = tiles_z(ldz);
         if (ns > 0) build_tile_lists();
     }
     int ntiles() const { return ntx * ntz; }
@@ -153,6 +151,7 @@ struct Engine {
             return f;
         };
         std::vector<float> tz = axis(g.nz, g.taper_top, g.taper_bottom);
+        tz.resize(ldz, 0.f);
         std::vector<float> tx = axis(g.nx, g.taper_left, g.taper_right);
         d_tz.reserve(tz.size() * sizeof(float));
         d_tx.reserve(tx.size() * sizeof(float));
@@ -167,11 +166,12 @@ struct Engine {
     void build_divisors()
     {
         const int nx = g.nx, nz = g.nz;
+        const long long nc = (long long)nx * nz;  // host model is [np][nx][nz]
         const float* la = h_model.data();
-        const float* mu = la + nxz;
-        const float* rho = la + 2 * nxz;
-        const float* eps = vti ? la + 4 * nxz : nullptr;
-        std::vector<double> dvx(nxz, 1.0), dvz(nxz, 1.0);
+        const float* mu = la + nc;
+        const float* rho = la + 2 * nc;
+        const float* eps = vti ? la + 4 * nc : nullptr;
+        std::vector<double> dvx(nc, 1.0), dvz(nc, 1.0);
         const double h0 = kH0, dt = g.dt, dx = g.dx, dz = g.dz;
         auto vpx2 = [&](long long i) { return (vti ? (1 + 2 * (double)eps[i]) : 1.0) * ((double)la[i] + 2 * (double)mu[i]); };
         auto scale = [&](long long i, double sx, double sz) { dvx[i] = 1 + sx; dvz[i] = 1 + sz; };
@@ -262,16 +262,20 @@ struct Engine {
 
     void set_model(const float* m)
     {
-        const size_t n = (size_t)np * nxz;
+        const size_t n = (size_t)np * g.nx * g.nz;  // host model is [np][nx][nz]
         for (size_t i = 0; i < n; i++)
             if (!std::isfinite(m[i])) throw std::runtime_error("model contains a non-finite value");
         h_model.assign(m, m + n);
-        const float* rho = m + 2 * nxz;
-        const float* mu = m + nxz;
-        for (long long i = 0; i < nxz; i++)
+        const long long nc = (long long)g.nx * g.nz;
+        const float* rho = m + 2 * nc;
+        const float* mu = m + nc;
+        for (long long i = 0; i < nc; i++)
             if (!(rho[i] > 0) || !(mu[i] > 0)) throw std::runtime_error("model must have rho > 0 and mu > 0");
-        d_model.reserve(n * sizeof(float));
-        HIPCHECK(hipMemcpyAsync(d_model.p, m, n * sizeof(float), hipMemcpyHostToDevice, st));
+        // device copy with z pitch ldz; padding cells hold lambda = mu = c13 = eps = 0, rho = 1
+        d_model.reserve((size_t)np * nxz * sizeof(float));
+        for (int c = 0; c < np; c++) launch_fill(d_model.as<float>() + c * nxz, nxz, c == 2 ? 1.f : 0.f, st);
+        HIPCHECK(hipMemcpy2DAsync(d_model.p, (size_t)ldz * sizeof(float), m, (size_t)g.nz * sizeof(float),
+                                  (size_t)g.nz * sizeof(float), (size_t)np * g.nx, hipMemcpyHostToDevice, st));
         build_divisors();
         model_epoch++;
     }
@@ -373,10 +377,10 @@ struct Engine {
                 const PointEntry& e = ents[k];
                 for (int tx = 0; tx < ntx; tx++) {
                     int sx, ex;
-                    tile_range_host(tx, ntx, tile_x, g.nx, sx, ex);
+                    tile_range_x(tx, ntx, tile_x, g.nx, sx, ex);
                     for (int tz = 0; tz < ntz; tz++) {
                         int sz, ez;
-                        tile_range_host(tz, ntz, kTileZ, g.nz, sz, ez);
+                        tile_range_z(tz, ntz, g.nz, ldz, sz, ez);
                         bool hit;
                         if (owner) hit = (e.xi0 + 1 >= sx && e.xi0 + 1 < ex && e.zi0 + 1 >= sz && e.zi0 + 1 < ez);
                         else hit = (e.xi0 + 2 >= sx && e.xi0 < ex && e.zi0 + 2 >= sz && e.zi0 < ez);
@@ -408,6 +412,7 @@ struct Engine {
         StepArgs a{};
         a.nx = g.nx;
         a.nz = g.nz;
+        a.ldz = ldz;
         a.ntx = ntx;
         a.ntz = ntz;
         a.nshots = ns;
@@ -473,10 +478,10 @@ struct Engine {
         L.curr = 1;
         L.next = 2;
         if (adjoint)
-            launch_init_prev(L.P(), 2 * nxz, g.nz, ns, d_rcv_shot_ptr.as<int>(), d_rcv_ents.as<PointEntry>(), traces,
+            launch_init_prev(L.P(), 2 * nxz, nxz, ldz, ns, d_rcv_shot_ptr.as<int>(), d_rcv_ents.as<PointEntry>(), traces,
                              (long long)nr * g.nt, st);
         else
-            launch_init_prev(L.P(), 2 * nxz, g.nz, ns, d_src_shot_ptr.as<int>(), d_src_ents.as<PointEntry>(), traces,
+            launch_init_prev(L.P(), 2 * nxz, nxz, ldz, ns, d_src_shot_ptr.as<int>(), d_src_ents.as<PointEntry>(), traces,
                              (long long)ns * g.nt, st);
         launch_scale_prev(L.P(), d_model.as<float>() + 2 * nxz, (float)(0.5 * g.dt * g.dt), nxz, ns, st);
     }
@@ -562,13 +567,29 @@ struct Engine {
             jlast = nseg - 1;
         }
         const int nt = g.nt;
+        hipEvent_t ev0 = nullptr, ev1 = nullptr;
+        if (opt_timing) {
+            HIPCHECK(hipEventCreate(&ev0));
+            HIPCHECK(hipEventCreate(&ev1));
+            HIPCHECK(hipEventRecord(ev0, st));
+        }
         for (int it = 0; it < nt - 1; it++) {
             if (keep) capture(L, it, jlast);
             step(L, it, false, dsrc, nr > 0 ? drcv : nullptr);
         }
+        if (opt_timing) {
+            HIPCHECK(hipEventRecord(ev1, st));
+            HIPCHECK(hipEventSynchronize(ev1));
+            float ms = 0.f;
+            HIPCHECK(hipEventElapsedTime(&ms, ev0, ev1));
+            last_fwd_loop_ms = ms;
+            last_fwd_loop_steps = nt - 1;
+            (void)hipEventDestroy(ev0);
+            (void)hipEventDestroy(ev1);
+        }
         if (keep) capture(L, nt - 1, jlast);
         if (nr > 0)
-            launch_extract(L.C(), 2 * nxz, g.nz, nxz, d_rcv_ext_ents.as<PointEntry>(), nr, drcv, (long long)nr * nt,
+            launch_extract(L.C(), 2 * nxz, nxz, ldz, d_rcv_ext_ents.as<PointEntry>(), nr, drcv, (long long)nr * nt,
                            nt - 1, st);
         if (!on_device && nr > 0)
             HIPCHECK(hipMemcpyAsync(rcv, drcv, (size_t)ntr_r * sizeof(float), hipMemcpyDeviceToHost, st));
@@ -634,6 +655,7 @@ struct Engine {
         ImageArgs a{};
         a.nx = g.nx;
         a.nz = g.nz;
+        a.ldz = ldz;
         a.ntx = ntx;
         a.ntz = ntz;
         a.nshots = ns;
@@ -688,13 +710,13 @@ struct Engine {
             step(A, it, true, dadj, nullptr);
         }
         image(A, 0, dsrc);
-        float* out = grad;
-        if (!on_device) {
-            d_gout.reserve((size_t)np * nxz * sizeof(float));
-            out = d_gout.as<float>();
-        }
+        d_gout.reserve((size_t)np * nxz * sizeof(float));
+        float* out = d_gout.as<float>();
         launch_sum_shots(d_gshot.as<float>(), (long long)np * nxz, ns, (long long)np * nxz, out, 0, st);
-        if (!on_device) HIPCHECK(hipMemcpyAsync(grad, out, (size_t)np * nxz * sizeof(float), hipMemcpyDeviceToHost, st));
+        // [np][nx][ldz] -> [np][nx][nz]
+        HIPCHECK(hipMemcpy2DAsync(grad, (size_t)g.nz * sizeof(float), out, (size_t)ldz * sizeof(float),
+                                  (size_t)g.nz * sizeof(float), (size_t)np * g.nx,
+                                  on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, st));
         HIPCHECK(hipStreamSynchronize(st));
         HIPCHECK(hipGetLastError());
     }
@@ -811,6 +833,8 @@ int fwi2d_engine_set_option(fwi2d_engine* e, const char* key, long long value)
             x->ckpt_epoch = -1;
         } else if (k == "lag") {
             x->opt_lag = value ? 1 : 0;
+        } else if (k == "timing") {
+            x->opt_timing = value ? 1 : 0;
         } else if (k == "graph") {
             x->opt_graph = value ? 1 : 0;
         } else {
@@ -833,6 +857,8 @@ long long fwi2d_engine_get_stat(fwi2d_engine* e, const char* key)
     if (k == "ntiles") return x->ntiles();
     if (k == "nrec") return x->nr;
     if (k == "nshots") return x->ns;
+    if (k == "fwd_loop_ns") return (long long)(x->last_fwd_loop_ms * 1e6);
+    if (k == "fwd_loop_steps") return x->last_fwd_loop_steps;
     return -1;
 }
 

@@ -1,11 +1,15 @@
 // Kernel argument blocks and launchers of the elastic propagation engine (gfx950).
+//
+// Device layout: every field is [x][ldz] with z contiguous and ldz = nz rounded up to a
+// multiple of 4, so that each thread moves 4 consecutive z cells (a "quad") with one
+// 128-bit access; the padding cells are zero and never read as stencil data.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
 namespace fwi2d {
 
-constexpr int kTileZ = 64;     // one wavefront of lanes along z (the contiguous axis)
+constexpr int kTileZ = 64;     // z extent of a tile: 16 quads
 constexpr int kThreads = 256;  // 4 wavefronts per workgroup
 
 // One point injector (delta_m3): 3x3 footprint starting at (xi0, zi0) with separable
@@ -19,14 +23,15 @@ struct PointEntry {
 };
 
 struct StepArgs {
-    int nx, nz, ntx, ntz, nshots, nt, it, taper_prev;
+    int nx, nz, ldz, ntx, ntz, nshots, nt, it, taper_prev;
     float rdx, rdz, rdx2, rdz2, dt, dt2, sat_x, sat_z;
     int bc_top, bc_bottom, bc_left, bc_right;
     const float *lam, *mu, *rho, *c13, *eps;
     const float *div_tb;   // [2 sides][2 comps][nx] absorbing divisors, top then bottom
     const float *div_lr;   // [2 sides][2 comps][nz]
-    const float *taper_x, *taper_z;
-    const float* prev;     // shot 0 base of [2][nx][nz]; shot stride below
+    const float *taper_x;  // [nx]
+    const float *taper_z;  // [ldz]
+    const float* prev;     // shot 0 base of [2][nx][ldz]; shot stride below
     const float* curr;
     float* next;
     long long shot_stride;
@@ -45,17 +50,17 @@ struct StepArgs {
 };
 
 struct ImageArgs {
-    int nx, nz, ntx, ntz, nshots, mode;   // mode 0 interior, 1 first (idx 0), 2 last
-    float rdx, rdz, wlm, wrho;            // weights: lambda/mu (w*dtg or w*dtg/2), rho (w/dtg)
+    int nx, nz, ldz, ntx, ntz, nshots, mode;  // mode 0 interior, 1 first (idx 0), 2 last
+    float rdx, rdz, wlm, wrho;                // weights: lambda/mu (w*dtg or w*dtg/2), rho (w/dtg)
     const float *lam, *mu, *c13, *eps;
     const float* adj;        // current adjoint level, shot 0 base
     const float* adj_prev;   // previous adjoint level (its derivatives are imaged)
     long long shot_stride;
-    const float* f0;         // forward snapshot idx   [shot][2][nxz] with stride fshot_stride
+    const float* f0;         // forward snapshot idx   [shot][2][nx][ldz], shot stride fshot_stride
     const float* fm;         // forward snapshot idx-1 (mode 0, 2)
     const float* fp;         // forward snapshot idx+1 (mode 0, 1)
     long long fshot_stride;
-    float* grad;             // per-shot raw gradients [shot][ncomp][nxz]
+    float* grad;             // per-shot raw gradients [shot][ncomp][nx][ldz]
     int ncomp;
     long long gshot_stride;
     int lag;                 // 1: derivatives of adj_prev (reference), 0: of adj
@@ -63,12 +68,16 @@ struct ImageArgs {
 
 void upload_constants(hipStream_t s);
 
-int tiles_along(int n, int t);
+int tiles_x(int nx, int tile_x);
+int tiles_z(int ldz);
+void tile_range_x(int k, int ntile, int T, int n, int& s, int& e);
+void tile_range_z(int k, int ntile, int nz, int ldz, int& s, int& e);
+
 void launch_step(const StepArgs& a, bool vti, int tile_x, hipStream_t s);
-void launch_init_prev(float* prev, long long shot_stride, int nz, int nshots, const int* shot_ptr,
+void launch_init_prev(float* prev, long long shot_stride, long long nxz, int ldz, int nshots, const int* shot_ptr,
                       const PointEntry* ents, const float* trace, long long comp_stride, hipStream_t s);
 void launch_scale_prev(float* prev, const float* rho, float half_dt2, long long nxz, int nshots, hipStream_t s);
-void launch_extract(const float* state, long long shot_stride, int nz, long long nxz, const PointEntry* ents,
+void launch_extract(const float* state, long long shot_stride, long long nxz, int ldz, const PointEntry* ents,
                     int n, float* trace, long long comp_stride, int it, hipStream_t s);
 void launch_image(const ImageArgs& a, bool vti, int tile_x, hipStream_t s);
 void launch_sum_shots(const float* per_shot, long long shot_stride, int nshots, long long n, float* out,

@@ -60,6 +60,13 @@ typedef struct {
 
 int orc_real_size(void) { return (int)sizeof(real); }
 
+#ifdef _OPENMP
+#include <omp.h>
+int orc_num_threads(void) { return omp_get_max_threads(); }
+#else
+int orc_num_threads(void) { return 1; }
+#endif
+
 /* ------------------------------------------------------------------------- */
 /* SBP coefficient tables                                                      */
 /* ------------------------------------------------------------------------- */

@@ -99,7 +99,7 @@ def test_forward_small_and_ragged_grids(lib, orc64, orc32, nx, nz):
     case = engine_case(nx=nx, nz=nz, nt=200, taper=4, ns=1)
     ref = oracle_forward(orc64, *case)
     gap = rel(oracle_forward(orc32, *case), ref)
-    for tx in (8, 16, 32):
+    for tx in (16, 32):
         _, out = run_engine(lib, *case, tile_x=tx)
         err = rel(out, ref)
         print("nx=%d nz=%d tile_x=%d err %.3e gap %.3e" % (nx, nz, tx, err, gap))
@@ -110,9 +110,8 @@ def test_tile_width_and_batch_are_bit_exact(lib):
     case = engine_case(ns=3)
     prob, model, sxz, rxz, src, nt, dt = case
     _, a = run_engine(lib, *case, tile_x=16)
-    _, b = run_engine(lib, *case, tile_x=8)
     _, c = run_engine(lib, *case, tile_x=32)
-    assert np.array_equal(a, b) and np.array_equal(a, c)
+    assert np.array_equal(a, c)
     # each shot alone gives the same traces as the batch
     r0 = 0
     for s in range(3):

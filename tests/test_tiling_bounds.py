@@ -59,22 +59,46 @@ def tiles(n, T):
     return [tile_range(k, nt, T, n) for k in range(nt)]
 
 
-def check_axis(n, T):
+def tile_range_quads(k, ntile, T, n):
+    """z tiling in 4-cell quads: tiles start on quads; the last one is anchored to the
+    last quad of the padded extent ceil(n/4)*4 and writes up to n."""
+    nq, tq = (n + 3) // 4, T // 4
+    if ntile == 1:
+        return 0, n
+    if k == ntile - 1:
+        return 4 * (nq - tq), n
+    s = k * T
+    return s, min(s + T, 4 * (nq - tq))
+
+
+def tiles_quads(n, T):
+    nq, tq = (n + 3) // 4, T // 4
+    nt = (nq + tq - 1) // tq
+    return [tile_range_quads(k, nt, T, n) for k in range(nt)]
+
+
+def check_axis(n, T, quads=False):
     """Per-axis check (the 2-D stencils are separable, so axes can be checked apart)."""
     W = T + 2 * H_STEP
-    I = T + 4
+    I = T + 8 if quads else T + 4
     cover = np.zeros(n, int)
-    for s, e in tiles(n, T):
+    for s, e in (tiles_quads(n, T) if quads else tiles(n, T)):
+        if quads:
+            assert s % 4 == 0
         assert 0 <= s < e <= n and e - s <= T
         cover[s:e] += 1
         ws = window_start(s, H_STEP)
+        if quads:
+            assert ws % 4 == 0
         loaded = set(p for p in range(ws, ws + W) if 0 <= p < n)
         # first-derivative products: computed on the needed range [plo, phi), stored from plo
         plo, phi = needed_products(s, e, n)
         inter = set(range(plo, phi))
         assert phi - plo <= I, (n, T, s, plo, phi)
+        porg = ws if quads else plo  # quad kernels store products on the state window
         assert inter <= loaded  # cross-axis reads of a product use its own row/column
         for p in inter:
+            assert 0 <= p - porg < I
             for o in d1_offsets(p, n):
                 assert 0 <= p + o - ws < W and (p + o) in loaded, (n, T, s, p, o)
         for g in range(s, e):
@@ -82,7 +106,7 @@ def check_axis(n, T):
                 assert 0 <= g + o - ws < W and (g + o) in loaded, (n, T, s, g, o)
             for o in d1_offsets(g, n):
                 # D1 of the products along this axis and of the state (tangential SAT rows)
-                assert 0 <= g + o - plo < I and (g + o) in inter, (n, T, s, g, o)
+                assert 0 <= g + o - porg < I and (g + o) in inter, (n, T, s, g, o)
                 assert (g + o) in loaded
             # SAT normal rows (4 one-sided points, 6 closure points) at the edges
             if g == 0:
@@ -106,6 +130,11 @@ def check_axis(n, T):
 def test_tiles_partition_and_windows(T):
     for n in list(range(12, 3 * T + 20)) + [201, 300, 500, 601, 1000, 1700, 3000]:
         check_axis(n, T)
+
+
+def test_quad_tiles_along_z():
+    for n in list(range(12, 300)) + [501, 601, 1000, 3001]:
+        check_axis(n, 64, quads=True)
 
 
 def xcd_remap(bid, nb):
