@@ -39,8 +39,84 @@ def parse():
     ap.add_argument("--nt", type=int, default=4000)
     ap.add_argument("--plain-shots", type=int, default=4)
     ap.add_argument("--tile-x", type=int, default=16)
-    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-check", action="store_true", help="skip the small-case gradient error check")
+    ap.add_argument("--cpu-steps", type=int, default=1600, help="time steps of the CPU baseline sample")
+    ap.add_argument("--summarize", metavar="DIR",
+                    help="only summarize the rocprofv3 CSV output under DIR (kernel time per launch shape, "
+                         "counters per dispatch) and exit")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a 1 GiB device-to-device copy 5 times (HBM counter calibration) and exit")
     return ap.parse_args()
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_step_iso_64shots_500x1700_traffic.md")
+
+
+def measured_traffic():
+    """HBM bytes per k_step launch from the committed counter summary (FETCH_SIZE and
+    WRITE_SIZE per dispatch, calibrated against a copy kernel of known size), or None."""
+    try:
+        for line in open(TRAFFIC_FILE):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
+def log(msg):
+    print("[bench %.1fs] %s" % (time.perf_counter() - T0, msg), file=sys.stderr, flush=True)
+
+
+T0 = time.perf_counter()
+
+
+def summarize(d):
+    """Per (kernel, grid, workgroup) launch counts and mean durations from rocprofv3's
+    kernel_trace.csv, and per-dispatch counter means from counter_collection.csv."""
+    import csv
+    import glob
+    from collections import defaultdict
+
+    def shape(r, key):
+        if key in r:
+            return r[key]
+        return "x".join(r.get(key + "_" + a, "?") for a in "XYZ")
+
+    for path in sorted(glob.glob(os.path.join(d, "**", "*kernel_trace.csv"), recursive=True)):
+        acc = defaultdict(lambda: [0, 0.0])
+        for r in csv.DictReader(open(path)):
+            k = (r["Kernel_Name"][:60], shape(r, "Grid_Size"), shape(r, "Workgroup_Size"))
+            acc[k][0] += 1
+            acc[k][1] += float(r["End_Timestamp"]) - float(r["Start_Timestamp"])
+        tot = sum(v[1] for v in acc.values())
+        print("# %s: %d dispatches, %.3f s of kernel time" % (os.path.basename(path), sum(v[0] for v in acc.values()),
+                                                            tot * 1e-9))
+        print("%-60s %-16s %-10s %8s %12s %7s" % ("kernel", "grid", "wg", "calls", "avg_us", "pct"))
+        for k, (n, t) in sorted(acc.items(), key=lambda kv: -kv[1][1])[:25]:
+            print("%-60s %-16s %-10s %8d %12.2f %6.2f%%" % (k[0], k[1], k[2], n, t / n * 1e-3, 100 * t / tot))
+    for path in sorted(glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)):
+        acc = defaultdict(lambda: [0, 0.0])
+        for r in csv.DictReader(open(path)):
+            k = (r["Kernel_Name"][:60], shape(r, "Grid_Size"), r["Counter_Name"])
+            acc[k][0] += 1
+            acc[k][1] += float(r["Counter_Value"])
+        print("# %s" % os.path.basename(path))
+        print("%-60s %-16s %-12s %8s %16s" % ("kernel", "grid", "counter", "calls", "avg_per_dispatch"))
+        for k, (n, v) in sorted(acc.items(), key=lambda kv: -kv[1][1]):
+            print("%-60s %-16s %-12s %8d %16.1f" % (k[0], k[1], k[2], n, v / n))
+
+
+def calibrate_copy():
+    import torch
+    n = 1 << 28  # 1 GiB of float32
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 5}))
 
 
 def converted(m):
@@ -125,7 +201,7 @@ def grad_error_small():
     return max(float(np.linalg.norm(gk[c] - gref[c]) / np.linalg.norm(gref[c])) for c in range(3))
 
 
-def cpu_baseline(nt_sample=120):
+def cpu_baseline(nt_sample=400):
     """Oracle FP32 restatement (OpenMP, all host cores) on one shot of the same grid for
     nt_sample steps, forward with snapshots + adjoint with imaging; scaled to shots/s."""
     from fwi2d_amd import workloads as W
@@ -155,6 +231,12 @@ def cpu_baseline(nt_sample=120):
 
 def main():
     args = parse()
+    if args.summarize:
+        summarize(args.summarize)
+        return
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build runs on a single MI355X; see DESIGN.md for "
               "how the batched engine shards shots over GPUs.", file=sys.stderr)
@@ -168,19 +250,24 @@ def main():
     cells_per_grad = 2.0 * nt * nx * nz  # forward + adjoint, per shot
 
     # ---- batched form -------------------------------------------------------
+    log("workload ready: %d shots, %dx%d, nt=%d" % (ns, nz, nx, nt))
     B = Batch(fwi2d_amd, torch, prob, m_true, m0, sxz, rxz, src, nt, dt, args.tile_x)
+    log("observed data modelled")
     for _ in range(args.warmup):
         B.gradient()
     torch.cuda.synchronize()
+    log("warmup done")
     t0 = time.perf_counter()
     for _ in range(args.steps):
         B.gradient()
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
+    log("timed gradients done: %.3f s" % el)
     ms_per_step = 1000.0 * el / args.steps
     shots_per_s = ns * args.steps / el
     gcells = shots_per_s * cells_per_grad / 1e9
     grad_norm = float(torch.linalg.vector_norm(B.grad).item())
+    traffic = measured_traffic() if ns == 64 else None
 
     # ---- dominant kernel: fused step, timed with HIP events on its stream ----
     B.eng.set_option("timing", 1)
@@ -198,6 +285,7 @@ def main():
     # ---- plain form: one shot per launch, host-driven steps -----------------
     P = args.plain_shots
     t_plain = 0.0
+    plain_shots_per_s = None
     for s in range(P):
         one = Batch(fwi2d_amd, torch, prob, m_true, m0, sxz[s:s + 1], rxz[s:s + 1], src[:, s:s + 1], nt, dt,
                     args.tile_x)
@@ -208,10 +296,14 @@ def main():
         torch.cuda.synchronize()
         t_plain += time.perf_counter() - t1
         del one
-    plain_shots_per_s = P / t_plain
+    if P > 0:
+        plain_shots_per_s = P / t_plain
+        log("plain form done")
 
-    grad_err = grad_error_small()
-    cpu = None if args.no_cpu else cpu_baseline()
+    grad_err = None if args.no_check else grad_error_small()
+    log("gradient check done")
+    cpu = None if args.no_cpu else cpu_baseline(args.cpu_steps)
+    log("cpu baseline done")
 
     line = {
         "metric": METRIC,
@@ -231,14 +323,14 @@ def main():
                                % (ns, nt),
                    "shots_per_step": ns, "nx": nx, "nz": nz, "nt": nt, "tile_x": args.tile_x},
         "gcell_updates_per_s": round(gcells, 2),
-        "plain_one_shot_per_launch_shots_per_s": round(plain_shots_per_s, 4),
-        "batched_vs_plain": round(shots_per_s / plain_shots_per_s, 3),
+        "plain_one_shot_per_launch_shots_per_s": plain_shots_per_s and round(plain_shots_per_s, 4),
+        "batched_vs_plain": plain_shots_per_s and round(shots_per_s / plain_shots_per_s, 3),
         "recompute_steps_per_gradient": int(recompute // max(args.steps + args.warmup, 1)),
         "grad_rel_err_vs_fp64": grad_err,
         "grad_err_sample": "1 shot, 160x100 grid, nt=600, raw lambda/mu/rho gradients vs FP64 oracle",
         "grad_norm": grad_norm,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "k_step (fused time step, all shots)", "avg_launch_us": round(step_s * 1e6, 2),
                      "bytes_per_cell_update": BYTES_PER_CELL_UPDATE_FWD, "cells_per_launch": cells_per_launch},
     }
