@@ -70,13 +70,16 @@ struct Engine {
     int ldz = 0;          // device z pitch: nz rounded up to a multiple of 4
     long long nxz = 0;    // nx * ldz, the device size of one field
     int tile_x = 16, ntx = 0, ntz = 0;
+    int n_tiles_in = 0, n_tiles_edge = 0;
+    DevMem d_tiles_in, d_tiles_edge;
     int opt_tile_x = 16, opt_segment = 0, opt_lag = 1, opt_graph = 0, opt_timing = 0;
+    int opt_spb = 4;      // shots advanced one after the other by each step workgroup
     double last_fwd_loop_ms = 0.0;  // HIP-event time of the last forward step loop (timing option)
     int last_fwd_loop_steps = 0;
     bool vti = false;
 
-    DevMem d_model, d_div_tb, d_div_lr, d_tx, d_tz;
-    std::vector<float> h_model;
+    DevMem d_model, d_deriv, d_div_tb, d_div_lr, d_tx, d_tz;
+    std::vector<float> h_model, h_deriv;  // h_deriv: dt^2/rho[, (1+2eps) mu, (1+2eps) lambda], [nd][nx][nz]
     long long model_epoch = 0, ckpt_epoch = -1;
 
     // acquisition
@@ -129,6 +132,21 @@ struct Engine {
         tile_x = tx;
         ntx = tiles_x(g.nx, tile_x);
         ntz = tiles_z(ldz);
+        // interior tiles run the step kernel without closure code; tiles of each class
+        // are listed x-major so that neighbouring tiles land on one XCD
+        std::vector<int> in, edge;
+        for (int tix = 0; tix < ntx; tix++)
+            for (int tiz = 0; tiz < ntz; tiz++)
+                (interior_tile(tix, ntx, tile_x, tiz, ntz, g.nx, g.nz, ldz) ? in : edge).push_back(tix * ntz + tiz);
+        n_tiles_in = (int)in.size();
+        n_tiles_edge = (int)edge.size();
+        d_tiles_in.reserve(std::max<size_t>(in.size(), 1) * sizeof(int));
+        d_tiles_edge.reserve(std::max<size_t>(edge.size(), 1) * sizeof(int));
+        if (!in.empty())
+            HIPCHECK(hipMemcpyAsync(d_tiles_in.p, in.data(), in.size() * sizeof(int), hipMemcpyHostToDevice, st));
+        if (!edge.empty())
+            HIPCHECK(hipMemcpyAsync(d_tiles_edge.p, edge.data(), edge.size() * sizeof(int), hipMemcpyHostToDevice, st));
+        HIPCHECK(hipStreamSynchronize(st));
         if (ns > 0) build_tile_lists();
     }
     int ntiles() const { return ntx * ntz; }
@@ -276,6 +294,29 @@ struct Engine {
         for (int c = 0; c < np; c++) launch_fill(d_model.as<float>() + c * nxz, nxz, c == 2 ? 1.f : 0.f, st);
         HIPCHECK(hipMemcpy2DAsync(d_model.p, (size_t)ldz * sizeof(float), m, (size_t)g.nz * sizeof(float),
                                   (size_t)g.nz * sizeof(float), (size_t)np * g.nx, hipMemcpyHostToDevice, st));
+        // per-cell factors of the step, formed in double and rounded once
+        // VTI imaging factors (we_op.cpp:1510-1519): delta recovered from c13, then
+        // d c13 / d lambda and d c13 / d mu
+        const int nd = vti ? 6 : 1;
+        h_deriv.resize((size_t)nd * nc);
+        for (long long i = 0; i < nc; i++) {
+            h_deriv[i] = (float)(g.dt * g.dt / (double)rho[i]);
+            if (vti) {
+                const double l = m[i], u = mu[i], c = m[3 * nc + i];
+                const double e = 1.0 + 2.0 * (double)m[4 * nc + i];
+                h_deriv[nc + i] = (float)(e * u);
+                h_deriv[2 * nc + i] = (float)(e * l);
+                const double del = ((c + u) * (c + u) - (l + u) * (l + u)) / (2.0 * (l + u) * (l + 2.0 * u));
+                const double v1 = std::sqrt(2.0 * (l + 2.0 * u) * (l + u) * del + (l + u) * (l + u));
+                h_deriv[3 * nc + i] = (float)e;
+                h_deriv[4 * nc + i] = (float)(((1.0 + 2.0 * del) * l + (1.0 + 3.0 * del) * u) / v1);
+                h_deriv[5 * nc + i] = (float)(((1.0 + 3.0 * del) * l + (1.0 + 4.0 * del) * u) / v1 - 1.0);
+            }
+        }
+        d_deriv.reserve((size_t)nd * nxz * sizeof(float));
+        launch_fill(d_deriv.as<float>(), nd * nxz, 0.f, st);
+        HIPCHECK(hipMemcpy2DAsync(d_deriv.p, (size_t)ldz * sizeof(float), h_deriv.data(), (size_t)g.nz * sizeof(float),
+                                  (size_t)g.nz * sizeof(float), (size_t)nd * g.nx, hipMemcpyHostToDevice, st));
         build_divisors();
         model_epoch++;
     }
@@ -435,6 +476,12 @@ struct Engine {
         a.rho = m + 2 * nxz;
         a.c13 = vti ? m + 3 * nxz : nullptr;
         a.eps = vti ? m + 4 * nxz : nullptr;
+        const float* dv = d_deriv.as<float>();
+        a.irho = dv;
+        a.mux = vti ? dv + nxz : nullptr;
+        a.lamx = vti ? dv + 2 * nxz : nullptr;
+        a.spb = std::max(1, std::min(opt_spb, ns));
+        a.ngroups = (ns + a.spb - 1) / a.spb;
         a.div_tb = d_div_tb.as<float>();
         a.div_lr = d_div_lr.as<float>();
         a.taper_x = d_tx.as<float>();
@@ -486,9 +533,13 @@ struct Engine {
         launch_scale_prev(L.P(), d_model.as<float>() + 2 * nxz, (float)(0.5 * g.dt * g.dt), nxz, ns, st);
     }
 
-    void step(Levels& L, int it, bool adjoint, const float* traces, float* rcv)
+    // img_idx >= 0: image forward snapshot img_idx in this adjoint step (its segment must
+    // be loaded): the step's current level is the lagged adjoint level and its new level
+    // the adjoint level of the reference's compute_gradients call
+    void step(Levels& L, int it, bool adjoint, const float* traces, float* rcv, int img_idx = -1)
     {
         StepArgs a = base_args();
+        if (img_idx >= 0) set_image_args(a, img_idx);
         a.it = it;
         a.taper_prev = it > 0;
         a.prev = L.P();
@@ -507,7 +558,8 @@ struct Engine {
             a.ext_trace = rcv;
             a.ext_comp_stride = (long long)nr * g.nt;
         }
-        launch_step(a, vti, tile_x, st);
+        launch_step(a, vti, tile_x, d_tiles_in.as<int>(), n_tiles_in, false, img_idx >= 0, st);
+        launch_step(a, vti, tile_x, d_tiles_edge.as<int>(), n_tiles_edge, true, img_idx >= 0, st);
         L.rotate();
         stat_steps++;
     }
@@ -648,6 +700,37 @@ struct Engine {
         seg_loaded = j;
     }
 
+    // imaging weights of snapshot idx (we_op.cpp:527-561): gdamp window, endpoint halves
+    void image_weights(int idx, int& mode, float& wlm, float& wrho) const
+    {
+        mode = (idx > 0 && idx < g.nt / g.sub) ? 0 : (idx == 0 ? 1 : 2);
+        const double dtg = g.sub * g.dt;
+        const double t = idx * dtg;
+        double w = 1.0;
+        if (g.gdamp_tmax - t > 0 && t - g.gdamp_tmin >= 0)
+            w = std::cos(g.gdamp_power * M_PI * (t - g.gdamp_tmin) / (g.gdamp_tmax - g.gdamp_tmin));
+        w *= w;
+        wlm = (float)(idx == 0 ? w * 0.5 * dtg : w * dtg);
+        wrho = (float)(w * 1.0 / dtg);
+    }
+
+    void set_image_args(StepArgs& a, int idx)
+    {
+        const int j = idx / seg_K;
+        if (seg_loaded != j) throw std::logic_error("imaging segment not loaded");
+        image_weights(idx, a.img_mode, a.wlm, a.wrho);
+        a.f0 = seg_slot(j, idx);
+        a.fm = (a.img_mode != 1) ? seg_slot(j, idx - 1) : nullptr;
+        a.fp = (a.img_mode != 2) ? seg_slot(j, idx + 1) : nullptr;
+        a.fshot_stride = 2 * nxz;
+        a.grad = d_gshot.as<float>();
+        a.ggroup_stride = (long long)np * nxz;
+        const float* dv = d_deriv.as<float>();
+        a.ge = vti ? dv + 3 * nxz : nullptr;
+        a.gv2 = vti ? dv + 4 * nxz : nullptr;
+        a.gv3 = vti ? dv + 5 * nxz : nullptr;
+    }
+
     void image(const Levels& A, int idx, const float* dsrc)
     {
         const int j = idx / seg_K;
@@ -659,17 +742,9 @@ struct Engine {
         a.ntx = ntx;
         a.ntz = ntz;
         a.nshots = ns;
-        a.mode = (idx > 0 && idx < g.nt / g.sub) ? 0 : (idx == 0 ? 1 : 2);
         a.rdx = (float)(1.0 / g.dx);
         a.rdz = (float)(1.0 / g.dz);
-        const double dtg = g.sub * g.dt;
-        const double t = idx * dtg;
-        double w = 1.0;
-        if (g.gdamp_tmax - t > 0 && t - g.gdamp_tmin >= 0)
-            w = std::cos(g.gdamp_power * M_PI * (t - g.gdamp_tmin) / (g.gdamp_tmax - g.gdamp_tmin));
-        w *= w;
-        a.wlm = (float)(idx == 0 ? w * 0.5 * dtg : w * dtg);
-        a.wrho = (float)(w * 1.0 / dtg);
+        image_weights(idx, a.mode, a.wlm, a.wrho);
         const float* m = d_model.as<float>();
         a.lam = m;
         a.mu = m + nxz;
@@ -700,19 +775,40 @@ struct Engine {
         const long long ntr_r = (long long)2 * nr * g.nt;
         const float* dadj = on_device ? adj : upload_traces(d_adjsrc, adj, std::max<long long>(ntr_r, 1));
         const float* dsrc = fwd_src;
-        d_gshot.reserve((size_t)ns * np * nxz * sizeof(float));
-        launch_fill(d_gshot.as<float>(), (long long)ns * np * nxz, 0.f, st);
+        // with the reference's lag the imaging is fused into the adjoint steps and the
+        // gradients are kept per shot group; without it (a test option) the separate
+        // imaging kernel keeps them per shot
+        const bool fused = opt_lag == 1;
+        const int spb = std::max(1, std::min(opt_spb, ns));
+        const int nslab = fused ? (ns + spb - 1) / spb : ns;
+        d_gshot.reserve((size_t)nslab * np * nxz * sizeof(float));
+        launch_fill(d_gshot.as<float>(), (long long)nslab * np * nxz, 0.f, st);
         Levels A = levels_in(d_adj);
         init_levels(A, true, dadj);
         const int nt = g.nt, sub = g.sub;
-        for (int it = 0; it < nt - 1; it++) {
-            if (it != 0 && (nt - 1 - it) % sub == 0) image(A, (nt - 1 - it) / sub, dsrc);
-            step(A, it, true, dadj, nullptr);
+        if (fused) {
+            // step it takes the adjoint from level it to it+1; the pair (it, it+1) images
+            // snapshot (nt-2-it)/sub when that divides (the reference images at its
+            // iteration it+1, after the step, with the derivatives of level it)
+            for (int it = 0; it < nt - 1; it++) {
+                int idx = -1;
+                if ((nt - 2 - it) % sub == 0) {
+                    idx = (nt - 2 - it) / sub;
+                    load_segment(idx / seg_K, dsrc);
+                    stat_image_calls++;
+                }
+                step(A, it, true, dadj, nullptr, idx);
+            }
+        } else {
+            for (int it = 0; it < nt - 1; it++) {
+                if (it != 0 && (nt - 1 - it) % sub == 0) image(A, (nt - 1 - it) / sub, dsrc);
+                step(A, it, true, dadj, nullptr);
+            }
+            image(A, 0, dsrc);
         }
-        image(A, 0, dsrc);
         d_gout.reserve((size_t)np * nxz * sizeof(float));
         float* out = d_gout.as<float>();
-        launch_sum_shots(d_gshot.as<float>(), (long long)np * nxz, ns, (long long)np * nxz, out, 0, st);
+        launch_sum_shots(d_gshot.as<float>(), (long long)np * nxz, nslab, (long long)np * nxz, out, 0, st);
         // [np][nx][ldz] -> [np][nx][nz]
         HIPCHECK(hipMemcpy2DAsync(grad, (size_t)g.nz * sizeof(float), out, (size_t)ldz * sizeof(float),
                                   (size_t)g.nz * sizeof(float), (size_t)np * g.nx,
@@ -837,6 +933,9 @@ int fwi2d_engine_set_option(fwi2d_engine* e, const char* key, long long value)
             x->opt_timing = value ? 1 : 0;
         } else if (k == "graph") {
             x->opt_graph = value ? 1 : 0;
+        } else if (k == "shots_per_block") {
+            if (value < 1 || value > 64) throw std::runtime_error("shots_per_block must be in [1, 64]");
+            x->opt_spb = (int)value;
         } else {
             throw std::runtime_error("unknown option " + k);
         }
@@ -854,7 +953,9 @@ long long fwi2d_engine_get_stat(fwi2d_engine* e, const char* key)
     if (k == "segment") return x->seg_K;
     if (k == "nseg") return x->nseg;
     if (k == "tile_x") return x->tile_x;
+    if (k == "shots_per_block") return x->opt_spb;
     if (k == "ntiles") return x->ntiles();
+    if (k == "interior_tiles") return x->n_tiles_in;
     if (k == "nrec") return x->nr;
     if (k == "nshots") return x->ns;
     if (k == "fwd_loop_ns") return (long long)(x->last_fwd_loop_ms * 1e6);

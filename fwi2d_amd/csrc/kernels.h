@@ -24,9 +24,12 @@ struct PointEntry {
 
 struct StepArgs {
     int nx, nz, ldz, ntx, ntz, nshots, nt, it, taper_prev;
+    int spb, ngroups;      // shots per workgroup, shot groups (ceil(nshots / spb))
     float rdx, rdz, rdx2, rdz2, dt, dt2, sat_x, sat_z;
     int bc_top, bc_bottom, bc_left, bc_right;
     const float *lam, *mu, *rho, *c13, *eps;
+    const float* irho;     // dt^2 / rho
+    const float *mux, *lamx;  // VTI: (1 + 2 eps) mu, (1 + 2 eps) lambda
     const float *div_tb;   // [2 sides][2 comps][nx] absorbing divisors, top then bottom
     const float *div_lr;   // [2 sides][2 comps][nz]
     const float *taper_x;  // [nx]
@@ -47,6 +50,15 @@ struct StepArgs {
     const PointEntry* ext;
     float* ext_trace;
     long long ext_comp_stride;
+    // imaging fused into an adjoint step: the derivatives of the step's current level
+    // and the new level image forward snapshot idx (compute_gradients, we_op.cpp:505-562)
+    int img_mode;              // 0 interior idx, 1 first (idx 0), 2 last
+    float wlm, wrho;           // lambda/mu and rho weights of this snapshot
+    const float *f0, *fm, *fp; // forward snapshots idx, idx-1, idx+1 (shot 0 base)
+    long long fshot_stride;
+    float* grad;               // raw gradients per shot group [group][np][nx][ldz]
+    long long ggroup_stride;
+    const float *ge, *gv2, *gv3;  // VTI: 1 + 2 eps, d c13 / d lambda, d c13 / d mu
 };
 
 struct ImageArgs {
@@ -73,7 +85,11 @@ int tiles_z(int ldz);
 void tile_range_x(int k, int ntile, int T, int n, int& s, int& e);
 void tile_range_z(int k, int ntile, int nz, int ldz, int& s, int& e);
 
-void launch_step(const StepArgs& a, bool vti, int tile_x, hipStream_t s);
+// tiles: ntile tile indices (tix * ntz + tiz); edge selects the instantiation with the
+// closure and SAT paths (interior_tile() false) or without them
+void launch_step(const StepArgs& a, bool vti, int tile_x, const int* tiles, int ntile, bool edge, bool image,
+                 hipStream_t s);
+bool interior_tile(int tix, int ntx, int tile_x, int tiz, int ntz, int nx, int nz, int ldz);
 void launch_init_prev(float* prev, long long shot_stride, long long nxz, int ldz, int nshots, const int* shot_ptr,
                       const PointEntry* ents, const float* trace, long long comp_stride, hipStream_t s);
 void launch_scale_prev(float* prev, const float* rho, float half_dt2, long long nxz, int nshots, hipStream_t s);

@@ -1,9 +1,9 @@
 // Device kernels of the elastic SBP-SAT propagation engine, written for gfx950.
 //
-// Layout: every field is [x][z] with z contiguous (index ix*nz + iz), as in the
-// reference.  A workgroup of 256 threads owns a tile of 64 (z) x TX (x) cells for one
-// shot; lanes run along z so every global and LDS access of a wavefront is a
-// contiguous 64-float row.
+// Layout: every field is [x][ldz] with z contiguous (index ix*ldz + iz), the reference's
+// [x][z] order with z padded to a multiple of 4.  A workgroup of 256 threads owns a tile
+// of 64 (z) x TX (x) cells and a group of shots; lanes run along z and each moves 4
+// consecutive cells, so a wavefront's global accesses are contiguous 256-byte columns.
 //
 // Tiling: tiles start at multiples of the tile size except the last one in each
 // direction, which is anchored to end at the grid edge (start = n - T).  Each tile
@@ -79,12 +79,12 @@ __device__ __forceinline__ int window_start(int s, int H)
 
 // Range [lo, hi) of first-derivative products a tile writing [s, e) of n cells reads:
 // its cells +/- 2, widened to the 6 closure inputs of D1 rows at either grid edge.
-__device__ __forceinline__ void product_range(int s, int e, int n, int& lo, int& hi)
+__host__ __device__ __forceinline__ void product_range(int s, int e, int n, int& lo, int& hi)
 {
-    lo = max(0, s - 2);
-    hi = min(n, e + 2);
-    if (s < 4) { lo = 0; hi = max(hi, 6); }
-    if (e > n - 4) { lo = min(lo, n - 6); hi = n; }
+    lo = s - 2 > 0 ? s - 2 : 0;
+    hi = e + 2 < n ? e + 2 : n;
+    if (s < 4) { lo = 0; hi = hi > 6 ? hi : 6; }
+    if (e > n - 4) { lo = lo < n - 6 ? lo : n - 6; hi = n; }
 }
 
 // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a contiguous run
@@ -194,131 +194,38 @@ __device__ __forceinline__ float d1_row0(const float* p, int s)
 // ---------------------------------------------------------------------------
 // Fused time step
 //
-// A workgroup owns a 64 (z) x TX (x) tile of one shot; each thread owns quads of 4
-// consecutive z cells.  Phase 1 stages the state and mu with a 4-cell halo in LDS,
-// phase 2 forms the products of first derivatives, phase 3 computes the new level.
-// Quads whose stencils are all interior take the vectorised path; quads touching
-// closure rows or SAT lines take the per-cell path.
+// A workgroup owns a 64 (z) x TX (x) tile and advances a group of shots through it one
+// after the other, so that the coefficient window and the per-cell factors are read
+// once per tile, not once per shot.  Per shot: the state window (halo 4 in x, 8 in z)
+// is staged in LDS, the products of first derivatives are formed on the tile +/- 2 (x)
+// and +/- 4 (z), then each thread computes the new level of its quads (4 consecutive z
+// cells).  The x and z parts of every stencil choose between the vectorised interior
+// path and the closure path independently, so only quads that really touch a closure
+// row take it.  The next shot's window is loaded into registers before the current
+// shot's arithmetic, which hides its latency.
 // ---------------------------------------------------------------------------
 template <int TX>
 struct StepShape {
-    static constexpr int TZ = kTileZ, H = 4;
-    static constexpr int WZ = TZ + 2 * H, WX = TX + 2 * H;  // state/coefficient window (cells)
-    static constexpr int WQ = WZ / 4;                        // window quads per column
-    static constexpr int IX = TX + 4;                        // product columns (z extent = WZ)
+    static constexpr int TZ = kTileZ;
+    static constexpr int HX = 4, HZ = 8;
+    static constexpr int WX = TX + 2 * HX, WZ = TZ + 2 * HZ;  // state window (cells)
+    static constexpr int WQ = WZ / 4;                          // window quads per column
+    static constexpr int IX = TX + 4;                          // product columns
+    static constexpr int PZ = TZ + 8;                          // product rows [sz-4, sz+TZ+4)
+    static constexpr int PQ = PZ / 4;
     static constexpr int NQ1 = (WX * WQ + kThreads - 1) / kThreads;  // window quads per thread
-    static constexpr int NQ2 = (IX * WQ + kThreads - 1) / kThreads;  // product quads per thread
-    static constexpr int NQ3 = (TX * 16) / kThreads;                  // new-level quads per thread
+    static constexpr int NQ2 = (IX * PQ + kThreads - 1) / kThreads;  // product quads per thread
+    static constexpr int NQ3 = (TX * (TZ / 4)) / kThreads;           // new-level quads per thread
 };
 
 struct TileInfo {
-    int shot, tile, sx, ex, sz, ez, wsx, wsz, plx, phx, plz, phz;
+    int tile, sx, ex, sz, ez, wsx, wsz, plx, phx, plz, phz, pz0;
 };
 
 struct StepSmem {
     float *ux, *uz, *mu, *mux, *px, *pz, *qx, *qz;
 };
 
-// ---- per-cell path (closure rows, SAT lines) ------------------------------------
-// products at one cell; w = window index, returns (px, pz, qx, qz)
-template <bool VTI, int TX>
-__device__ __forceinline__ float4 product_cell(const StepArgs& a, const StepSmem& sm, int w, int gx, int gz,
-                                               float lam, float c13, float eps)
-{
-    using S = StepShape<TX>;
-    const float dxux = d1<false>(&sm.ux[w], S::WZ, gx, a.nx) * a.rdx;
-    const float dzuz = d1<false>(&sm.uz[w], 1, gz, a.nz) * a.rdz;
-    const float dxuz = d1<false>(&sm.uz[w], S::WZ, gx, a.nx) * a.rdx;
-    const float dzux = d1<false>(&sm.ux[w], 1, gz, a.nz) * a.rdz;
-    const float mu = sm.mu[w];
-    float px, pz = 0.f;
-    if (!VTI) {
-        px = lam * (dxux + dzuz);
-    } else {
-        const float lamx = (1.f + 2.f * eps) * lam;
-        px = lamx * dxux + c13 * dzuz;
-        pz = lam * dzuz + c13 * dxux;
-    }
-    return make_float4(px, pz, mu * dxuz, mu * dzux);
-}
-
-// right-hand side r = (rx, rz) of one cell before sources: volume terms (we_op.cpp:690-720,
-// 1665-1700) and SAT terms (we_op.cpp:726-822, 1705-1800)
-template <bool VTI, int TX>
-__device__ __forceinline__ void rhs_cell(const StepArgs& a, const StepSmem& sm, const TileInfo& t, int gx, int gz,
-                                         float prx, float prz, float& rxo, float& rzo)
-{
-    using S = StepShape<TX>;
-    const int nx = a.nx, nz = a.nz;
-    const int w = (gx - t.wsx) * S::WZ + (gz - t.wsz);
-    const int q = (gx - t.plx) * S::WZ + (gz - t.wsz);
-    float rx, rz, d2zx, d2xz;
-    if (gx >= 6 && gx < nx - 6) {
-        const W5 wx = d2_weights(&sm.mu[w], S::WZ);
-        rx = 2.f * d2_apply(VTI ? d2_weights(&sm.mux[w], S::WZ) : wx, &sm.ux[w], S::WZ);
-        d2xz = d2_apply(wx, &sm.uz[w], S::WZ);
-    } else {
-        rx = 2.f * d2_closure(&sm.ux[w], VTI ? &sm.mux[w] : &sm.mu[w], S::WZ, gx, nx);
-        d2xz = d2_closure(&sm.uz[w], &sm.mu[w], S::WZ, gx, nx);
-    }
-    if (gz >= 6 && gz < nz - 6) {
-        const W5 wz = d2_weights(&sm.mu[w], 1);
-        rz = 2.f * d2_apply(wz, &sm.uz[w], 1);
-        d2zx = d2_apply(wz, &sm.ux[w], 1);
-    } else {
-        rz = 2.f * d2_closure(&sm.uz[w], &sm.mu[w], 1, gz, nz);
-        d2zx = d2_closure(&sm.ux[w], &sm.mu[w], 1, gz, nz);
-    }
-    rx *= a.rdx2;
-    rz *= a.rdz2;
-    rz += d2xz * a.rdx2;
-    rx += d2zx * a.rdz2;
-    rx += d1<false>(&sm.px[q], S::WZ, gx, nx) * a.rdx;
-    rz += d1<false>(VTI ? &sm.pz[q] : &sm.px[q], 1, gz, nz) * a.rdz;
-    rz += d1<false>(&sm.qz[q], S::WZ, gx, nx) * a.rdx;
-    rx += d1<false>(&sm.qx[q], 1, gz, nz) * a.rdz;
-
-    const bool edge = (gz == 0) | (gz == nz - 1) | (gx == 0) | (gx == nx - 1);
-    if (edge) {
-        const int g = gx * a.ldz + gz;
-        const float mu0 = sm.mu[w];
-        const float lam0 = a.lam[g];
-        const float rho0 = a.rho[g];
-        const float f1tb = VTI ? a.c13[g] : lam0;
-        const float epsf = VTI ? 1.f + 2.f * a.eps[g] : 1.f;
-        for (int side = 0; side < 2; side++) {
-            const int bc = side ? a.bc_bottom : a.bc_top;
-            if (bc == 0 || gz != (side ? nz - 1 : 0)) continue;
-            const float sgn = side ? 1.f : -1.f;
-            const int sn = side ? -1 : 1;  // inward step along z
-            float tt = sgn * mu0 * d1<false>(&sm.uz[w], S::WZ, gx, nx) * a.rdx + mu0 * bnd_deriv(&sm.ux[w], sn) * a.rdz;
-            if (bc == 2) tt -= (0.5f * sqrtf(mu0 * rho0)) * prx / a.dt;
-            rx -= a.sat_z * tt;
-            tt = sgn * f1tb * d1<false>(&sm.ux[w], S::WZ, gx, nx) * a.rdx + (2.f * mu0) * bnd_deriv(&sm.uz[w], sn) * a.rdz;
-            if (bc == 2) tt -= (0.5f * sqrtf((lam0 + 2.f * mu0) * rho0)) * prz / a.dt;
-            rz -= a.sat_z * tt;
-            rz += (a.sat_z * a.rdz) * lam0 * d1_row0(&sm.uz[w], sn);
-        }
-        for (int side = 0; side < 2; side++) {
-            const int bc = side ? a.bc_right : a.bc_left;
-            if (bc == 0 || gx != (side ? nx - 1 : 0)) continue;
-            const float sgn = side ? 1.f : -1.f;
-            const int sn = side ? -S::WZ : S::WZ;  // inward step along x
-            const float f2 = epsf * (2.f * mu0);
-            float tt = sgn * f1tb * d1<false>(&sm.uz[w], 1, gz, nz) * a.rdz + f2 * bnd_deriv(&sm.ux[w], sn) * a.rdx;
-            if (bc == 2) tt -= (0.5f * sqrtf(epsf * (lam0 + 2.f * mu0) * rho0)) * prx / a.dt;
-            rx -= a.sat_x * tt;
-            rx += (a.sat_x * a.rdx) * (epsf * lam0) * d1_row0(&sm.ux[w], sn);
-            tt = sgn * mu0 * d1<false>(&sm.ux[w], 1, gz, nz) * a.rdz + mu0 * bnd_deriv(&sm.uz[w], sn) * a.rdx;
-            if (bc == 2) tt -= (0.5f * sqrtf(mu0 * rho0)) * prz / a.dt;
-            rz -= a.sat_x * tt;
-        }
-    }
-    rxo = rx;
-    rzo = rz;
-}
-
-// ---- vectorised path: interior quad ------------------------------------------------
 // D1 of a 12-cell column (cells z0-4 .. z0+7) at the 4 quad cells
 __device__ __forceinline__ void d1_quad_z(const float* c, float* out)
 {
@@ -345,15 +252,133 @@ __device__ __forceinline__ void load_col12(const float* base, int i, float* c)
     unpack(ld4(base + i + 4), c + 8);
 }
 
-template <bool VTI, int TX>
-__device__ __forceinline__ void rhs_quad(const StepArgs& a, const StepSmem& sm, const TileInfo& t, int gx, int z0,
-                                         float* rx, float* rz)
+// D2(c) u at global index g of n along stride s: interior weights or closure rows
+__device__ __forceinline__ float d2_any(const float* pu, const float* pc, int s, int g, int n)
+{
+    if (g >= 6 && g < n - 6) return d2_apply(d2_weights(pc, s), pu, s);
+    return d2_closure(pu, pc, s, g, n);
+}
+
+// ---- products of first derivatives on one quad --------------------------------------
+template <bool VTI, int TX, bool EDGE>
+__device__ __forceinline__ void product_quad(const StepArgs& a, const StepSmem& sm, const TileInfo& t, int gx, int z0,
+                                             float4 lam4, float4 lamx4, float4 c134, float* p, float* pz, float* qx,
+                                             float* qz)
 {
     using S = StepShape<TX>;
+    const int nx = a.nx, nz = a.nz;
     const int w = (gx - t.wsx) * S::WZ + (z0 - t.wsz);
-    const int q = (gx - t.plx) * S::WZ + (z0 - t.wsz);
-    // D2 along x of ux (coefficient 2mu, or (1+2eps)2mu) and uz (mu)
-    {
+    float dxux[4], dxuz[4], dzux[4], dzuz[4];
+    if (!EDGE || (gx >= 4 && gx < nx - 4)) {
+        float X[4][4], Z[4][4];
+        const int off[4] = {-2, -1, 1, 2};
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            unpack(ld4(&sm.ux[w + off[c] * S::WZ]), X[c]);
+            unpack(ld4(&sm.uz[w + off[c] * S::WZ]), Z[c]);
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            dxux[k] = (cD1I[0] * (X[2][k] - X[1][k]) + cD1I[1] * (X[3][k] - X[0][k])) * a.rdx;
+            dxuz[k] = (cD1I[0] * (Z[2][k] - Z[1][k]) + cD1I[1] * (Z[3][k] - Z[0][k])) * a.rdx;
+        }
+    } else if constexpr (EDGE) {
+        for (int k = 0; k < 4; k++) {
+            dxux[k] = d1_closure(&sm.ux[w + k], S::WZ, gx, nx) * a.rdx;
+            dxuz[k] = d1_closure(&sm.uz[w + k], S::WZ, gx, nx) * a.rdx;
+        }
+    }
+    if (!EDGE || (z0 >= 4 && z0 + 3 < nz - 4)) {
+        float Uc[12], Zc[12];
+        load_col12(sm.ux, w, Uc);
+        load_col12(sm.uz, w, Zc);
+        d1_quad_z(Uc, dzux);
+        d1_quad_z(Zc, dzuz);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            dzux[k] *= a.rdz;
+            dzuz[k] *= a.rdz;
+        }
+    } else if constexpr (EDGE) {
+        for (int k = 0; k < 4; k++) {
+            const int gz = z0 + k;
+            const bool in = gz >= 0 && gz < nz;
+            dzux[k] = in ? d1<false>(&sm.ux[w + k], 1, gz, nz) * a.rdz : 0.f;
+            dzuz[k] = in ? d1<false>(&sm.uz[w + k], 1, gz, nz) * a.rdz : 0.f;
+        }
+    }
+    float mu[4], lam[4], lamx[4], c13[4];
+    unpack(ld4(&sm.mu[w]), mu);
+    unpack(lam4, lam);
+    unpack(lamx4, lamx);
+    unpack(c134, c13);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        if (!VTI) {
+            p[k] = lam[k] * (dxux[k] + dzuz[k]);
+            pz[k] = 0.f;
+        } else {
+            p[k] = lamx[k] * dxux[k] + c13[k] * dzuz[k];
+            pz[k] = lam[k] * dzuz[k] + c13[k] * dxux[k];
+        }
+        qx[k] = mu[k] * dxuz[k];
+        qz[k] = mu[k] * dzux[k];
+    }
+}
+
+// ---- SAT terms of one perimeter cell (we_op.cpp:726-822, 1705-1800) -------------------
+template <bool VTI, int TX>
+__device__ __forceinline__ void sat_cell(const StepArgs& a, const float* sux, const float* suz, float mu0, int gx,
+                                         int gz, float prx, float prz, float& rx, float& rz)
+{
+    using S = StepShape<TX>;
+    const int nx = a.nx, nz = a.nz;
+    const int g = gx * a.ldz + gz;
+    const float lam0 = a.lam[g];
+    const float rho0 = a.rho[g];
+    const float f1tb = VTI ? a.c13[g] : lam0;
+    const float epsf = VTI ? 1.f + 2.f * a.eps[g] : 1.f;
+    for (int side = 0; side < 2; side++) {
+        const int bc = side ? a.bc_bottom : a.bc_top;
+        if (bc == 0 || gz != (side ? nz - 1 : 0)) continue;
+        const float sgn = side ? 1.f : -1.f;
+        const int sn = side ? -1 : 1;  // inward step along z
+        float tt = sgn * mu0 * d1<false>(suz, S::WZ, gx, nx) * a.rdx + mu0 * bnd_deriv(sux, sn) * a.rdz;
+        if (bc == 2) tt -= (0.5f * sqrtf(mu0 * rho0)) * prx / a.dt;
+        rx -= a.sat_z * tt;
+        tt = sgn * f1tb * d1<false>(sux, S::WZ, gx, nx) * a.rdx + (2.f * mu0) * bnd_deriv(suz, sn) * a.rdz;
+        if (bc == 2) tt -= (0.5f * sqrtf((lam0 + 2.f * mu0) * rho0)) * prz / a.dt;
+        rz -= a.sat_z * tt;
+        rz += (a.sat_z * a.rdz) * lam0 * d1_row0(suz, sn);
+    }
+    for (int side = 0; side < 2; side++) {
+        const int bc = side ? a.bc_right : a.bc_left;
+        if (bc == 0 || gx != (side ? nx - 1 : 0)) continue;
+        const float sgn = side ? 1.f : -1.f;
+        const int sn = side ? -S::WZ : S::WZ;  // inward step along x
+        const float f2 = epsf * (2.f * mu0);
+        float tt = sgn * f1tb * d1<false>(suz, 1, gz, nz) * a.rdz + f2 * bnd_deriv(sux, sn) * a.rdx;
+        if (bc == 2) tt -= (0.5f * sqrtf(epsf * (lam0 + 2.f * mu0) * rho0)) * prx / a.dt;
+        rx -= a.sat_x * tt;
+        rx += (a.sat_x * a.rdx) * (epsf * lam0) * d1_row0(sux, sn);
+        tt = sgn * mu0 * d1<false>(sux, 1, gz, nz) * a.rdz + mu0 * bnd_deriv(suz, sn) * a.rdx;
+        if (bc == 2) tt -= (0.5f * sqrtf(mu0 * rho0)) * prz / a.dt;
+        rz -= a.sat_x * tt;
+    }
+}
+
+// ---- right-hand side of one quad before sources: volume terms (we_op.cpp:690-720,
+// 1665-1700) and SAT terms ----------------------------------------------------------
+template <bool VTI, int TX, bool EDGE>
+__device__ __forceinline__ void rhs_quad(const StepArgs& a, const StepSmem& sm, const TileInfo& t, int gx, int z0,
+                                         const float* prx, const float* prz, float* rx, float* rz)
+{
+    using S = StepShape<TX>;
+    const int nx = a.nx, nz = a.nz;
+    const int w = (gx - t.wsx) * S::WZ + (z0 - t.wsz);
+    const int q = (gx - t.plx) * S::PZ + (z0 - t.pz0);
+    // x part: D2x[2mu or 2(1+2eps)mu] ux, D2x[mu] uz, Dx of the products
+    if (!EDGE || (gx >= 6 && gx < nx - 6)) {
         float U[5][4], Z[5][4], M[5][4], MX[VTI ? 5 : 1][4];
 #pragma unroll
         for (int c = 0; c < 5; c++) {
@@ -362,47 +387,46 @@ __device__ __forceinline__ void rhs_quad(const StepArgs& a, const StepSmem& sm, 
             unpack(ld4(&sm.mu[w + (c - 2) * S::WZ]), M[c]);
             if (VTI) unpack(ld4(&sm.mux[w + (c - 2) * S::WZ]), MX[c]);
         }
+        float P[4][4], Q[4][4];
+        const int off[4] = {-2, -1, 1, 2};
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            unpack(ld4(&sm.px[q + off[c] * S::PZ]), P[c]);
+            unpack(ld4(&sm.qz[q + off[c] * S::PZ]), Q[c]);
+        }
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             const float mk[5] = {M[0][k], M[1][k], M[2][k], M[3][k], M[4][k]};
             const float uk[5] = {U[0][k], U[1][k], U[2][k], U[3][k], U[4][k]};
             const float zk[5] = {Z[0][k], Z[1][k], Z[2][k], Z[3][k], Z[4][k]};
             const W5 wx = d2_weights(&mk[2], 1);
+            float d2u;
             if (VTI) {
                 const float xk[5] = {MX[0][k], MX[1][k], MX[2][k], MX[3][k], MX[4][k]};
-                rx[k] = 2.f * d2_apply(d2_weights(&xk[2], 1), &uk[2], 1);
+                d2u = d2_apply(d2_weights(&xk[2], 1), &uk[2], 1);
             } else {
-                rx[k] = 2.f * d2_apply(wx, &uk[2], 1);
+                d2u = d2_apply(wx, &uk[2], 1);
             }
-            rz[k] = d2_apply(wx, &zk[2], 1);  // D2x[mu] uz, added after D2z below
+            rx[k] = 2.f * d2u * a.rdx2;
+            rz[k] = d2_apply(wx, &zk[2], 1) * a.rdx2;
+            rx[k] += (cD1I[0] * (P[2][k] - P[1][k]) + cD1I[1] * (P[3][k] - P[0][k])) * a.rdx;
+            rz[k] += (cD1I[0] * (Q[2][k] - Q[1][k]) + cD1I[1] * (Q[3][k] - Q[0][k])) * a.rdx;
+        }
+    } else if constexpr (EDGE) {
+        for (int k = 0; k < 4; k++) {
+            rx[k] = 2.f * d2_closure(&sm.ux[w + k], VTI ? &sm.mux[w + k] : &sm.mu[w + k], S::WZ, gx, nx) * a.rdx2;
+            rz[k] = d2_closure(&sm.uz[w + k], &sm.mu[w + k], S::WZ, gx, nx) * a.rdx2;
+            rx[k] += d1<false>(&sm.px[q + k], S::PZ, gx, nx) * a.rdx;
+            rz[k] += d1<false>(&sm.qz[q + k], S::PZ, gx, nx) * a.rdx;
         }
     }
-    // D2 along z
-    float d2zx[4], d2zz[4];
-    {
-        float Uc[12], Zc[12], Mc[12];
+    // z part: D2z[2mu] uz, D2z[mu] ux, Dz of the products
+    if (!EDGE || (z0 >= 6 && z0 + 3 < nz - 6)) {
+        float Uc[12], Zc[12], Mc[12], d2zx[4], d2zz[4];
         load_col12(sm.ux, w, Uc);
         load_col12(sm.uz, w, Zc);
         load_col12(sm.mu, w, Mc);
         d2_quad_z(Mc, Zc, Uc, d2zz, d2zx);
-    }
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        const float d2xz = rz[k];
-        rx[k] *= a.rdx2;
-        rz[k] = 2.f * d2zz[k] * a.rdz2;
-        rz[k] += d2xz * a.rdx2;
-        rx[k] += d2zx[k] * a.rdz2;
-    }
-    // first derivatives of the products
-    {
-        float P[4][4], Q[4][4];
-        const int off[4] = {-2, -1, 1, 2};
-#pragma unroll
-        for (int c = 0; c < 4; c++) {
-            unpack(ld4(&sm.px[q + off[c] * S::WZ]), P[c]);
-            unpack(ld4(&sm.qz[q + off[c] * S::WZ]), Q[c]);
-        }
         float Pc[12], Qc[12], dpz[4], dqz[4];
         load_col12(VTI ? sm.pz : sm.px, q, Pc);
         load_col12(sm.qx, q, Qc);
@@ -410,127 +434,172 @@ __device__ __forceinline__ void rhs_quad(const StepArgs& a, const StepSmem& sm, 
         d1_quad_z(Qc, dqz);
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            const float dpx = cD1I[0] * (P[2][k] - P[1][k]) + cD1I[1] * (P[3][k] - P[0][k]);
-            const float dqx = cD1I[0] * (Q[2][k] - Q[1][k]) + cD1I[1] * (Q[3][k] - Q[0][k]);
-            rx[k] += dpx * a.rdx;
+            rz[k] += 2.f * d2zz[k] * a.rdz2;
+            rx[k] += d2zx[k] * a.rdz2;
             rz[k] += dpz[k] * a.rdz;
-            rz[k] += dqx * a.rdx;
             rx[k] += dqz[k] * a.rdz;
         }
-    }
-}
-
-// products of one interior quad (all four cells have interior D1 stencils)
-template <bool VTI, int TX>
-__device__ __forceinline__ void product_quad(const StepArgs& a, const StepSmem& sm, int w, const float* lam,
-                                             const float* c13, const float* eps, float4& P, float4& PZ, float4& QX,
-                                             float4& QZ)
-{
-    using S = StepShape<TX>;
-    float X[4][4], XZ[4][4];
-    const int off[4] = {-2, -1, 1, 2};
-#pragma unroll
-    for (int c = 0; c < 4; c++) {
-        unpack(ld4(&sm.ux[w + off[c] * S::WZ]), X[c]);
-        unpack(ld4(&sm.uz[w + off[c] * S::WZ]), XZ[c]);
-    }
-    float Uc[12], Zc[12], dzux[4], dzuz[4], mu[4];
-    load_col12(sm.ux, w, Uc);
-    load_col12(sm.uz, w, Zc);
-    d1_quad_z(Uc, dzux);
-    d1_quad_z(Zc, dzuz);
-    unpack(ld4(&sm.mu[w]), mu);
-    float p[4], pz[4], qx[4], qz[4];
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        const float dxux = (cD1I[0] * (X[2][k] - X[1][k]) + cD1I[1] * (X[3][k] - X[0][k])) * a.rdx;
-        const float dxuz = (cD1I[0] * (XZ[2][k] - XZ[1][k]) + cD1I[1] * (XZ[3][k] - XZ[0][k])) * a.rdx;
-        const float duz = dzuz[k] * a.rdz;
-        const float dux = dzux[k] * a.rdz;
-        if (!VTI) {
-            p[k] = lam[k] * (dxux + duz);
-            pz[k] = 0.f;
-        } else {
-            const float lamx = (1.f + 2.f * eps[k]) * lam[k];
-            p[k] = lamx * dxux + c13[k] * duz;
-            pz[k] = lam[k] * duz + c13[k] * dxux;
+    } else if constexpr (EDGE) {
+        for (int k = 0; k < 4; k++) {
+            const int gz = z0 + k;
+            if (gz >= nz) continue;
+            rz[k] += 2.f * d2_any(&sm.uz[w + k], &sm.mu[w + k], 1, gz, nz) * a.rdz2;
+            rx[k] += d2_any(&sm.ux[w + k], &sm.mu[w + k], 1, gz, nz) * a.rdz2;
+            rz[k] += d1<false>(VTI ? &sm.pz[q + k] : &sm.px[q + k], 1, gz, nz) * a.rdz;
+            rx[k] += d1<false>(&sm.qx[q + k], 1, gz, nz) * a.rdz;
         }
-        qx[k] = mu[k] * dxuz;
-        qz[k] = mu[k] * dux;
     }
-    P = make_float4(p[0], p[1], p[2], p[3]);
-    PZ = make_float4(pz[0], pz[1], pz[2], pz[3]);
-    QX = make_float4(qx[0], qx[1], qx[2], qx[3]);
-    QZ = make_float4(qz[0], qz[1], qz[2], qz[3]);
+    // SAT lines
+    if (EDGE && (gx == 0 || gx == nx - 1 || z0 == 0 || z0 + 3 >= nz - 1)) {
+        for (int k = 0; k < 4; k++) {
+            const int gz = z0 + k;
+            if (gz >= nz) continue;
+            if (gz == 0 || gz == nz - 1 || gx == 0 || gx == nx - 1)
+                sat_cell<VTI, TX>(a, &sm.ux[w + k], &sm.uz[w + k], sm.mu[w + k], gx, gz, prx[k], prz[k], rx[k], rz[k]);
+        }
+    }
 }
 
-template <bool VTI, int TX>
-__global__ void __launch_bounds__(kThreads) k_step(const StepArgs a)
+// First derivatives of a pair of fields (u, v) at the 4 cells of a quad from an LDS
+// array with column stride CS: (dx u, dz u, dx v, dz v), scaled by 1/h.
+template <bool EDGE, int CS>
+__device__ __forceinline__ void quad_derivs(const float* u, const float* v, int w, int gx, int z0, int nx, int nz,
+                                            float rdx, float rdz, float* xu, float* zu, float* xv, float* zv)
+{
+    if (!EDGE || (gx >= 4 && gx < nx - 4)) {
+        float U[4][4], V[4][4];
+        const int off[4] = {-2, -1, 1, 2};
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            unpack(ld4(&u[w + off[c] * CS]), U[c]);
+            unpack(ld4(&v[w + off[c] * CS]), V[c]);
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            xu[k] = (cD1I[0] * (U[2][k] - U[1][k]) + cD1I[1] * (U[3][k] - U[0][k])) * rdx;
+            xv[k] = (cD1I[0] * (V[2][k] - V[1][k]) + cD1I[1] * (V[3][k] - V[0][k])) * rdx;
+        }
+    } else if constexpr (EDGE) {
+        for (int k = 0; k < 4; k++) {
+            xu[k] = d1_closure(&u[w + k], CS, gx, nx) * rdx;
+            xv[k] = d1_closure(&v[w + k], CS, gx, nx) * rdx;
+        }
+    }
+    if (!EDGE || (z0 >= 4 && z0 + 3 < nz - 4)) {
+        float Uc[12], Vc[12];
+        load_col12(u, w, Uc);
+        load_col12(v, w, Vc);
+        d1_quad_z(Uc, zu);
+        d1_quad_z(Vc, zv);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            zu[k] *= rdz;
+            zv[k] *= rdz;
+        }
+    } else if constexpr (EDGE) {
+        for (int k = 0; k < 4; k++) {
+            const int gz = z0 + k;
+            const bool in = gz < nz;
+            zu[k] = in ? d1<false>(&u[w + k], 1, gz, nz) * rdz : 0.f;
+            zv[k] = in ? d1<false>(&v[w + k], 1, gz, nz) * rdz : 0.f;
+        }
+    }
+}
+
+// Hides a value from the optimiser so that address arithmetic derived from it is
+// recomputed inside the shot loop instead of being hoisted out and kept live in
+// registers across it (which multiplies the register footprint of the loop body).
+__device__ __forceinline__ void opaque(int& v) { __asm__ volatile("" : "+s"(v)); }
+__device__ __forceinline__ void opaque_v(int& v) { __asm__ volatile("" : "+v"(v)); }
+__device__ __forceinline__ void opaque(TileInfo& t)
+{
+    opaque(t.tile);
+    opaque(t.sx);
+    opaque(t.ex);
+    opaque(t.sz);
+    opaque(t.ez);
+    opaque(t.wsx);
+    opaque(t.wsz);
+    opaque(t.plx);
+    opaque(t.phx);
+    opaque(t.plz);
+    opaque(t.phz);
+    opaque(t.pz0);
+}
+
+// EDGE = false: tiles whose every quad takes the interior paths (the host classifies
+// tiles with the same conditions); no closure or SAT code is compiled in.
+// IMG: adjoint step with the imaging condition of forward snapshot a.img_mode/idx fused in;
+// the gradient contributions of the workgroup's shots are summed in registers and added
+// to the shot group's gradient once per launch.
+template <bool VTI, int TX, bool EDGE, bool IMG>
+__global__ void __launch_bounds__(kThreads) k_step(const StepArgs a, const int* tiles)
 {
     using S = StepShape<TX>;
     __shared__ __attribute__((aligned(16))) float sUx[S::WX * S::WZ];
     __shared__ __attribute__((aligned(16))) float sUz[S::WX * S::WZ];
     __shared__ __attribute__((aligned(16))) float sMu[S::WX * S::WZ];
     __shared__ __attribute__((aligned(16))) float sMuX[VTI ? S::WX * S::WZ : 4];  // (1+2eps) mu
-    __shared__ __attribute__((aligned(16))) float sPx[S::IX * S::WZ];
-    __shared__ __attribute__((aligned(16))) float sPz[VTI ? S::IX * S::WZ : 4];
-    __shared__ __attribute__((aligned(16))) float sQx[S::IX * S::WZ];
-    __shared__ __attribute__((aligned(16))) float sQz[S::IX * S::WZ];
-    const StepSmem sm{sUx, sUz, sMu, sMuX, sPx, sPz, sQx, sQz};
+    __shared__ __attribute__((aligned(16))) float sPx[S::IX * S::PZ];
+    __shared__ __attribute__((aligned(16))) float sPz[VTI ? S::IX * S::PZ : 4];
+    __shared__ __attribute__((aligned(16))) float sQx[S::IX * S::PZ];
+    __shared__ __attribute__((aligned(16))) float sQz[S::IX * S::PZ];
+    __shared__ __attribute__((aligned(16))) float sFx[IMG ? S::IX * S::PZ : 4];  // forward snapshot idx,
+    __shared__ __attribute__((aligned(16))) float sFz[IMG ? S::IX * S::PZ : 4];  // product-range layout
+    StepSmem sm{sUx, sUz, sMu, sMuX, sPx, sPz, sQx, sQz};
 
     const int ntiles = a.ntx * a.ntz;
     const int lin = xcd_remap(blockIdx.x, gridDim.x);
     TileInfo t;
-    t.shot = lin / ntiles;
-    t.tile = lin - t.shot * ntiles;
+    const int slot = lin / a.ngroups;
+    const int group = lin - slot * a.ngroups;
+    t.tile = tiles[slot];
     const int tix = t.tile / a.ntz;
     const int tiz = t.tile - tix * a.ntz;
     const int nx = a.nx, nz = a.nz, ldz = a.ldz;
     tile_range_x_hd(tix, a.ntx, TX, nx, t.sx, t.ex);
     tile_range_z_hd(tiz, a.ntz, nz, ldz, t.sz, t.ez);
-    t.wsx = window_start(t.sx, S::H);
-    t.wsz = window_start(t.sz, S::H);
+    t.wsx = window_start(t.sx, S::HX);
+    t.wsz = t.sz - S::HZ;
+    t.pz0 = t.sz - 4;
     product_range(t.sx, t.ex, nx, t.plx, t.phx);
     product_range(t.sz, t.ez, nz, t.plz, t.phz);
     const long long nxz = (long long)nx * ldz;
-    const float* cux = a.curr + t.shot * a.shot_stride;
-    const float* cuz = cux + nxz;
-    const float* pux = a.prev + t.shot * a.shot_stride;
     const int tid = threadIdx.x;
+    const int s0 = group * a.spb;
+    const int s1 = min(a.nshots, s0 + a.spb);
 
-    // ---- phase 1: every global load of the thread is issued first ----
-    float4 wux[S::NQ1], wuz[S::NQ1], wmu[S::NQ1], wep[VTI ? S::NQ1 : 1];
+    // ---- per tile: coefficient window, product coefficients, new-level factors ----
 #pragma unroll
     for (int m = 0; m < S::NQ1; m++) {
         const int i = tid + m * kThreads;
-        const int lx = i / S::WQ, lq = i - lx * S::WQ;
-        const int gx = t.wsx + lx, gz = t.wsz + 4 * lq;
-        const bool ok = i < S::WX * S::WQ && gx >= 0 && gx < nx && gz >= 0 && gz < ldz;
-        const int g = ok ? gx * ldz + gz : 0;
-        const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
-        wux[m] = ok ? ld4(cux + g) : z4;
-        wuz[m] = ok ? ld4(cuz + g) : z4;
-        wmu[m] = ok ? ld4(a.mu + g) : z4;
-        if (VTI) wep[m] = ok ? ld4(a.eps + g) : z4;
+        if (i < S::WX * S::WQ) {
+            const int lx = i / S::WQ, lq = i - lx * S::WQ;
+            const int gx = t.wsx + lx, gz = t.wsz + 4 * lq;
+            const bool ok = gx >= 0 && gx < nx && gz >= 0 && gz < ldz;
+            const int g = ok ? gx * ldz + gz : 0;
+            const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+            const int o = lx * S::WZ + 4 * lq;
+            st4(&sMu[o], ok ? ld4(a.mu + g) : z4);
+            if (VTI) st4(&sMuX[o], ok ? ld4(a.mux + g) : z4);
+        }
     }
-    // products' coefficients
-    float4 plam[S::NQ2], pc13[VTI ? S::NQ2 : 1], peps[VTI ? S::NQ2 : 1];
+    float4 plam[S::NQ2], plamx[VTI ? S::NQ2 : 1], pc13[VTI ? S::NQ2 : 1];
 #pragma unroll
     for (int m = 0; m < S::NQ2; m++) {
         const int i = tid + m * kThreads;
-        const int lc = i / S::WQ, lq = i - lc * S::WQ;
-        const int gx = t.plx + lc, gz = t.wsz + 4 * lq;
-        const bool ok = i < S::IX * S::WQ && gx < t.phx && gz + 3 >= t.plz && gz < t.phz;
-        const int g = ok ? gx * ldz + gz : 0;
+        const int lc = i / S::PQ, lq = i - lc * S::PQ;
+        const int gx = t.plx + lc, z0 = t.pz0 + 4 * lq;
+        const bool ok = i < S::IX * S::PQ && gx < t.phx && z0 + 3 >= t.plz && z0 < t.phz;
+        const int g = ok ? gx * ldz + z0 : 0;
         const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
         plam[m] = ok ? ld4(a.lam + g) : z4;
         if (VTI) {
+            plamx[m] = ok ? ld4(a.lamx + g) : z4;
             pc13[m] = ok ? ld4(a.c13 + g) : z4;
-            peps[m] = ok ? ld4(a.eps + g) : z4;
         }
     }
-    // new-level cells: previous level, density, tapers
-    float4 qprx[S::NQ3], qprz[S::NQ3], qrho[S::NQ3], qtz[S::NQ3];
+    float4 qir[S::NQ3], qtz[S::NQ3];
     float qtx[S::NQ3];
 #pragma unroll
     for (int m = 0; m < S::NQ3; m++) {
@@ -539,180 +608,269 @@ __global__ void __launch_bounds__(kThreads) k_step(const StepArgs a)
         const bool ok = gx < t.ex && gz < t.ez;
         const int g = ok ? gx * ldz + gz : 0;
         const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
-        qprx[m] = ok ? ld4(pux + g) : z4;
-        qprz[m] = ok ? ld4(pux + nxz + g) : z4;
-        qrho[m] = ok ? ld4(a.rho + g) : make_float4(1.f, 1.f, 1.f, 1.f);
+        qir[m] = ok ? ld4(a.irho + g) : z4;
         qtz[m] = ok ? ld4(a.taper_z + gz) : z4;
         qtx[m] = ok ? a.taper_x[gx] : 0.f;
     }
+
+    // ---- state window of one shot into registers ----
+    float4 wux[S::NQ1], wuz[S::NQ1];
+    auto load_window = [&](int shot, int tid, const TileInfo& t) {
+        const float* cux = a.curr + shot * a.shot_stride;
 #pragma unroll
-    for (int m = 0; m < S::NQ1; m++) {
-        const int i = tid + m * kThreads;
-        if (i < S::WX * S::WQ) {
+        for (int m = 0; m < S::NQ1; m++) {
+            const int i = tid + m * kThreads;
             const int lx = i / S::WQ, lq = i - lx * S::WQ;
-            const int o = lx * S::WZ + 4 * lq;
-            st4(&sUx[o], wux[m]);
-            st4(&sUz[o], wuz[m]);
-            st4(&sMu[o], wmu[m]);
-            if (VTI) {
-                const float4 e = wep[m], mu = wmu[m];
-                st4(&sMuX[o], make_float4((1.f + 2.f * e.x) * mu.x, (1.f + 2.f * e.y) * mu.y, (1.f + 2.f * e.z) * mu.z,
-                                          (1.f + 2.f * e.w) * mu.w));
-            }
+            const int gx = t.wsx + lx, gz = t.wsz + 4 * lq;
+            const bool ok = i < S::WX * S::WQ && gx >= 0 && gx < nx && gz >= 0 && gz < ldz;
+            const int g = ok ? gx * ldz + gz : 0;
+            const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+            wux[m] = ok ? ld4(cux + g) : z4;
+            wuz[m] = ok ? ld4(cux + nxz + g) : z4;
         }
-    }
-    __syncthreads();
+    };
+    if (s0 < s1) load_window(s0, tid, t);
 
-    // ---- extraction of the current level by the owning tile ----
-    if (a.ext_ptr) {
-        const int b = a.ext_ptr[t.shot * ntiles + t.tile], e = a.ext_ptr[t.shot * ntiles + t.tile + 1];
-        for (int k = b * 2 + tid; k < e * 2; k += kThreads) {
-            const PointEntry en = a.ext[a.ext_idx[k >> 1]];
-            const int comp = k & 1;
-            const float* su = comp ? sUz : sUx;
-            float v = 0.f;
-            for (int ix = 0; ix < 3; ix++)
-                for (int iz = 0; iz < 3; iz++)
-                    v += en.wx[ix] * en.wz[iz] * su[(en.xi0 + ix - t.wsx) * S::WZ + (en.zi0 + iz - t.wsz)];
-            a.ext_trace[en.trace + comp * a.ext_comp_stride + a.it] = v;
-        }
-    }
-
-    // ---- phase 2: products of first derivatives on [plx, phx) x [plz, phz) ----
+    float gl[IMG ? S::NQ3 : 1][4], gm[IMG ? S::NQ3 : 1][4], gr[IMG ? S::NQ3 : 1][4];
+    if constexpr (IMG) {
 #pragma unroll
-    for (int m = 0; m < S::NQ2; m++) {
-        const int i = tid + m * kThreads;
-        if (i >= S::IX * S::WQ) break;
-        const int lc = i / S::WQ, lq = i - lc * S::WQ;
-        const int gx = t.plx + lc, z0 = t.wsz + 4 * lq;
-        const int o = lc * S::WZ + 4 * lq;
-        const int w = (gx - t.wsx) * S::WZ + 4 * lq;
-        float4 P = make_float4(0.f, 0.f, 0.f, 0.f), PZ = P, QX = P, QZ = P;
-        if (gx < t.phx && z0 + 3 >= t.plz && z0 < t.phz) {
-            const bool fast = gx >= 4 && gx < nx - 4 && z0 >= 4 && z0 + 3 < nz - 4 && lq >= 1 && lq + 1 < S::WQ &&
-                              z0 >= t.plz && z0 + 3 < t.phz;
-            float lam[4], c13[4] = {0.f, 0.f, 0.f, 0.f}, eps[4] = {0.f, 0.f, 0.f, 0.f};
-            unpack(plam[m], lam);
-            if (VTI) {
-                unpack(pc13[m], c13);
-                unpack(peps[m], eps);
+        for (int m = 0; m < S::NQ3; m++)
+#pragma unroll
+            for (int k = 0; k < 4; k++) gl[m][k] = gm[m][k] = gr[m][k] = 0.f;
+    }
+
+    const TileInfo t0 = t;
+    const int tid0 = tid;
+    for (int shot = s0; shot < s1; shot++) {
+        TileInfo t = t0;
+        int tid = tid0;
+        opaque(t);
+        opaque_v(tid);
+        // previous level of this shot's new-level quads
+        float4 qprx[S::NQ3], qprz[S::NQ3];
+        const float* pux = a.prev + shot * a.shot_stride;
+#pragma unroll
+        for (int m = 0; m < S::NQ3; m++) {
+            const int i = tid + m * kThreads;
+            const int gx = t.sx + (i >> 4), gz = t.sz + 4 * (i & 15);
+            const bool ok = gx < t.ex && gz < t.ez;
+            const int g = ok ? gx * ldz + gz : 0;
+            const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+            qprx[m] = ok ? ld4(pux + g) : z4;
+            qprz[m] = ok ? ld4(pux + nxz + g) : z4;
+        }
+        __syncthreads();  // the previous shot is done with the LDS state and products
+#pragma unroll
+        for (int m = 0; m < S::NQ1; m++) {
+            const int i = tid + m * kThreads;
+            if (i < S::WX * S::WQ) {
+                const int lx = i / S::WQ, lq = i - lx * S::WQ;
+                const int o = lx * S::WZ + 4 * lq;
+                st4(&sUx[o], wux[m]);
+                st4(&sUz[o], wuz[m]);
             }
-            if (fast) {
-                product_quad<VTI, TX>(a, sm, w, lam, c13, eps, P, PZ, QX, QZ);
-            } else {
-                float p[4] = {0.f, 0.f, 0.f, 0.f}, pz[4] = {0.f, 0.f, 0.f, 0.f}, qx[4] = {0.f, 0.f, 0.f, 0.f},
-                      qz[4] = {0.f, 0.f, 0.f, 0.f};
+        }
+        if constexpr (IMG) {
+            const float* F0 = a.f0 + shot * a.fshot_stride;
+#pragma unroll
+            for (int m = 0; m < S::NQ2; m++) {
+                const int i = tid + m * kThreads;
+                if (i >= S::IX * S::PQ) break;
+                const int lc = i / S::PQ, lq = i - lc * S::PQ;
+                const int gx = t.plx + lc, z0 = t.pz0 + 4 * lq;
+                const bool ok = gx < t.phx && z0 + 3 >= t.plz && z0 < t.phz;
+                const int g = ok ? gx * ldz + z0 : 0;
+                const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+                const int o = lc * S::PZ + 4 * lq;
+                st4(&sFx[o], ok ? ld4(F0 + g) : z4);
+                st4(&sFz[o], ok ? ld4(F0 + nxz + g) : z4);
+            }
+        }
+        __syncthreads();
+        if (shot + 1 < s1) load_window(shot + 1, tid, t);
+
+        // extraction of the current level by the owning tile
+        if (a.ext_ptr) {
+            const int b = a.ext_ptr[shot * ntiles + t.tile], e = a.ext_ptr[shot * ntiles + t.tile + 1];
+            for (int k = b * 2 + tid; k < e * 2; k += kThreads) {
+                const PointEntry en = a.ext[a.ext_idx[k >> 1]];
+                const int comp = k & 1;
+                const float* su = comp ? sUz : sUx;
+                float v = 0.f;
+                for (int ix = 0; ix < 3; ix++)
+                    for (int iz = 0; iz < 3; iz++)
+                        v += en.wx[ix] * en.wz[iz] * su[(en.xi0 + ix - t.wsx) * S::WZ + (en.zi0 + iz - t.wsz)];
+                a.ext_trace[en.trace + comp * a.ext_comp_stride + a.it] = v;
+            }
+        }
+
+        // products of first derivatives
+#pragma unroll
+        for (int m = 0; m < S::NQ2; m++) {
+            const int i = tid + m * kThreads;
+            if (i >= S::IX * S::PQ) break;
+            const int lc = i / S::PQ, lq = i - lc * S::PQ;
+            const int gx = t.plx + lc, z0 = t.pz0 + 4 * lq;
+            float p[4] = {0.f, 0.f, 0.f, 0.f}, pz[4] = {0.f, 0.f, 0.f, 0.f}, qx[4] = {0.f, 0.f, 0.f, 0.f},
+                  qz[4] = {0.f, 0.f, 0.f, 0.f};
+            if (gx < t.phx && z0 + 3 >= t.plz && z0 < t.phz)
+                product_quad<VTI, TX, EDGE>(a, sm, t, gx, z0, plam[m], VTI ? plamx[m] : plam[m], VTI ? pc13[m] : plam[m],
+                                      p, pz, qx, qz);
+            const int o = lc * S::PZ + 4 * lq;
+            st4(&sPx[o], make_float4(p[0], p[1], p[2], p[3]));
+            if (VTI) st4(&sPz[o], make_float4(pz[0], pz[1], pz[2], pz[3]));
+            st4(&sQx[o], make_float4(qx[0], qx[1], qx[2], qx[3]));
+            st4(&sQz[o], make_float4(qz[0], qz[1], qz[2], qz[3]));
+        }
+        __syncthreads();
+
+        // new level
+        float* nux = a.next + shot * a.shot_stride;
+        float* nuz = nux + nxz;
+        const int ib = a.inj_ptr[shot * ntiles + t.tile], ie = a.inj_ptr[shot * ntiles + t.tile + 1];
+#pragma unroll
+        for (int m = 0; m < S::NQ3; m++) {
+            const int i = tid + m * kThreads;
+            const int gx = t.sx + (i >> 4), z0 = t.sz + 4 * (i & 15);
+            if (gx >= t.ex || z0 >= t.ez) continue;
+            float prx[4], prz[4], ir[4], tz[4];
+            unpack(qprx[m], prx);
+            unpack(qprz[m], prz);
+            unpack(qir[m], ir);
+            unpack(qtz[m], tz);
+            const float txf = qtx[m];
+            if (a.taper_prev) {
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    prx[k] = (prx[k] * tz[k]) * txf;
+                    prz[k] = (prz[k] * tz[k]) * txf;
+                }
+            }
+            float rx[4], rz[4];
+            rhs_quad<VTI, TX, EDGE>(a, sm, t, gx, z0, prx, prz, rx, rz);
+            // sources (we_op.cpp:722-724)
+            for (int k2 = ib; k2 < ie; k2++) {
+                const PointEntry& en = a.inj[a.inj_idx[k2]];
+                const int ax = gx - en.xi0;
+                if ((unsigned)ax >= 3u) continue;
+                const float fx = a.inj_trace[en.trace + a.it];
+                const float fz = a.inj_trace[en.trace + a.inj_comp_stride + a.it];
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const int az = z0 + k - en.zi0;
+                    if ((unsigned)az < 3u) {
+                        const float wgt = en.wx[ax] * en.wz[az];
+                        rx[k] += wgt * fx;
+                        rz[k] += wgt * fz;
+                    }
+                }
+            }
+            const int wq = (gx - t.wsx) * S::WZ + (z0 - t.wsz);
+            float cx[4], cz[4], vx[4], vz[4];
+            unpack(ld4(&sUx[wq]), cx);
+            unpack(ld4(&sUz[wq]), cz);
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                // leapfrog (we_op.cpp:824-830) with ir = dt^2 / rho
+                vx[k] = ir[k] * rx[k] + (2.f * cx[k] - prx[k]);
+                vz[k] = ir[k] * rz[k] + (2.f * cz[k] - prz[k]);
+            }
+            if (EDGE && (gx == 0 || gx == nx - 1 || z0 == 0 || z0 + 3 >= nz - 1)) {
+                // implicit part of the absorbing SAT on the perimeter (spatial_operators.cpp:307-521)
                 for (int k = 0; k < 4; k++) {
                     const int gz = z0 + k;
-                    if (gz < t.plz || gz >= t.phz) continue;
-                    const float4 r = product_cell<VTI, TX>(a, sm, w + k, gx, gz, lam[k], c13[k], eps[k]);
-                    p[k] = r.x;
-                    pz[k] = r.y;
-                    qx[k] = r.z;
-                    qz[k] = r.w;
+                    if (gz >= nz) continue;
+                    const bool edge = (gz == 0) | (gz == nz - 1) | (gx == 0) | (gx == nx - 1);
+                    if (!edge) continue;
+                    float dvx, dvz;
+                    if (gz == 0) { dvx = a.div_tb[gx]; dvz = a.div_tb[nx + gx]; }
+                    else if (gz == nz - 1) { dvx = a.div_tb[2 * nx + gx]; dvz = a.div_tb[3 * nx + gx]; }
+                    else if (gx == 0) { dvx = a.div_lr[gz]; dvz = a.div_lr[nz + gz]; }
+                    else { dvx = a.div_lr[2 * nz + gz]; dvz = a.div_lr[3 * nz + gz]; }
+                    vx[k] = vx[k] / dvx;
+                    vz[k] = vz[k] / dvz;
                 }
-                P = make_float4(p[0], p[1], p[2], p[3]);
-                PZ = make_float4(pz[0], pz[1], pz[2], pz[3]);
-                QX = make_float4(qx[0], qx[1], qx[2], qx[3]);
-                QZ = make_float4(qz[0], qz[1], qz[2], qz[3]);
+            }
+            // cos^2 taper of the new level (we_op.cpp:846-861); cells past nz stay zero
+            float ox[4], oz[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const bool in = z0 + k < nz;
+                ox[k] = in ? (vx[k] * tz[k]) * txf : 0.f;
+                oz[k] = in ? (vz[k] * tz[k]) * txf : 0.f;
+            }
+            const int g = gx * ldz + z0;
+            st4(nux + g, make_float4(ox[0], ox[1], ox[2], ox[3]));
+            st4(nuz + g, make_float4(oz[0], oz[1], oz[2], oz[3]));
+            if constexpr (IMG) {
+                // adjoint derivatives of the step's current level (the reference's lagged
+                // u_x, u_z), forward derivatives of snapshot idx (we_op.cpp:519-541)
+                float axx[4], azx[4], axz[4], azz[4], t0x[4], t1z[4], t2x[4], t3z[4];
+                quad_derivs<EDGE, S::WZ>(sUx, sUz, wq, gx, z0, nx, nz, a.rdx, a.rdz, axx, azx, axz, azz);
+                const int q = (gx - t.plx) * S::PZ + (z0 - t.pz0);
+                quad_derivs<EDGE, S::PZ>(sFx, sFz, q, gx, z0, nx, nz, a.rdx, a.rdz, t0x, t3z, t2x, t1z);
+                float f0x[4], f0z[4], fmx[4], fmz[4], fpx[4], fpz[4];
+                unpack(ld4(&sFx[q]), f0x);
+                unpack(ld4(&sFz[q]), f0z);
+                const long long fo = shot * a.fshot_stride + g;
+                if (a.img_mode != 1) {
+                    unpack(ld4(a.fm + fo), fmx);
+                    unpack(ld4(a.fm + fo + nxz), fmz);
+                }
+                if (a.img_mode != 2) {
+                    unpack(ld4(a.fp + fo), fpx);
+                    unpack(ld4(a.fp + fo + nxz), fpz);
+                }
+                float e[4], v2[4], v3[4];
+                if (VTI) {
+                    unpack(ld4(a.ge + g), e);
+                    unpack(ld4(a.gv2 + g), v2);
+                    unpack(ld4(a.gv3 + g), v3);
+                }
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    if (z0 + k >= nz) continue;
+                    float d2x, d2z;
+                    if (a.img_mode == 0) {
+                        d2x = fpx[k] - 2.f * f0x[k] + fmx[k];
+                        d2z = fpz[k] - 2.f * f0z[k] + fmz[k];
+                    } else if (a.img_mode == 1) {
+                        d2x = fpx[k] - f0x[k];
+                        d2z = fpz[k] - f0z[k];
+                    } else {
+                        d2x = -f0x[k] + fmx[k];
+                        d2z = -f0z[k] + fmz[k];
+                    }
+                    if (!VTI) {
+                        gl[m][k] += a.wlm * (axx[k] + azz[k]) * (t0x[k] + t1z[k]);
+                        gm[m][k] += a.wlm * ((azx[k] + axz[k]) * (t2x[k] + t3z[k]) + 2.f * axx[k] * t0x[k] +
+                                             2.f * azz[k] * t1z[k]);
+                    } else {
+                        const float cr = axx[k] * t1z[k] + azz[k] * t0x[k];
+                        gl[m][k] += a.wlm * (e[k] * axx[k] * t0x[k] + azz[k] * t1z[k] + v2[k] * cr);
+                        gm[m][k] += a.wlm * ((azx[k] + axz[k]) * (t2x[k] + t3z[k]) + 2.f * e[k] * axx[k] * t0x[k] +
+                                             2.f * azz[k] * t1z[k] + v3[k] * cr);
+                    }
+                    gr[m][k] += a.wrho * (ox[k] * d2x + oz[k] * d2z);
+                }
             }
         }
-        st4(&sPx[o], P);
-        if (VTI) st4(&sPz[o], PZ);
-        st4(&sQx[o], QX);
-        st4(&sQz[o], QZ);
     }
-    __syncthreads();
-
-    // ---- phase 3: new level ----
-    float* nux = a.next + t.shot * a.shot_stride;
-    float* nuz = nux + nxz;
-    const int ib = a.inj_ptr[t.shot * ntiles + t.tile], ie = a.inj_ptr[t.shot * ntiles + t.tile + 1];
+    if constexpr (IMG) {
+        float* G = a.grad + group * a.ggroup_stride;
 #pragma unroll
-    for (int m = 0; m < S::NQ3; m++) {
-        const int i = tid + m * kThreads;
-        const int gx = t.sx + (i >> 4), z0 = t.sz + 4 * (i & 15);
-        if (gx >= t.ex || z0 >= t.ez) continue;
-        float prx[4], prz[4], rho[4], tz[4];
-        unpack(qprx[m], prx);
-        unpack(qprz[m], prz);
-        unpack(qrho[m], rho);
-        unpack(qtz[m], tz);
-        const float txf = qtx[m];
-        if (a.taper_prev) {
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                prx[k] = (prx[k] * tz[k]) * txf;
-                prz[k] = (prz[k] * tz[k]) * txf;
-            }
+        for (int m = 0; m < S::NQ3; m++) {
+            const int i = tid + m * kThreads;
+            const int gx = t.sx + (i >> 4), z0 = t.sz + 4 * (i & 15);
+            if (gx >= t.ex || z0 >= t.ez || s0 >= s1) continue;
+            const long long g = (long long)gx * ldz + z0;
+            float4 v = ld4(G + g);
+            st4(G + g, make_float4(v.x + gl[m][0], v.y + gl[m][1], v.z + gl[m][2], v.w + gl[m][3]));
+            v = ld4(G + nxz + g);
+            st4(G + nxz + g, make_float4(v.x + gm[m][0], v.y + gm[m][1], v.z + gm[m][2], v.w + gm[m][3]));
+            v = ld4(G + 2 * nxz + g);
+            st4(G + 2 * nxz + g, make_float4(v.x + gr[m][0], v.y + gr[m][1], v.z + gr[m][2], v.w + gr[m][3]));
         }
-        float rx[4], rz[4];
-        const bool fast = gx >= 6 && gx < nx - 6 && z0 >= 6 && z0 + 3 < nz - 6;
-        if (fast) {
-            rhs_quad<VTI, TX>(a, sm, t, gx, z0, rx, rz);
-        } else {
-            for (int k = 0; k < 4; k++) {
-                rx[k] = rz[k] = 0.f;
-                if (z0 + k < t.ez) rhs_cell<VTI, TX>(a, sm, t, gx, z0 + k, prx[k], prz[k], rx[k], rz[k]);
-            }
-        }
-        // sources (we_op.cpp:722-724), added after the volume terms and before the SAT
-        // terms in the reference; the SAT lines never carry sources' footprints away
-        // from their cells, so the order only changes rounding
-        for (int k2 = ib; k2 < ie; k2++) {
-            const PointEntry& en = a.inj[a.inj_idx[k2]];
-            const int ax = gx - en.xi0;
-            if ((unsigned)ax >= 3u) continue;
-            const float sx = a.inj_trace[en.trace + a.it];
-            const float sz = a.inj_trace[en.trace + a.inj_comp_stride + a.it];
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                const int az = z0 + k - en.zi0;
-                if ((unsigned)az < 3u) {
-                    const float wgt = en.wx[ax] * en.wz[az];
-                    rx[k] += wgt * sx;
-                    rz[k] += wgt * sz;
-                }
-            }
-        }
-        float vx[4], vz[4];
-        const int wq = (gx - t.wsx) * S::WZ + (z0 - t.wsz);
-        float cx[4], cz[4];
-        unpack(ld4(&sUx[wq]), cx);
-        unpack(ld4(&sUz[wq]), cz);
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            // leapfrog (we_op.cpp:824-830)
-            vx[k] = a.dt2 * rx[k] / rho[k] + 2.f * cx[k] - prx[k];
-            vz[k] = a.dt2 * rz[k] / rho[k] + 2.f * cz[k] - prz[k];
-        }
-        if (!fast) {
-            // implicit part of the absorbing SAT on the perimeter (spatial_operators.cpp:307-521)
-            for (int k = 0; k < 4; k++) {
-                const int gz = z0 + k;
-                if (gz >= t.ez) continue;
-                const bool edge = (gz == 0) | (gz == nz - 1) | (gx == 0) | (gx == nx - 1);
-                if (!edge) continue;
-                float dvx, dvz;
-                if (gz == 0) { dvx = a.div_tb[gx]; dvz = a.div_tb[nx + gx]; }
-                else if (gz == nz - 1) { dvx = a.div_tb[2 * nx + gx]; dvz = a.div_tb[3 * nx + gx]; }
-                else if (gx == 0) { dvx = a.div_lr[gz]; dvz = a.div_lr[nz + gz]; }
-                else { dvx = a.div_lr[2 * nz + gz]; dvz = a.div_lr[3 * nz + gz]; }
-                vx[k] = vx[k] / dvx;
-                vz[k] = vz[k] / dvz;
-            }
-        }
-        // cos^2 taper of the new level (we_op.cpp:846-861); cells past nz stay zero
-        float ox[4], oz[4];
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const bool in = z0 + k < nz;
-            ox[k] = in ? (vx[k] * tz[k]) * txf : 0.f;
-            oz[k] = in ? (vz[k] * tz[k]) * txf : 0.f;
-        }
-        const int g = gx * ldz + z0;
-        st4(nux + g, make_float4(ox[0], ox[1], ox[2], ox[3]));
-        st4(nuz + g, make_float4(oz[0], oz[1], oz[2], oz[3]));
     }
 }
 
@@ -921,18 +1079,52 @@ static inline int grid_for(long long n, int cap = 4096)
     return (int)(b < 1 ? 1 : (b > cap ? cap : b));
 }
 
-template <bool VTI>
-static void step_dispatch(const StepArgs& a, int tile_x, hipStream_t s)
+// True when every product quad and every new-level quad of the tile takes the interior
+// paths of k_step (the conditions below are those of product_quad and rhs_quad).
+bool interior_tile(int tix, int ntx, int tile_x, int tiz, int ntz, int nx, int nz, int ldz)
 {
-    const int nb = a.ntx * a.ntz * a.nshots;
-    if (tile_x == 32) hipLaunchKernelGGL((k_step<VTI, 32>), dim3(nb), dim3(kThreads), 0, s, a);
-    else hipLaunchKernelGGL((k_step<VTI, 16>), dim3(nb), dim3(kThreads), 0, s, a);
+    int sx, ex, sz, ez, plx, phx, plz, phz;
+    tile_range_x_hd(tix, ntx, tile_x, nx, sx, ex);
+    tile_range_z_hd(tiz, ntz, nz, ldz, sz, ez);
+    product_range(sx, ex, nx, plx, phx);
+    product_range(sz, ez, nz, plz, phz);
+    if (plx < 4 || phx > nx - 4 || sx < 6 || ex > nx - 6) return false;
+    const int pz0 = sz - 4;
+    for (int lq = 0; lq < (kTileZ + 8) / 4; lq++) {
+        const int z0 = pz0 + 4 * lq;
+        if (z0 + 3 >= plz && z0 < phz && !(z0 >= 4 && z0 + 3 < nz - 4)) return false;
+    }
+    for (int z0 = sz; z0 < ez; z0 += 4)
+        if (!(z0 >= 6 && z0 + 3 < nz - 6)) return false;
+    return true;
 }
 
-void launch_step(const StepArgs& a, bool vti, int tile_x, hipStream_t s)
+template <bool VTI, bool EDGE, bool IMG>
+static void step_dispatch(const StepArgs& a, int tile_x, const int* tiles, int ntile, hipStream_t s)
 {
-    if (vti) step_dispatch<true>(a, tile_x, s);
-    else step_dispatch<false>(a, tile_x, s);
+    const int nb = ntile * a.ngroups;
+    if (nb <= 0) return;
+    if (tile_x == 32) hipLaunchKernelGGL((k_step<VTI, 32, EDGE, IMG>), dim3(nb), dim3(kThreads), 0, s, a, tiles);
+    else hipLaunchKernelGGL((k_step<VTI, 16, EDGE, IMG>), dim3(nb), dim3(kThreads), 0, s, a, tiles);
+}
+
+template <bool VTI, bool EDGE>
+static void step_dispatch_img(const StepArgs& a, int tile_x, const int* tiles, int ntile, bool image, hipStream_t s)
+{
+    if (image) step_dispatch<VTI, EDGE, true>(a, tile_x, tiles, ntile, s);
+    else step_dispatch<VTI, EDGE, false>(a, tile_x, tiles, ntile, s);
+}
+
+void launch_step(const StepArgs& a, bool vti, int tile_x, const int* tiles, int ntile, bool edge, bool image,
+                 hipStream_t s)
+{
+    if (vti) {
+        if (edge) step_dispatch_img<true, true>(a, tile_x, tiles, ntile, image, s);
+        else step_dispatch_img<true, false>(a, tile_x, tiles, ntile, image, s);
+    } else {
+        if (edge) step_dispatch_img<false, true>(a, tile_x, tiles, ntile, image, s);
+        else step_dispatch_img<false, false>(a, tile_x, tiles, ntile, image, s);
+    }
 }
 
 template <bool VTI>

@@ -85,9 +85,13 @@ int fwi2d_engine_adjoint(fwi2d_engine* e, const float* adj, float* grad, int on_
 int fwi2d_engine_dot_f64(fwi2d_engine* e, const float* a, const float* b, const float* w, long long n,
                          double* out);
 
-/* Tuning and test options: "tile_x" (8|16|32, 0 = autotune), "segment" (snapshots per
-   checkpoint segment, 0 = auto), "lag" (1 reference imaging lag, 0 none), "graph"
-   (1 capture the step sequence in a hipGraph), "batch" (shots per launch, 0 = all). */
+/* Tuning and test options (no reference counterpart): "tile_x" (16 | 32 cells per tile
+   along x), "shots_per_block" (1..64 shots advanced in turn by one step workgroup),
+   "segment" (snapshots per checkpoint segment, 0 = about sqrt(2 nt/sub)), "lag" (1 the
+   reference's one-step imaging lag, 0 none), "timing" (1 time the forward step loop with
+   HIP events), "graph" (reserved).  Statistics: "steps", "recompute_steps",
+   "image_calls", "segment", "nseg", "tile_x", "shots_per_block", "ntiles", "interior_tiles", "nrec",
+   "nshots", "fwd_loop_ns", "fwd_loop_steps"; -1 for an unknown key. */
 int fwi2d_engine_set_option(fwi2d_engine* e, const char* key, long long value);
 long long fwi2d_engine_get_stat(fwi2d_engine* e, const char* key);
 

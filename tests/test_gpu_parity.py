@@ -72,9 +72,11 @@ def oracle_forward(orc, prob, model, sxz, rxz, src, nt, dt, sub=0, full=None):
     return np.concatenate(out, axis=1)
 
 
-def run_engine(lib, prob, model, sxz, rxz, src, nt, dt, sub=0, tile_x=16, keep=False):
+def run_engine(lib, prob, model, sxz, rxz, src, nt, dt, sub=0, tile_x=16, keep=False, spb=None):
     eng = lib.Engine(lib.grid_from(prob, nt, dt, sub))
     eng.set_option("tile_x", tile_x)
+    if spb is not None:
+        eng.set_option("shots_per_block", spb)
     eng.set_model(model)
     eng.set_shots(sxz, rxz)
     return eng, eng.forward(src, keep_checkpoints=keep)
@@ -112,6 +114,10 @@ def test_tile_width_and_batch_are_bit_exact(lib):
     _, a = run_engine(lib, *case, tile_x=16)
     _, c = run_engine(lib, *case, tile_x=32)
     assert np.array_equal(a, c)
+    # shots advanced one after the other in a workgroup, or one per workgroup
+    for spb in (1, 2, 3):
+        _, c = run_engine(lib, *case, spb=spb)
+        assert np.array_equal(a, c), spb
     # each shot alone gives the same traces as the batch
     r0 = 0
     for s in range(3):
@@ -160,6 +166,12 @@ def test_raw_gradient_matches_oracle(lib, orc64, vti):
         eng.set_option("segment", K)
         eng.forward(src, keep_checkpoints=True)
         assert np.array_equal(eng.adjoint(adj), gk), K
+    # one shot per workgroup sums the shots in another order: equal to rounding
+    eng.set_option("shots_per_block", 1)
+    eng.forward(src, keep_checkpoints=True)
+    g1 = eng.adjoint(adj)
+    for c in range(3):
+        assert rel(g1[c], gk[c].astype(np.float64)) < 1e-6
 
 
 def test_operator_level_forward_and_gradient(lib, orc64):
