@@ -70,10 +70,33 @@ struct Engine {
     int ldz = 0;          // device z pitch: nz rounded up to a multiple of 4
     long long nxz = 0;    // nx * ldz, the device size of one field
     int tile_x = 16, ntx = 0, ntz = 0;
-    int n_tiles_in = 0, n_tiles_edge = 0;
-    DevMem d_tiles_in, d_tiles_edge;
+    std::vector<int4> boxes;                        // {sx, ex, sz, ez} of every step box
+    std::vector<int> list_core, list_bz, list_bx;   // box indices per box class
+    DevMem d_boxes, d_list_core, d_list_bz, d_list_bx;
     int opt_tile_x = 16, opt_segment = 0, opt_lag = 1, opt_graph = 0, opt_timing = 0;
-    int opt_spb = 4;      // shots advanced one after the other by each step workgroup
+    // hipGraph replay of whole time loops: a loop is captured once per configuration
+    // (every pointer and size it uses) and replayed while that configuration holds
+    struct LoopGraph {
+        std::vector<long long> key;
+        hipGraphExec_t exec = nullptr;
+        int seg_loaded_after = -1;
+        long long steps = 0, recompute = 0, images = 0;
+        void reset()
+        {
+            if (exec) (void)hipGraphExecDestroy(exec);
+            exec = nullptr;
+            key.clear();
+        }
+    };
+    LoopGraph graph_fwd, graph_adj;
+    long long graph_captures = 0, graph_replays = 0;
+    int opt_autotune = 0;  // pick tile_x and shots per workgroup by timing steps at the next forward run
+    bool tuned = false;
+    std::string tune_report;
+    int opt_spb = 4;       // shots advanced one after the other by each interior-tile workgroup
+    int opt_spb_edge = 1;  // the same for edge tiles (their workgroups are latency-bound)
+    hipStream_t st_edge = nullptr;       // edge-tile launches run beside the interior ones
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     double last_fwd_loop_ms = 0.0;  // HIP-event time of the last forward step loop (timing option)
     int last_fwd_loop_steps = 0;
     bool vti = false;
@@ -114,6 +137,9 @@ struct Engine {
         if (nxz >= (1LL << 31)) throw std::runtime_error("grid too large: nx * nz must stay below 2^31 cells");
         HIPCHECK(hipSetDevice(device));
         HIPCHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        HIPCHECK(hipStreamCreateWithFlags(&st_edge, hipStreamNonBlocking));
+        HIPCHECK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+        HIPCHECK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
         upload_constants(st);
         set_tile(opt_tile_x);
         build_tapers();
@@ -124,6 +150,14 @@ struct Engine {
             (void)hipStreamSynchronize(st);
             (void)hipStreamDestroy(st);
         }
+        if (st_edge) {
+            (void)hipStreamSynchronize(st_edge);
+            (void)hipStreamDestroy(st_edge);
+        }
+        graph_fwd.reset();
+        graph_adj.reset();
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_join) (void)hipEventDestroy(ev_join);
     }
 
     void set_tile(int tx)
@@ -132,24 +166,60 @@ struct Engine {
         tile_x = tx;
         ntx = tiles_x(g.nx, tile_x);
         ntz = tiles_z(ldz);
-        // interior tiles run the step kernel without closure code; tiles of each class
-        // are listed x-major so that neighbouring tiles land on one XCD
-        std::vector<int> in, edge;
-        for (int tix = 0; tix < ntx; tix++)
-            for (int tiz = 0; tiz < ntz; tiz++)
-                (interior_tile(tix, ntx, tile_x, tiz, ntz, g.nx, g.nz, ldz) ? in : edge).push_back(tix * ntz + tiz);
-        n_tiles_in = (int)in.size();
-        n_tiles_edge = (int)edge.size();
-        d_tiles_in.reserve(std::max<size_t>(in.size(), 1) * sizeof(int));
-        d_tiles_edge.reserve(std::max<size_t>(edge.size(), 1) * sizeof(int));
-        if (!in.empty())
-            HIPCHECK(hipMemcpyAsync(d_tiles_in.p, in.data(), in.size() * sizeof(int), hipMemcpyHostToDevice, st));
-        if (!edge.empty())
-            HIPCHECK(hipMemcpyAsync(d_tiles_edge.p, edge.data(), edge.size() * sizeof(int), hipMemcpyHostToDevice, st));
-        HIPCHECK(hipStreamSynchronize(st));
+        build_boxes();
         if (ns > 0) build_tile_lists();
     }
-    int ntiles() const { return ntx * ntz; }
+
+    // Step decomposition: the core [6, nx-6) x [8, core_z_end) where every stencil is
+    // interior, cut into tile_x x 64 boxes (the tile grid clipped to the core), and the
+    // edge bands around it: 64 x 16 boxes along the top and bottom (corners included),
+    // 8 x 128 boxes along the left and right sides.  Boxes partition the grid.
+    void build_boxes()
+    {
+        boxes.clear();
+        list_core.clear();
+        list_bz.clear();
+        list_bx.clear();
+        const int nx = g.nx, nz = g.nz;
+        const int cx0 = kCoreX0, cx1 = nx - kCoreX0;
+        const int cz0 = kCoreZ0, cz1 = std::max(core_z_end(nz), kCoreZ0);
+        auto add = [&](std::vector<int>& list, int sx, int ex, int sz, int ez) {
+            if (ex <= sx || ez <= sz) return;
+            list.push_back((int)boxes.size());
+            boxes.push_back(make_int4(sx, ex, sz, ez));
+        };
+        for (int tix = 0; tix < ntx; tix++) {
+            int sx, ex;
+            tile_range_x(tix, ntx, tile_x, nx, sx, ex);
+            for (int tiz = 0; tiz < ntz; tiz++) {
+                int sz, ez;
+                tile_range_z(tiz, ntz, nz, ldz, sz, ez);
+                add(list_core, std::max(sx, cx0), std::min(ex, cx1), std::max(sz, cz0), std::min(ez, cz1));
+            }
+        }
+        const int nbx = (nx + 63) / 64;
+        for (int k = 0; k < nbx; k++) {
+            int sx, ex;
+            tile_range_x(k, nbx, 64, nx, sx, ex);
+            add(list_bz, sx, ex, 0, cz0);
+            add(list_bz, sx, ex, cz1, nz);
+        }
+        for (int sz = cz0; sz < cz1; sz += 128) {
+            const int ez = std::min(sz + 128, cz1);
+            add(list_bx, 0, cx0, sz, ez);
+            add(list_bx, cx1, nx, sz, ez);
+        }
+        auto up = [&](DevMem& d, const void* h, size_t b) {
+            d.reserve(std::max<size_t>(b, 16));
+            if (b) HIPCHECK(hipMemcpyAsync(d.p, h, b, hipMemcpyHostToDevice, st));
+        };
+        up(d_boxes, boxes.data(), boxes.size() * sizeof(int4));
+        up(d_list_core, list_core.data(), list_core.size() * sizeof(int));
+        up(d_list_bz, list_bz.data(), list_bz.size() * sizeof(int));
+        up(d_list_bx, list_bx.data(), list_bx.size() * sizeof(int));
+        HIPCHECK(hipStreamSynchronize(st));
+    }
+    int ntiles() const { return (int)boxes.size(); }
 
     // cos^2 taper factors per axis (taperz/taperx, spatial_operators.cpp:523-549)
     void build_tapers()
@@ -404,11 +474,12 @@ struct Engine {
         up(d_rcv_shot_ptr, rcv_shot_ptr.data(), rcv_shot_ptr.size() * sizeof(int));
         build_tile_lists();
         ckpt_epoch = -1;
+        tuned = false;
         HIPCHECK(hipStreamSynchronize(st));
     }
 
-    // per-(shot, tile) lists: injectors whose 3x3 footprint meets the tile's write range,
-    // and extractors whose pivot lies in it (the owner reads the footprint from LDS)
+    // per-(shot, box) lists: injectors whose 3x3 footprint meets the box, and extractors
+    // whose pivot lies in it (the owner reads the footprint from LDS)
     void build_tile_lists()
     {
         const int nt_ = ntiles();
@@ -416,17 +487,12 @@ struct Engine {
             std::vector<std::vector<int>> per((size_t)ns * nt_);
             for (int k = 0; k < (int)ents.size(); k++) {
                 const PointEntry& e = ents[k];
-                for (int tx = 0; tx < ntx; tx++) {
-                    int sx, ex;
-                    tile_range_x(tx, ntx, tile_x, g.nx, sx, ex);
-                    for (int tz = 0; tz < ntz; tz++) {
-                        int sz, ez;
-                        tile_range_z(tz, ntz, g.nz, ldz, sz, ez);
-                        bool hit;
-                        if (owner) hit = (e.xi0 + 1 >= sx && e.xi0 + 1 < ex && e.zi0 + 1 >= sz && e.zi0 + 1 < ez);
-                        else hit = (e.xi0 + 2 >= sx && e.xi0 < ex && e.zi0 + 2 >= sz && e.zi0 < ez);
-                        if (hit) per[(size_t)e.shot * nt_ + tx * ntz + tz].push_back(k);
-                    }
+                for (int b = 0; b < nt_; b++) {
+                    const int sx = boxes[b].x, ex = boxes[b].y, sz = boxes[b].z, ez = boxes[b].w;
+                    bool hit;
+                    if (owner) hit = (e.xi0 + 1 >= sx && e.xi0 + 1 < ex && e.zi0 + 1 >= sz && e.zi0 + 1 < ez);
+                    else hit = (e.xi0 + 2 >= sx && e.xi0 < ex && e.zi0 + 2 >= sz && e.zi0 < ez);
+                    if (hit) per[(size_t)e.shot * nt_ + b].push_back(k);
                 }
             }
             L.ptr.assign(per.size() + 1, 0);
@@ -480,6 +546,7 @@ struct Engine {
         a.irho = dv;
         a.mux = vti ? dv + nxz : nullptr;
         a.lamx = vti ? dv + 2 * nxz : nullptr;
+        a.nbox = (int)boxes.size();
         a.spb = std::max(1, std::min(opt_spb, ns));
         a.ngroups = (ns + a.spb - 1) / a.spb;
         a.div_tb = d_div_tb.as<float>();
@@ -558,8 +625,20 @@ struct Engine {
             a.ext_trace = rcv;
             a.ext_comp_stride = (long long)nr * g.nt;
         }
-        launch_step(a, vti, tile_x, d_tiles_in.as<int>(), n_tiles_in, false, img_idx >= 0, st);
-        launch_step(a, vti, tile_x, d_tiles_edge.as<int>(), n_tiles_edge, true, img_idx >= 0, st);
+        // edge bands on their own stream, concurrent with the core boxes (disjoint cells)
+        StepArgs e = a;
+        e.spb = std::max(1, std::min(opt_spb_edge, ns));
+        e.ngroups = (ns + e.spb - 1) / e.spb;
+        const bool img = img_idx >= 0;
+        const int4* bx = d_boxes.as<int4>();
+        HIPCHECK(hipEventRecord(ev_fork, st));
+        HIPCHECK(hipStreamWaitEvent(st_edge, ev_fork, 0));
+        launch_step(e, vti, BoxClass::band_z, bx, d_list_bz.as<int>(), (int)list_bz.size(), img, st_edge);
+        launch_step(e, vti, BoxClass::band_x, bx, d_list_bx.as<int>(), (int)list_bx.size(), img, st_edge);
+        HIPCHECK(hipEventRecord(ev_join, st_edge));
+        launch_step(a, vti, tile_x == 32 ? BoxClass::core32 : BoxClass::core16, bx, d_list_core.as<int>(),
+                    (int)list_core.size(), img, st);
+        HIPCHECK(hipStreamWaitEvent(st, ev_join, 0));
         L.rotate();
         stat_steps++;
     }
@@ -596,6 +675,120 @@ struct Engine {
         if (model_epoch == 0) throw std::runtime_error("no model: call fwi2d_engine_set_model first");
     }
 
+    // Every value a captured loop depends on besides the contents of device memory.
+    std::vector<long long> loop_key(int kind, const void* p0, const void* p1, int flag) const
+    {
+        auto P = [](const void* p) { return (long long)(uintptr_t)p; };
+        return {kind, P(p0), P(p1), flag, g.nt, ns, nr, tile_x, opt_spb, opt_spb_edge, opt_lag, seg_K, nseg,
+                P(d_fwd.p), P(d_adj.p), P(d_ckpt.p), P(d_seg.p), P(d_gshot.p), P(d_model.p), P(d_deriv.p),
+                P(d_boxes.p), P(d_list_core.p), P(d_list_bz.p), P(d_list_bx.p), P(fwd_inj.d_ptr.p), P(fwd_inj.d_idx.p),
+                P(fwd_ext.d_ptr.p), P(fwd_ext.d_idx.p), P(adj_inj.d_ptr.p), P(adj_inj.d_idx.p), P(d_src_ents.p),
+                P(d_rcv_ents.p), P(d_rcv_ext_ents.p), P(d_src_shot_ptr.p), P(d_rcv_shot_ptr.p), P(d_tx.p), P(d_tz.p),
+                P(d_div_tb.p), P(d_div_lr.p), (long long)list_core.size(), (long long)list_bz.size(),
+                (long long)list_bx.size(), (long long)boxes.size()};
+    }
+
+    // Runs body (which only enqueues device work on st / st_edge) directly, or, with the
+    // graph option, replays its capture when the key matches and captures it otherwise.
+    template <class F>
+    void run_loop(LoopGraph& G, const std::vector<long long>& key, F body)
+    {
+        if (!opt_graph) {
+            body();
+            return;
+        }
+        if (G.exec && G.key == key) {
+            HIPCHECK(hipGraphLaunch(G.exec, st));
+            seg_loaded = G.seg_loaded_after;
+            stat_steps += G.steps;
+            stat_recompute_steps += G.recompute;
+            stat_image_calls += G.images;
+            graph_replays++;
+            return;
+        }
+        G.reset();
+        const long long s0 = stat_steps, r0 = stat_recompute_steps, i0 = stat_image_calls;
+        hipGraph_t graph = nullptr;
+        HIPCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        try {
+            body();
+        } catch (...) {
+            (void)hipStreamEndCapture(st, &graph);
+            if (graph) (void)hipGraphDestroy(graph);
+            throw;
+        }
+        HIPCHECK(hipStreamEndCapture(st, &graph));
+        hipGraphExec_t exec = nullptr;
+        const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        HIPCHECK(ie);
+        G.exec = exec;
+        G.key = key;
+        G.seg_loaded_after = seg_loaded;
+        G.steps = stat_steps - s0;
+        G.recompute = stat_recompute_steps - r0;
+        G.images = stat_image_calls - i0;
+        graph_captures++;
+        HIPCHECK(hipGraphLaunch(G.exec, st));
+    }
+
+    // Step-shape autotuner: times a few forward steps of the current shots for every
+    // (tile_x, shots per core workgroup, shots per edge workgroup) and keeps the fastest.
+    // Forward results do not depend on the choice (bit for bit); gradients only through
+    // the order in which a workgroup sums its shots.
+    void autotune(const float* dsrc)
+    {
+        const int warm = 2, timed = 4;
+        const int nsteps = std::min(warm + timed, g.nt - 1);
+        if (nsteps < 2) return;
+        hipEvent_t e0, e1;
+        HIPCHECK(hipEventCreate(&e0));
+        HIPCHECK(hipEventCreate(&e1));
+        double best = 1e300;
+        int btx = opt_tile_x, bspb = opt_spb, bspe = opt_spb_edge;
+        tune_report.clear();
+        const long long steps0 = stat_steps;
+        for (int tx : {16, 32})
+            for (int spb : {1, 2, 4, 8}) {
+                if (spb > 1 && spb > ns) continue;
+                for (int spe : {1, 2}) {
+                    if (spe > 1 && spe > ns) continue;
+                    set_tile(tx);
+                    opt_spb = spb;
+                    opt_spb_edge = spe;
+                    Levels L = levels_in(d_fwd);
+                    init_levels(L, false, dsrc);
+                    for (int it = 0; it < nsteps; it++) {
+                        if (it == nsteps - timed) HIPCHECK(hipEventRecord(e0, st));
+                        step(L, it, false, dsrc, nullptr);
+                    }
+                    HIPCHECK(hipEventRecord(e1, st));
+                    HIPCHECK(hipEventSynchronize(e1));
+                    float ms = 0.f;
+                    HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+                    const double us = 1e3 * ms / std::min(timed, nsteps);
+                    char buf[96];
+                    snprintf(buf, sizeof(buf), "%stile_x=%d spb=%d spb_edge=%d: %.1f us", tune_report.empty() ? "" : "; ",
+                             tx, spb, spe, us);
+                    tune_report += buf;
+                    if (us < best) {
+                        best = us;
+                        btx = tx;
+                        bspb = spb;
+                        bspe = spe;
+                    }
+                }
+            }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        set_tile(btx);
+        opt_tile_x = btx;
+        opt_spb = bspb;
+        opt_spb_edge = bspe;
+        stat_steps = steps0;
+        tuned = true;
+    }
+
     void forward(const float* src, float* rcv, bool keep, bool on_device)
     {
         require_ready();
@@ -603,6 +796,7 @@ struct Engine {
         ckpt_epoch = -1;  // any forward run replaces the recomputation sources
         const long long ntr_s = (long long)2 * ns * g.nt, ntr_r = (long long)2 * nr * g.nt;
         const float* dsrc = on_device ? src : upload_traces(d_src, src, ntr_s);
+        if (opt_autotune && !tuned) autotune(dsrc);
         float* drcv = rcv;
         if (!on_device) {
             d_rcv.reserve((size_t)std::max<long long>(ntr_r, 1) * sizeof(float));
@@ -625,10 +819,15 @@ struct Engine {
             HIPCHECK(hipEventCreate(&ev1));
             HIPCHECK(hipEventRecord(ev0, st));
         }
-        for (int it = 0; it < nt - 1; it++) {
-            if (keep) capture(L, it, jlast);
-            step(L, it, false, dsrc, nr > 0 ? drcv : nullptr);
-        }
+        const Levels L0 = L;
+        run_loop(graph_fwd, loop_key(0, dsrc, drcv, keep ? jlast : -1), [&] {
+            Levels Lc = L0;
+            for (int it = 0; it < nt - 1; it++) {
+                if (keep) capture(Lc, it, jlast);
+                step(Lc, it, false, dsrc, nr > 0 ? drcv : nullptr);
+            }
+        });
+        for (int it = 0; it < nt - 1; it++) L.rotate();  // the loop's final level rotation
         if (opt_timing) {
             HIPCHECK(hipEventRecord(ev1, st));
             HIPCHECK(hipEventSynchronize(ev1));
@@ -724,7 +923,7 @@ struct Engine {
         a.fp = (a.img_mode != 2) ? seg_slot(j, idx + 1) : nullptr;
         a.fshot_stride = 2 * nxz;
         a.grad = d_gshot.as<float>();
-        a.ggroup_stride = (long long)np * nxz;
+        a.gslab_stride = (long long)np * nxz;
         const float* dv = d_deriv.as<float>();
         a.ge = vti ? dv + 3 * nxz : nullptr;
         a.gv2 = vti ? dv + 4 * nxz : nullptr;
@@ -775,12 +974,11 @@ struct Engine {
         const long long ntr_r = (long long)2 * nr * g.nt;
         const float* dadj = on_device ? adj : upload_traces(d_adjsrc, adj, std::max<long long>(ntr_r, 1));
         const float* dsrc = fwd_src;
-        // with the reference's lag the imaging is fused into the adjoint steps and the
-        // gradients are kept per shot group; without it (a test option) the separate
-        // imaging kernel keeps them per shot
+        // with the reference's lag the imaging is fused into the adjoint steps (each
+        // workgroup adds its shots' sum to the slab of its first shot); without it (a test
+        // option) the separate imaging kernel fills one slab per shot
         const bool fused = opt_lag == 1;
-        const int spb = std::max(1, std::min(opt_spb, ns));
-        const int nslab = fused ? (ns + spb - 1) / spb : ns;
+        const int nslab = ns;
         d_gshot.reserve((size_t)nslab * np * nxz * sizeof(float));
         launch_fill(d_gshot.as<float>(), (long long)nslab * np * nxz, 0.f, st);
         Levels A = levels_in(d_adj);
@@ -790,15 +988,19 @@ struct Engine {
             // step it takes the adjoint from level it to it+1; the pair (it, it+1) images
             // snapshot (nt-2-it)/sub when that divides (the reference images at its
             // iteration it+1, after the step, with the derivatives of level it)
-            for (int it = 0; it < nt - 1; it++) {
-                int idx = -1;
-                if ((nt - 2 - it) % sub == 0) {
-                    idx = (nt - 2 - it) / sub;
-                    load_segment(idx / seg_K, dsrc);
-                    stat_image_calls++;
+            const int seg0 = seg_loaded;
+            run_loop(graph_adj, loop_key(1, dadj, dsrc, seg0), [&] {
+                Levels Ac = A;
+                for (int it = 0; it < nt - 1; it++) {
+                    int idx = -1;
+                    if ((nt - 2 - it) % sub == 0) {
+                        idx = (nt - 2 - it) / sub;
+                        load_segment(idx / seg_K, dsrc);
+                        stat_image_calls++;
+                    }
+                    step(Ac, it, true, dadj, nullptr, idx);
                 }
-                step(A, it, true, dadj, nullptr, idx);
-            }
+            });
         } else {
             for (int it = 0; it < nt - 1; it++) {
                 if (it != 0 && (nt - 1 - it) % sub == 0) image(A, (nt - 1 - it) / sub, dsrc);
@@ -932,10 +1134,16 @@ int fwi2d_engine_set_option(fwi2d_engine* e, const char* key, long long value)
         } else if (k == "timing") {
             x->opt_timing = value ? 1 : 0;
         } else if (k == "graph") {
-            x->opt_graph = value ? 1 : 0;
+            x->opt_graph = value ? 1 : 0;  // captured loops stay cached, keyed on their configuration
         } else if (k == "shots_per_block") {
             if (value < 1 || value > 64) throw std::runtime_error("shots_per_block must be in [1, 64]");
             x->opt_spb = (int)value;
+        } else if (k == "autotune") {
+            x->opt_autotune = value ? 1 : 0;
+            x->tuned = false;
+        } else if (k == "shots_per_block_edge") {
+            if (value < 1 || value > 64) throw std::runtime_error("shots_per_block_edge must be in [1, 64]");
+            x->opt_spb_edge = (int)value;
         } else {
             throw std::runtime_error("unknown option " + k);
         }
@@ -954,13 +1162,22 @@ long long fwi2d_engine_get_stat(fwi2d_engine* e, const char* key)
     if (k == "nseg") return x->nseg;
     if (k == "tile_x") return x->tile_x;
     if (k == "shots_per_block") return x->opt_spb;
+    if (k == "shots_per_block_edge") return x->opt_spb_edge;
     if (k == "ntiles") return x->ntiles();
-    if (k == "interior_tiles") return x->n_tiles_in;
+    if (k == "interior_tiles") return (long long)x->list_core.size();
+    if (k == "graph_captures") return x->graph_captures;
+    if (k == "graph_replays") return x->graph_replays;
     if (k == "nrec") return x->nr;
     if (k == "nshots") return x->ns;
     if (k == "fwd_loop_ns") return (long long)(x->last_fwd_loop_ms * 1e6);
     if (k == "fwd_loop_steps") return x->last_fwd_loop_steps;
     return -1;
+}
+
+const char* fwi2d_engine_tune_report(fwi2d_engine* e)
+{
+    if (!e) return "";
+    return e->impl->tune_report.c_str();
 }
 
 void fwi2d_engine_destroy(fwi2d_engine* e)

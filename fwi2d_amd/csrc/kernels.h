@@ -24,6 +24,7 @@ struct PointEntry {
 
 struct StepArgs {
     int nx, nz, ldz, ntx, ntz, nshots, nt, it, taper_prev;
+    int nbox;              // boxes of the step decomposition (CSR lists are per shot and box)
     int spb, ngroups;      // shots per workgroup, shot groups (ceil(nshots / spb))
     float rdx, rdz, rdx2, rdz2, dt, dt2, sat_x, sat_z;
     int bc_top, bc_bottom, bc_left, bc_right;
@@ -56,8 +57,8 @@ struct StepArgs {
     float wlm, wrho;           // lambda/mu and rho weights of this snapshot
     const float *f0, *fm, *fp; // forward snapshots idx, idx-1, idx+1 (shot 0 base)
     long long fshot_stride;
-    float* grad;               // raw gradients per shot group [group][np][nx][ldz]
-    long long ggroup_stride;
+    float* grad;               // raw gradient slabs [shot][np][nx][ldz]; a workgroup adds its
+    long long gslab_stride;    // shots' sum to the slab of its first shot
     const float *ge, *gv2, *gv3;  // VTI: 1 + 2 eps, d c13 / d lambda, d c13 / d mu
 };
 
@@ -85,11 +86,18 @@ int tiles_z(int ldz);
 void tile_range_x(int k, int ntile, int T, int n, int& s, int& e);
 void tile_range_z(int k, int ntile, int nz, int ldz, int& s, int& e);
 
-// tiles: ntile tile indices (tix * ntz + tiz); edge selects the instantiation with the
-// closure and SAT paths (interior_tile() false) or without them
-void launch_step(const StepArgs& a, bool vti, int tile_x, const int* tiles, int ntile, bool edge, bool image,
+// Box shapes of the step decomposition: core boxes (TX x 64, every stencil interior) and
+// edge-band boxes (64 x 16 along the top and bottom, 8 x 128 along the sides).
+enum class BoxClass { core16, core32, band_z, band_x };
+constexpr int kCoreX0 = 6;   // core: x in [6, nx - 6), z in [8, core_z_end(nz))
+constexpr int kCoreZ0 = 8;
+inline int core_z_end(int nz) { return ((nz - 8) / 4) * 4; }
+inline int box_width(BoxClass c) { return c == BoxClass::core16 ? 16 : c == BoxClass::core32 ? 32 : c == BoxClass::band_z ? 64 : 8; }
+inline int box_height(BoxClass c) { return c == BoxClass::band_z ? 16 : c == BoxClass::band_x ? 128 : 64; }
+
+// n boxes listed in list (indices into boxes: {sx, ex, sz, ez}) of one class
+void launch_step(const StepArgs& a, bool vti, BoxClass c, const int4* boxes, const int* list, int n, bool image,
                  hipStream_t s);
-bool interior_tile(int tix, int ntx, int tile_x, int tiz, int ntz, int nx, int nz, int ldz);
 void launch_init_prev(float* prev, long long shot_stride, long long nxz, int ldz, int nshots, const int* shot_ptr,
                       const PointEntry* ents, const float* trace, long long comp_stride, hipStream_t s);
 void launch_scale_prev(float* prev, const float* rho, float half_dt2, long long nxz, int nshots, hipStream_t s);

@@ -116,7 +116,7 @@ __device__ __forceinline__ float d1_int(const float* p, int s)
     return cD1I[0] * (p[s] - p[-s]) + cD1I[1] * (p[2 * s] - p[-2 * s]);
 }
 
-__device__ __noinline__ float d1_closure(const float* p, int s, int g, int n)
+__device__ __forceinline__ float d1_closure(const float* p, int s, int g, int n)
 {
     float v = 0.f;
     if (g < 4) {
@@ -154,13 +154,19 @@ __device__ __forceinline__ W5 d2_weights(const float* pc, int s)
     return w;
 }
 
+// D2 weights of the 4 cells of a quad along x and z (coefficient mu); they depend on the
+// model only, so a workgroup forms them once and reuses them for all its shots
+struct QuadW {
+    W5 x[4], z[4];
+};
+
 __device__ __forceinline__ float d2_apply(const W5& w, const float* pu, int s)
 {
     return w.c * pu[0] + w.m1 * pu[-s] + w.p1 * pu[s] + w.m2 * pu[-2 * s] + w.p2 * pu[2 * s];
 }
 
 // closure rows of D2(c) u (the 6x8x8 table)
-__device__ __noinline__ float d2_closure(const float* pu, const float* pc, int s, int g, int n)
+__device__ __forceinline__ float d2_closure(const float* pu, const float* pc, int s, int g, int n)
 {
     float out = 0.f;
     const bool top = g < 6;
@@ -204,23 +210,34 @@ __device__ __forceinline__ float d1_row0(const float* p, int s)
 // row take it.  The next shot's window is loaded into registers before the current
 // shot's arithmetic, which hides its latency.
 // ---------------------------------------------------------------------------
-template <int TX>
+template <int TX, int TZ_>
 struct StepShape {
-    static constexpr int TZ = kTileZ;
+    static constexpr int TZ = TZ_;
     static constexpr int HX = 4, HZ = 8;
     static constexpr int WX = TX + 2 * HX, WZ = TZ + 2 * HZ;  // state window (cells)
     static constexpr int WQ = WZ / 4;                          // window quads per column
     static constexpr int IX = TX + 4;                          // product columns
     static constexpr int PZ = TZ + 8;                          // product rows [sz-4, sz+TZ+4)
     static constexpr int PQ = PZ / 4;
+    static constexpr int QC = TZ / 4;                          // new-level quads per column
     static constexpr int NQ1 = (WX * WQ + kThreads - 1) / kThreads;  // window quads per thread
     static constexpr int NQ2 = (IX * PQ + kThreads - 1) / kThreads;  // product quads per thread
-    static constexpr int NQ3 = (TX * (TZ / 4)) / kThreads;           // new-level quads per thread
+    static constexpr int NQ3 = (TX * QC) / kThreads;                  // new-level quads per thread
+    static_assert((TX * QC) % kThreads == 0, "box shape must give every thread whole quads");
 };
 
 struct TileInfo {
     int tile, sx, ex, sz, ez, wsx, wsz, plx, phx, plz, phz, pz0;
 };
+
+// new-level quad i of a box: column-major, so that a wavefront covers whole columns
+// (contiguous 16-byte quads along z)
+template <class S>
+__device__ __forceinline__ void out_quad(const TileInfo& t, int i, int& gx, int& z0)
+{
+    gx = t.sx + i / S::QC;
+    z0 = t.sz + 4 * (i % S::QC);
+}
 
 struct StepSmem {
     float *ux, *uz, *mu, *mux, *px, *pz, *qx, *qz;
@@ -260,12 +277,11 @@ __device__ __forceinline__ float d2_any(const float* pu, const float* pc, int s,
 }
 
 // ---- products of first derivatives on one quad --------------------------------------
-template <bool VTI, int TX, bool EDGE>
+template <bool VTI, class S, bool EDGE>
 __device__ __forceinline__ void product_quad(const StepArgs& a, const StepSmem& sm, const TileInfo& t, int gx, int z0,
                                              float4 lam4, float4 lamx4, float4 c134, float* p, float* pz, float* qx,
                                              float* qz)
 {
-    using S = StepShape<TX>;
     const int nx = a.nx, nz = a.nz;
     const int w = (gx - t.wsx) * S::WZ + (z0 - t.wsz);
     float dxux[4], dxuz[4], dzux[4], dzuz[4];
@@ -327,11 +343,10 @@ __device__ __forceinline__ void product_quad(const StepArgs& a, const StepSmem& 
 }
 
 // ---- SAT terms of one perimeter cell (we_op.cpp:726-822, 1705-1800) -------------------
-template <bool VTI, int TX>
+template <bool VTI, class S>
 __device__ __forceinline__ void sat_cell(const StepArgs& a, const float* sux, const float* suz, float mu0, int gx,
                                          int gz, float prx, float prz, float& rx, float& rz)
 {
-    using S = StepShape<TX>;
     const int nx = a.nx, nz = a.nz;
     const int g = gx * a.ldz + gz;
     const float lam0 = a.lam[g];
@@ -369,22 +384,21 @@ __device__ __forceinline__ void sat_cell(const StepArgs& a, const float* sux, co
 
 // ---- right-hand side of one quad before sources: volume terms (we_op.cpp:690-720,
 // 1665-1700) and SAT terms ----------------------------------------------------------
-template <bool VTI, int TX, bool EDGE>
+template <bool VTI, class S, bool EDGE, bool CW>
 __device__ __forceinline__ void rhs_quad(const StepArgs& a, const StepSmem& sm, const TileInfo& t, int gx, int z0,
-                                         const float* prx, const float* prz, float* rx, float* rz)
+                                         const float* prx, const float* prz, float* rx, float* rz, const QuadW& qw)
 {
-    using S = StepShape<TX>;
     const int nx = a.nx, nz = a.nz;
     const int w = (gx - t.wsx) * S::WZ + (z0 - t.wsz);
     const int q = (gx - t.plx) * S::PZ + (z0 - t.pz0);
     // x part: D2x[2mu or 2(1+2eps)mu] ux, D2x[mu] uz, Dx of the products
     if (!EDGE || (gx >= 6 && gx < nx - 6)) {
-        float U[5][4], Z[5][4], M[5][4], MX[VTI ? 5 : 1][4];
+        float U[5][4], Z[5][4], M[CW ? 1 : 5][4], MX[VTI ? 5 : 1][4];
 #pragma unroll
         for (int c = 0; c < 5; c++) {
             unpack(ld4(&sm.ux[w + (c - 2) * S::WZ]), U[c]);
             unpack(ld4(&sm.uz[w + (c - 2) * S::WZ]), Z[c]);
-            unpack(ld4(&sm.mu[w + (c - 2) * S::WZ]), M[c]);
+            if (!CW) unpack(ld4(&sm.mu[w + (c - 2) * S::WZ]), M[c]);
             if (VTI) unpack(ld4(&sm.mux[w + (c - 2) * S::WZ]), MX[c]);
         }
         float P[4][4], Q[4][4];
@@ -396,10 +410,16 @@ __device__ __forceinline__ void rhs_quad(const StepArgs& a, const StepSmem& sm, 
         }
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            const float mk[5] = {M[0][k], M[1][k], M[2][k], M[3][k], M[4][k]};
             const float uk[5] = {U[0][k], U[1][k], U[2][k], U[3][k], U[4][k]};
             const float zk[5] = {Z[0][k], Z[1][k], Z[2][k], Z[3][k], Z[4][k]};
-            const W5 wx = d2_weights(&mk[2], 1);
+            W5 wx;
+            if constexpr (CW) {
+                wx = qw.x[k];
+            } else {
+                const float mk[5] = {M[CW ? 0 : 0][k], M[CW ? 0 : 1][k], M[CW ? 0 : 2][k], M[CW ? 0 : 3][k],
+                                     M[CW ? 0 : 4][k]};
+                wx = d2_weights(&mk[2], 1);
+            }
             float d2u;
             if (VTI) {
                 const float xk[5] = {MX[0][k], MX[1][k], MX[2][k], MX[3][k], MX[4][k]};
@@ -422,11 +442,20 @@ __device__ __forceinline__ void rhs_quad(const StepArgs& a, const StepSmem& sm, 
     }
     // z part: D2z[2mu] uz, D2z[mu] ux, Dz of the products
     if (!EDGE || (z0 >= 6 && z0 + 3 < nz - 6)) {
-        float Uc[12], Zc[12], Mc[12], d2zx[4], d2zz[4];
+        float Uc[12], Zc[12], d2zx[4], d2zz[4];
         load_col12(sm.ux, w, Uc);
         load_col12(sm.uz, w, Zc);
-        load_col12(sm.mu, w, Mc);
-        d2_quad_z(Mc, Zc, Uc, d2zz, d2zx);
+        if constexpr (CW) {
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                d2zz[k] = d2_apply(qw.z[k], &Zc[4 + k], 1);
+                d2zx[k] = d2_apply(qw.z[k], &Uc[4 + k], 1);
+            }
+        } else {
+            float Mc[12];
+            load_col12(sm.mu, w, Mc);
+            d2_quad_z(Mc, Zc, Uc, d2zz, d2zx);
+        }
         float Pc[12], Qc[12], dpz[4], dqz[4];
         load_col12(VTI ? sm.pz : sm.px, q, Pc);
         load_col12(sm.qx, q, Qc);
@@ -455,7 +484,7 @@ __device__ __forceinline__ void rhs_quad(const StepArgs& a, const StepSmem& sm, 
             const int gz = z0 + k;
             if (gz >= nz) continue;
             if (gz == 0 || gz == nz - 1 || gx == 0 || gx == nx - 1)
-                sat_cell<VTI, TX>(a, &sm.ux[w + k], &sm.uz[w + k], sm.mu[w + k], gx, gz, prx[k], prz[k], rx[k], rz[k]);
+                sat_cell<VTI, S>(a, &sm.ux[w + k], &sm.uz[w + k], sm.mu[w + k], gx, gz, prx[k], prz[k], rx[k], rz[k]);
         }
     }
 }
@@ -527,15 +556,17 @@ __device__ __forceinline__ void opaque(TileInfo& t)
     opaque(t.pz0);
 }
 
-// EDGE = false: tiles whose every quad takes the interior paths (the host classifies
-// tiles with the same conditions); no closure or SAT code is compiled in.
+// A workgroup advances the shots of one group through one box of the grid.  Boxes of the
+// core (EDGE = false) hold only cells whose stencils are all interior, so no closure or
+// SAT code is compiled in; boxes of the edge bands (EDGE = true) take the closure and
+// SAT paths where a quad needs them.
 // IMG: adjoint step with the imaging condition of forward snapshot a.img_mode/idx fused in;
 // the gradient contributions of the workgroup's shots are summed in registers and added
-// to the shot group's gradient once per launch.
-template <bool VTI, int TX, bool EDGE, bool IMG>
-__global__ void __launch_bounds__(kThreads) k_step(const StepArgs a, const int* tiles)
+// once per launch to the gradient slab of the group's first shot.
+template <bool VTI, int TX, int TZ, bool EDGE, bool IMG>
+__global__ void __launch_bounds__(kThreads) k_step(const StepArgs a, const int4* boxes, const int* list)
 {
-    using S = StepShape<TX>;
+    using S = StepShape<TX, TZ>;
     __shared__ __attribute__((aligned(16))) float sUx[S::WX * S::WZ];
     __shared__ __attribute__((aligned(16))) float sUz[S::WX * S::WZ];
     __shared__ __attribute__((aligned(16))) float sMu[S::WX * S::WZ];
@@ -548,17 +579,18 @@ __global__ void __launch_bounds__(kThreads) k_step(const StepArgs a, const int* 
     __shared__ __attribute__((aligned(16))) float sFz[IMG ? S::IX * S::PZ : 4];  // product-range layout
     StepSmem sm{sUx, sUz, sMu, sMuX, sPx, sPz, sQx, sQz};
 
-    const int ntiles = a.ntx * a.ntz;
+    const int ntiles = a.nbox;  // boxes of the CSR injector / extractor lists
     const int lin = xcd_remap(blockIdx.x, gridDim.x);
     TileInfo t;
     const int slot = lin / a.ngroups;
     const int group = lin - slot * a.ngroups;
-    t.tile = tiles[slot];
-    const int tix = t.tile / a.ntz;
-    const int tiz = t.tile - tix * a.ntz;
+    t.tile = list[slot];
+    const int4 box = boxes[t.tile];
     const int nx = a.nx, nz = a.nz, ldz = a.ldz;
-    tile_range_x_hd(tix, a.ntx, TX, nx, t.sx, t.ex);
-    tile_range_z_hd(tiz, a.ntz, nz, ldz, t.sz, t.ez);
+    t.sx = box.x;
+    t.ex = box.y;
+    t.sz = box.z;
+    t.ez = box.w;
     t.wsx = window_start(t.sx, S::HX);
     t.wsz = t.sz - S::HZ;
     t.pz0 = t.sz - 4;
@@ -604,13 +636,39 @@ __global__ void __launch_bounds__(kThreads) k_step(const StepArgs a, const int* 
 #pragma unroll
     for (int m = 0; m < S::NQ3; m++) {
         const int i = tid + m * kThreads;
-        const int gx = t.sx + (i >> 4), gz = t.sz + 4 * (i & 15);
+        int gx, gz;
+        out_quad<S>(t, i, gx, gz);
         const bool ok = gx < t.ex && gz < t.ez;
         const int g = ok ? gx * ldz + gz : 0;
         const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
         qir[m] = ok ? ld4(a.irho + g) : z4;
         qtz[m] = ok ? ld4(a.taper_z + gz) : z4;
         qtx[m] = ok ? a.taper_x[gx] : 0.f;
+    }
+
+    // ---- D2 weights of the new-level quads (core boxes, isotropic): once per workgroup ----
+    constexpr bool CW = !EDGE && !VTI && S::NQ3 == 1;
+    QuadW qw[CW ? S::NQ3 : 1];
+    if constexpr (CW) {
+        __syncthreads();  // the coefficient window is in LDS
+#pragma unroll
+        for (int m = 0; m < S::NQ3; m++) {
+            const int i = tid + m * kThreads;
+            int gx, z0;
+            out_quad<S>(t, i, gx, z0);
+            if (gx >= t.ex || z0 >= t.ez) continue;
+            const int w = (gx - t.wsx) * S::WZ + (z0 - t.wsz);
+            float M[5][4], Mc[12];
+#pragma unroll
+            for (int c = 0; c < 5; c++) unpack(ld4(&sMu[w + (c - 2) * S::WZ]), M[c]);
+            load_col12(sMu, w, Mc);
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const float mk[5] = {M[0][k], M[1][k], M[2][k], M[3][k], M[4][k]};
+                qw[m].x[k] = d2_weights(&mk[2], 1);
+                qw[m].z[k] = d2_weights(&Mc[4 + k], 1);
+            }
+        }
     }
 
     // ---- state window of one shot into registers ----
@@ -652,7 +710,8 @@ __global__ void __launch_bounds__(kThreads) k_step(const StepArgs a, const int* 
 #pragma unroll
         for (int m = 0; m < S::NQ3; m++) {
             const int i = tid + m * kThreads;
-            const int gx = t.sx + (i >> 4), gz = t.sz + 4 * (i & 15);
+            int gx, gz;
+        out_quad<S>(t, i, gx, gz);
             const bool ok = gx < t.ex && gz < t.ez;
             const int g = ok ? gx * ldz + gz : 0;
             const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -714,7 +773,7 @@ __global__ void __launch_bounds__(kThreads) k_step(const StepArgs a, const int* 
             float p[4] = {0.f, 0.f, 0.f, 0.f}, pz[4] = {0.f, 0.f, 0.f, 0.f}, qx[4] = {0.f, 0.f, 0.f, 0.f},
                   qz[4] = {0.f, 0.f, 0.f, 0.f};
             if (gx < t.phx && z0 + 3 >= t.plz && z0 < t.phz)
-                product_quad<VTI, TX, EDGE>(a, sm, t, gx, z0, plam[m], VTI ? plamx[m] : plam[m], VTI ? pc13[m] : plam[m],
+                product_quad<VTI, S, EDGE>(a, sm, t, gx, z0, plam[m], VTI ? plamx[m] : plam[m], VTI ? pc13[m] : plam[m],
                                       p, pz, qx, qz);
             const int o = lc * S::PZ + 4 * lq;
             st4(&sPx[o], make_float4(p[0], p[1], p[2], p[3]));
@@ -731,7 +790,8 @@ __global__ void __launch_bounds__(kThreads) k_step(const StepArgs a, const int* 
 #pragma unroll
         for (int m = 0; m < S::NQ3; m++) {
             const int i = tid + m * kThreads;
-            const int gx = t.sx + (i >> 4), z0 = t.sz + 4 * (i & 15);
+            int gx, z0;
+            out_quad<S>(t, i, gx, z0);
             if (gx >= t.ex || z0 >= t.ez) continue;
             float prx[4], prz[4], ir[4], tz[4];
             unpack(qprx[m], prx);
@@ -747,7 +807,7 @@ __global__ void __launch_bounds__(kThreads) k_step(const StepArgs a, const int* 
                 }
             }
             float rx[4], rz[4];
-            rhs_quad<VTI, TX, EDGE>(a, sm, t, gx, z0, prx, prz, rx, rz);
+            rhs_quad<VTI, S, EDGE, CW>(a, sm, t, gx, z0, prx, prz, rx, rz, qw[CW ? m : 0]);
             // sources (we_op.cpp:722-724)
             for (int k2 = ib; k2 < ie; k2++) {
                 const PointEntry& en = a.inj[a.inj_idx[k2]];
@@ -857,11 +917,12 @@ __global__ void __launch_bounds__(kThreads) k_step(const StepArgs a, const int* 
         }
     }
     if constexpr (IMG) {
-        float* G = a.grad + group * a.ggroup_stride;
+        float* G = a.grad + s0 * a.gslab_stride;
 #pragma unroll
         for (int m = 0; m < S::NQ3; m++) {
             const int i = tid + m * kThreads;
-            const int gx = t.sx + (i >> 4), z0 = t.sz + 4 * (i & 15);
+            int gx, z0;
+            out_quad<S>(t, i, gx, z0);
             if (gx >= t.ex || z0 >= t.ez || s0 >= s1) continue;
             const long long g = (long long)gx * ldz + z0;
             float4 v = ld4(G + g);
@@ -1079,52 +1140,32 @@ static inline int grid_for(long long n, int cap = 4096)
     return (int)(b < 1 ? 1 : (b > cap ? cap : b));
 }
 
-// True when every product quad and every new-level quad of the tile takes the interior
-// paths of k_step (the conditions below are those of product_quad and rhs_quad).
-bool interior_tile(int tix, int ntx, int tile_x, int tiz, int ntz, int nx, int nz, int ldz)
+template <bool VTI, int TX, int TZ, bool EDGE>
+static void step_launch(const StepArgs& a, const int4* boxes, const int* list, int n, bool image, hipStream_t s)
 {
-    int sx, ex, sz, ez, plx, phx, plz, phz;
-    tile_range_x_hd(tix, ntx, tile_x, nx, sx, ex);
-    tile_range_z_hd(tiz, ntz, nz, ldz, sz, ez);
-    product_range(sx, ex, nx, plx, phx);
-    product_range(sz, ez, nz, plz, phz);
-    if (plx < 4 || phx > nx - 4 || sx < 6 || ex > nx - 6) return false;
-    const int pz0 = sz - 4;
-    for (int lq = 0; lq < (kTileZ + 8) / 4; lq++) {
-        const int z0 = pz0 + 4 * lq;
-        if (z0 + 3 >= plz && z0 < phz && !(z0 >= 4 && z0 + 3 < nz - 4)) return false;
-    }
-    for (int z0 = sz; z0 < ez; z0 += 4)
-        if (!(z0 >= 6 && z0 + 3 < nz - 6)) return false;
-    return true;
-}
-
-template <bool VTI, bool EDGE, bool IMG>
-static void step_dispatch(const StepArgs& a, int tile_x, const int* tiles, int ntile, hipStream_t s)
-{
-    const int nb = ntile * a.ngroups;
+    const int nb = n * a.ngroups;
     if (nb <= 0) return;
-    if (tile_x == 32) hipLaunchKernelGGL((k_step<VTI, 32, EDGE, IMG>), dim3(nb), dim3(kThreads), 0, s, a, tiles);
-    else hipLaunchKernelGGL((k_step<VTI, 16, EDGE, IMG>), dim3(nb), dim3(kThreads), 0, s, a, tiles);
+    if (image) hipLaunchKernelGGL((k_step<VTI, TX, TZ, EDGE, true>), dim3(nb), dim3(kThreads), 0, s, a, boxes, list);
+    else hipLaunchKernelGGL((k_step<VTI, TX, TZ, EDGE, false>), dim3(nb), dim3(kThreads), 0, s, a, boxes, list);
 }
 
-template <bool VTI, bool EDGE>
-static void step_dispatch_img(const StepArgs& a, int tile_x, const int* tiles, int ntile, bool image, hipStream_t s)
+template <bool VTI>
+static void step_class(const StepArgs& a, BoxClass c, const int4* boxes, const int* list, int n, bool image,
+                       hipStream_t s)
 {
-    if (image) step_dispatch<VTI, EDGE, true>(a, tile_x, tiles, ntile, s);
-    else step_dispatch<VTI, EDGE, false>(a, tile_x, tiles, ntile, s);
+    switch (c) {
+    case BoxClass::core16: step_launch<VTI, 16, 64, false>(a, boxes, list, n, image, s); break;
+    case BoxClass::core32: step_launch<VTI, 32, 64, false>(a, boxes, list, n, image, s); break;
+    case BoxClass::band_z: step_launch<VTI, 64, 16, true>(a, boxes, list, n, image, s); break;
+    case BoxClass::band_x: step_launch<VTI, 8, 128, true>(a, boxes, list, n, image, s); break;
+    }
 }
 
-void launch_step(const StepArgs& a, bool vti, int tile_x, const int* tiles, int ntile, bool edge, bool image,
+void launch_step(const StepArgs& a, bool vti, BoxClass c, const int4* boxes, const int* list, int n, bool image,
                  hipStream_t s)
 {
-    if (vti) {
-        if (edge) step_dispatch_img<true, true>(a, tile_x, tiles, ntile, image, s);
-        else step_dispatch_img<true, false>(a, tile_x, tiles, ntile, image, s);
-    } else {
-        if (edge) step_dispatch_img<false, true>(a, tile_x, tiles, ntile, image, s);
-        else step_dispatch_img<false, false>(a, tile_x, tiles, ntile, image, s);
-    }
+    if (vti) step_class<true>(a, c, boxes, list, n, image, s);
+    else step_class<false>(a, c, boxes, list, n, image, s);
 }
 
 template <bool VTI>

@@ -43,7 +43,7 @@ class Fwi2dOpDesc(C.Structure):
 EXPORTS = [
     "fwi2d_last_error", "fwi2d_engine_create", "fwi2d_engine_set_model", "fwi2d_engine_set_shots",
     "fwi2d_engine_forward", "fwi2d_engine_adjoint", "fwi2d_engine_dot_f64", "fwi2d_engine_set_option",
-    "fwi2d_engine_get_stat", "fwi2d_engine_destroy", "fwi2d_op_create", "fwi2d_op_apply_forward",
+    "fwi2d_engine_get_stat", "fwi2d_engine_tune_report", "fwi2d_engine_destroy", "fwi2d_op_create", "fwi2d_op_apply_forward",
     "fwi2d_op_apply_jacobianT", "fwi2d_op_engine", "fwi2d_op_time_axis", "fwi2d_op_destroy",
 ]
 
@@ -71,6 +71,8 @@ def load():
     lib.fwi2d_engine_set_option.argtypes = [P, C.c_char_p, C.c_longlong]
     lib.fwi2d_engine_get_stat.argtypes = [P, C.c_char_p]
     lib.fwi2d_engine_get_stat.restype = C.c_longlong
+    lib.fwi2d_engine_tune_report.argtypes = [P]
+    lib.fwi2d_engine_tune_report.restype = C.c_char_p
     lib.fwi2d_engine_destroy.argtypes = [P]
     lib.fwi2d_op_create.argtypes = [C.POINTER(Fwi2dOpDesc), P, C.POINTER(P)]
     lib.fwi2d_op_apply_forward.argtypes = [P, C.c_int, P, P]
@@ -169,6 +171,9 @@ class Engine:
 
     def stat(self, key):
         return load().fwi2d_engine_get_stat(self._h, key.encode())
+
+    def tune_report(self):
+        return load().fwi2d_engine_tune_report(self._h).decode()
 
 
 def op_desc_from(prob, device=0):

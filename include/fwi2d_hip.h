@@ -86,14 +86,23 @@ int fwi2d_engine_dot_f64(fwi2d_engine* e, const float* a, const float* b, const 
                          double* out);
 
 /* Tuning and test options (no reference counterpart): "tile_x" (16 | 32 cells per tile
-   along x), "shots_per_block" (1..64 shots advanced in turn by one step workgroup),
+   along x), "shots_per_block" (1..64 shots advanced in turn by one step workgroup of an
+   interior tile), "shots_per_block_edge" (the same for tiles at the grid edge),
+   "autotune" (1: at the next forward run, time a few steps of every tile_x and
+   shots-per-workgroup choice and keep the fastest; forward results are unchanged),
    "segment" (snapshots per checkpoint segment, 0 = about sqrt(2 nt/sub)), "lag" (1 the
    reference's one-step imaging lag, 0 none), "timing" (1 time the forward step loop with
-   HIP events), "graph" (reserved).  Statistics: "steps", "recompute_steps",
-   "image_calls", "segment", "nseg", "tile_x", "shots_per_block", "ntiles", "interior_tiles", "nrec",
+   HIP events), "graph" (1: capture each forward / adjoint time loop in a hipGraph the
+   first time it runs in a configuration and replay it afterwards).  Statistics: "steps", "recompute_steps",
+   "image_calls", "segment", "nseg", "tile_x", "shots_per_block", "shots_per_block_edge",
+   "ntiles", "interior_tiles", "graph_captures", "graph_replays", "nrec",
    "nshots", "fwd_loop_ns", "fwd_loop_steps"; -1 for an unknown key. */
 int fwi2d_engine_set_option(fwi2d_engine* e, const char* key, long long value);
 long long fwi2d_engine_get_stat(fwi2d_engine* e, const char* key);
+
+/* Timings of the last autotuning run, "tile_x=16 spb=4 spb_edge=1: 512.3 us; ..."
+   ("" before any); valid until the next call on the engine. */
+const char* fwi2d_engine_tune_report(fwi2d_engine* e);
 
 void fwi2d_engine_destroy(fwi2d_engine* e);
 

@@ -1,16 +1,23 @@
 """CPU emulation of the step and imaging kernels' index arithmetic.
 
-Mirrors tile_range / window_start of fwi2d_amd/csrc/kernels.hip and checks, for many
-grid shapes and every tile, that each stencil read of the kernels (SBP interior and
-closure rows, first-derivative products, SAT rows, extraction footprints) lands inside
-the LDS arrays and on a cell of the grid that the tile loaded -- so the kernels never
-read outside their shared-memory windows nor use a zero-filled halo cell as data.
-Also checks that the tiles' write ranges partition the grid.
+Mirrors the box decomposition of the engine (fwi2d_amd/csrc/engine.hip: build_boxes)
+and the index arithmetic of k_step / k_image (fwi2d_amd/csrc/kernels.hip), and checks,
+for many grid shapes and every box, that
+  * the boxes partition the grid and fit their class's shape;
+  * core boxes (compiled without closure code) only ever take the interior paths;
+  * each stencil read (SBP interior and closure rows, first-derivative products, SAT
+    rows, extraction footprints) lands inside the LDS arrays and on a grid cell the box
+    loaded, so no kernel reads outside its shared-memory windows or uses a zero-filled
+    halo cell as data.
+The 2-D stencils are separable, so each axis of a box is checked on its own.
 """
 import numpy as np
 import pytest
 
-TZ, H_STEP, H_IMG = 64, 4, 2
+TZ, H_IMG = 64, 2
+CORE_X0, CORE_Z0 = 6, 8
+SHAPES = {"core16": (16, 64, False), "core32": (32, 64, False), "band_z": (64, 16, True),
+          "band_x": (8, 128, True)}
 
 
 def tile_range(k, ntile, T, n):
@@ -45,7 +52,7 @@ def d2_offsets(g, n):
 
 
 def needed_products(s, e, n):
-    """Range of first-derivative products a tile writing [s, e) uses (mirrors the kernel)."""
+    """product_range of kernels.hip: the products a box writing [s, e) uses."""
     lo, hi = max(0, s - 2), min(n, e + 2)
     if s < 4:
         lo, hi = 0, max(hi, 6)
@@ -60,8 +67,6 @@ def tiles(n, T):
 
 
 def tile_range_quads(k, ntile, T, n):
-    """z tiling in 4-cell quads: tiles start on quads; the last one is anchored to the
-    last quad of the padded extent ceil(n/4)*4 and writes up to n."""
     nq, tq = (n + 3) // 4, T // 4
     if ntile == 1:
         return 0, n
@@ -77,127 +82,160 @@ def tiles_quads(n, T):
     return [tile_range_quads(k, nt, T, n) for k in range(nt)]
 
 
-def check_x_axis(n, T):
-    """x axis of the step kernel (cells): window halo 4, products on [plo, phi) stored
-    from plo in T + 4 columns; interior/closure choice per column."""
-    W, I = T + 2 * H_STEP, T + 4
-    cover = np.zeros(n, int)
-    for s, e in tiles(n, T):
-        assert 0 <= s < e <= n and e - s <= T
-        cover[s:e] += 1
-        ws = window_start(s, H_STEP)
-        loaded = set(p for p in range(ws, ws + W) if 0 <= p < n)
-        plo, phi = needed_products(s, e, n)
-        need = set(range(plo, phi))
-        assert phi - plo <= I, (n, T, s, plo, phi)
-        for p in need:  # product columns: Dx of the state (fast: +-1, +-2; closure: 6 points)
-            taps = [-2, -1, 1, 2] if 4 <= p < n - 4 else d1_offsets(p, n)
-            for o in taps:
-                assert 0 <= p + o - ws < W and (p + o) in loaded, (n, T, s, p, o)
-        for g in range(s, e):
-            d2 = [-2, -1, 0, 1, 2] if 6 <= g < n - 6 else d2_offsets(g, n)
-            for o in d2:
-                assert 0 <= g + o - ws < W and (g + o) in loaded, (n, T, s, g, o)
-            for o in d1_offsets(g, n):  # Dx of the products, tangential SAT derivative
-                assert 0 <= g + o - plo < I and (g + o) in need, (n, T, s, g, o)
-                assert (g + o) in loaded
-            if g == 0:
-                assert all(k in loaded for k in range(6))
-            if g == n - 1:
-                assert all(n - 1 - k in loaded for k in range(6))
-            if 1 <= g <= n - 2:  # extraction footprint of a pivot owned by this tile
-                assert all((g + o) in loaded and 0 <= g + o - ws < W for o in (-1, 0, 1))
-        check_image_axis(n, T, s, e)
-    assert np.all(cover == 1), (n, T, cover)
+def core_z_end(nz):
+    return ((nz - 8) // 4) * 4
 
 
-HZ_STEP, PZ = 8, TZ + 8
+def build_boxes(nx, nz, tile_x):
+    """Mirror of Engine::build_boxes: list of (class, sx, ex, sz, ez)."""
+    out = []
+    cx0, cx1 = CORE_X0, nx - CORE_X0
+    cz0, cz1 = CORE_Z0, max(core_z_end(nz), CORE_Z0)
+
+    def add(c, sx, ex, sz, ez):
+        if ex > sx and ez > sz:
+            out.append((c, sx, ex, sz, ez))
+    core = "core16" if tile_x == 16 else "core32"
+    for sx, ex in tiles(nx, tile_x):
+        for sz, ez in tiles_quads(nz, TZ):
+            add(core, max(sx, cx0), min(ex, cx1), max(sz, cz0), min(ez, cz1))
+    for sx, ex in tiles(nx, 64):
+        add("band_z", sx, ex, 0, cz0)
+        add("band_z", sx, ex, cz1, nz)
+    for sz in range(cz0, cz1, 128):
+        ez = min(sz + 128, cz1)
+        add("band_x", 0, cx0, sz, ez)
+        add("band_x", cx1, nx, sz, ez)
+    return out
 
 
-def check_z_axis(n):
-    """z axis of the step kernel (quads of 4 cells): window rows [s-8, s+72), product
-    rows [s-4, s+68) in 18 quads; each quad takes the interior path only when all its
-    cells are interior, else per-cell closure code."""
+def check_x(sx, ex, TX, n, edge):
+    assert 0 <= sx < ex <= n and ex - sx <= TX
+    ws, W, I = window_start(sx, 4), TX + 8, TX + 4
+    loaded = set(p for p in range(ws, ws + W) if 0 <= p < n)
+    plo, phi = needed_products(sx, ex, n)
+    need = set(range(plo, phi))
+    assert phi - plo <= I, (n, sx, plo, phi)
+    for p in need:
+        fast = 4 <= p < n - 4
+        assert fast or edge, ("closure product column in a core box", n, sx, p)
+        for o in ([-2, -1, 1, 2] if fast else d1_offsets(p, n)):
+            assert 0 <= p + o - ws < W and (p + o) in loaded, (n, sx, p, o)
+    for g in range(sx, ex):
+        fast = 6 <= g < n - 6
+        assert fast or edge, ("closure column in a core box", n, sx, g)
+        for o in ([-2, -1, 0, 1, 2] if fast else d2_offsets(g, n)):
+            assert 0 <= g + o - ws < W and (g + o) in loaded, (n, sx, g, o)
+        for o in ([-2, -1, 1, 2] if fast else d1_offsets(g, n)):
+            assert 0 <= g + o - plo < I and (g + o) in need, (n, sx, g, o)
+            assert (g + o) in loaded
+        if g == 0:
+            assert all(k in loaded for k in range(6))
+        if g == n - 1:
+            assert all(n - 1 - k in loaded for k in range(6))
+        if 1 <= g <= n - 2:  # extraction footprint of a pivot owned by this box
+            assert all((g + o) in loaded and 0 <= g + o - ws < W for o in (-1, 0, 1))
+
+
+def check_z(sz, ez, TZb, n, edge):
     ldz = (n + 3) // 4 * 4
-    WZ = TZ + 2 * HZ_STEP
-    cover = np.zeros(n, int)
-    for s, e in tiles_quads(n, TZ):
-        assert s % 4 == 0 and 0 <= s < e <= n and e - s <= TZ
-        cover[s:e] += 1
-        ws, pz0 = s - HZ_STEP, s - 4
-        assert ws + WZ >= min(ldz, s + TZ + 8)
-        loaded = set(p for p in range(ws, ws + WZ) if 0 <= p < n)  # padding rows are not data
-        plo, phi = needed_products(s, e, n)
-        need = set(range(plo, phi))
-        assert plo >= pz0 and phi <= pz0 + PZ, (n, s, plo, phi)
-        for lq in range(PZ // 4):
-            z0 = pz0 + 4 * lq
-            if not (z0 + 3 >= plo and z0 < phi):
-                continue
-            assert 0 <= z0 and z0 + 3 < ldz  # coefficient float4 read stays in the column
-            if z0 >= 4 and z0 + 3 < n - 4:
-                assert ws <= z0 - 4 and z0 + 7 < ws + WZ  # 12-row LDS read
-                for k in range(4):
-                    if z0 + k in need:
-                        assert all(z0 + k + o in loaded for o in (-2, -1, 1, 2))
-            else:
-                for k in range(4):
-                    g = z0 + k
-                    if g < 0 or g >= n:
-                        continue
+    assert sz % 4 == 0 and 0 <= sz < ez <= n and ez - sz <= TZb
+    ws, WZ, pz0, PZ = sz - 8, TZb + 16, sz - 4, TZb + 8
+    loaded = set(p for p in range(ws, ws + WZ) if 0 <= p < n)  # padding rows are not data
+    plo, phi = needed_products(sz, ez, n)
+    need = set(range(plo, phi))
+    assert plo >= pz0 and phi <= pz0 + PZ, (n, sz, plo, phi)
+    for lq in range(PZ // 4):
+        z0 = pz0 + 4 * lq
+        if not (z0 + 3 >= plo and z0 < phi):
+            continue
+        assert 0 <= z0 and z0 + 3 < ldz  # coefficient / snapshot float4 reads
+        fast = z0 >= 4 and z0 + 3 < n - 4
+        assert fast or edge, ("closure product quad in a core box", n, sz, z0)
+        if fast:
+            assert ws <= z0 - 4 and z0 + 7 < ws + WZ  # 12-row LDS read
+            for k in range(4):
+                if z0 + k in need:
+                    assert all(z0 + k + o in loaded for o in (-2, -1, 1, 2))
+        else:
+            for k in range(4):
+                g = z0 + k
+                if 0 <= g < n:
                     for o in d1_offsets(g, n):
-                        assert 0 <= g + o - ws < WZ and (g + o) in loaded, (n, s, g, o)
-        for j in range(TZ // 4):
-            z0 = s + 4 * j
-            if z0 >= e:
-                continue
-            assert z0 + 3 < ldz
-            if z0 >= 6 and z0 + 3 < n - 6:
-                assert ws <= z0 - 4 and z0 + 7 < ws + WZ
-                assert 0 <= z0 - 4 - pz0 and z0 + 7 - pz0 < PZ
-                for k in range(4):
-                    g = z0 + k
-                    assert all(g + o in loaded for o in (-2, -1, 0, 1, 2))
-                    assert all(g + o in need for o in (-2, -1, 1, 2))
-            else:
-                for k in range(4):
-                    g = z0 + k
-                    if g >= n:
-                        continue
-                    for o in d2_offsets(g, n):
-                        assert 0 <= g + o - ws < WZ and (g + o) in loaded, (n, s, g, o)
-                    for o in d1_offsets(g, n):
-                        assert 0 <= g + o - pz0 < PZ and (g + o) in need, (n, s, g, o)
+                        assert 0 <= g + o - ws < WZ and (g + o) in loaded, (n, sz, g, o)
+    for j in range(TZb // 4):
+        z0 = sz + 4 * j
+        if z0 >= ez:
+            continue
+        assert z0 + 3 < ldz
+        fast = z0 >= 6 and z0 + 3 < n - 6
+        assert fast or edge, ("closure quad in a core box", n, sz, z0)
+        if fast:
+            assert ws <= z0 - 4 and z0 + 7 < ws + WZ
+            assert 0 <= z0 - 4 - pz0 and z0 + 7 - pz0 < PZ
+            for k in range(4):
+                g = z0 + k
+                assert all(g + o in loaded for o in (-2, -1, 0, 1, 2))
+                assert all(g + o in need for o in (-2, -1, 1, 2))
+        else:
+            for k in range(4):
+                g = z0 + k
+                if g >= n:
+                    continue
+                for o in d2_offsets(g, n):
+                    assert 0 <= g + o - ws < WZ and (g + o) in loaded, (n, sz, g, o)
+                for o in d1_offsets(g, n):
+                    assert 0 <= g + o - pz0 < PZ and (g + o) in need, (n, sz, g, o)
+    for g in range(sz, ez):
+        if g == 0:
+            assert all(k in loaded for k in range(6))
+        if g == n - 1:
+            assert all(n - 1 - k in loaded for k in range(6))
+        if 1 <= g <= n - 2:
+            assert all((g + o) in loaded and 0 <= g + o - ws < WZ for o in (-1, 0, 1))
+
+
+def check_grid(nx, nz, tile_x):
+    cover = np.zeros((nx, nz), int)
+    for c, sx, ex, sz, ez in build_boxes(nx, nz, tile_x):
+        TX, TZb, edge = SHAPES[c]
+        cover[sx:ex, sz:ez] += 1
+        check_x(sx, ex, TX, nx, edge)
+        check_z(sz, ez, TZb, nz, edge)
+    assert np.all(cover == 1), (nx, nz)
+
+
+SIZES = list(range(12, 90)) + [129, 131, 201, 300, 500, 601, 1000, 1700, 3000]
+
+
+@pytest.mark.parametrize("tile_x", [16, 32])
+def test_boxes_partition_and_windows_x(tile_x):
+    for nx in SIZES:
+        check_grid(nx, 70, tile_x)
+
+
+def test_boxes_partition_and_windows_z():
+    for nz in list(range(12, 300)) + [501, 601, 1000, 3001]:
+        check_grid(40, nz, 16)
+
+
+def check_image_axis(n, T, quads):
+    """Imaging kernel (used without the imaging lag): halo 2 windows on the tile grid."""
+    for s, e in (tiles_quads(n, T) if quads else tiles(n, T)):
+        wsi = window_start(s, H_IMG)
+        Wi = T + 2 * H_IMG
+        loaded_i = set(p for p in range(wsi, wsi + Wi) if 0 <= p < n)
         for g in range(s, e):
-            if g == 0:
-                assert all(k in loaded for k in range(6))
-            if g == n - 1:
-                assert all(n - 1 - k in loaded for k in range(6))
-            if 1 <= g <= n - 2:
-                assert all((g + o) in loaded and 0 <= g + o - ws < WZ for o in (-1, 0, 1))
-        check_image_axis(n, TZ, s, e)
-    assert np.all(cover == 1), (n, cover)
+            for o in d1_offsets(g, n):
+                assert 0 <= g + o - wsi < Wi and (g + o) in loaded_i, ("image", n, T, s, g, o)
 
 
-def check_image_axis(n, T, s, e):
-    """Imaging kernel: halo 2 window from window_start(s, 2), D1 taps of every cell."""
-    wsi = window_start(s, H_IMG)
-    Wi = T + 2 * H_IMG
-    loaded_i = set(p for p in range(wsi, wsi + Wi) if 0 <= p < n)
-    for g in range(s, e):
-        for o in d1_offsets(g, n):
-            assert 0 <= g + o - wsi < Wi and (g + o) in loaded_i, ("image", n, T, s, g, o)
-
-
-@pytest.mark.parametrize("T", [16, 32])
-def test_tiles_partition_and_windows(T):
-    for n in list(range(12, 3 * T + 20)) + [201, 300, 500, 601, 1000, 1700, 3000]:
-        check_x_axis(n, T)
-
-
-def test_quad_tiles_along_z():
-    for n in list(range(12, 300)) + [501, 601, 1000, 3001]:
-        check_z_axis(n)
+def test_image_tiles():
+    for n in SIZES:
+        check_image_axis(n, 16, False)
+        check_image_axis(n, 32, False)
+    for n in list(range(12, 300)) + [501, 1000]:
+        check_image_axis(n, TZ, True)
 
 
 def xcd_remap(bid, nb):
