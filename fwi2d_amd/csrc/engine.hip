@@ -114,6 +114,9 @@ struct Engine {
 
     // state and checkpoints
     DevMem d_fwd, d_adj, d_ckpt, d_seg, d_gshot, d_gout, d_src, d_rcv, d_adjsrc, d_partial, d_scalar;
+    DevMem d_ckpt16, d_ckpt_scale, d_amax;  // BF16 checkpoint levels, their per-shot scales
+    int opt_bf16 = 0;                       // checkpoint levels stored in BF16 (max-abs scaled)
+    int opt_stiff = 0;                      // VTI raw gradients w.r.t. c11, c13, c33, c55, rho
     int seg_K = 0, nseg = 0, nsnap = 0, seg_loaded = -1;
     const float* fwd_src = nullptr;  // source traces of the checkpointed forward run (device)
     long long stat_steps = 0, stat_recompute_steps = 0, stat_image_calls = 0;
@@ -661,6 +664,25 @@ struct Engine {
     // checkpoints exist for segments j = 1 .. nseg-1
     float* ckpt_curr(int j) { return d_ckpt.as<float>() + (long long)(2 * (j - 1)) * state_stride(); }
     float* ckpt_prev(int j) { return d_ckpt.as<float>() + (long long)(2 * (j - 1) + 1) * state_stride(); }
+    // checkpoint slot k (2(j-1) curr, 2(j-1)+1 prev): store a level / restore it
+    void ckpt_store(int k, const float* u)
+    {
+        const long long ss = state_stride();
+        if (opt_bf16)
+            launch_pack_bf16(u, d_ckpt16.as<unsigned short>() + k * ss, d_ckpt_scale.as<float>() + (long long)k * ns,
+                             d_amax.as<unsigned>(), 2 * nxz, ns, st);
+        else
+            launch_copy(d_ckpt.as<float>() + k * ss, u, ss, st);
+    }
+    void ckpt_load(int k, float* u)
+    {
+        const long long ss = state_stride();
+        if (opt_bf16)
+            launch_unpack_bf16(d_ckpt16.as<unsigned short>() + k * ss, d_ckpt_scale.as<float>() + (long long)k * ns,
+                               u, 2 * nxz, ns, st);
+        else
+            launch_copy(u, d_ckpt.as<float>() + k * ss, ss, st);
+    }
 
     const float* upload_traces(DevMem& d, const float* h, long long n)
     {
@@ -680,6 +702,7 @@ struct Engine {
     {
         auto P = [](const void* p) { return (long long)(uintptr_t)p; };
         return {kind, P(p0), P(p1), flag, g.nt, ns, nr, tile_x, opt_spb, opt_spb_edge, opt_lag, seg_K, nseg,
+                opt_bf16, opt_stiff, P(d_ckpt16.p), P(d_ckpt_scale.p), P(d_amax.p),
                 P(d_fwd.p), P(d_adj.p), P(d_ckpt.p), P(d_seg.p), P(d_gshot.p), P(d_model.p), P(d_deriv.p),
                 P(d_boxes.p), P(d_list_core.p), P(d_list_bz.p), P(d_list_bx.p), P(fwd_inj.d_ptr.p), P(fwd_inj.d_idx.p),
                 P(fwd_ext.d_ptr.p), P(fwd_ext.d_idx.p), P(adj_inj.d_ptr.p), P(adj_inj.d_idx.p), P(d_src_ents.p),
@@ -808,7 +831,14 @@ struct Engine {
         int jlast = -1;
         if (keep) {
             plan_segments();
-            d_ckpt.reserve((size_t)std::max(2 * (nseg - 1), 1) * state_stride() * sizeof(float));
+            const size_t nslot = (size_t)std::max(2 * (nseg - 1), 1);
+            if (opt_bf16) {
+                d_ckpt16.reserve(nslot * state_stride() * sizeof(unsigned short));
+                d_ckpt_scale.reserve(nslot * ns * sizeof(float));
+                d_amax.reserve((size_t)ns * sizeof(unsigned));
+            } else {
+                d_ckpt.reserve(nslot * state_stride() * sizeof(float));
+            }
             d_seg.reserve((size_t)(seg_K + 2) * state_stride() * sizeof(float));
             jlast = nseg - 1;
         }
@@ -859,8 +889,8 @@ struct Engine {
         const long long ss = state_stride();
         for (int j = 1; j < nseg; j++)
             if (level == (j * seg_K - 1) * g.sub && level < g.nt - 1) {
-                launch_copy(ckpt_curr(j), L.C(), ss, st);
-                launch_copy(ckpt_prev(j), L.P(), ss, st);
+                ckpt_store(2 * (j - 1), L.C());
+                ckpt_store(2 * (j - 1) + 1, L.P());
             }
         if (level % g.sub == 0) {
             int idx = level / g.sub;
@@ -882,8 +912,8 @@ struct Engine {
             F.prev = 0;
             F.curr = 1;
             F.next = 2;
-            launch_copy(F.C(), ckpt_curr(j), ss, st);
-            launch_copy(F.P(), ckpt_prev(j), ss, st);
+            ckpt_load(2 * (j - 1), F.C());
+            ckpt_load(2 * (j - 1) + 1, F.P());
             start = seg_base(j) * g.sub;
         }
         const int end = seg_last_idx(j) * g.sub;
@@ -928,6 +958,7 @@ struct Engine {
         a.ge = vti ? dv + 3 * nxz : nullptr;
         a.gv2 = vti ? dv + 4 * nxz : nullptr;
         a.gv3 = vti ? dv + 5 * nxz : nullptr;
+        a.stiff = vti && opt_stiff;
     }
 
     void image(const Levels& A, int idx, const float* dsrc)
@@ -978,6 +1009,8 @@ struct Engine {
         // workgroup adds its shots' sum to the slab of its first shot); without it (a test
         // option) the separate imaging kernel fills one slab per shot
         const bool fused = opt_lag == 1;
+        if (opt_stiff && (!vti || !fused))
+            throw std::runtime_error("stiffness gradients need a VTI grid and the imaging lag");
         const int nslab = ns;
         d_gshot.reserve((size_t)nslab * np * nxz * sizeof(float));
         launch_fill(d_gshot.as<float>(), (long long)nslab * np * nxz, 0.f, st);
@@ -1138,6 +1171,11 @@ int fwi2d_engine_set_option(fwi2d_engine* e, const char* key, long long value)
         } else if (k == "shots_per_block") {
             if (value < 1 || value > 64) throw std::runtime_error("shots_per_block must be in [1, 64]");
             x->opt_spb = (int)value;
+        } else if (k == "stiffness_gradients") {
+            x->opt_stiff = value ? 1 : 0;
+        } else if (k == "bf16_checkpoints") {
+            x->opt_bf16 = value ? 1 : 0;
+            x->ckpt_epoch = -1;
         } else if (k == "autotune") {
             x->opt_autotune = value ? 1 : 0;
             x->tuned = false;

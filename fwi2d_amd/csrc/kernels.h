@@ -60,6 +60,7 @@ struct StepArgs {
     float* grad;               // raw gradient slabs [shot][np][nx][ldz]; a workgroup adds its
     long long gslab_stride;    // shots' sum to the slab of its first shot
     const float *ge, *gv2, *gv3;  // VTI: 1 + 2 eps, d c13 / d lambda, d c13 / d mu
+    int stiff;                 // VTI: image c11, c13, c33, c55, rho instead of lambda, mu, rho
 };
 
 struct ImageArgs {
@@ -108,6 +109,11 @@ void launch_sum_shots(const float* per_shot, long long shot_stride, int nshots, 
                       int accumulate, hipStream_t s);
 void launch_fill(float* p, long long n, float v, hipStream_t s);
 void launch_copy(float* dst, const float* src, long long n, hipStream_t s);
+// per-shot max-abs scaled BF16 packing of a time level (amax: nshots words of scratch)
+void launch_pack_bf16(const float* u, unsigned short* q, float* scale, unsigned* amax, long long per_shot, int nshots,
+                      hipStream_t s);
+void launch_unpack_bf16(const unsigned short* q, const float* scale, float* u, long long per_shot, int nshots,
+                        hipStream_t s);
 void launch_dot_f64(const float* a, const float* b, const float* w, long long n, double* partial,
                     int nblocks, hipStream_t s);
 void launch_reduce_f64(const double* partial, int n, double* out, hipStream_t s);

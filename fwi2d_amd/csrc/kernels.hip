@@ -690,11 +690,15 @@ __global__ void __launch_bounds__(kThreads) k_step(const StepArgs a, const int4*
     if (s0 < s1) load_window(s0, tid, t);
 
     float gl[IMG ? S::NQ3 : 1][4], gm[IMG ? S::NQ3 : 1][4], gr[IMG ? S::NQ3 : 1][4];
+    float g3[IMG && VTI ? S::NQ3 : 1][4], g4[IMG && VTI ? S::NQ3 : 1][4];  // VTI stiffness mode
     if constexpr (IMG) {
 #pragma unroll
         for (int m = 0; m < S::NQ3; m++)
 #pragma unroll
-            for (int k = 0; k < 4; k++) gl[m][k] = gm[m][k] = gr[m][k] = 0.f;
+            for (int k = 0; k < 4; k++) {
+                gl[m][k] = gm[m][k] = gr[m][k] = 0.f;
+                if (VTI) g3[m][k] = g4[m][k] = 0.f;
+            }
     }
 
     const TileInfo t0 = t;
@@ -905,13 +909,20 @@ __global__ void __launch_bounds__(kThreads) k_step(const StepArgs a, const int4*
                         gl[m][k] += a.wlm * (axx[k] + azz[k]) * (t0x[k] + t1z[k]);
                         gm[m][k] += a.wlm * ((azx[k] + axz[k]) * (t2x[k] + t3z[k]) + 2.f * axx[k] * t0x[k] +
                                              2.f * azz[k] * t1z[k]);
+                    } else if (a.stiff) {
+                        // d/dc11, d/dc13, d/dc33, d/dc55 of the same bilinear form
+                        gl[m][k] += a.wlm * axx[k] * t0x[k];
+                        gm[m][k] += a.wlm * (axx[k] * t1z[k] + azz[k] * t0x[k]);
+                        gr[m][k] += a.wlm * azz[k] * t1z[k];
+                        g3[m][k] += a.wlm * (azx[k] + axz[k]) * (t2x[k] + t3z[k]);
                     } else {
                         const float cr = axx[k] * t1z[k] + azz[k] * t0x[k];
                         gl[m][k] += a.wlm * (e[k] * axx[k] * t0x[k] + azz[k] * t1z[k] + v2[k] * cr);
                         gm[m][k] += a.wlm * ((azx[k] + axz[k]) * (t2x[k] + t3z[k]) + 2.f * e[k] * axx[k] * t0x[k] +
                                              2.f * azz[k] * t1z[k] + v3[k] * cr);
                     }
-                    gr[m][k] += a.wrho * (ox[k] * d2x + oz[k] * d2z);
+                    if (VTI && a.stiff) g4[m][k] += a.wrho * (ox[k] * d2x + oz[k] * d2z);
+                    else gr[m][k] += a.wrho * (ox[k] * d2x + oz[k] * d2z);
                 }
             }
         }
@@ -931,6 +942,12 @@ __global__ void __launch_bounds__(kThreads) k_step(const StepArgs a, const int4*
             st4(G + nxz + g, make_float4(v.x + gm[m][0], v.y + gm[m][1], v.z + gm[m][2], v.w + gm[m][3]));
             v = ld4(G + 2 * nxz + g);
             st4(G + 2 * nxz + g, make_float4(v.x + gr[m][0], v.y + gr[m][1], v.z + gr[m][2], v.w + gr[m][3]));
+            if (VTI && a.stiff) {
+                v = ld4(G + 3 * nxz + g);
+                st4(G + 3 * nxz + g, make_float4(v.x + g3[m][0], v.y + g3[m][1], v.z + g3[m][2], v.w + g3[m][3]));
+                v = ld4(G + 4 * nxz + g);
+                st4(G + 4 * nxz + g, make_float4(v.x + g4[m][0], v.y + g4[m][1], v.z + g4[m][2], v.w + g4[m][3]));
+            }
         }
     }
 }
@@ -1089,6 +1106,51 @@ __global__ void k_fill(float* p, long long n, float v)
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) p[i] = v;
 }
 
+// BF16 checkpoint levels with a max-abs scale per (level, shot): x = bf16(u / s), u = s x.
+// Scales are kept as the bit patterns of non-negative floats, whose integer order is the
+// float order, so the per-shot maximum is an integer atomic max.
+__global__ void k_absmax_shots(const float* u, long long per_shot, int nshots, unsigned* amax)
+{
+    const int shot = blockIdx.y;
+    const float* p = u + shot * per_shot;
+    float m = 0.f;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < per_shot;
+         i += (long long)gridDim.x * blockDim.x)
+        m = fmaxf(m, fabsf(p[i]));
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0) atomicMax(&amax[shot], __float_as_uint(m));
+}
+
+__device__ __forceinline__ unsigned short to_bf16(float x)
+{
+    const unsigned b = __float_as_uint(x);
+    return (unsigned short)((b + 0x7FFFu + ((b >> 16) & 1u)) >> 16);  // round to nearest even
+}
+
+__global__ void k_pack_bf16(const float* u, unsigned short* q, const unsigned* amax, float* scale, long long per_shot)
+{
+    const int shot = blockIdx.y;
+    const float s = __uint_as_float(amax[shot]);
+    const float inv = s > 0.f ? 1.f / s : 0.f;
+    if (blockIdx.x == 0 && threadIdx.x == 0) scale[shot] = s;
+    const float* p = u + shot * per_shot;
+    unsigned short* d = q + shot * per_shot;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < per_shot;
+         i += (long long)gridDim.x * blockDim.x)
+        d[i] = to_bf16(p[i] * inv);
+}
+
+__global__ void k_unpack_bf16(const unsigned short* q, const float* scale, float* u, long long per_shot)
+{
+    const int shot = blockIdx.y;
+    const float s = scale[shot];
+    const unsigned short* p = q + shot * per_shot;
+    float* d = u + shot * per_shot;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < per_shot;
+         i += (long long)gridDim.x * blockDim.x)
+        d[i] = __uint_as_float((unsigned)p[i] << 16) * s;
+}
+
 __global__ void k_copy(float* dst, const float* src, long long n)
 {
     const long long n4 = n >> 2;
@@ -1212,6 +1274,22 @@ void launch_sum_shots(const float* per_shot, long long shot_stride, int nshots, 
 void launch_fill(float* p, long long n, float v, hipStream_t s)
 {
     hipLaunchKernelGGL(k_fill, dim3(grid_for(n)), dim3(kThreads), 0, s, p, n, v);
+}
+
+void launch_pack_bf16(const float* u, unsigned short* q, float* scale, unsigned* amax, long long per_shot, int nshots,
+                      hipStream_t s)
+{
+    const int gx = (int)std::min<long long>((per_shot + kThreads - 1) / kThreads, 512);
+    hipLaunchKernelGGL(k_fill, dim3(1), dim3(kThreads), 0, s, (float*)amax, (long long)nshots, 0.f);
+    hipLaunchKernelGGL(k_absmax_shots, dim3(gx, nshots), dim3(kThreads), 0, s, u, per_shot, nshots, amax);
+    hipLaunchKernelGGL(k_pack_bf16, dim3(gx, nshots), dim3(kThreads), 0, s, u, q, amax, scale, per_shot);
+}
+
+void launch_unpack_bf16(const unsigned short* q, const float* scale, float* u, long long per_shot, int nshots,
+                        hipStream_t s)
+{
+    const int gx = (int)std::min<long long>((per_shot + kThreads - 1) / kThreads, 512);
+    hipLaunchKernelGGL(k_unpack_bf16, dim3(gx, nshots), dim3(kThreads), 0, s, q, scale, u, per_shot);
 }
 
 void launch_copy(float* dst, const float* src, long long n, hipStream_t s)

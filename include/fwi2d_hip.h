@@ -90,6 +90,10 @@ int fwi2d_engine_dot_f64(fwi2d_engine* e, const float* a, const float* b, const 
    interior tile), "shots_per_block_edge" (the same for tiles at the grid edge),
    "autotune" (1: at the next forward run, time a few steps of every tile_x and
    shots-per-workgroup choice and keep the fastest; forward results are unchanged),
+   "bf16_checkpoints" (1: checkpoint levels stored in BF16 with a max-abs scale per level
+   and shot; halves their memory, the gradient then carries the recomputation error),
+   "stiffness_gradients" (VTI, 1: fwi2d_engine_adjoint returns raw gradients w.r.t.
+   c11, c13, c33, c55, rho instead of lambda, mu, rho, 0, 0),
    "segment" (snapshots per checkpoint segment, 0 = about sqrt(2 nt/sub)), "lag" (1 the
    reference's one-step imaging lag, 0 none), "timing" (1 time the forward step loop with
    HIP events), "graph" (1: capture each forward / adjoint time loop in a hipGraph the

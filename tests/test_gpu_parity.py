@@ -168,6 +168,59 @@ def test_graph_replay_matches_eager(lib):
     assert eng.stat("graph_replays") >= 2
 
 
+@pytest.mark.parametrize("vti", [False, True])
+def test_bf16_checkpoints(lib, vti):
+    """BF16 checkpoint levels (per-shot max-abs scaling) leave the forward traces bit for
+    bit unchanged and move the gradient by the recomputation error they introduce; the
+    error against FP32 checkpoints is printed and bounded."""
+    prob, model, sxz, rxz, src, nt, dt = engine_case(vti=vti, nt=401, ns=2)
+    rng = np.random.default_rng(11)
+    adj = rng.standard_normal((2, sum(len(r) for r in rxz), nt)) * np.hanning(nt)
+    eng = lib.Engine(lib.grid_from(prob, nt, dt, 1))
+    eng.set_option("segment", 9)
+    eng.set_model(model)
+    eng.set_shots(sxz, rxz)
+    d32 = eng.forward(src, keep_checkpoints=True)
+    g32 = eng.adjoint(adj)
+    eng.set_option("bf16_checkpoints", 1)
+    d16 = eng.forward(src, keep_checkpoints=True)
+    g16 = eng.adjoint(adj)
+    assert np.array_equal(d16, d32)
+    for c in range(3):
+        err = rel(g16[c], g32[c].astype(np.float64))
+        print("vti=%s component %d: BF16-checkpoint gradient vs FP32 checkpoints rel err %.2e" % (vti, c, err))
+        assert err < 2e-2
+
+
+def test_vti_stiffness_gradients_chain_rule(lib):
+    """Stiffness mode (c11, c13, c33, c55, rho) against the reference's VTI lambda / mu
+    gradients through the chain rule of c11 = (1+2eps)(lambda+2mu), c33 = lambda+2mu,
+    c55 = mu and c13(lambda, mu, delta) (we_op.cpp:1510-1519)."""
+    prob, model, sxz, rxz, src, nt, dt = engine_case(vti=True, nt=301, ns=2)
+    rng = np.random.default_rng(2)
+    adj = rng.standard_normal((2, sum(len(r) for r in rxz), nt)) * np.hanning(nt)
+    eng = lib.Engine(lib.grid_from(prob, nt, dt, 1))
+    eng.set_model(model)
+    eng.set_shots(sxz, rxz)
+    eng.forward(src, keep_checkpoints=True)
+    g_lm = eng.adjoint(adj).astype(np.float64)
+    eng.set_option("stiffness_gradients", 1)
+    g_c = eng.adjoint(adj).astype(np.float64)
+    l, u, c = (model[i].astype(np.float64) for i in (0, 1, 3))
+    e = 1.0 + 2.0 * model[4].astype(np.float64)
+    dl = ((c + u) ** 2 - (l + u) ** 2) / (2.0 * (l + u) * (l + 2.0 * u))
+    v1 = np.sqrt(2.0 * (l + 2.0 * u) * (l + u) * dl + (l + u) ** 2)
+    v2 = ((1 + 2 * dl) * l + (1 + 3 * dl) * u) / v1
+    v3 = ((1 + 3 * dl) * l + (1 + 4 * dl) * u) / v1 - 1.0
+    g11, g13, g33, g55, grho = g_c
+    checks = [(e * g11 + g33 + v2 * g13, g_lm[0]), (2 * e * g11 + 2 * g33 + v3 * g13 + g55, g_lm[1]),
+              (grho, g_lm[2])]
+    for i, (a_, b_) in enumerate(checks):
+        err = rel(a_, b_)
+        print("chain rule component %d rel err %.2e" % (i, err))
+        assert err < 1e-5
+
+
 def adjoint_case(vti=False, nt=401, sub=1):
     case = list(engine_case(vti=vti, nt=nt, ns=2))
     return case
