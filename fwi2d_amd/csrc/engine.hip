@@ -114,8 +114,8 @@ struct Engine {
 
     // state and checkpoints
     DevMem d_fwd, d_adj, d_ckpt, d_seg, d_gshot, d_gout, d_src, d_rcv, d_adjsrc, d_partial, d_scalar;
-    DevMem d_ckpt16, d_ckpt_scale, d_amax;  // BF16 checkpoint levels, their per-shot scales
-    int opt_bf16 = 0;                       // checkpoint levels stored in BF16 (max-abs scaled)
+    DevMem d_ckpt16, d_ckpt_scale, d_amax;  // 16-bit checkpoint levels, their per-shot scales
+    int opt_bf16 = 0;                       // checkpoint levels: 0 FP32, 1 BF16, 2 FP16 (max-abs scaled)
     int opt_stiff = 0;                      // VTI raw gradients w.r.t. c11, c13, c33, c55, rho
     int seg_K = 0, nseg = 0, nsnap = 0, seg_loaded = -1;
     const float* fwd_src = nullptr;  // source traces of the checkpointed forward run (device)
@@ -669,8 +669,8 @@ struct Engine {
     {
         const long long ss = state_stride();
         if (opt_bf16)
-            launch_pack_bf16(u, d_ckpt16.as<unsigned short>() + k * ss, d_ckpt_scale.as<float>() + (long long)k * ns,
-                             d_amax.as<unsigned>(), 2 * nxz, ns, st);
+            launch_pack16(opt_bf16 == 2, u, d_ckpt16.as<unsigned short>() + k * ss,
+                          d_ckpt_scale.as<float>() + (long long)k * ns, d_amax.as<unsigned>(), 2 * nxz, ns, st);
         else
             launch_copy(d_ckpt.as<float>() + k * ss, u, ss, st);
     }
@@ -678,8 +678,8 @@ struct Engine {
     {
         const long long ss = state_stride();
         if (opt_bf16)
-            launch_unpack_bf16(d_ckpt16.as<unsigned short>() + k * ss, d_ckpt_scale.as<float>() + (long long)k * ns,
-                               u, 2 * nxz, ns, st);
+            launch_unpack16(opt_bf16 == 2, d_ckpt16.as<unsigned short>() + k * ss,
+                            d_ckpt_scale.as<float>() + (long long)k * ns, u, 2 * nxz, ns, st);
         else
             launch_copy(u, d_ckpt.as<float>() + k * ss, ss, st);
     }
@@ -1173,8 +1173,9 @@ int fwi2d_engine_set_option(fwi2d_engine* e, const char* key, long long value)
             x->opt_spb = (int)value;
         } else if (k == "stiffness_gradients") {
             x->opt_stiff = value ? 1 : 0;
-        } else if (k == "bf16_checkpoints") {
-            x->opt_bf16 = value ? 1 : 0;
+        } else if (k == "checkpoint_format") {
+            if (value < 0 || value > 2) throw std::runtime_error("checkpoint_format must be 0 (fp32), 1 (bf16) or 2 (fp16)");
+            x->opt_bf16 = (int)value;
             x->ckpt_epoch = -1;
         } else if (k == "autotune") {
             x->opt_autotune = value ? 1 : 0;

@@ -109,11 +109,12 @@ void launch_sum_shots(const float* per_shot, long long shot_stride, int nshots, 
                       int accumulate, hipStream_t s);
 void launch_fill(float* p, long long n, float v, hipStream_t s);
 void launch_copy(float* dst, const float* src, long long n, hipStream_t s);
-// per-shot max-abs scaled BF16 packing of a time level (amax: nshots words of scratch)
-void launch_pack_bf16(const float* u, unsigned short* q, float* scale, unsigned* amax, long long per_shot, int nshots,
-                      hipStream_t s);
-void launch_unpack_bf16(const unsigned short* q, const float* scale, float* u, long long per_shot, int nshots,
-                        hipStream_t s);
+// per-shot max-abs scaled 16-bit packing of a time level, FP16 (f16) or BF16 (amax:
+// nshots words of scratch)
+void launch_pack16(bool f16, const float* u, unsigned short* q, float* scale, unsigned* amax, long long per_shot,
+                   int nshots, hipStream_t s);
+void launch_unpack16(bool f16, const unsigned short* q, const float* scale, float* u, long long per_shot, int nshots,
+                     hipStream_t s);
 void launch_dot_f64(const float* a, const float* b, const float* w, long long n, double* partial,
                     int nblocks, hipStream_t s);
 void launch_reduce_f64(const double* partial, int n, double* out, hipStream_t s);

@@ -12,6 +12,7 @@
 // the anchored tile, and a halo of 4 cells holds every stencil input.
 #include "kernels.h"
 #include "sbp_tables.h"
+#include <hip/hip_fp16.h>
 
 namespace fwi2d {
 
@@ -1106,7 +1107,8 @@ __global__ void k_fill(float* p, long long n, float v)
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) p[i] = v;
 }
 
-// BF16 checkpoint levels with a max-abs scale per (level, shot): x = bf16(u / s), u = s x.
+// 16-bit checkpoint levels (BF16 or FP16) with a max-abs scale per (level, shot):
+// x = round16(u / s), u = s x.
 // Scales are kept as the bit patterns of non-negative floats, whose integer order is the
 // float order, so the per-shot maximum is an integer atomic max.
 __global__ void k_absmax_shots(const float* u, long long per_shot, int nshots, unsigned* amax)
@@ -1127,7 +1129,8 @@ __device__ __forceinline__ unsigned short to_bf16(float x)
     return (unsigned short)((b + 0x7FFFu + ((b >> 16) & 1u)) >> 16);  // round to nearest even
 }
 
-__global__ void k_pack_bf16(const float* u, unsigned short* q, const unsigned* amax, float* scale, long long per_shot)
+template <bool F16>
+__global__ void k_pack16(const float* u, unsigned short* q, const unsigned* amax, float* scale, long long per_shot)
 {
     const int shot = blockIdx.y;
     const float s = __uint_as_float(amax[shot]);
@@ -1137,10 +1140,11 @@ __global__ void k_pack_bf16(const float* u, unsigned short* q, const unsigned* a
     unsigned short* d = q + shot * per_shot;
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < per_shot;
          i += (long long)gridDim.x * blockDim.x)
-        d[i] = to_bf16(p[i] * inv);
+        d[i] = F16 ? __half_as_ushort(__float2half_rn(p[i] * inv)) : to_bf16(p[i] * inv);
 }
 
-__global__ void k_unpack_bf16(const unsigned short* q, const float* scale, float* u, long long per_shot)
+template <bool F16>
+__global__ void k_unpack16(const unsigned short* q, const float* scale, float* u, long long per_shot)
 {
     const int shot = blockIdx.y;
     const float s = scale[shot];
@@ -1148,7 +1152,7 @@ __global__ void k_unpack_bf16(const unsigned short* q, const float* scale, float
     float* d = u + shot * per_shot;
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < per_shot;
          i += (long long)gridDim.x * blockDim.x)
-        d[i] = __uint_as_float((unsigned)p[i] << 16) * s;
+        d[i] = (F16 ? __half2float(__ushort_as_half(p[i])) : __uint_as_float((unsigned)p[i] << 16)) * s;
 }
 
 __global__ void k_copy(float* dst, const float* src, long long n)
@@ -1276,20 +1280,22 @@ void launch_fill(float* p, long long n, float v, hipStream_t s)
     hipLaunchKernelGGL(k_fill, dim3(grid_for(n)), dim3(kThreads), 0, s, p, n, v);
 }
 
-void launch_pack_bf16(const float* u, unsigned short* q, float* scale, unsigned* amax, long long per_shot, int nshots,
-                      hipStream_t s)
+void launch_pack16(bool f16, const float* u, unsigned short* q, float* scale, unsigned* amax, long long per_shot,
+                   int nshots, hipStream_t s)
 {
     const int gx = (int)std::min<long long>((per_shot + kThreads - 1) / kThreads, 512);
     hipLaunchKernelGGL(k_fill, dim3(1), dim3(kThreads), 0, s, (float*)amax, (long long)nshots, 0.f);
     hipLaunchKernelGGL(k_absmax_shots, dim3(gx, nshots), dim3(kThreads), 0, s, u, per_shot, nshots, amax);
-    hipLaunchKernelGGL(k_pack_bf16, dim3(gx, nshots), dim3(kThreads), 0, s, u, q, amax, scale, per_shot);
+    if (f16) hipLaunchKernelGGL(k_pack16<true>, dim3(gx, nshots), dim3(kThreads), 0, s, u, q, amax, scale, per_shot);
+    else hipLaunchKernelGGL(k_pack16<false>, dim3(gx, nshots), dim3(kThreads), 0, s, u, q, amax, scale, per_shot);
 }
 
-void launch_unpack_bf16(const unsigned short* q, const float* scale, float* u, long long per_shot, int nshots,
-                        hipStream_t s)
+void launch_unpack16(bool f16, const unsigned short* q, const float* scale, float* u, long long per_shot, int nshots,
+                     hipStream_t s)
 {
     const int gx = (int)std::min<long long>((per_shot + kThreads - 1) / kThreads, 512);
-    hipLaunchKernelGGL(k_unpack_bf16, dim3(gx, nshots), dim3(kThreads), 0, s, q, scale, u, per_shot);
+    if (f16) hipLaunchKernelGGL(k_unpack16<true>, dim3(gx, nshots), dim3(kThreads), 0, s, q, scale, u, per_shot);
+    else hipLaunchKernelGGL(k_unpack16<false>, dim3(gx, nshots), dim3(kThreads), 0, s, q, scale, u, per_shot);
 }
 
 void launch_copy(float* dst, const float* src, long long n, hipStream_t s)

@@ -90,8 +90,9 @@ int fwi2d_engine_dot_f64(fwi2d_engine* e, const float* a, const float* b, const 
    interior tile), "shots_per_block_edge" (the same for tiles at the grid edge),
    "autotune" (1: at the next forward run, time a few steps of every tile_x and
    shots-per-workgroup choice and keep the fastest; forward results are unchanged),
-   "bf16_checkpoints" (1: checkpoint levels stored in BF16 with a max-abs scale per level
-   and shot; halves their memory, the gradient then carries the recomputation error),
+   "checkpoint_format" (0 FP32; 1 BF16, 2 FP16: checkpoint levels stored in 16 bits with
+   a max-abs scale per level and shot, which halves their memory; the gradient then
+   carries the error of the recomputation from rounded levels),
    "stiffness_gradients" (VTI, 1: fwi2d_engine_adjoint returns raw gradients w.r.t.
    c11, c13, c33, c55, rho instead of lambda, mu, rho, 0, 0),
    "segment" (snapshots per checkpoint segment, 0 = about sqrt(2 nt/sub)), "lag" (1 the
