@@ -93,7 +93,7 @@ struct Engine {
     int opt_autotune = 0;  // pick tile_x and shots per workgroup by timing steps at the next forward run
     bool tuned = false;
     std::string tune_report;
-    int opt_spb = 4;       // shots advanced one after the other by each interior-tile workgroup
+    int opt_spb = 8;       // shots advanced one after the other by each core-box workgroup
     int opt_spb_edge = 1;  // the same for edge tiles (their workgroups are latency-bound)
     hipStream_t st_edge = nullptr;       // edge-tile launches run beside the interior ones
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
@@ -772,7 +772,7 @@ struct Engine {
         tune_report.clear();
         const long long steps0 = stat_steps;
         for (int tx : {16, 32})
-            for (int spb : {1, 2, 4, 8}) {
+            for (int spb : {1, 2, 4, 8, 16}) {
                 if (spb > 1 && spb > ns) continue;
                 for (int spe : {1, 2}) {
                     if (spe > 1 && spe > ns) continue;
