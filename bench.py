@@ -288,7 +288,8 @@ def main():
     # ---- dominant kernel: fused step, timed with HIP events on its stream ----
     # (eager launches: one step = the core-box launch plus the edge-band launches beside it)
     tune = {"tile_x": B.eng.stat("tile_x"), "shots_per_block": B.eng.stat("shots_per_block"),
-            "shots_per_block_edge": B.eng.stat("shots_per_block_edge"), "report": B.eng.tune_report()}
+            "shots_per_block_edge": B.eng.stat("shots_per_block_edge"), "edge_stream": B.eng.stat("edge_stream"),
+            "report": B.eng.tune_report()}
     graph_replays = B.eng.stat("graph_replays")
     B.eng.set_option("graph", 0)
     B.eng.set_option("timing", 1)

@@ -71,8 +71,8 @@ struct Engine {
     long long nxz = 0;    // nx * ldz, the device size of one field
     int tile_x = 16, ntx = 0, ntz = 0;
     std::vector<int4> boxes;                        // {sx, ex, sz, ez} of every step box
-    std::vector<int> list_core, list_bz, list_bx;   // box indices per box class
-    DevMem d_boxes, d_list_core, d_list_bz, d_list_bx;
+    std::vector<int> list_core, list_bz, list_bz8, list_bx;  // box indices per box class
+    DevMem d_boxes, d_list_core, d_list_bz, d_list_bz8, d_list_bx;
     int opt_tile_x = 16, opt_segment = 0, opt_lag = 1, opt_graph = 0, opt_timing = 0;
     // hipGraph replay of whole time loops: a loop is captured once per configuration
     // (every pointer and size it uses) and replayed while that configuration holds
@@ -95,6 +95,7 @@ struct Engine {
     std::string tune_report;
     int opt_spb = 8;       // shots advanced one after the other by each core-box workgroup
     int opt_spb_edge = 1;  // the same for edge tiles (their workgroups are latency-bound)
+    int opt_edge_stream = 1;  // edge-band launches on their own stream
     hipStream_t st_edge = nullptr;       // edge-tile launches run beside the interior ones
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     double last_fwd_loop_ms = 0.0;  // HIP-event time of the last forward step loop (timing option)
@@ -182,6 +183,7 @@ struct Engine {
         boxes.clear();
         list_core.clear();
         list_bz.clear();
+        list_bz8.clear();
         list_bx.clear();
         const int nx = g.nx, nz = g.nz;
         const int cx0 = kCoreX0, cx1 = nx - kCoreX0;
@@ -200,13 +202,20 @@ struct Engine {
                 add(list_core, std::max(sx, cx0), std::min(ex, cx1), std::max(sz, cz0), std::min(ez, cz1));
             }
         }
-        const int nbx = (nx + 63) / 64;
-        for (int k = 0; k < nbx; k++) {
-            int sx, ex;
-            tile_range_x(k, nbx, 64, nx, sx, ex);
-            add(list_bz, sx, ex, 0, cz0);
-            add(list_bz, sx, ex, cz1, nz);
-        }
+        // top and bottom bands: 8-row bands in 128 x 8 boxes, taller ones (the bottom band
+        // holds 8 to 11 rows) in 64 x 16 boxes
+        auto zband = [&](int sz, int ez) {
+            const bool thin = ez - sz <= 8;
+            const int w = thin ? 128 : 64;
+            const int nb = (nx + w - 1) / w;
+            for (int k = 0; k < nb; k++) {
+                int sx, ex;
+                tile_range_x(k, nb, w, nx, sx, ex);
+                add(thin ? list_bz8 : list_bz, sx, ex, sz, ez);
+            }
+        };
+        zband(0, cz0);
+        zband(cz1, nz);
         for (int sz = cz0; sz < cz1; sz += 128) {
             const int ez = std::min(sz + 128, cz1);
             add(list_bx, 0, cx0, sz, ez);
@@ -219,6 +228,7 @@ struct Engine {
         up(d_boxes, boxes.data(), boxes.size() * sizeof(int4));
         up(d_list_core, list_core.data(), list_core.size() * sizeof(int));
         up(d_list_bz, list_bz.data(), list_bz.size() * sizeof(int));
+        up(d_list_bz8, list_bz8.data(), list_bz8.size() * sizeof(int));
         up(d_list_bx, list_bx.data(), list_bx.size() * sizeof(int));
         HIPCHECK(hipStreamSynchronize(st));
     }
@@ -628,20 +638,25 @@ struct Engine {
             a.ext_trace = rcv;
             a.ext_comp_stride = (long long)nr * g.nt;
         }
-        // edge bands on their own stream, concurrent with the core boxes (disjoint cells)
+        // edge bands on their own stream, concurrent with the core boxes (disjoint cells),
+        // or after them on the same stream
         StepArgs e = a;
         e.spb = std::max(1, std::min(opt_spb_edge, ns));
         e.ngroups = (ns + e.spb - 1) / e.spb;
         const bool img = img_idx >= 0;
         const int4* bx = d_boxes.as<int4>();
-        HIPCHECK(hipEventRecord(ev_fork, st));
-        HIPCHECK(hipStreamWaitEvent(st_edge, ev_fork, 0));
-        launch_step(e, vti, BoxClass::band_z, bx, d_list_bz.as<int>(), (int)list_bz.size(), img, st_edge);
-        launch_step(e, vti, BoxClass::band_x, bx, d_list_bx.as<int>(), (int)list_bx.size(), img, st_edge);
-        HIPCHECK(hipEventRecord(ev_join, st_edge));
+        hipStream_t se = opt_edge_stream ? st_edge : st;
+        if (opt_edge_stream) {
+            HIPCHECK(hipEventRecord(ev_fork, st));
+            HIPCHECK(hipStreamWaitEvent(st_edge, ev_fork, 0));
+        }
+        launch_step(e, vti, BoxClass::band_z8, bx, d_list_bz8.as<int>(), (int)list_bz8.size(), img, se);
+        launch_step(e, vti, BoxClass::band_z, bx, d_list_bz.as<int>(), (int)list_bz.size(), img, se);
+        launch_step(e, vti, BoxClass::band_x, bx, d_list_bx.as<int>(), (int)list_bx.size(), img, se);
+        if (opt_edge_stream) HIPCHECK(hipEventRecord(ev_join, st_edge));
         launch_step(a, vti, tile_x == 32 ? BoxClass::core32 : BoxClass::core16, bx, d_list_core.as<int>(),
                     (int)list_core.size(), img, st);
-        HIPCHECK(hipStreamWaitEvent(st, ev_join, 0));
+        if (opt_edge_stream) HIPCHECK(hipStreamWaitEvent(st, ev_join, 0));
         L.rotate();
         stat_steps++;
     }
@@ -702,12 +717,12 @@ struct Engine {
     {
         auto P = [](const void* p) { return (long long)(uintptr_t)p; };
         return {kind, P(p0), P(p1), flag, g.nt, ns, nr, tile_x, opt_spb, opt_spb_edge, opt_lag, seg_K, nseg,
-                opt_bf16, opt_stiff, P(d_ckpt16.p), P(d_ckpt_scale.p), P(d_amax.p),
+                opt_bf16, opt_stiff, opt_edge_stream, P(d_ckpt16.p), P(d_ckpt_scale.p), P(d_amax.p),
                 P(d_fwd.p), P(d_adj.p), P(d_ckpt.p), P(d_seg.p), P(d_gshot.p), P(d_model.p), P(d_deriv.p),
-                P(d_boxes.p), P(d_list_core.p), P(d_list_bz.p), P(d_list_bx.p), P(fwd_inj.d_ptr.p), P(fwd_inj.d_idx.p),
+                P(d_boxes.p), P(d_list_core.p), P(d_list_bz.p), P(d_list_bz8.p), P(d_list_bx.p), P(fwd_inj.d_ptr.p), P(fwd_inj.d_idx.p),
                 P(fwd_ext.d_ptr.p), P(fwd_ext.d_idx.p), P(adj_inj.d_ptr.p), P(adj_inj.d_idx.p), P(d_src_ents.p),
                 P(d_rcv_ents.p), P(d_rcv_ext_ents.p), P(d_src_shot_ptr.p), P(d_rcv_shot_ptr.p), P(d_tx.p), P(d_tz.p),
-                P(d_div_tb.p), P(d_div_lr.p), (long long)list_core.size(), (long long)list_bz.size(),
+                P(d_div_tb.p), P(d_div_lr.p), (long long)list_core.size(), (long long)list_bz.size(), (long long)list_bz8.size(),
                 (long long)list_bx.size(), (long long)boxes.size()};
     }
 
@@ -771,36 +786,40 @@ struct Engine {
         int btx = opt_tile_x, bspb = opt_spb, bspe = opt_spb_edge;
         tune_report.clear();
         const long long steps0 = stat_steps;
+        int bes = opt_edge_stream;
         for (int tx : {16, 32})
             for (int spb : {1, 2, 4, 8, 16}) {
                 if (spb > 1 && spb > ns) continue;
-                for (int spe : {1, 2}) {
-                    if (spe > 1 && spe > ns) continue;
-                    set_tile(tx);
-                    opt_spb = spb;
-                    opt_spb_edge = spe;
-                    Levels L = levels_in(d_fwd);
-                    init_levels(L, false, dsrc);
-                    for (int it = 0; it < nsteps; it++) {
-                        if (it == nsteps - timed) HIPCHECK(hipEventRecord(e0, st));
-                        step(L, it, false, dsrc, nullptr);
+                for (int spe : {1, 2, 4})
+                    for (int es : {1, 0}) {
+                        if (spe > 1 && spe > ns) continue;
+                        set_tile(tx);
+                        opt_spb = spb;
+                        opt_spb_edge = spe;
+                        opt_edge_stream = es;
+                        Levels L = levels_in(d_fwd);
+                        init_levels(L, false, dsrc);
+                        for (int it = 0; it < nsteps; it++) {
+                            if (it == nsteps - timed) HIPCHECK(hipEventRecord(e0, st));
+                            step(L, it, false, dsrc, nullptr);
+                        }
+                        HIPCHECK(hipEventRecord(e1, st));
+                        HIPCHECK(hipEventSynchronize(e1));
+                        float ms = 0.f;
+                        HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+                        const double us = 1e3 * ms / std::min(timed, nsteps);
+                        char buf[112];
+                        snprintf(buf, sizeof(buf), "%stile_x=%d spb=%d spb_edge=%d edge_stream=%d: %.1f us",
+                                 tune_report.empty() ? "" : "; ", tx, spb, spe, es, us);
+                        tune_report += buf;
+                        if (us < best) {
+                            best = us;
+                            btx = tx;
+                            bspb = spb;
+                            bspe = spe;
+                            bes = es;
+                        }
                     }
-                    HIPCHECK(hipEventRecord(e1, st));
-                    HIPCHECK(hipEventSynchronize(e1));
-                    float ms = 0.f;
-                    HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
-                    const double us = 1e3 * ms / std::min(timed, nsteps);
-                    char buf[96];
-                    snprintf(buf, sizeof(buf), "%stile_x=%d spb=%d spb_edge=%d: %.1f us", tune_report.empty() ? "" : "; ",
-                             tx, spb, spe, us);
-                    tune_report += buf;
-                    if (us < best) {
-                        best = us;
-                        btx = tx;
-                        bspb = spb;
-                        bspe = spe;
-                    }
-                }
             }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
@@ -808,6 +827,7 @@ struct Engine {
         opt_tile_x = btx;
         opt_spb = bspb;
         opt_spb_edge = bspe;
+        opt_edge_stream = bes;
         stat_steps = steps0;
         tuned = true;
     }
@@ -1177,6 +1197,8 @@ int fwi2d_engine_set_option(fwi2d_engine* e, const char* key, long long value)
             if (value < 0 || value > 2) throw std::runtime_error("checkpoint_format must be 0 (fp32), 1 (bf16) or 2 (fp16)");
             x->opt_bf16 = (int)value;
             x->ckpt_epoch = -1;
+        } else if (k == "edge_stream") {
+            x->opt_edge_stream = value ? 1 : 0;
         } else if (k == "autotune") {
             x->opt_autotune = value ? 1 : 0;
             x->tuned = false;
@@ -1202,6 +1224,7 @@ long long fwi2d_engine_get_stat(fwi2d_engine* e, const char* key)
     if (k == "tile_x") return x->tile_x;
     if (k == "shots_per_block") return x->opt_spb;
     if (k == "shots_per_block_edge") return x->opt_spb_edge;
+    if (k == "edge_stream") return x->opt_edge_stream;
     if (k == "ntiles") return x->ntiles();
     if (k == "interior_tiles") return (long long)x->list_core.size();
     if (k == "graph_captures") return x->graph_captures;

@@ -88,13 +88,12 @@ void tile_range_x(int k, int ntile, int T, int n, int& s, int& e);
 void tile_range_z(int k, int ntile, int nz, int ldz, int& s, int& e);
 
 // Box shapes of the step decomposition: core boxes (TX x 64, every stencil interior) and
-// edge-band boxes (64 x 16 along the top and bottom, 8 x 128 along the sides).
-enum class BoxClass { core16, core32, band_z, band_x };
+// edge-band boxes (128 x 8 for 8-row bands and 64 x 16 for taller ones along the top and
+// bottom, 8 x 128 along the sides).
+enum class BoxClass { core16, core32, band_z, band_x, band_z8 };
 constexpr int kCoreX0 = 6;   // core: x in [6, nx - 6), z in [8, core_z_end(nz))
 constexpr int kCoreZ0 = 8;
 inline int core_z_end(int nz) { return ((nz - 8) / 4) * 4; }
-inline int box_width(BoxClass c) { return c == BoxClass::core16 ? 16 : c == BoxClass::core32 ? 32 : c == BoxClass::band_z ? 64 : 8; }
-inline int box_height(BoxClass c) { return c == BoxClass::band_z ? 16 : c == BoxClass::band_x ? 128 : 64; }
 
 // n boxes listed in list (indices into boxes: {sx, ex, sz, ez}) of one class
 void launch_step(const StepArgs& a, bool vti, BoxClass c, const int4* boxes, const int* list, int n, bool image,

@@ -175,12 +175,36 @@ __device__ __forceinline__ float d2_closure(const float* pu, const float* pc, in
     const int dir = top ? s : -s;
     const float* qu = top ? pu - g * s : pu + r * s;
     const float* qc = top ? pc - g * s : pc + r * s;
+#pragma unroll 1
     for (int j = 0; j < 8; j++) {
         float val = 0.f;
         for (int k = 0; k < 8; k++) val += cD2B[r][j][k] * qc[k * dir];
         out += val * qu[j * dir];
     }
     return out;
+}
+
+// closure rows of D2(c) for two fields sharing the coefficient (weights formed once)
+__device__ __forceinline__ void d2_closure2(const float* pu, const float* pv, const float* pc, int s, int g, int n,
+                                            float& ou, float& ov)
+{
+    const bool top = g < 6;
+    const int r = top ? g : n - 1 - g;
+    const int dir = top ? s : -s;
+    const int o = top ? -g * s : r * s;
+    const float* qu = pu + o;
+    const float* qv = pv + o;
+    const float* qc = pc + o;
+    float cu = 0.f, cv = 0.f;
+#pragma unroll 1
+    for (int j = 0; j < 8; j++) {
+        float val = 0.f;
+        for (int k = 0; k < 8; k++) val += cD2B[r][j][k] * qc[k * dir];
+        cu += val * qu[j * dir];
+        cv += val * qv[j * dir];
+    }
+    ou = cu;
+    ov = cv;
 }
 
 // one-sided outward boundary derivative sum_k SC[k] u[k*s] (times h)
@@ -271,6 +295,18 @@ __device__ __forceinline__ void load_col12(const float* base, int i, float* c)
 }
 
 // D2(c) u at global index g of n along stride s: interior weights or closure rows
+__device__ __forceinline__ void d2_any2(const float* pu, const float* pv, const float* pc, int s, int g, int n,
+                                        float& ou, float& ov)
+{
+    if (g >= 6 && g < n - 6) {
+        const W5 w = d2_weights(pc, s);
+        ou = d2_apply(w, pu, s);
+        ov = d2_apply(w, pv, s);
+    } else {
+        d2_closure2(pu, pv, pc, s, g, n, ou, ov);
+    }
+}
+
 __device__ __forceinline__ float d2_any(const float* pu, const float* pc, int s, int g, int n)
 {
     if (g >= 6 && g < n - 6) return d2_apply(d2_weights(pc, s), pu, s);
@@ -300,6 +336,7 @@ __device__ __forceinline__ void product_quad(const StepArgs& a, const StepSmem& 
             dxuz[k] = (cD1I[0] * (Z[2][k] - Z[1][k]) + cD1I[1] * (Z[3][k] - Z[0][k])) * a.rdx;
         }
     } else if constexpr (EDGE) {
+#pragma unroll 1
         for (int k = 0; k < 4; k++) {
             dxux[k] = d1_closure(&sm.ux[w + k], S::WZ, gx, nx) * a.rdx;
             dxuz[k] = d1_closure(&sm.uz[w + k], S::WZ, gx, nx) * a.rdx;
@@ -317,6 +354,7 @@ __device__ __forceinline__ void product_quad(const StepArgs& a, const StepSmem& 
             dzuz[k] *= a.rdz;
         }
     } else if constexpr (EDGE) {
+#pragma unroll 1
         for (int k = 0; k < 4; k++) {
             const int gz = z0 + k;
             const bool in = gz >= 0 && gz < nz;
@@ -434,9 +472,17 @@ __device__ __forceinline__ void rhs_quad(const StepArgs& a, const StepSmem& sm, 
             rz[k] += (cD1I[0] * (Q[2][k] - Q[1][k]) + cD1I[1] * (Q[3][k] - Q[0][k])) * a.rdx;
         }
     } else if constexpr (EDGE) {
+#pragma unroll 1
         for (int k = 0; k < 4; k++) {
-            rx[k] = 2.f * d2_closure(&sm.ux[w + k], VTI ? &sm.mux[w + k] : &sm.mu[w + k], S::WZ, gx, nx) * a.rdx2;
-            rz[k] = d2_closure(&sm.uz[w + k], &sm.mu[w + k], S::WZ, gx, nx) * a.rdx2;
+            if (VTI) {
+                rx[k] = 2.f * d2_closure(&sm.ux[w + k], &sm.mux[w + k], S::WZ, gx, nx) * a.rdx2;
+                rz[k] = d2_closure(&sm.uz[w + k], &sm.mu[w + k], S::WZ, gx, nx) * a.rdx2;
+            } else {
+                float dxx, dxz;
+                d2_closure2(&sm.ux[w + k], &sm.uz[w + k], &sm.mu[w + k], S::WZ, gx, nx, dxx, dxz);
+                rx[k] = 2.f * dxx * a.rdx2;
+                rz[k] = dxz * a.rdx2;
+            }
             rx[k] += d1<false>(&sm.px[q + k], S::PZ, gx, nx) * a.rdx;
             rz[k] += d1<false>(&sm.qz[q + k], S::PZ, gx, nx) * a.rdx;
         }
@@ -470,17 +516,21 @@ __device__ __forceinline__ void rhs_quad(const StepArgs& a, const StepSmem& sm, 
             rx[k] += dqz[k] * a.rdz;
         }
     } else if constexpr (EDGE) {
+#pragma unroll 1
         for (int k = 0; k < 4; k++) {
             const int gz = z0 + k;
             if (gz >= nz) continue;
-            rz[k] += 2.f * d2_any(&sm.uz[w + k], &sm.mu[w + k], 1, gz, nz) * a.rdz2;
-            rx[k] += d2_any(&sm.ux[w + k], &sm.mu[w + k], 1, gz, nz) * a.rdz2;
+            float dzz, dzx;
+            d2_any2(&sm.uz[w + k], &sm.ux[w + k], &sm.mu[w + k], 1, gz, nz, dzz, dzx);
+            rz[k] += 2.f * dzz * a.rdz2;
+            rx[k] += dzx * a.rdz2;
             rz[k] += d1<false>(VTI ? &sm.pz[q + k] : &sm.px[q + k], 1, gz, nz) * a.rdz;
             rx[k] += d1<false>(&sm.qx[q + k], 1, gz, nz) * a.rdz;
         }
     }
     // SAT lines
     if (EDGE && (gx == 0 || gx == nx - 1 || z0 == 0 || z0 + 3 >= nz - 1)) {
+#pragma unroll 1
         for (int k = 0; k < 4; k++) {
             const int gz = z0 + k;
             if (gz >= nz) continue;
@@ -510,6 +560,7 @@ __device__ __forceinline__ void quad_derivs(const float* u, const float* v, int 
             xv[k] = (cD1I[0] * (V[2][k] - V[1][k]) + cD1I[1] * (V[3][k] - V[0][k])) * rdx;
         }
     } else if constexpr (EDGE) {
+#pragma unroll 1
         for (int k = 0; k < 4; k++) {
             xu[k] = d1_closure(&u[w + k], CS, gx, nx) * rdx;
             xv[k] = d1_closure(&v[w + k], CS, gx, nx) * rdx;
@@ -527,6 +578,7 @@ __device__ __forceinline__ void quad_derivs(const float* u, const float* v, int 
             zv[k] *= rdz;
         }
     } else if constexpr (EDGE) {
+#pragma unroll 1
         for (int k = 0; k < 4; k++) {
             const int gz = z0 + k;
             const bool in = gz < nz;
@@ -1223,6 +1275,7 @@ static void step_class(const StepArgs& a, BoxClass c, const int4* boxes, const i
     case BoxClass::core16: step_launch<VTI, 16, 64, false>(a, boxes, list, n, image, s); break;
     case BoxClass::core32: step_launch<VTI, 32, 64, false>(a, boxes, list, n, image, s); break;
     case BoxClass::band_z: step_launch<VTI, 64, 16, true>(a, boxes, list, n, image, s); break;
+    case BoxClass::band_z8: step_launch<VTI, 128, 8, true>(a, boxes, list, n, image, s); break;
     case BoxClass::band_x: step_launch<VTI, 8, 128, true>(a, boxes, list, n, image, s); break;
     }
 }

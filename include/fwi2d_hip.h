@@ -88,6 +88,7 @@ int fwi2d_engine_dot_f64(fwi2d_engine* e, const float* a, const float* b, const 
 /* Tuning and test options (no reference counterpart): "tile_x" (16 | 32 cells per tile
    along x), "shots_per_block" (1..64 shots advanced in turn by one step workgroup of an
    interior tile), "shots_per_block_edge" (the same for tiles at the grid edge),
+   "edge_stream" (1: edge-band launches run on a second stream beside the core launch),
    "autotune" (1: at the next forward run, time a few steps of every tile_x and
    shots-per-workgroup choice and keep the fastest; forward results are unchanged),
    "checkpoint_format" (0 FP32; 1 BF16, 2 FP16: checkpoint levels stored in 16 bits with
@@ -99,7 +100,7 @@ int fwi2d_engine_dot_f64(fwi2d_engine* e, const float* a, const float* b, const 
    reference's one-step imaging lag, 0 none), "timing" (1 time the forward step loop with
    HIP events), "graph" (1: capture each forward / adjoint time loop in a hipGraph the
    first time it runs in a configuration and replay it afterwards).  Statistics: "steps", "recompute_steps",
-   "image_calls", "segment", "nseg", "tile_x", "shots_per_block", "shots_per_block_edge",
+   "image_calls", "segment", "nseg", "tile_x", "shots_per_block", "shots_per_block_edge", "edge_stream",
    "ntiles", "interior_tiles", "graph_captures", "graph_replays", "nrec",
    "nshots", "fwd_loop_ns", "fwd_loop_steps"; -1 for an unknown key. */
 int fwi2d_engine_set_option(fwi2d_engine* e, const char* key, long long value);

@@ -17,7 +17,7 @@ import pytest
 TZ, H_IMG = 64, 2
 CORE_X0, CORE_Z0 = 6, 8
 SHAPES = {"core16": (16, 64, False), "core32": (32, 64, False), "band_z": (64, 16, True),
-          "band_x": (8, 128, True)}
+          "band_z8": (128, 8, True), "band_x": (8, 128, True)}
 
 
 def tile_range(k, ntile, T, n):
@@ -99,9 +99,10 @@ def build_boxes(nx, nz, tile_x):
     for sx, ex in tiles(nx, tile_x):
         for sz, ez in tiles_quads(nz, TZ):
             add(core, max(sx, cx0), min(ex, cx1), max(sz, cz0), min(ez, cz1))
-    for sx, ex in tiles(nx, 64):
-        add("band_z", sx, ex, 0, cz0)
-        add("band_z", sx, ex, cz1, nz)
+    for sz, ez in ((0, cz0), (cz1, nz)):
+        thin = ez - sz <= 8
+        for sx, ex in tiles(nx, 128 if thin else 64):
+            add("band_z8" if thin else "band_z", sx, ex, sz, ez)
     for sz in range(cz0, cz1, 128):
         ez = min(sz + 128, cz1)
         add("band_x", 0, cx0, sz, ez)
