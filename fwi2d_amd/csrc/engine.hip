@@ -48,16 +48,56 @@ struct DevMem {
     ~DevMem() { release(); }
 };
 
-// CSR of point entries per (shot, tile) -- host build, device copy
-struct TileLists {
-    std::vector<int> ptr, idx;
-    DevMem d_ptr, d_idx;
-    void upload(hipStream_t s)
+// Cells touched by the point injectors of all shots, with their contributions (trace
+// offset of the x component, weight) in injector order -- host build, device copy
+struct InjList {
+    std::vector<InjCell> cells;
+    std::vector<int> ptr;
+    std::vector<long long> ctrace;
+    std::vector<float> cw;
+    DevMem d_cells, d_ptr, d_ctrace, d_cw;
+    void build(const std::vector<PointEntry>& ents, hipStream_t s)
     {
-        d_ptr.reserve(std::max<size_t>(ptr.size(), 1) * sizeof(int));
-        d_idx.reserve(std::max<size_t>(idx.size(), 1) * sizeof(int));
-        if (!ptr.empty()) HIPCHECK(hipMemcpyAsync(d_ptr.p, ptr.data(), ptr.size() * sizeof(int), hipMemcpyHostToDevice, s));
-        if (!idx.empty()) HIPCHECK(hipMemcpyAsync(d_idx.p, idx.data(), idx.size() * sizeof(int), hipMemcpyHostToDevice, s));
+        struct C {
+            int shot, gx, gz, k;
+            long long trace;
+            float w;
+        };
+        std::vector<C> all;
+        all.reserve(ents.size() * 9);
+        for (int k = 0; k < (int)ents.size(); k++) {
+            const PointEntry& e = ents[k];
+            for (int ix = 0; ix < 3; ix++)
+                for (int iz = 0; iz < 3; iz++)
+                    all.push_back({e.shot, e.xi0 + ix, e.zi0 + iz, k, e.trace, e.wx[ix] * e.wz[iz]});
+        }
+        std::stable_sort(all.begin(), all.end(), [](const C& a, const C& b) {
+            if (a.shot != b.shot) return a.shot < b.shot;
+            if (a.gx != b.gx) return a.gx < b.gx;
+            if (a.gz != b.gz) return a.gz < b.gz;
+            return a.k < b.k;
+        });
+        cells.clear();
+        ptr.assign(1, 0);
+        ctrace.clear();
+        cw.clear();
+        for (size_t i = 0; i < all.size(); i++) {
+            if (i == 0 || all[i].shot != all[i - 1].shot || all[i].gx != all[i - 1].gx || all[i].gz != all[i - 1].gz) {
+                if (i > 0) ptr.push_back((int)ctrace.size());
+                cells.push_back({all[i].shot, all[i].gx, all[i].gz, 0});
+            }
+            ctrace.push_back(all[i].trace);
+            cw.push_back(all[i].w);
+        }
+        if (!all.empty()) ptr.push_back((int)ctrace.size());
+        auto up = [&](DevMem& d, const void* h, size_t b) {
+            d.reserve(std::max<size_t>(b, 16));
+            if (b) HIPCHECK(hipMemcpyAsync(d.p, h, b, hipMemcpyHostToDevice, s));
+        };
+        up(d_cells, cells.data(), cells.size() * sizeof(InjCell));
+        up(d_ptr, ptr.data(), ptr.size() * sizeof(int));
+        up(d_ctrace, ctrace.data(), ctrace.size() * sizeof(long long));
+        up(d_cw, cw.data(), cw.size() * sizeof(float));
     }
 };
 
@@ -111,7 +151,7 @@ struct Engine {
     std::vector<PointEntry> src_ents, rcv_ents;  // flat, ordered by shot
     std::vector<int> src_shot_ptr, rcv_shot_ptr;
     DevMem d_src_ents, d_rcv_ents, d_src_shot_ptr, d_rcv_shot_ptr;
-    TileLists fwd_inj, fwd_ext, adj_inj;
+    InjList inj_src, inj_rcv;  // sources (forward) and receivers (adjoint) as injectors
 
     // state and checkpoints
     DevMem d_fwd, d_adj, d_ckpt, d_seg, d_gshot, d_gout, d_src, d_rcv, d_adjsrc, d_partial, d_scalar;
@@ -171,7 +211,6 @@ struct Engine {
         ntx = tiles_x(g.nx, tile_x);
         ntz = tiles_z(ldz);
         build_boxes();
-        if (ns > 0) build_tile_lists();
     }
 
     // Step decomposition: the core [6, nx-6) x [8, core_z_end) where every stencil is
@@ -485,40 +524,10 @@ struct Engine {
         up(d_rcv_ext_ents, rcv_ext_ents.data(), rcv_ext_ents.size() * sizeof(PointEntry));
         up(d_src_shot_ptr, src_shot_ptr.data(), src_shot_ptr.size() * sizeof(int));
         up(d_rcv_shot_ptr, rcv_shot_ptr.data(), rcv_shot_ptr.size() * sizeof(int));
-        build_tile_lists();
+        inj_src.build(src_ents, st);
+        inj_rcv.build(rcv_ents, st);
         ckpt_epoch = -1;
         tuned = false;
-        HIPCHECK(hipStreamSynchronize(st));
-    }
-
-    // per-(shot, box) lists: injectors whose 3x3 footprint meets the box, and extractors
-    // whose pivot lies in it (the owner reads the footprint from LDS)
-    void build_tile_lists()
-    {
-        const int nt_ = ntiles();
-        auto build = [&](TileLists& L, const std::vector<PointEntry>& ents, bool owner) {
-            std::vector<std::vector<int>> per((size_t)ns * nt_);
-            for (int k = 0; k < (int)ents.size(); k++) {
-                const PointEntry& e = ents[k];
-                for (int b = 0; b < nt_; b++) {
-                    const int sx = boxes[b].x, ex = boxes[b].y, sz = boxes[b].z, ez = boxes[b].w;
-                    bool hit;
-                    if (owner) hit = (e.xi0 + 1 >= sx && e.xi0 + 1 < ex && e.zi0 + 1 >= sz && e.zi0 + 1 < ez);
-                    else hit = (e.xi0 + 2 >= sx && e.xi0 < ex && e.zi0 + 2 >= sz && e.zi0 < ez);
-                    if (hit) per[(size_t)e.shot * nt_ + b].push_back(k);
-                }
-            }
-            L.ptr.assign(per.size() + 1, 0);
-            L.idx.clear();
-            for (size_t i = 0; i < per.size(); i++) {
-                L.idx.insert(L.idx.end(), per[i].begin(), per[i].end());
-                L.ptr[i + 1] = (int)L.idx.size();
-            }
-            L.upload(st);
-        };
-        build(fwd_inj, src_ents, false);
-        build(fwd_ext, rcv_ext_ents, true);
-        build(adj_inj, rcv_ents, false);
         HIPCHECK(hipStreamSynchronize(st));
     }
 
@@ -559,7 +568,6 @@ struct Engine {
         a.irho = dv;
         a.mux = vti ? dv + nxz : nullptr;
         a.lamx = vti ? dv + 2 * nxz : nullptr;
-        a.nbox = (int)boxes.size();
         a.spb = std::max(1, std::min(opt_spb, ns));
         a.ngroups = (ns + a.spb - 1) / a.spb;
         a.div_tb = d_div_tb.as<float>();
@@ -625,19 +633,10 @@ struct Engine {
         a.prev = L.P();
         a.curr = L.C();
         a.next = L.N();
-        const TileLists& inj = adjoint ? adj_inj : fwd_inj;
-        a.inj_ptr = inj.d_ptr.as<int>();
-        a.inj_idx = inj.d_idx.as<int>();
-        a.inj = adjoint ? d_rcv_ents.as<PointEntry>() : d_src_ents.as<PointEntry>();
-        a.inj_trace = traces;
-        a.inj_comp_stride = (long long)(adjoint ? nr : ns) * g.nt;
-        if (rcv) {
-            a.ext_ptr = fwd_ext.d_ptr.as<int>();
-            a.ext_idx = fwd_ext.d_idx.as<int>();
-            a.ext = d_rcv_ext_ents.as<PointEntry>();
-            a.ext_trace = rcv;
-            a.ext_comp_stride = (long long)nr * g.nt;
-        }
+        // extraction of the current level (forward)
+        if (rcv)
+            launch_extract(L.C(), 2 * nxz, nxz, ldz, d_rcv_ext_ents.as<PointEntry>(), nr, rcv, (long long)nr * g.nt, it,
+                           st);
         // edge bands on their own stream, concurrent with the core boxes (disjoint cells),
         // or after them on the same stream
         StepArgs e = a;
@@ -657,6 +656,26 @@ struct Engine {
         launch_step(a, vti, tile_x == 32 ? BoxClass::core32 : BoxClass::core16, bx, d_list_core.as<int>(),
                     (int)list_core.size(), img, st);
         if (opt_edge_stream) HIPCHECK(hipStreamWaitEvent(st, ev_join, 0));
+        // point sources of this step into the new level
+        const InjList& inj = adjoint ? inj_rcv : inj_src;
+        InjImage im{};
+        if (img) {
+            im.on = 1;
+            im.mode = a.img_mode;
+            im.wrho = a.wrho;
+            im.f0 = a.f0;
+            im.fm = a.fm;
+            im.fp = a.fp;
+            im.fshot_stride = a.fshot_stride;
+            im.grad = a.grad;
+            im.gslab_stride = a.gslab_stride;
+            im.rho_off = (long long)(a.stiff ? 4 : 2) * nxz;
+        }
+        launch_inject_cells(inj.d_cells.as<InjCell>(), (int)inj.cells.size(), inj.d_ptr.as<int>(),
+                            inj.d_ctrace.as<long long>(), inj.d_cw.as<float>(), traces,
+                            (long long)(adjoint ? nr : ns) * g.nt, it, L.N(), 2 * nxz, nxz, g.nx, g.nz, ldz,
+                            d_deriv.as<float>(), d_tx.as<float>(), d_tz.as<float>(), d_div_tb.as<float>(),
+                            d_div_lr.as<float>(), im, st);
         L.rotate();
         stat_steps++;
     }
@@ -719,8 +738,9 @@ struct Engine {
         return {kind, P(p0), P(p1), flag, g.nt, ns, nr, tile_x, opt_spb, opt_spb_edge, opt_lag, seg_K, nseg,
                 opt_bf16, opt_stiff, opt_edge_stream, P(d_ckpt16.p), P(d_ckpt_scale.p), P(d_amax.p),
                 P(d_fwd.p), P(d_adj.p), P(d_ckpt.p), P(d_seg.p), P(d_gshot.p), P(d_model.p), P(d_deriv.p),
-                P(d_boxes.p), P(d_list_core.p), P(d_list_bz.p), P(d_list_bz8.p), P(d_list_bx.p), P(fwd_inj.d_ptr.p), P(fwd_inj.d_idx.p),
-                P(fwd_ext.d_ptr.p), P(fwd_ext.d_idx.p), P(adj_inj.d_ptr.p), P(adj_inj.d_idx.p), P(d_src_ents.p),
+                P(d_boxes.p), P(d_list_core.p), P(d_list_bz.p), P(d_list_bz8.p), P(d_list_bx.p), P(inj_src.d_cells.p), P(inj_src.d_ptr.p),
+                P(inj_src.d_ctrace.p), P(inj_src.d_cw.p), P(inj_rcv.d_cells.p), P(inj_rcv.d_ptr.p), P(inj_rcv.d_ctrace.p),
+                P(inj_rcv.d_cw.p), (long long)inj_src.cells.size(), (long long)inj_rcv.cells.size(), P(d_src_ents.p),
                 P(d_rcv_ents.p), P(d_rcv_ext_ents.p), P(d_src_shot_ptr.p), P(d_rcv_shot_ptr.p), P(d_tx.p), P(d_tz.p),
                 P(d_div_tb.p), P(d_div_lr.p), (long long)list_core.size(), (long long)list_bz.size(), (long long)list_bz8.size(),
                 (long long)list_bx.size(), (long long)boxes.size()};

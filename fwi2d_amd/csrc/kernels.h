@@ -22,9 +22,14 @@ struct PointEntry {
     int pad;
 };
 
+// A cell touched by point injectors: shot and grid position; its contributions are
+// listed in a CSR (trace offset of the x component, weight) in injector order.
+struct InjCell {
+    int shot, gx, gz, pad;
+};
+
 struct StepArgs {
     int nx, nz, ldz, ntx, ntz, nshots, nt, it, taper_prev;
-    int nbox;              // boxes of the step decomposition (CSR lists are per shot and box)
     int spb, ngroups;      // shots per workgroup, shot groups (ceil(nshots / spb))
     float rdx, rdz, rdx2, rdz2, dt, dt2, sat_x, sat_z;
     int bc_top, bc_bottom, bc_left, bc_right;
@@ -39,18 +44,6 @@ struct StepArgs {
     const float* curr;
     float* next;
     long long shot_stride;
-    // injection into the new level (sources forward, receivers adjoint)
-    const int* inj_ptr;    // [nshots * ntiles + 1] CSR over (shot, tile)
-    const int* inj_idx;    // indices into inj
-    const PointEntry* inj;
-    const float* inj_trace;
-    long long inj_comp_stride;
-    // extraction of the current level (forward only; ext_ptr may be null)
-    const int* ext_ptr;
-    const int* ext_idx;
-    const PointEntry* ext;
-    float* ext_trace;
-    long long ext_comp_stride;
     // imaging fused into an adjoint step: the derivatives of the step's current level
     // and the new level image forward snapshot idx (compute_gradients, we_op.cpp:505-562)
     int img_mode;              // 0 interior idx, 1 first (idx 0), 2 last
@@ -101,6 +94,21 @@ void launch_step(const StepArgs& a, bool vti, BoxClass c, const int4* boxes, con
 void launch_init_prev(float* prev, long long shot_stride, long long nxz, int ldz, int nshots, const int* shot_ptr,
                       const PointEntry* ents, const float* trace, long long comp_stride, hipStream_t s);
 void launch_scale_prev(float* prev, const float* rho, float half_dt2, long long nxz, int nshots, hipStream_t s);
+// Density term of a fused imaging step for the injected increment of the new level
+// (on = 0: no imaging); rho_off: offset of the density component in a gradient slab.
+struct InjImage {
+    int on, mode;
+    float wrho;
+    const float *f0, *fm, *fp;
+    long long fshot_stride;
+    float* grad;
+    long long gslab_stride, rho_off;
+};
+void launch_inject_cells(const InjCell* cells, int ncell, const int* ptr, const long long* ctrace, const float* cw,
+                         const float* trace, long long comp_stride, int it, float* next, long long shot_stride,
+                         long long nxz, int nx, int nz, int ldz, const float* irho, const float* taper_x,
+                         const float* taper_z, const float* div_tb, const float* div_lr, const InjImage& im,
+                         hipStream_t s);
 void launch_extract(const float* state, long long shot_stride, long long nxz, int ldz, const PointEntry* ents,
                     int n, float* trace, long long comp_stride, int it, hipStream_t s);
 void launch_image(const ImageArgs& a, bool vti, int tile_x, hipStream_t s);

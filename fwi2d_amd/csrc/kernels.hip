@@ -632,7 +632,6 @@ __global__ void __launch_bounds__(kThreads) k_step(const StepArgs a, const int4*
     __shared__ __attribute__((aligned(16))) float sFz[IMG ? S::IX * S::PZ : 4];  // product-range layout
     StepSmem sm{sUx, sUz, sMu, sMuX, sPx, sPz, sQx, sQz};
 
-    const int ntiles = a.nbox;  // boxes of the CSR injector / extractor lists
     const int lin = xcd_remap(blockIdx.x, gridDim.x);
     TileInfo t;
     const int slot = lin / a.ngroups;
@@ -805,21 +804,6 @@ __global__ void __launch_bounds__(kThreads) k_step(const StepArgs a, const int4*
         __syncthreads();
         if (shot + 1 < s1) load_window(shot + 1, tid, t);
 
-        // extraction of the current level by the owning tile
-        if (a.ext_ptr) {
-            const int b = a.ext_ptr[shot * ntiles + t.tile], e = a.ext_ptr[shot * ntiles + t.tile + 1];
-            for (int k = b * 2 + tid; k < e * 2; k += kThreads) {
-                const PointEntry en = a.ext[a.ext_idx[k >> 1]];
-                const int comp = k & 1;
-                const float* su = comp ? sUz : sUx;
-                float v = 0.f;
-                for (int ix = 0; ix < 3; ix++)
-                    for (int iz = 0; iz < 3; iz++)
-                        v += en.wx[ix] * en.wz[iz] * su[(en.xi0 + ix - t.wsx) * S::WZ + (en.zi0 + iz - t.wsz)];
-                a.ext_trace[en.trace + comp * a.ext_comp_stride + a.it] = v;
-            }
-        }
-
         // products of first derivatives
 #pragma unroll
         for (int m = 0; m < S::NQ2; m++) {
@@ -843,7 +827,6 @@ __global__ void __launch_bounds__(kThreads) k_step(const StepArgs a, const int4*
         // new level
         float* nux = a.next + shot * a.shot_stride;
         float* nuz = nux + nxz;
-        const int ib = a.inj_ptr[shot * ntiles + t.tile], ie = a.inj_ptr[shot * ntiles + t.tile + 1];
 #pragma unroll
         for (int m = 0; m < S::NQ3; m++) {
             const int i = tid + m * kThreads;
@@ -865,23 +848,6 @@ __global__ void __launch_bounds__(kThreads) k_step(const StepArgs a, const int4*
             }
             float rx[4], rz[4];
             rhs_quad<VTI, S, EDGE, CW>(a, sm, t, gx, z0, prx, prz, rx, rz, qw[CW ? m : 0]);
-            // sources (we_op.cpp:722-724)
-            for (int k2 = ib; k2 < ie; k2++) {
-                const PointEntry& en = a.inj[a.inj_idx[k2]];
-                const int ax = gx - en.xi0;
-                if ((unsigned)ax >= 3u) continue;
-                const float fx = a.inj_trace[en.trace + a.it];
-                const float fz = a.inj_trace[en.trace + a.inj_comp_stride + a.it];
-#pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    const int az = z0 + k - en.zi0;
-                    if ((unsigned)az < 3u) {
-                        const float wgt = en.wx[ax] * en.wz[az];
-                        rx[k] += wgt * fx;
-                        rz[k] += wgt * fz;
-                    }
-                }
-            }
             const int wq = (gx - t.wsx) * S::WZ + (z0 - t.wsz);
             float cx[4], cz[4], vx[4], vz[4];
             unpack(ld4(&sUx[wq]), cx);
@@ -1051,6 +1017,60 @@ __global__ void k_extract(const float* state, long long shot_stride, long long n
     for (int ix = 0; ix < 3; ix++)
         for (int iz = 0; iz < 3; iz++) v += en.wx[ix] * en.wz[iz] * u[(long long)(en.xi0 + ix) * ldz + en.zi0 + iz];
     trace[en.trace + comp * comp_stride + it] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Point sources of a step (we_op.cpp:722-724): added to the new level after the step,
+// one thread per touched cell summing its contributions in a fixed order.  The step
+// forms next = (dt^2/rho (r + s) + 2 curr - prev) / d * taper, which is linear in the
+// source term s, so the sources add dt^2/rho * s / d * taper (d: the absorbing divisor on
+// the perimeter, 1 elsewhere).
+// ---------------------------------------------------------------------------
+__global__ void k_inject_cells(const InjCell* cells, int ncell, const int* ptr, const long long* ctrace,
+                               const float* cw, const float* trace, long long comp_stride, int it, float* next,
+                               long long shot_stride, long long nxz, int nx, int nz, int ldz, const float* irho,
+                               const float* taper_x, const float* taper_z, const float* div_tb, const float* div_lr,
+                               const InjImage im)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    const InjCell cell = cells[c];
+    float fx = 0.f, fz = 0.f;
+    for (int k = ptr[c]; k < ptr[c + 1]; k++) {
+        const long long tr = ctrace[k] + it;
+        fx += cw[k] * trace[tr];
+        fz += cw[k] * trace[tr + comp_stride];
+    }
+    const int gx = cell.gx, gz = cell.gz;
+    const long long g = (long long)gx * ldz + gz;
+    float dx = 1.f, dz = 1.f;
+    if (gz == 0) { dx = div_tb[gx]; dz = div_tb[nx + gx]; }
+    else if (gz == nz - 1) { dx = div_tb[2 * nx + gx]; dz = div_tb[3 * nx + gx]; }
+    else if (gx == 0) { dx = div_lr[gz]; dz = div_lr[nz + gz]; }
+    else if (gx == nx - 1) { dx = div_lr[2 * nz + gz]; dz = div_lr[3 * nz + gz]; }
+    const float f = irho[g] * taper_z[gz] * taper_x[gx];
+    float* u = next + cell.shot * shot_stride + g;
+    const float ix = f * fx / dx, iz = f * fz / dz;
+    u[0] += ix;
+    u[nxz] += iz;
+    if (im.on) {
+        // the fused imaging step took the new level before this increment: add its
+        // density-term contribution (we_op.cpp:533-541) to the shot's own gradient slab
+        const long long fo = cell.shot * im.fshot_stride + g;
+        const float f0x = im.f0[fo], f0z = im.f0[fo + nxz];
+        float d2x, d2z;
+        if (im.mode == 0) {
+            d2x = im.fp[fo] - 2.f * f0x + im.fm[fo];
+            d2z = im.fp[fo + nxz] - 2.f * f0z + im.fm[fo + nxz];
+        } else if (im.mode == 1) {
+            d2x = im.fp[fo] - f0x;
+            d2z = im.fp[fo + nxz] - f0z;
+        } else {
+            d2x = -f0x + im.fm[fo];
+            d2z = -f0z + im.fm[fo + nxz];
+        }
+        im.grad[cell.shot * im.gslab_stride + im.rho_off + g] += im.wrho * (ix * d2x + iz * d2z);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1312,6 +1332,18 @@ void launch_scale_prev(float* prev, const float* rho, float half_dt2, long long 
 {
     hipLaunchKernelGGL(k_scale_prev, dim3(grid_for(2 * nxz * nshots)), dim3(kThreads), 0, s, prev, rho, half_dt2, nxz,
                        nshots);
+}
+
+void launch_inject_cells(const InjCell* cells, int ncell, const int* ptr, const long long* ctrace, const float* cw,
+                         const float* trace, long long comp_stride, int it, float* next, long long shot_stride,
+                         long long nxz, int nx, int nz, int ldz, const float* irho, const float* taper_x,
+                         const float* taper_z, const float* div_tb, const float* div_lr, const InjImage& im,
+                         hipStream_t s)
+{
+    if (ncell <= 0) return;
+    hipLaunchKernelGGL(k_inject_cells, dim3((ncell + kThreads - 1) / kThreads), dim3(kThreads), 0, s, cells, ncell,
+                       ptr, ctrace, cw, trace, comp_stride, it, next, shot_stride, nxz, nx, nz, ldz, irho, taper_x,
+                       taper_z, div_tb, div_lr, im);
 }
 
 void launch_extract(const float* state, long long shot_stride, long long nxz, int ldz, const PointEntry* ents, int n,
