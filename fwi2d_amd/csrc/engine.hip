@@ -135,6 +135,7 @@ struct Engine {
     std::string tune_report;
     int opt_spb = 8;       // shots advanced one after the other by each core-box workgroup
     int opt_spb_edge = 1;  // the same for edge tiles (their workgroups are latency-bound)
+    int opt_spb_img = 0;   // shots per core workgroup in imaging steps (0: as opt_spb)
     int opt_edge_stream = 1;  // edge-band launches on their own stream
     hipStream_t st_edge = nullptr;       // edge-tile launches run beside the interior ones
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
@@ -627,7 +628,13 @@ struct Engine {
     void step(Levels& L, int it, bool adjoint, const float* traces, float* rcv, int img_idx = -1)
     {
         StepArgs a = base_args();
-        if (img_idx >= 0) set_image_args(a, img_idx);
+        if (img_idx >= 0) {
+            set_image_args(a, img_idx);
+            if (opt_spb_img > 0) {
+                a.spb = std::max(1, std::min(opt_spb_img, ns));
+                a.ngroups = (ns + a.spb - 1) / a.spb;
+            }
+        }
         a.it = it;
         a.taper_prev = it > 0;
         a.prev = L.P();
@@ -736,7 +743,7 @@ struct Engine {
     {
         auto P = [](const void* p) { return (long long)(uintptr_t)p; };
         return {kind, P(p0), P(p1), flag, g.nt, ns, nr, tile_x, opt_spb, opt_spb_edge, opt_lag, seg_K, nseg,
-                opt_bf16, opt_stiff, opt_edge_stream, P(d_ckpt16.p), P(d_ckpt_scale.p), P(d_amax.p),
+                opt_bf16, opt_stiff, opt_edge_stream, opt_spb_img, P(d_ckpt16.p), P(d_ckpt_scale.p), P(d_amax.p),
                 P(d_fwd.p), P(d_adj.p), P(d_ckpt.p), P(d_seg.p), P(d_gshot.p), P(d_model.p), P(d_deriv.p),
                 P(d_boxes.p), P(d_list_core.p), P(d_list_bz.p), P(d_list_bz8.p), P(d_list_bx.p), P(inj_src.d_cells.p), P(inj_src.d_ptr.p),
                 P(inj_src.d_ctrace.p), P(inj_src.d_cw.p), P(inj_rcv.d_cells.p), P(inj_rcv.d_ptr.p), P(inj_rcv.d_ctrace.p),
@@ -1245,6 +1252,9 @@ int fwi2d_engine_set_option(fwi2d_engine* e, const char* key, long long value)
         } else if (k == "autotune") {
             x->opt_autotune = value ? 1 : 0;
             x->tuned = false;
+        } else if (k == "shots_per_block_img") {
+            if (value < 0 || value > 64) throw std::runtime_error("shots_per_block_img must be in [0, 64]");
+            x->opt_spb_img = (int)value;
         } else if (k == "shots_per_block_edge") {
             if (value < 1 || value > 64) throw std::runtime_error("shots_per_block_edge must be in [1, 64]");
             x->opt_spb_edge = (int)value;
@@ -1267,6 +1277,7 @@ long long fwi2d_engine_get_stat(fwi2d_engine* e, const char* key)
     if (k == "tile_x") return x->tile_x;
     if (k == "shots_per_block") return x->opt_spb;
     if (k == "shots_per_block_edge") return x->opt_spb_edge;
+    if (k == "shots_per_block_img") return x->opt_spb_img;
     if (k == "edge_stream") return x->opt_edge_stream;
     if (k == "ntiles") return x->ntiles();
     if (k == "interior_tiles") return (long long)x->list_core.size();

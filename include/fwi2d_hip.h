@@ -89,6 +89,8 @@ int fwi2d_engine_dot_f64(fwi2d_engine* e, const float* a, const float* b, const 
    along x), "shots_per_block" (1..64 shots advanced in turn by one step workgroup of an
    interior tile), "shots_per_block_edge" (the same for tiles at the grid edge),
    "edge_stream" (1: edge-band launches run on a second stream beside the core launch),
+   "shots_per_block_img" (shots per core workgroup in adjoint steps with imaging; 0: as
+   "shots_per_block"),
    "autotune" (1: at the next forward run, time a few steps of every tile_x and
    shots-per-workgroup choice and keep the fastest; forward results are unchanged),
    "checkpoint_format" (0 FP32; 1 BF16, 2 FP16: checkpoint levels stored in 16 bits with
