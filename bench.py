@@ -35,8 +35,10 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--shots", type=int, default=64)
-    ap.add_argument("--nt", type=int, default=4000)
+    ap.add_argument("--workload", choices=["c2", "c3"], default="c2",
+                    help="c2: 64 isotropic shots on 500x1700 (default); c3: 16 VTI shots on 1000x3000, h = 5 m")
+    ap.add_argument("--shots", type=int, default=None, help="shots per batch (default 64 for c2, 16 for c3)")
+    ap.add_argument("--nt", type=int, default=None, help="time steps (default 4000 for c2, 6000 for c3)")
     ap.add_argument("--plain-shots", type=int, default=4)
     ap.add_argument("--tile-x", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -120,20 +122,38 @@ def calibrate_copy():
 
 
 def converted(m):
+    """(vp, vs, rho[, delta, eps]) -> (lambda, mu, rho[, c13, eps]), convert_model
+    (we_op.cpp:476-494, 1441-1462)."""
     lam = m[2] * (m[0] ** 2 - 2 * m[1] ** 2)
     mu = m[2] * m[1] ** 2
-    return np.stack([lam, mu, m[2]]).astype(np.float32)
+    out = [lam, mu, m[2]]
+    if len(m) == 5:
+        out += [np.sqrt(2 * (lam + 2 * mu) * (lam + mu) * m[3] + (lam + mu) ** 2) - mu, m[4]]
+    return np.stack(out).astype(np.float32)
 
 
-def workload(nshots, nt):
+WORKLOADS = {
+    # name: (nz, nx, h km, dt s, shots, nt, vti, geometry seed, text)
+    "c2": (500, 1700, 0.01, 0.001, 64, 4000, False, 2,
+           "batched isotropic FWI gradients (forward + checkpointed adjoint): %d shots, 500x1700 grid h=10 m, "
+           "nt=%d dt=1 ms, 100-400 receivers/shot, 8-20 Hz Ricker"),
+    "c3": (1000, 3000, 0.005, 0.0005, 16, 6000, True, 3,
+           "batched VTI FWI gradients (forward + checkpointed adjoint): %d shots, 1000x3000 grid h=5 m, "
+           "nt=%d dt=0.5 ms, Thomsen eps in [0, 0.25], delta in [-0.05, 0.15], 8-20 Hz Ricker"),
+}
+
+
+def workload(kind, nshots, nt):
     from fwi2d_amd import workloads as W
-    nz, nx, h, dt = 500, 1700, 0.01, 0.001
+    nz, nx, h, dt, _, _, vti, gseed, _ = WORKLOADS[kind]
     true = W.gradient_model(nx, nz, h, h, seed=0)
+    if vti:
+        true = np.concatenate([true, W.vti_fields(nx, nz, h, h)], 0)
     start = W.smooth_model(true, h, h, 0.3)
-    sxz, rxz, freqs = W.config2_geometry(nshots, nx, h)
+    sxz, rxz, freqs = W.config2_geometry(nshots, nx, h, seed=gseed)
     wav = np.stack([W.ricker(nt, dt, f) for f in freqs])
     src = np.stack([wav * np.cos(1.5708), wav * np.sin(1.5708)]).astype(np.float32)
-    prob = W.base_problem(nx, nz, h, h, nt, dt, sxz, rxz, dt=dt, sub=1)
+    prob = W.base_problem(nx, nz, h, h, nt, dt, sxz, rxz, vti=int(vti), dt=dt, sub=1)
     return prob, converted(true), converted(start), sxz, rxz, src, dt
 
 
@@ -161,7 +181,8 @@ class Batch:
         self.adj = torch.zeros_like(self.rcv)
         self.res = torch.zeros_like(self.rcv)
         self.misfit = self.grad_norm = None
-        self.grad = torch.zeros((3, prob["nx"], prob["nz"]), dtype=torch.float32, device=dev)
+        self.grad = torch.zeros((5 if prob.get("vti") else 3, prob["nx"], prob["nz"]), dtype=torch.float32,
+                                device=dev)
         # observed data from the true model
         self.eng.set_model(model_true)
         self.eng.forward_device(self.src.data_ptr(), self.rcv.data_ptr(), keep_checkpoints=False)
@@ -184,17 +205,20 @@ class Batch:
         self.grad_norm = self.eng.dot_f64(self.grad.data_ptr(), 0, 0, self.grad.numel()) ** 0.5
 
 
-def grad_error_small():
-    """Gradient rel-L2 error vs the FP64 oracle on a reduced single-shot case."""
+def grad_error_small(vti=False, fmt=0):
+    """Gradient rel-L2 error vs the FP64 oracle on a reduced single-shot case (largest over
+    the lambda / mu / rho components); fmt: checkpoint format (0 FP32, 1 BF16, 2 FP16)."""
     import fwi2d_amd
     from fwi2d_amd import workloads as W
     from oracle.oracle import Oracle, grid_struct
     nx, nz, h, nt, dt = 160, 100, 0.01, 600, 0.001
     true = W.add_ellipses(W.gradient_model(nx, nz, h, h, seed=0), h, h)
+    if vti:
+        true = np.concatenate([true, W.vti_fields(nx, nz, h, h)], 0)
     start = W.smooth_model(true, h, h, 0.1)
     m0, m1 = converted(true).astype(np.float64), converted(start).astype(np.float64)
     prob = dict(nx=nx, nz=nz, dx=h, dz=h, bc_top=1, bc_bottom=2, bc_left=2, bc_right=2, taper_top=0,
-                taper_bottom=20, taper_left=20, taper_right=20, taper_strength=0.05, vti=0)
+                taper_bottom=20, taper_left=20, taper_right=20, taper_strength=0.05, vti=int(vti))
     sxz = np.array([[0.8, 0.02]])
     rxz = [np.stack([np.arange(2, nx - 2, 4) * h, np.full(len(range(2, nx - 2, 4)), 0.02)], 1)]
     w = W.ricker(nt, dt, 12.0)
@@ -207,9 +231,11 @@ def grad_error_small():
     full = np.zeros((1 + nt, 2, nx, nz))
     d = orc.propagate(g, m1, inj, src, ext=ext, full=full)
     adj = -(d - dobs)[:, :, ::-1].copy()
-    gref = np.zeros((3, nx, nz))
+    gref = np.zeros((5 if vti else 3, nx, nz))
     orc.propagate(g, m1, ext, adj, full=full, grad=gref)
     eng = fwi2d_amd.Engine(fwi2d_amd.grid_from(prob, nt, dt, 1))
+    eng.set_option("checkpoint_format", fmt)
+    eng.set_option("segment", 25)
     eng.set_model(m1)
     eng.set_shots(sxz, rxz)
     eng.forward(src, keep_checkpoints=True)
@@ -217,32 +243,35 @@ def grad_error_small():
     return max(float(np.linalg.norm(gk[c] - gref[c]) / np.linalg.norm(gref[c])) for c in range(3))
 
 
-def cpu_baseline(nt_sample=400):
+def cpu_baseline(nt_sample=400, kind="c2"):
     """Oracle FP32 restatement (OpenMP, all host cores) on one shot of the same grid for
     nt_sample steps, forward with snapshots + adjoint with imaging; scaled to shots/s."""
     from fwi2d_amd import workloads as W
     from oracle.oracle import Oracle, grid_struct
     orc = Oracle(32)
-    nz, nx, h, dt = 500, 1700, 0.01, 0.001
-    m = converted(W.gradient_model(nx, nz, h, h, seed=0))
+    nz, nx, h, dt, _, _, vti, gseed, _ = WORKLOADS[kind]
+    m = W.gradient_model(nx, nz, h, h, seed=0)
+    if vti:
+        m = np.concatenate([m, W.vti_fields(nx, nz, h, h)], 0)
+    m = converted(m)
     prob = dict(nx=nx, nz=nz, dx=h, dz=h, bc_top=1, bc_bottom=2, bc_left=2, bc_right=2, taper_top=0,
-                taper_bottom=30, taper_left=30, taper_right=30, taper_strength=0.05, vti=0)
-    sxz, rxz, freqs = W.config2_geometry(1, nx, h)
+                taper_bottom=30, taper_left=30, taper_right=30, taper_strength=0.05, vti=int(vti))
+    sxz, rxz, freqs = W.config2_geometry(1, nx, h, seed=gseed)
     g = grid_struct(dict(prob, nt=nt_sample, dt=dt, sub=1))
     w = W.ricker(nt_sample, dt, float(freqs[0]))
     src = np.stack([0 * w[None], w[None]]).astype(np.float32)
     inj = orc.delta_m3(nx, nz, h, h, 0, 0, sxz[:, 0], sxz[:, 1])
     ext = orc.delta_m3(nx, nz, h, h, 0, 0, rxz[0][:, 0], rxz[0][:, 1])
     full = np.zeros((1 + nt_sample, 2, nx, nz), np.float32)
-    grad = np.zeros((3, nx, nz), np.float32)
+    grad = np.zeros((5 if vti else 3, nx, nz), np.float32)
     t0 = time.perf_counter()
     d = orc.propagate(g, m, inj, src, ext=ext, full=full)
     orc.propagate(g, m, ext, d[:, :, ::-1].copy(), full=full, grad=grad)
     el = time.perf_counter() - t0
     cells = 2.0 * nt_sample * nx * nz
     return dict(cell_updates_per_s=cells / el, seconds=el, threads=int(orc.lib.orc_num_threads()),
-                sample="1 shot, 500x1700 grid, %d forward + %d adjoint steps (sub=1, imaging every step)"
-                % (nt_sample, nt_sample))
+                sample="1 shot, %dx%d grid%s, %d forward + %d adjoint steps (sub=1, imaging every step)"
+                % (nz, nx, " (VTI)" if vti else "", nt_sample, nt_sample))
 
 
 def main():
@@ -260,8 +289,12 @@ def main():
     import torch
     import fwi2d_amd
     fwi2d_amd.load()
-    nt, ns = args.nt, args.shots
-    prob, m_true, m0, sxz, rxz, src, dt = workload(ns, nt)
+    wl = WORKLOADS[args.workload]
+    ns = args.shots or wl[4]
+    nt = args.nt or wl[5]
+    vti = wl[6]
+    bytes_per_cell = 44.0 if vti else BYTES_PER_CELL_UPDATE_FWD
+    prob, m_true, m0, sxz, rxz, src, dt = workload(args.workload, ns, nt)
     nx, nz = prob["nx"], prob["nz"]
     cells_per_grad = 2.0 * nt * nx * nz  # forward + adjoint, per shot
 
@@ -283,7 +316,7 @@ def main():
     shots_per_s = ns * args.steps / el
     gcells = shots_per_s * cells_per_grad / 1e9
     grad_norm, misfit = B.grad_norm, B.misfit
-    traffic = measured_traffic() if ns == 64 else None
+    traffic = measured_traffic() if (args.workload, ns) == ("c2", 64) else None
 
     # ---- dominant kernel: fused step, timed with HIP events on its stream ----
     # (eager launches: one step = the core-box launch plus the edge-band launches beside it)
@@ -299,7 +332,7 @@ def main():
     B.eng.set_option("timing", 0)
     step_s = loop_ns * 1e-9 / nsteps
     cells_per_launch = ns * nx * nz
-    achieved = BYTES_PER_CELL_UPDATE_FWD * cells_per_launch / step_s / 1e9
+    achieved = bytes_per_cell * cells_per_launch / step_s / 1e9
     recompute = B.eng.stat("recompute_steps")
     del B
     torch.cuda.empty_cache()
@@ -322,9 +355,13 @@ def main():
         plain_shots_per_s = P / t_plain
         log("plain form done")
 
-    grad_err = None if args.no_check else grad_error_small()
+    grad_err = None if args.no_check else grad_error_small(vti)
+    grad_err16 = None
+    if vti and not args.no_check:
+        grad_err16 = {"bf16": grad_error_small(True, 1), "fp16": grad_error_small(True, 2)}
     log("gradient check done")
-    cpu = None if args.no_cpu else cpu_baseline(args.cpu_steps)
+    cpu = None if args.no_cpu else cpu_baseline(args.cpu_steps if not vti else max(args.cpu_steps // 4, 50),
+                                                  args.workload)
     log("cpu baseline done")
 
     line = {
@@ -340,9 +377,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "batched isotropic FWI gradients (forward + checkpointed adjoint): %d shots, "
-                               "500x1700 grid h=10 m, nt=%d dt=1 ms, 100-400 receivers/shot, 8-20 Hz Ricker"
-                               % (ns, nt),
+        "config": {"workload": wl[8] % (ns, nt),
                    "shots_per_step": ns, "nx": nx, "nz": nz, "nt": nt, "autotuned": tune,
                    "hipgraph_replays": graph_replays},
         "gcell_updates_per_s": round(gcells, 2),
@@ -350,15 +385,18 @@ def main():
         "batched_vs_plain": plain_shots_per_s and round(shots_per_s / plain_shots_per_s, 3),
         "recompute_steps_per_gradient": int(recompute // max(args.steps + args.warmup, 1)),
         "grad_rel_err_vs_fp64": grad_err,
-        "grad_err_sample": "1 shot, 160x100 grid, nt=600, raw lambda/mu/rho gradients vs FP64 oracle",
+        "grad_err_sample": "1 shot, 160x100 grid%s, nt=600, raw lambda/mu/rho gradients vs FP64 oracle"
+                           % (" (VTI)" if vti else ""),
         "grad_norm_f64": grad_norm,
         "misfit_f64": misfit,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "k_step (fused time step of all shots: core-box launch + edge-band launches)",
                      "avg_launch_us": round(step_s * 1e6, 2),
-                     "bytes_per_cell_update": BYTES_PER_CELL_UPDATE_FWD, "cells_per_launch": cells_per_launch},
+                     "bytes_per_cell_update": bytes_per_cell, "cells_per_launch": cells_per_launch},
     }
+    if grad_err16:
+        line["grad_rel_err_vs_fp64_16bit_checkpoints"] = grad_err16
     if cpu:
         line["cpu_baseline"] = {"value": round(cpu["cell_updates_per_s"] / cells_per_grad, 6), "unit": "shots/s",
                                 "cores": cpu["threads"], "kind": "port", "sample": cpu["sample"],
