@@ -368,3 +368,49 @@ def test_engine_gradient_against_finite_differences(lib):
         err = abs(fd - an) / abs(fd)
         print("%s: finite difference %.6e  gradient %.6e  rel %.2e" % (name, fd, an, err))
         assert err < tol[name], name
+
+
+def test_full_size_properties(lib):
+    """Size-independent properties on the full 64-shot 500 x 1700 grid (a short time axis
+    keeps the test brief; the grid, batch and launch shapes are those of the bench):
+    scaling in the source amplitude (a power of two scales FP32 exactly except where values
+    cross the subnormal range, so to 1e-7), superposition of two source sets, batch
+    invariance of one shot (bit for bit), and scaling of the gradient in the adjoint
+    source."""
+    nz, nx, h, dt, nt, ns = 500, 1700, 0.01, 0.001, 240, 64
+    m = W.gradient_model(nx, nz, h, h, seed=0)
+    model = np.stack([m[2] * (m[0] ** 2 - 2 * m[1] ** 2), m[2] * m[1] ** 2, m[2]]).astype(np.float32)
+    sxz, rxz, freqs = W.config2_geometry(ns, nx, h)
+    prob = W.base_problem(nx, nz, h, h, nt, dt, sxz, rxz, dt=dt, sub=1)
+    wav = np.stack([W.ricker(nt, dt, f) for f in freqs])
+    s1 = np.stack([0.3 * wav, wav]).astype(np.float32)
+    s2 = np.stack([wav, -0.5 * wav]).astype(np.float32)[..., ::-1].copy()
+    eng = lib.Engine(lib.grid_from(prob, nt, dt, 1))
+    eng.set_option("shots_per_block", 16)
+    eng.set_model(model)
+    eng.set_shots(sxz, rxz)
+    d1 = eng.forward(s1)
+    err = rel(eng.forward(2 * s1), 2 * d1.astype(np.float64))
+    print("scaling rel err %.2e" % err)
+    assert err < 1e-7
+    d2 = eng.forward(s2)
+    d12 = eng.forward(s1 + s2)
+    err = rel(d12, d1.astype(np.float64) + d2)
+    print("superposition rel err %.2e" % err)
+    assert err < 1e-5
+    k = 37
+    one = lib.Engine(lib.grid_from(prob, nt, dt, 1))
+    one.set_option("shots_per_block", 16)
+    one.set_model(model)
+    one.set_shots(sxz[k:k + 1], rxz[k:k + 1])
+    r0 = sum(len(r) for r in rxz[:k])
+    assert np.array_equal(one.forward(s1[:, k:k + 1]), d1[:, r0:r0 + len(rxz[k])])
+    rng = np.random.default_rng(4)
+    adj = (rng.standard_normal(d1.shape) * np.hanning(nt)).astype(np.float32)
+    eng.forward(s1, keep_checkpoints=True)
+    g1 = eng.adjoint(adj)
+    g2 = eng.adjoint(2 * adj)
+    err = rel(g2, 2 * g1.astype(np.float64))
+    print("gradient scaling rel err %.2e" % err)
+    assert err < 1e-7
+    assert np.all(np.isfinite(g1)) and np.linalg.norm(g1) > 0
