@@ -135,7 +135,8 @@ struct Engine {
     std::string tune_report;
     int opt_spb = 8;       // shots advanced one after the other by each core-box workgroup
     int opt_spb_edge = 1;  // the same for edge tiles (their workgroups are latency-bound)
-    int opt_spb_img = 0;   // shots per core workgroup in imaging steps (0: as opt_spb)
+    int opt_spb_img = 0;   // shots per imaging workgroup (0: as opt_spb)
+    int opt_image_fused = 0;  // 1: imaging condition fused into the adjoint step kernels
     int opt_edge_stream = 1;  // edge-band launches on their own stream
     hipStream_t st_edge = nullptr;       // edge-tile launches run beside the interior ones
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
@@ -628,12 +629,18 @@ struct Engine {
     void step(Levels& L, int it, bool adjoint, const float* traces, float* rcv, int img_idx = -1)
     {
         StepArgs a = base_args();
+        // imaging fused into the step kernels, or (default) a separate k_img pass after the
+        // step and its point sources
+        const bool fuse_img = img_idx >= 0 && opt_image_fused;
+        StepArgs ia{};
         if (img_idx >= 0) {
             set_image_args(a, img_idx);
+            ia = a;
             if (opt_spb_img > 0) {
-                a.spb = std::max(1, std::min(opt_spb_img, ns));
-                a.ngroups = (ns + a.spb - 1) / a.spb;
+                ia.spb = std::max(1, std::min(opt_spb_img, ns));
+                ia.ngroups = (ns + ia.spb - 1) / ia.spb;
             }
+            if (fuse_img) a = ia;
         }
         a.it = it;
         a.taper_prev = it > 0;
@@ -649,7 +656,7 @@ struct Engine {
         StepArgs e = a;
         e.spb = std::max(1, std::min(opt_spb_edge, ns));
         e.ngroups = (ns + e.spb - 1) / e.spb;
-        const bool img = img_idx >= 0;
+        const bool img = fuse_img;
         const int4* bx = d_boxes.as<int4>();
         hipStream_t se = opt_edge_stream ? st_edge : st;
         if (opt_edge_stream) {
@@ -683,6 +690,11 @@ struct Engine {
                             (long long)(adjoint ? nr : ns) * g.nt, it, L.N(), 2 * nxz, nxz, g.nx, g.nz, ldz,
                             d_deriv.as<float>(), d_tx.as<float>(), d_tz.as<float>(), d_div_tb.as<float>(),
                             d_div_lr.as<float>(), im, st);
+        if (img_idx >= 0 && !fuse_img) {
+            ia.curr = L.C();
+            ia.next = L.N();
+            launch_img(ia, vti, tile_x, st);
+        }
         L.rotate();
         stat_steps++;
     }
@@ -743,7 +755,7 @@ struct Engine {
     {
         auto P = [](const void* p) { return (long long)(uintptr_t)p; };
         return {kind, P(p0), P(p1), flag, g.nt, ns, nr, tile_x, opt_spb, opt_spb_edge, opt_lag, seg_K, nseg,
-                opt_bf16, opt_stiff, opt_edge_stream, opt_spb_img, P(d_ckpt16.p), P(d_ckpt_scale.p), P(d_amax.p),
+                opt_bf16, opt_stiff, opt_edge_stream, opt_spb_img, opt_image_fused, P(d_ckpt16.p), P(d_ckpt_scale.p), P(d_amax.p),
                 P(d_fwd.p), P(d_adj.p), P(d_ckpt.p), P(d_seg.p), P(d_gshot.p), P(d_model.p), P(d_deriv.p),
                 P(d_boxes.p), P(d_list_core.p), P(d_list_bz.p), P(d_list_bz8.p), P(d_list_bx.p), P(inj_src.d_cells.p), P(inj_src.d_ptr.p),
                 P(inj_src.d_ctrace.p), P(inj_src.d_cw.p), P(inj_rcv.d_cells.p), P(inj_rcv.d_ptr.p), P(inj_rcv.d_ctrace.p),
@@ -1258,6 +1270,8 @@ int fwi2d_engine_set_option(fwi2d_engine* e, const char* key, long long value)
         } else if (k == "shots_per_block_edge") {
             if (value < 1 || value > 64) throw std::runtime_error("shots_per_block_edge must be in [1, 64]");
             x->opt_spb_edge = (int)value;
+        } else if (k == "image_fused") {
+            x->opt_image_fused = value ? 1 : 0;
         } else {
             throw std::runtime_error("unknown option " + k);
         }
@@ -1279,6 +1293,7 @@ long long fwi2d_engine_get_stat(fwi2d_engine* e, const char* key)
     if (k == "shots_per_block_edge") return x->opt_spb_edge;
     if (k == "shots_per_block_img") return x->opt_spb_img;
     if (k == "edge_stream") return x->opt_edge_stream;
+    if (k == "image_fused") return x->opt_image_fused;
     if (k == "ntiles") return x->ntiles();
     if (k == "interior_tiles") return (long long)x->list_core.size();
     if (k == "graph_captures") return x->graph_captures;

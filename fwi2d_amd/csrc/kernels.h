@@ -112,6 +112,9 @@ void launch_inject_cells(const InjCell* cells, int ncell, const int* ptr, const 
 void launch_extract(const float* state, long long shot_stride, long long nxz, int ldz, const PointEntry* ents,
                     int n, float* trace, long long comp_stride, int it, hipStream_t s);
 void launch_image(const ImageArgs& a, bool vti, int tile_x, hipStream_t s);
+// imaging condition after an adjoint step over the tile_x x 64 tile grid (a.ntx x a.ntz
+// tiles, a.ngroups groups of a.spb shots); a.curr lagged adjoint level, a.next the new one
+void launch_img(const StepArgs& a, bool vti, int tile_x, hipStream_t s);
 void launch_sum_shots(const float* per_shot, long long shot_stride, int nshots, long long n, float* out,
                       int accumulate, hipStream_t s);
 void launch_fill(float* p, long long n, float v, hipStream_t s);

@@ -235,20 +235,28 @@ __device__ __forceinline__ float d1_row0(const float* p, int s)
 // row take it.  The next shot's window is loaded into registers before the current
 // shot's arithmetic, which hides its latency.
 // ---------------------------------------------------------------------------
+// Thin z bands (TZ <= 16) map lanes row-major: a wavefront holds one quad row of 64
+// consecutive columns, so the z closure row of every lane of a wave is the same and the
+// closure coefficients are read with scalar loads.  Their LDS columns then hold an odd
+// number of quads, which keeps the column-strided 16-byte reads free of bank conflicts.
 template <int TX, int TZ_>
 struct StepShape {
-    static constexpr int TZ = TZ_;
+    static constexpr int TZ = TZ_, BX = TX;
+    static constexpr bool ROWMAJ = TZ <= 16;
     static constexpr int HX = 4, HZ = 8;
-    static constexpr int WX = TX + 2 * HX, WZ = TZ + 2 * HZ;  // state window (cells)
-    static constexpr int WQ = WZ / 4;                          // window quads per column
+    static constexpr int WX = TX + 2 * HX;                     // state window columns
+    static constexpr int WQ = (TZ + 2 * HZ) / 4;               // state window quads per column
+    static constexpr int WZ = 4 * (ROWMAJ ? (WQ | 1) : WQ);    // LDS column stride of the window
     static constexpr int IX = TX + 4;                          // product columns
-    static constexpr int PZ = TZ + 8;                          // product rows [sz-4, sz+TZ+4)
-    static constexpr int PQ = PZ / 4;
+    static constexpr int PQ = (TZ + 8) / 4;                    // product quads: rows [sz-4, sz+TZ+4)
+    static constexpr int PZ = 4 * (ROWMAJ ? (PQ | 1) : PQ);    // LDS column stride of the products
+    static constexpr int IXP = ROWMAJ ? (IX + 63) / 64 * 64 : IX;  // product slots per quad row
     static constexpr int QC = TZ / 4;                          // new-level quads per column
     static constexpr int NQ1 = (WX * WQ + kThreads - 1) / kThreads;  // window quads per thread
-    static constexpr int NQ2 = (IX * PQ + kThreads - 1) / kThreads;  // product quads per thread
+    static constexpr int NQ2 = (IXP * PQ + kThreads - 1) / kThreads; // product quads per thread
     static constexpr int NQ3 = (TX * QC) / kThreads;                  // new-level quads per thread
     static_assert((TX * QC) % kThreads == 0, "box shape must give every thread whole quads");
+    static_assert(!ROWMAJ || TX % 64 == 0, "row-major boxes need whole wavefronts per quad row");
 };
 
 struct TileInfo {
@@ -256,12 +264,35 @@ struct TileInfo {
 };
 
 // new-level quad i of a box: column-major, so that a wavefront covers whole columns
-// (contiguous 16-byte quads along z)
+// (contiguous 16-byte quads along z); row-major in thin z bands, where the quad row is
+// uniform over a wavefront and is made a scalar
 template <class S>
 __device__ __forceinline__ void out_quad(const TileInfo& t, int i, int& gx, int& z0)
 {
-    gx = t.sx + i / S::QC;
-    z0 = t.sz + 4 * (i % S::QC);
+    if constexpr (S::ROWMAJ) {
+        const int row = __builtin_amdgcn_readfirstlane(i / S::BX);
+        gx = t.sx + (i - row * S::BX);
+        z0 = t.sz + 4 * row;
+    } else {
+        gx = t.sx + i / S::QC;
+        z0 = t.sz + 4 * (i % S::QC);
+    }
+}
+
+// product quad of slot i: column lc of the IX product columns, quad row lq of PQ;
+// false for a slot that holds none
+template <class S>
+__device__ __forceinline__ bool prod_slot(int i, int& lc, int& lq)
+{
+    if constexpr (S::ROWMAJ) {
+        lq = __builtin_amdgcn_readfirstlane(i / S::IXP);
+        lc = i - lq * S::IXP;
+        return lc < S::IX && lq < S::PQ;
+    } else {
+        lc = i / S::PQ;
+        lq = i - lc * S::PQ;
+        return i < S::IX * S::PQ;
+    }
 }
 
 struct StepSmem {
@@ -588,6 +619,60 @@ __device__ __forceinline__ void quad_derivs(const float* u, const float* v, int 
     }
 }
 
+// Imaging condition of one quad (compute_gradients, we_op.cpp:519-561): a* are the
+// derivatives of the lagged adjoint level (x_x, x_z, z_x, z_z), t* those of forward
+// snapshot idx (x_x, z_z, z_x, x_z), f0/fm/fp the snapshots idx, idx-1, idx+1 at the
+// cells, o the adjoint level of the density term.  Adds to the accumulators of the
+// lambda, mu, rho gradients (VTI stiffness mode: c11, c13, c33, c55, rho).
+template <bool VTI>
+__device__ __forceinline__ void image_accumulate(const StepArgs& a, long long g, int z0, const float* axx,
+                                                 const float* azx, const float* axz, const float* azz,
+                                                 const float* t0x, const float* t1z, const float* t2x,
+                                                 const float* t3z, const float* f0x, const float* f0z,
+                                                 const float* fmx, const float* fmz, const float* fpx,
+                                                 const float* fpz, const float* ox, const float* oz, float* gl,
+                                                 float* gm, float* gr, float* g3, float* g4)
+{
+    float e[4], v2[4], v3[4];
+    if (VTI) {
+        unpack(ld4(a.ge + g), e);
+        unpack(ld4(a.gv2 + g), v2);
+        unpack(ld4(a.gv3 + g), v3);
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        if (z0 + k >= a.nz) continue;
+        float d2x, d2z;
+        if (a.img_mode == 0) {
+            d2x = fpx[k] - 2.f * f0x[k] + fmx[k];
+            d2z = fpz[k] - 2.f * f0z[k] + fmz[k];
+        } else if (a.img_mode == 1) {
+            d2x = fpx[k] - f0x[k];
+            d2z = fpz[k] - f0z[k];
+        } else {
+            d2x = -f0x[k] + fmx[k];
+            d2z = -f0z[k] + fmz[k];
+        }
+        if (!VTI) {
+            gl[k] += a.wlm * (axx[k] + azz[k]) * (t0x[k] + t1z[k]);
+            gm[k] += a.wlm * ((azx[k] + axz[k]) * (t2x[k] + t3z[k]) + 2.f * axx[k] * t0x[k] + 2.f * azz[k] * t1z[k]);
+        } else if (a.stiff) {
+            // d/dc11, d/dc13, d/dc33, d/dc55 of the same bilinear form
+            gl[k] += a.wlm * axx[k] * t0x[k];
+            gm[k] += a.wlm * (axx[k] * t1z[k] + azz[k] * t0x[k]);
+            gr[k] += a.wlm * azz[k] * t1z[k];
+            g3[k] += a.wlm * (azx[k] + axz[k]) * (t2x[k] + t3z[k]);
+        } else {
+            const float cr = axx[k] * t1z[k] + azz[k] * t0x[k];
+            gl[k] += a.wlm * (e[k] * axx[k] * t0x[k] + azz[k] * t1z[k] + v2[k] * cr);
+            gm[k] += a.wlm * ((azx[k] + axz[k]) * (t2x[k] + t3z[k]) + 2.f * e[k] * axx[k] * t0x[k] +
+                              2.f * azz[k] * t1z[k] + v3[k] * cr);
+        }
+        if (VTI && a.stiff) g4[k] += a.wrho * (ox[k] * d2x + oz[k] * d2z);
+        else gr[k] += a.wrho * (ox[k] * d2x + oz[k] * d2z);
+    }
+}
+
 // Hides a value from the optimiser so that address arithmetic derived from it is
 // recomputed inside the shot loop instead of being hoisted out and kept live in
 // registers across it (which multiplies the register footprint of the loop body).
@@ -672,9 +757,10 @@ __global__ void __launch_bounds__(kThreads) k_step(const StepArgs a, const int4*
 #pragma unroll
     for (int m = 0; m < S::NQ2; m++) {
         const int i = tid + m * kThreads;
-        const int lc = i / S::PQ, lq = i - lc * S::PQ;
+        int lc, lq;
+        const bool slot = prod_slot<S>(i, lc, lq);
         const int gx = t.plx + lc, z0 = t.pz0 + 4 * lq;
-        const bool ok = i < S::IX * S::PQ && gx < t.phx && z0 + 3 >= t.plz && z0 < t.phz;
+        const bool ok = slot && gx < t.phx && z0 + 3 >= t.plz && z0 < t.phz;
         const int g = ok ? gx * ldz + z0 : 0;
         const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
         plam[m] = ok ? ld4(a.lam + g) : z4;
@@ -790,8 +876,8 @@ __global__ void __launch_bounds__(kThreads) k_step(const StepArgs a, const int4*
 #pragma unroll
             for (int m = 0; m < S::NQ2; m++) {
                 const int i = tid + m * kThreads;
-                if (i >= S::IX * S::PQ) break;
-                const int lc = i / S::PQ, lq = i - lc * S::PQ;
+                int lc, lq;
+                if (!prod_slot<S>(i, lc, lq)) continue;
                 const int gx = t.plx + lc, z0 = t.pz0 + 4 * lq;
                 const bool ok = gx < t.phx && z0 + 3 >= t.plz && z0 < t.phz;
                 const int g = ok ? gx * ldz + z0 : 0;
@@ -808,8 +894,8 @@ __global__ void __launch_bounds__(kThreads) k_step(const StepArgs a, const int4*
 #pragma unroll
         for (int m = 0; m < S::NQ2; m++) {
             const int i = tid + m * kThreads;
-            if (i >= S::IX * S::PQ) break;
-            const int lc = i / S::PQ, lq = i - lc * S::PQ;
+            int lc, lq;
+            if (!prod_slot<S>(i, lc, lq)) continue;
             const int gx = t.plx + lc, z0 = t.pz0 + 4 * lq;
             float p[4] = {0.f, 0.f, 0.f, 0.f}, pz[4] = {0.f, 0.f, 0.f, 0.f}, qx[4] = {0.f, 0.f, 0.f, 0.f},
                   qz[4] = {0.f, 0.f, 0.f, 0.f};
@@ -904,45 +990,8 @@ __global__ void __launch_bounds__(kThreads) k_step(const StepArgs a, const int4*
                     unpack(ld4(a.fp + fo), fpx);
                     unpack(ld4(a.fp + fo + nxz), fpz);
                 }
-                float e[4], v2[4], v3[4];
-                if (VTI) {
-                    unpack(ld4(a.ge + g), e);
-                    unpack(ld4(a.gv2 + g), v2);
-                    unpack(ld4(a.gv3 + g), v3);
-                }
-#pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    if (z0 + k >= nz) continue;
-                    float d2x, d2z;
-                    if (a.img_mode == 0) {
-                        d2x = fpx[k] - 2.f * f0x[k] + fmx[k];
-                        d2z = fpz[k] - 2.f * f0z[k] + fmz[k];
-                    } else if (a.img_mode == 1) {
-                        d2x = fpx[k] - f0x[k];
-                        d2z = fpz[k] - f0z[k];
-                    } else {
-                        d2x = -f0x[k] + fmx[k];
-                        d2z = -f0z[k] + fmz[k];
-                    }
-                    if (!VTI) {
-                        gl[m][k] += a.wlm * (axx[k] + azz[k]) * (t0x[k] + t1z[k]);
-                        gm[m][k] += a.wlm * ((azx[k] + axz[k]) * (t2x[k] + t3z[k]) + 2.f * axx[k] * t0x[k] +
-                                             2.f * azz[k] * t1z[k]);
-                    } else if (a.stiff) {
-                        // d/dc11, d/dc13, d/dc33, d/dc55 of the same bilinear form
-                        gl[m][k] += a.wlm * axx[k] * t0x[k];
-                        gm[m][k] += a.wlm * (axx[k] * t1z[k] + azz[k] * t0x[k]);
-                        gr[m][k] += a.wlm * azz[k] * t1z[k];
-                        g3[m][k] += a.wlm * (azx[k] + axz[k]) * (t2x[k] + t3z[k]);
-                    } else {
-                        const float cr = axx[k] * t1z[k] + azz[k] * t0x[k];
-                        gl[m][k] += a.wlm * (e[k] * axx[k] * t0x[k] + azz[k] * t1z[k] + v2[k] * cr);
-                        gm[m][k] += a.wlm * ((azx[k] + axz[k]) * (t2x[k] + t3z[k]) + 2.f * e[k] * axx[k] * t0x[k] +
-                                             2.f * azz[k] * t1z[k] + v3[k] * cr);
-                    }
-                    if (VTI && a.stiff) g4[m][k] += a.wrho * (ox[k] * d2x + oz[k] * d2z);
-                    else gr[m][k] += a.wrho * (ox[k] * d2x + oz[k] * d2z);
-                }
+                image_accumulate<VTI>(a, g, z0, axx, azx, axz, azz, t0x, t1z, t2x, t3z, f0x, f0z, fmx, fmz, fpx, fpz,
+                                      ox, oz, gl[m], gm[m], gr[m], g3[VTI ? m : 0], g4[VTI ? m : 0]);
             }
         }
     }
@@ -967,6 +1016,168 @@ __global__ void __launch_bounds__(kThreads) k_step(const StepArgs a, const int4*
                 v = ld4(G + 4 * nxz + g);
                 st4(G + 4 * nxz + g, make_float4(v.x + g4[m][0], v.y + g4[m][1], v.z + g4[m][2], v.w + g4[m][3]));
             }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Imaging condition after an adjoint step (compute_gradients, we_op.cpp:505-562).
+// a.curr is the lagged adjoint level (its first derivatives are imaged), a.next the
+// level after the step and its point sources (the density term), a.f0/fm/fp forward
+// snapshots idx, idx-1, idx+1.  A workgroup owns a TX x 64 tile of the tile grid and a
+// group of a.spb shots; per shot the adjoint and snapshot windows (halo 2 in x, 4 in z)
+// go through LDS, the next shot's windows are loaded into registers meanwhile, and the
+// contributions are summed in registers and added once to the gradient slab of the
+// group's first shot.  Tiles with every cell at least 4 from the edges take the interior
+// stencils only; the others choose per quad.
+// ---------------------------------------------------------------------------
+template <bool VTI, int TX>
+__global__ void __launch_bounds__(kThreads) k_img(const StepArgs a)
+{
+    constexpr int TZ = kTileZ, HX = 2, HZ = 4;
+    constexpr int WX = TX + 2 * HX, WZ = TZ + 2 * HZ, WQ = WZ / 4;
+    constexpr int NQ1 = (WX * WQ + kThreads - 1) / kThreads;
+    constexpr int QC = TZ / 4, NQ3 = TX * QC / kThreads;
+    static_assert((TX * QC) % kThreads == 0, "tile must give every thread whole quads");
+    __shared__ __attribute__((aligned(16))) float sAx[WX * WZ];
+    __shared__ __attribute__((aligned(16))) float sAz[WX * WZ];
+    __shared__ __attribute__((aligned(16))) float sFx[WX * WZ];
+    __shared__ __attribute__((aligned(16))) float sFz[WX * WZ];
+
+    const int lin = xcd_remap(blockIdx.x, gridDim.x);
+    const int tile = lin / a.ngroups;
+    const int group = lin - tile * a.ngroups;
+    const int tix = tile / a.ntz, tiz = tile - tix * a.ntz;
+    const int nx = a.nx, nz = a.nz, ldz = a.ldz;
+    int sx, ex, sz, ez;
+    tile_range_x_hd(tix, a.ntx, TX, nx, sx, ex);
+    tile_range_z_hd(tiz, a.ntz, nz, ldz, sz, ez);
+    const int wsx = window_start(sx, HX), wsz = sz - HZ;
+    const bool inner = sx >= 4 && ex <= nx - 4 && sz >= 4 && ez <= nz - 4;
+    const long long nxz = (long long)nx * ldz;
+    const int tid = threadIdx.x;
+    const int s0 = group * a.spb;
+    const int s1 = min(a.nshots, s0 + a.spb);
+
+    float4 rax[NQ1], raz[NQ1], rfx[NQ1], rfz[NQ1];
+    auto load_windows = [&](int shot, int tid, int wsx, int wsz) {
+        const float* A = a.curr + shot * a.shot_stride;
+        const float* F = a.f0 + shot * a.fshot_stride;
+#pragma unroll
+        for (int m = 0; m < NQ1; m++) {
+            const int i = tid + m * kThreads;
+            const int lx = i / WQ, lq = i - lx * WQ;
+            const int gx = wsx + lx, gz = wsz + 4 * lq;
+            const bool ok = i < WX * WQ && gx >= 0 && gx < nx && gz >= 0 && gz < ldz;
+            const long long g = ok ? (long long)gx * ldz + gz : 0;
+            const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+            rax[m] = ok ? ld4(A + g) : z4;
+            raz[m] = ok ? ld4(A + nxz + g) : z4;
+            rfx[m] = ok ? ld4(F + g) : z4;
+            rfz[m] = ok ? ld4(F + nxz + g) : z4;
+        }
+    };
+    if (s0 < s1) load_windows(s0, tid, wsx, wsz);
+
+    float gl[NQ3][4], gm[NQ3][4], gr[NQ3][4], g3[VTI ? NQ3 : 1][4], g4[VTI ? NQ3 : 1][4];
+#pragma unroll
+    for (int m = 0; m < NQ3; m++)
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            gl[m][k] = gm[m][k] = gr[m][k] = 0.f;
+            if (VTI) g3[m][k] = g4[m][k] = 0.f;
+        }
+
+    const int sx0 = sx, ex0 = ex, sz0 = sz, ez0 = ez, wsx0 = wsx, tid0 = tid;
+    for (int shot = s0; shot < s1; shot++) {
+        int sx = sx0, ex = ex0, sz = sz0, ez = ez0, wsx = wsx0, tid = tid0;
+        opaque(sx);
+        opaque(ex);
+        opaque(sz);
+        opaque(ez);
+        opaque(wsx);
+        opaque_v(tid);
+        const int wsz = sz - HZ;
+        // cell values of this shot: adjoint density level, snapshots idx-1 and idx+1
+        float4 qnx[NQ3], qnz[NQ3], qmx[NQ3], qmz[NQ3], qpx[NQ3], qpz[NQ3];
+#pragma unroll
+        for (int m = 0; m < NQ3; m++) {
+            const int i = tid + m * kThreads;
+            const int gx = sx + i / QC, z0 = sz + 4 * (i % QC);
+            const bool ok = gx < ex && z0 < ez;
+            const long long g = ok ? (long long)gx * ldz + z0 : 0;
+            const long long fo = shot * a.fshot_stride + g;
+            const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+            const float* N = a.next + shot * a.shot_stride;
+            qnx[m] = ok ? ld4(N + g) : z4;
+            qnz[m] = ok ? ld4(N + nxz + g) : z4;
+            qmx[m] = ok && a.img_mode != 1 ? ld4(a.fm + fo) : z4;
+            qmz[m] = ok && a.img_mode != 1 ? ld4(a.fm + fo + nxz) : z4;
+            qpx[m] = ok && a.img_mode != 2 ? ld4(a.fp + fo) : z4;
+            qpz[m] = ok && a.img_mode != 2 ? ld4(a.fp + fo + nxz) : z4;
+        }
+        __syncthreads();  // the previous shot is done with the windows
+#pragma unroll
+        for (int m = 0; m < NQ1; m++) {
+            const int i = tid + m * kThreads;
+            if (i < WX * WQ) {
+                const int lx = i / WQ, lq = i - lx * WQ;
+                const int o = lx * WZ + 4 * lq;
+                st4(&sAx[o], rax[m]);
+                st4(&sAz[o], raz[m]);
+                st4(&sFx[o], rfx[m]);
+                st4(&sFz[o], rfz[m]);
+            }
+        }
+        __syncthreads();
+        if (shot + 1 < s1) load_windows(shot + 1, tid, wsx, wsz);
+#pragma unroll
+        for (int m = 0; m < NQ3; m++) {
+            const int i = tid + m * kThreads;
+            const int gx = sx + i / QC, z0 = sz + 4 * (i % QC);
+            if (gx >= ex || z0 >= ez) continue;
+            const int w = (gx - wsx) * WZ + (z0 - wsz);
+            float axx[4], azx[4], axz[4], azz[4], t0x[4], t1z[4], t2x[4], t3z[4];
+            if (inner) {
+                quad_derivs<false, WZ>(sAx, sAz, w, gx, z0, nx, nz, a.rdx, a.rdz, axx, azx, axz, azz);
+                quad_derivs<false, WZ>(sFx, sFz, w, gx, z0, nx, nz, a.rdx, a.rdz, t0x, t3z, t2x, t1z);
+            } else {
+                quad_derivs<true, WZ>(sAx, sAz, w, gx, z0, nx, nz, a.rdx, a.rdz, axx, azx, axz, azz);
+                quad_derivs<true, WZ>(sFx, sFz, w, gx, z0, nx, nz, a.rdx, a.rdz, t0x, t3z, t2x, t1z);
+            }
+            float f0x[4], f0z[4], fmx[4], fmz[4], fpx[4], fpz[4], ox[4], oz[4];
+            unpack(ld4(&sFx[w]), f0x);
+            unpack(ld4(&sFz[w]), f0z);
+            unpack(qmx[m], fmx);
+            unpack(qmz[m], fmz);
+            unpack(qpx[m], fpx);
+            unpack(qpz[m], fpz);
+            unpack(qnx[m], ox);
+            unpack(qnz[m], oz);
+            const long long g = (long long)gx * ldz + z0;
+            image_accumulate<VTI>(a, g, z0, axx, azx, axz, azz, t0x, t1z, t2x, t3z, f0x, f0z, fmx, fmz, fpx, fpz, ox,
+                                  oz, gl[m], gm[m], gr[m], g3[VTI ? m : 0], g4[VTI ? m : 0]);
+        }
+    }
+    if (s0 >= s1) return;
+    float* G = a.grad + s0 * a.gslab_stride;
+#pragma unroll
+    for (int m = 0; m < NQ3; m++) {
+        const int i = tid + m * kThreads;
+        const int gx = sx + i / QC, z0 = sz + 4 * (i % QC);
+        if (gx >= ex || z0 >= ez) continue;
+        const long long g = (long long)gx * ldz + z0;
+        float4 v = ld4(G + g);
+        st4(G + g, make_float4(v.x + gl[m][0], v.y + gl[m][1], v.z + gl[m][2], v.w + gl[m][3]));
+        v = ld4(G + nxz + g);
+        st4(G + nxz + g, make_float4(v.x + gm[m][0], v.y + gm[m][1], v.z + gm[m][2], v.w + gm[m][3]));
+        v = ld4(G + 2 * nxz + g);
+        st4(G + 2 * nxz + g, make_float4(v.x + gr[m][0], v.y + gr[m][1], v.z + gr[m][2], v.w + gr[m][3]));
+        if (VTI && a.stiff) {
+            v = ld4(G + 3 * nxz + g);
+            st4(G + 3 * nxz + g, make_float4(v.x + g3[m][0], v.y + g3[m][1], v.z + g3[m][2], v.w + g3[m][3]));
+            v = ld4(G + 4 * nxz + g);
+            st4(G + 4 * nxz + g, make_float4(v.x + g4[m][0], v.y + g4[m][1], v.z + g4[m][2], v.w + g4[m][3]));
         }
     }
 }
@@ -1305,6 +1516,19 @@ void launch_step(const StepArgs& a, bool vti, BoxClass c, const int4* boxes, con
 {
     if (vti) step_class<true>(a, c, boxes, list, n, image, s);
     else step_class<false>(a, c, boxes, list, n, image, s);
+}
+
+void launch_img(const StepArgs& a, bool vti, int tile_x, hipStream_t s)
+{
+    const int n = a.ntx * a.ntz * a.ngroups;
+    if (n <= 0) return;
+    if (vti) {
+        if (tile_x == 32) k_img<true, 32><<<n, kThreads, 0, s>>>(a);
+        else k_img<true, 16><<<n, kThreads, 0, s>>>(a);
+    } else {
+        if (tile_x == 32) k_img<false, 32><<<n, kThreads, 0, s>>>(a);
+        else k_img<false, 16><<<n, kThreads, 0, s>>>(a);
+    }
 }
 
 template <bool VTI>

@@ -231,12 +231,113 @@ def check_image_axis(n, T, quads):
                 assert 0 <= g + o - wsi < Wi and (g + o) in loaded_i, ("image", n, T, s, g, o)
 
 
+def check_img_tiles(nx, nz, T):
+    """k_img: TX x 64 tiles of the tile grid, windows with halo 2 in x and 4 in z (quad
+    reads of rows z0-4 .. z0+7); a tile with every cell 4 or more from the edges takes
+    the interior stencils only."""
+    ldz = (nz + 3) // 4 * 4
+    for sx, ex in tiles(nx, T):
+        wsx, WX = window_start(sx, 2), T + 4
+        lx = set(p for p in range(wsx, wsx + WX) if 0 <= p < nx)
+        for sz, ez in tiles_quads(nz, 64):
+            wsz, WZ = sz - 4, 72
+            lz = set(p for p in range(wsz, wsz + WZ) if 0 <= p < nz)
+            inner = sx >= 4 and ex <= nx - 4 and sz >= 4 and ez <= nz - 4
+            for g in range(sx, ex):
+                fast = 4 <= g < nx - 4
+                assert fast or not inner
+                for o in ([-2, -1, 1, 2] if fast else d1_offsets(g, nx)):
+                    assert 0 <= g + o - wsx < WX and (g + o) in lx, ("img x", nx, T, sx, g, o)
+            for z0 in range(sz, ez, 4):
+                assert z0 + 3 < ldz
+                fast = z0 >= 4 and z0 + 3 < nz - 4
+                assert fast or not inner
+                if fast:
+                    assert wsz <= z0 - 4 and z0 + 7 < wsz + WZ
+                    assert all(z0 + k + o in lz for k in range(4) for o in (-2, -1, 1, 2))
+                else:
+                    for k in range(4):
+                        g = z0 + k
+                        if g < nz:
+                            for o in d1_offsets(g, nz):
+                                assert 0 <= g + o - wsz < WZ and (g + o) in lz, ("img z", nz, sz, g, o)
+
+
+@pytest.mark.parametrize("T", [16, 32])
+def test_img_tiles(T):
+    for nx in SIZES:
+        check_img_tiles(nx, 70, T)
+    for nz in list(range(12, 300)) + [501, 1000]:
+        check_img_tiles(40, nz, T)
+
+
 def test_image_tiles():
     for n in SIZES:
         check_image_axis(n, 16, False)
         check_image_axis(n, 32, False)
     for n in list(range(12, 300)) + [501, 1000]:
         check_image_axis(n, TZ, True)
+
+
+THREADS, WAVE = 256, 64
+
+
+def shape_consts(TX, TZb):
+    """StepShape of kernels.hip: lane mapping constants of one box class."""
+    rowmaj = TZb <= 16
+    WQ, PQ, IX = (TZb + 16) // 4, (TZb + 8) // 4, TX + 4
+    WZ = 4 * ((WQ | 1) if rowmaj else WQ)
+    PZ = 4 * ((PQ | 1) if rowmaj else PQ)
+    IXP = (IX + 63) // 64 * 64 if rowmaj else IX
+    QC = TZb // 4
+    return dict(rowmaj=rowmaj, WQ=WQ, PQ=PQ, IX=IX, WZ=WZ, PZ=PZ, IXP=IXP, QC=QC, WX=TX + 8,
+                NQ2=(IXP * PQ + THREADS - 1) // THREADS, NQ3=TX * QC // THREADS)
+
+
+def out_quad(c, TX, i):
+    """(column, quad row) of new-level slot i (out_quad of kernels.hip)."""
+    if c["rowmaj"]:
+        return i % TX, i // TX
+    return i // c["QC"], i % c["QC"]
+
+
+def prod_slot(c, i):
+    """(column, quad row, valid) of product slot i (prod_slot of kernels.hip)."""
+    if c["rowmaj"]:
+        lq, lc = i // c["IXP"], i % c["IXP"]
+        return lc, lq, lc < c["IX"] and lq < c["PQ"]
+    return i // c["PQ"], i % c["PQ"], i < c["IX"] * c["PQ"]
+
+
+@pytest.mark.parametrize("cls", sorted(SHAPES))
+def test_lane_mappings(cls):
+    """Every new-level quad and product quad of a box has exactly one slot; in row-major
+    boxes the quad row of a slot (made a scalar with readfirstlane in the kernel) is the
+    same on all 64 lanes of its wavefront; LDS column strides hold the windows."""
+    TX, TZb, _ = SHAPES[cls]
+    c = shape_consts(TX, TZb)
+    assert (TX * c["QC"]) % THREADS == 0
+    seen = set()
+    for i in range(c["NQ3"] * THREADS):
+        col, row = out_quad(c, TX, i)
+        assert 0 <= col < TX and 0 <= row < c["QC"]
+        seen.add((col, row))
+        if c["rowmaj"]:
+            assert out_quad(c, TX, i - i % WAVE)[1] == row
+    assert len(seen) == TX * c["QC"]
+    seen = set()
+    for i in range(c["NQ2"] * THREADS):
+        lc, lq, ok = prod_slot(c, i)
+        if c["rowmaj"]:
+            assert prod_slot(c, i - i % WAVE)[1] == lq
+        if ok:
+            assert (lc, lq) not in seen
+            seen.add((lc, lq))
+    assert len(seen) == c["IX"] * c["PQ"]
+    assert c["WZ"] >= TZb + 16 and c["PZ"] >= TZb + 8 and c["WZ"] % 4 == 0 and c["PZ"] % 4 == 0
+    if c["rowmaj"]:
+        # column-strided 16-byte LDS reads of consecutive lanes: odd quad strides
+        assert (c["WZ"] // 4) % 2 == 1 and (c["PZ"] // 4) % 2 == 1
 
 
 def xcd_remap(bid, nb):
