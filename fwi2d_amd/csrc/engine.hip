@@ -107,7 +107,7 @@ struct Engine {
     int device = 0;
     hipStream_t st = nullptr;
     int np = 3;
-    int ldz = 0;          // device z pitch: nz rounded up to a multiple of 4
+    int ldz = 0;          // device z pitch: nz rounded up to a multiple of 32 cells (128 bytes)
     long long nxz = 0;    // nx * ldz, the device size of one field
     int tile_x = 16, ntx = 0, ntz = 0;
     std::vector<int4> boxes;                        // {sx, ex, sz, ez} of every step box
@@ -135,8 +135,9 @@ struct Engine {
     std::string tune_report;
     int opt_spb = 8;       // shots advanced one after the other by each core-box workgroup
     int opt_spb_edge = 1;  // the same for edge tiles (their workgroups are latency-bound)
-    int opt_spb_img = 0;   // shots per imaging workgroup (0: as opt_spb)
+    int opt_spb_img = 0;   // shots per imaging workgroup (0: 4, or opt_spb when fused)
     int opt_image_fused = 0;  // 1: imaging condition fused into the adjoint step kernels
+    int opt_img_buffers = 1;  // LDS window buffers of the imaging kernel (1 or 2)
     int opt_edge_stream = 1;  // edge-band launches on their own stream
     hipStream_t st_edge = nullptr;       // edge-tile launches run beside the interior ones
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
@@ -178,7 +179,7 @@ struct Engine {
             if (b != 1 && b != 2) throw std::runtime_error("boundary conditions must be 1 (free surface) or 2 (absorbing)");
         vti = g.vti != 0;
         np = vti ? 5 : 3;
-        ldz = (g.nz + 3) & ~3;
+        ldz = (g.nz + 31) & ~31;
         nxz = (long long)g.nx * ldz;
         if (nxz >= (1LL << 31)) throw std::runtime_error("grid too large: nx * nz must stay below 2^31 cells");
         HIPCHECK(hipSetDevice(device));
@@ -636,10 +637,11 @@ struct Engine {
         if (img_idx >= 0) {
             set_image_args(a, img_idx);
             ia = a;
-            if (opt_spb_img > 0) {
-                ia.spb = std::max(1, std::min(opt_spb_img, ns));
-                ia.ngroups = (ns + ia.spb - 1) / ia.spb;
-            }
+            // shots per imaging workgroup: the separate kernel is fastest with small
+            // groups (more workgroups in flight; measured 4 < 16 < 64 at 64 shots)
+            const int spbi = opt_spb_img > 0 ? opt_spb_img : (fuse_img ? opt_spb : 4);
+            ia.spb = std::max(1, std::min(spbi, ns));
+            ia.ngroups = (ns + ia.spb - 1) / ia.spb;
             if (fuse_img) a = ia;
         }
         a.it = it;
@@ -693,7 +695,7 @@ struct Engine {
         if (img_idx >= 0 && !fuse_img) {
             ia.curr = L.C();
             ia.next = L.N();
-            launch_img(ia, vti, tile_x, st);
+            launch_img(ia, vti, tile_x, opt_img_buffers, st);
         }
         L.rotate();
         stat_steps++;
@@ -755,7 +757,7 @@ struct Engine {
     {
         auto P = [](const void* p) { return (long long)(uintptr_t)p; };
         return {kind, P(p0), P(p1), flag, g.nt, ns, nr, tile_x, opt_spb, opt_spb_edge, opt_lag, seg_K, nseg,
-                opt_bf16, opt_stiff, opt_edge_stream, opt_spb_img, opt_image_fused, P(d_ckpt16.p), P(d_ckpt_scale.p), P(d_amax.p),
+                opt_bf16, opt_stiff, opt_edge_stream, opt_spb_img, opt_image_fused, opt_img_buffers, P(d_ckpt16.p), P(d_ckpt_scale.p), P(d_amax.p),
                 P(d_fwd.p), P(d_adj.p), P(d_ckpt.p), P(d_seg.p), P(d_gshot.p), P(d_model.p), P(d_deriv.p),
                 P(d_boxes.p), P(d_list_core.p), P(d_list_bz.p), P(d_list_bz8.p), P(d_list_bx.p), P(inj_src.d_cells.p), P(inj_src.d_ptr.p),
                 P(inj_src.d_ctrace.p), P(inj_src.d_cw.p), P(inj_rcv.d_cells.p), P(inj_rcv.d_ptr.p), P(inj_rcv.d_ctrace.p),
@@ -1272,6 +1274,9 @@ int fwi2d_engine_set_option(fwi2d_engine* e, const char* key, long long value)
             x->opt_spb_edge = (int)value;
         } else if (k == "image_fused") {
             x->opt_image_fused = value ? 1 : 0;
+        } else if (k == "image_buffers") {
+            if (value != 1 && value != 2) throw std::runtime_error("image_buffers must be 1 or 2");
+            x->opt_img_buffers = (int)value;
         } else {
             throw std::runtime_error("unknown option " + k);
         }
@@ -1294,6 +1299,7 @@ long long fwi2d_engine_get_stat(fwi2d_engine* e, const char* key)
     if (k == "shots_per_block_img") return x->opt_spb_img;
     if (k == "edge_stream") return x->opt_edge_stream;
     if (k == "image_fused") return x->opt_image_fused;
+    if (k == "image_buffers") return x->opt_img_buffers;
     if (k == "ntiles") return x->ntiles();
     if (k == "interior_tiles") return (long long)x->list_core.size();
     if (k == "graph_captures") return x->graph_captures;

@@ -1,7 +1,8 @@
 // Kernel argument blocks and launchers of the elastic propagation engine (gfx950).
 //
 // Device layout: every field is [x][ldz] with z contiguous and ldz = nz rounded up to a
-// multiple of 4, so that each thread moves 4 consecutive z cells (a "quad") with one
+// multiple of 32 (128-byte columns, so that a 64-cell column segment of a tile is two
+// whole cache lines); each thread moves 4 consecutive z cells (a "quad") with one
 // 128-bit access; the padding cells are zero and never read as stencil data.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -113,8 +114,9 @@ void launch_extract(const float* state, long long shot_stride, long long nxz, in
                     int n, float* trace, long long comp_stride, int it, hipStream_t s);
 void launch_image(const ImageArgs& a, bool vti, int tile_x, hipStream_t s);
 // imaging condition after an adjoint step over the tile_x x 64 tile grid (a.ntx x a.ntz
-// tiles, a.ngroups groups of a.spb shots); a.curr lagged adjoint level, a.next the new one
-void launch_img(const StepArgs& a, bool vti, int tile_x, hipStream_t s);
+// tiles, a.ngroups groups of a.spb shots); a.curr lagged adjoint level, a.next the new
+// one; nbuf 2: the next shot's windows load into a second LDS buffer during this shot
+void launch_img(const StepArgs& a, bool vti, int tile_x, int nbuf, hipStream_t s);
 void launch_sum_shots(const float* per_shot, long long shot_stride, int nshots, long long n, float* out,
                       int accumulate, hipStream_t s);
 void launch_fill(float* p, long long n, float v, hipStream_t s);

@@ -16,30 +16,20 @@
 
 namespace fwi2d {
 
-__constant__ float cD1B[4][6];
 __constant__ float cD1I[2];
 __constant__ float cSC[4];
 __constant__ float cD2I[10];
-__constant__ float cD2B[6][8][8];
 
 void upload_constants(hipStream_t s)
 {
-    float d1b[4][6], d1i[2], sc[4], d2i[10], d2b[6][8][8];
-    for (int i = 0; i < 4; i++)
-        for (int j = 0; j < 6; j++) d1b[i][j] = (float)kD1Closure[i][j];
+    float d1i[2], sc[4], d2i[10];
     d1i[0] = (float)kD1Interior[0];
     d1i[1] = (float)kD1Interior[1];
     for (int i = 0; i < 4; i++) sc[i] = (float)kBoundaryDeriv[i];
     for (int i = 0; i < 10; i++) d2i[i] = (float)kD2Interior[i];
-    for (int r = 0; r < 6; r++)
-        for (int j = 0; j < 8; j++)
-            for (int k = 0; k < 8; k++) d2b[r][j][k] = 0.f;
-    for (const auto& e : kD2ClosureNZ) d2b[e.row][e.point][e.coef] = (float)(e.num / e.den);
-    (void)hipMemcpyToSymbolAsync(HIP_SYMBOL(cD1B), d1b, sizeof(d1b), 0, hipMemcpyHostToDevice, s);
     (void)hipMemcpyToSymbolAsync(HIP_SYMBOL(cD1I), d1i, sizeof(d1i), 0, hipMemcpyHostToDevice, s);
     (void)hipMemcpyToSymbolAsync(HIP_SYMBOL(cSC), sc, sizeof(sc), 0, hipMemcpyHostToDevice, s);
     (void)hipMemcpyToSymbolAsync(HIP_SYMBOL(cD2I), d2i, sizeof(d2i), 0, hipMemcpyHostToDevice, s);
-    (void)hipMemcpyToSymbolAsync(HIP_SYMBOL(cD2B), d2b, sizeof(d2b), 0, hipMemcpyHostToDevice, s);
     (void)hipStreamSynchronize(s);
 }
 
@@ -117,18 +107,35 @@ __device__ __forceinline__ float d1_int(const float* p, int s)
     return cD1I[0] * (p[s] - p[-s]) + cD1I[1] * (p[2 * s] - p[-2 * s]);
 }
 
-__device__ __forceinline__ float d1_closure(const float* p, int s, int g, int n)
+// Closure row R of D1 from the edge point q along stride st, with the coefficients as
+// immediates (zero ones skipped).
+template <int R>
+__device__ __forceinline__ float d1_closure_row(const float* q, int st)
 {
     float v = 0.f;
-    if (g < 4) {
-        const float* q = p - g * s;
-        for (int j = 0; j < 6; j++) v += cD1B[g][j] * q[j * s];
-    } else {
-        const int r = n - 1 - g;
-        const float* q = p + r * s;
-        for (int j = 0; j < 6; j++) v -= cD1B[r][j] * q[-j * s];
-    }
+#pragma unroll
+    for (int j = 0; j < 6; j++)
+        if (kD1Closure[R][j] != 0.0) v += (float)kD1Closure[R][j] * q[j * st];
     return v;
+}
+
+// D1 closure row of point g of n: rows 0..3 at the near edge, mirrored with opposite sign
+// at the far edge.  The row is dispatched to a compile-time instance (a uniform branch
+// where the row is the same for a whole wavefront).
+__device__ __forceinline__ float d1_closure(const float* p, int s, int g, int n)
+{
+    const bool near = g < 4;
+    const int r = near ? g : n - 1 - g;
+    const float* q = near ? p - g * s : p + r * s;
+    const int st = near ? s : -s;
+    float v;
+    switch (r) {
+    case 0: v = d1_closure_row<0>(q, st); break;
+    case 1: v = d1_closure_row<1>(q, st); break;
+    case 2: v = d1_closure_row<2>(q, st); break;
+    default: v = d1_closure_row<3>(q, st); break;
+    }
+    return near ? v : -v;
 }
 
 template <bool IN>
@@ -166,45 +173,70 @@ __device__ __forceinline__ float d2_apply(const W5& w, const float* pu, int s)
     return w.c * pu[0] + w.m1 * pu[-s] + w.p1 * pu[s] + w.m2 * pu[-2 * s] + w.p2 * pu[2 * s];
 }
 
-// closure rows of D2(c) u (the 6x8x8 table)
-__device__ __forceinline__ float d2_closure(const float* pu, const float* pc, int s, int g, int n)
+// Closure rows of D2(c): row R of the 6 x 8 x 8 table applied to one or two fields
+// sharing the coefficient, from the edge point along stride st.  The table's non-zeros
+// are compile-time immediates; the weights of the 8 field points are formed first.
+constexpr int kD2NZ = sizeof(kD2ClosureNZ) / sizeof(kD2ClosureNZ[0]);
+constexpr int d2_row_begin(int r)
 {
-    float out = 0.f;
-    const bool top = g < 6;
-    const int r = top ? g : n - 1 - g;
-    const int dir = top ? s : -s;
-    const float* qu = top ? pu - g * s : pu + r * s;
-    const float* qc = top ? pc - g * s : pc + r * s;
-#pragma unroll 1
-    for (int j = 0; j < 8; j++) {
-        float val = 0.f;
-        for (int k = 0; k < 8; k++) val += cD2B[r][j][k] * qc[k * dir];
-        out += val * qu[j * dir];
-    }
-    return out;
+    int i = 0;
+    while (i < kD2NZ && kD2ClosureNZ[i].row < r) i++;
+    return i;
 }
 
-// closure rows of D2(c) for two fields sharing the coefficient (weights formed once)
-__device__ __forceinline__ void d2_closure2(const float* pu, const float* pv, const float* pc, int s, int g, int n,
-                                            float& ou, float& ov)
+template <int R, bool TWO>
+__device__ __forceinline__ void d2_closure_row(const float* qu, const float* qv, const float* qc, int st, float& ou,
+                                               float& ov)
 {
-    const bool top = g < 6;
-    const int r = top ? g : n - 1 - g;
-    const int dir = top ? s : -s;
-    const int o = top ? -g * s : r * s;
-    const float* qu = pu + o;
-    const float* qv = pv + o;
-    const float* qc = pc + o;
+    float val[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) val[j] = 0.f;
+    constexpr int b = d2_row_begin(R), e = d2_row_begin(R + 1);
+#pragma unroll
+    for (int i = b; i < e; i++)
+        val[kD2ClosureNZ[i].point] +=
+            (float)(kD2ClosureNZ[i].num / kD2ClosureNZ[i].den) * qc[kD2ClosureNZ[i].coef * st];
     float cu = 0.f, cv = 0.f;
-#pragma unroll 1
+#pragma unroll
     for (int j = 0; j < 8; j++) {
-        float val = 0.f;
-        for (int k = 0; k < 8; k++) val += cD2B[r][j][k] * qc[k * dir];
-        cu += val * qu[j * dir];
-        cv += val * qv[j * dir];
+        cu += val[j] * qu[j * st];
+        if (TWO) cv += val[j] * qv[j * st];
     }
     ou = cu;
     ov = cv;
+}
+
+// closure rows of D2(c) for one field (TWO = false) or two fields sharing the coefficient
+template <bool TWO>
+__device__ __forceinline__ void d2_closure_any(const float* pu, const float* pv, const float* pc, int s, int g, int n,
+                                               float& ou, float& ov)
+{
+    const bool top = g < 6;
+    const int r = top ? g : n - 1 - g;
+    const int st = top ? s : -s;
+    const int o = top ? -g * s : r * s;
+    const float *qu = pu + o, *qv = pv + o, *qc = pc + o;
+    switch (r) {
+    case 0: d2_closure_row<0, TWO>(qu, qv, qc, st, ou, ov); break;
+    case 1: d2_closure_row<1, TWO>(qu, qv, qc, st, ou, ov); break;
+    case 2: d2_closure_row<2, TWO>(qu, qv, qc, st, ou, ov); break;
+    case 3: d2_closure_row<3, TWO>(qu, qv, qc, st, ou, ov); break;
+    case 4: d2_closure_row<4, TWO>(qu, qv, qc, st, ou, ov); break;
+    default: d2_closure_row<5, TWO>(qu, qv, qc, st, ou, ov); break;
+    }
+}
+
+__device__ __forceinline__ float d2_closure(const float* pu, const float* pc, int s, int g, int n)
+{
+    float ou, ov;
+    d2_closure_any<false>(pu, pu, pc, s, g, n, ou, ov);
+    return ou;
+}
+
+__device__ __forceinline__ void d2_closure2(const float* pu, const float* pv, const float* pc, int s, int g, int n,
+                                            float& ou, float& ov)
+{
+    d2_closure_any<true>(pu, pv, pc, s, g, n, ou, ov);
 }
 
 // one-sided outward boundary derivative sum_k SC[k] u[k*s] (times h)
@@ -214,13 +246,7 @@ __device__ __forceinline__ float bnd_deriv(const float* p, int s)
 }
 
 // sum_j B[0][j] u[j*s]: first closure row (times h), used by the normal complement terms
-__device__ __forceinline__ float d1_row0(const float* p, int s)
-{
-    float v = 0.f;
-#pragma unroll
-    for (int j = 0; j < 6; j++) v += cD1B[0][j] * p[j * s];
-    return v;
-}
+__device__ __forceinline__ float d1_row0(const float* p, int s) { return d1_closure_row<0>(p, s); }
 
 // ---------------------------------------------------------------------------
 // Fused time step
@@ -624,21 +650,17 @@ __device__ __forceinline__ void quad_derivs(const float* u, const float* v, int 
 // snapshot idx (x_x, z_z, z_x, x_z), f0/fm/fp the snapshots idx, idx-1, idx+1 at the
 // cells, o the adjoint level of the density term.  Adds to the accumulators of the
 // lambda, mu, rho gradients (VTI stiffness mode: c11, c13, c33, c55, rho).
+// Density term of the imaging condition of one quad: o is the adjoint level after the
+// step, f0/fm/fp the snapshots idx, idx-1, idx+1 at the cells (second time difference;
+// one-sided at the first and last snapshot, mode 1 and 2).  It goes to gr, or to g4 in
+// the VTI stiffness mode.  (The accumulators are updated unconditionally with selected
+// values: a branch on the mode that picks the array would put the arrays in scratch.)
 template <bool VTI>
-__device__ __forceinline__ void image_accumulate(const StepArgs& a, long long g, int z0, const float* axx,
-                                                 const float* azx, const float* axz, const float* azz,
-                                                 const float* t0x, const float* t1z, const float* t2x,
-                                                 const float* t3z, const float* f0x, const float* f0z,
-                                                 const float* fmx, const float* fmz, const float* fpx,
-                                                 const float* fpz, const float* ox, const float* oz, float* gl,
-                                                 float* gm, float* gr, float* g3, float* g4)
+__device__ __forceinline__ void image_rho(const StepArgs& a, int z0, const float* f0x, const float* f0z,
+                                          const float* fmx, const float* fmz, const float* fpx, const float* fpz,
+                                          const float* ox, const float* oz, float* gr, float* g4)
 {
-    float e[4], v2[4], v3[4];
-    if (VTI) {
-        unpack(ld4(a.ge + g), e);
-        unpack(ld4(a.gv2 + g), v2);
-        unpack(ld4(a.gv3 + g), v3);
-    }
+    const bool st = VTI && a.stiff;
 #pragma unroll
     for (int k = 0; k < 4; k++) {
         if (z0 + k >= a.nz) continue;
@@ -653,24 +675,68 @@ __device__ __forceinline__ void image_accumulate(const StepArgs& a, long long g,
             d2x = -f0x[k] + fmx[k];
             d2z = -f0z[k] + fmz[k];
         }
+        const float r = a.wrho * (ox[k] * d2x + oz[k] * d2z);
+        if (VTI) {
+            gr[k] += st ? 0.f : r;
+            g4[k] += st ? r : 0.f;
+        } else {
+            gr[k] += r;
+        }
+    }
+}
+
+// Stiffness terms of the imaging condition of one quad: a* are the derivatives of the
+// lagged adjoint level (x_x, x_z, z_x, z_z), t* those of forward snapshot idx (x_x,
+// z_z, z_x, x_z).  Adds to the lambda and mu accumulators gl, gm; in the VTI stiffness
+// mode to the c11, c13, c33, c55 accumulators gl, gm, gr, g3.
+template <bool VTI>
+__device__ __forceinline__ void image_stiff(const StepArgs& a, long long g, int z0, const float* axx, const float* azx,
+                                            const float* axz, const float* azz, const float* t0x, const float* t1z,
+                                            const float* t2x, const float* t3z, float* gl, float* gm, float* gr,
+                                            float* g3)
+{
+    const bool st = VTI && a.stiff;
+    float e[4], v2[4], v3[4];
+    if (VTI && !st) {
+        unpack(ld4(a.ge + g), e);
+        unpack(ld4(a.gv2 + g), v2);
+        unpack(ld4(a.gv3 + g), v3);
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        if (z0 + k >= a.nz) continue;
+        const float shear = (azx[k] + axz[k]) * (t2x[k] + t3z[k]);
         if (!VTI) {
             gl[k] += a.wlm * (axx[k] + azz[k]) * (t0x[k] + t1z[k]);
-            gm[k] += a.wlm * ((azx[k] + axz[k]) * (t2x[k] + t3z[k]) + 2.f * axx[k] * t0x[k] + 2.f * azz[k] * t1z[k]);
-        } else if (a.stiff) {
-            // d/dc11, d/dc13, d/dc33, d/dc55 of the same bilinear form
-            gl[k] += a.wlm * axx[k] * t0x[k];
-            gm[k] += a.wlm * (axx[k] * t1z[k] + azz[k] * t0x[k]);
-            gr[k] += a.wlm * azz[k] * t1z[k];
-            g3[k] += a.wlm * (azx[k] + axz[k]) * (t2x[k] + t3z[k]);
+            gm[k] += a.wlm * (shear + 2.f * axx[k] * t0x[k] + 2.f * azz[k] * t1z[k]);
         } else {
+            const float xx = axx[k] * t0x[k], zz = azz[k] * t1z[k];
             const float cr = axx[k] * t1z[k] + azz[k] * t0x[k];
-            gl[k] += a.wlm * (e[k] * axx[k] * t0x[k] + azz[k] * t1z[k] + v2[k] * cr);
-            gm[k] += a.wlm * ((azx[k] + axz[k]) * (t2x[k] + t3z[k]) + 2.f * e[k] * axx[k] * t0x[k] +
-                              2.f * azz[k] * t1z[k] + v3[k] * cr);
+            // stiffness mode: d/dc11, d/dc13, d/dc33, d/dc55 of the same bilinear form
+            const float dl = st ? xx : e[k] * xx + zz + v2[k] * cr;
+            const float dm = st ? cr : shear + 2.f * e[k] * xx + 2.f * zz + v3[k] * cr;
+            gl[k] += a.wlm * dl;
+            gm[k] += a.wlm * dm;
+            gr[k] += st ? a.wlm * zz : 0.f;
+            g3[k] += st ? a.wlm * shear : 0.f;
         }
-        if (VTI && a.stiff) g4[k] += a.wrho * (ox[k] * d2x + oz[k] * d2z);
-        else gr[k] += a.wrho * (ox[k] * d2x + oz[k] * d2z);
     }
+}
+
+// Imaging condition of one quad (compute_gradients, we_op.cpp:519-561) into the
+// accumulators of the lambda, mu, rho gradients (VTI stiffness mode: c11, c13, c33, c55,
+// rho in gl, gm, gr, g3, g4).
+template <bool VTI>
+__device__ __forceinline__ void image_accumulate(const StepArgs& a, long long g, int z0, const float* axx,
+                                                 const float* azx, const float* axz, const float* azz,
+                                                 const float* t0x, const float* t1z, const float* t2x,
+                                                 const float* t3z, const float* f0x, const float* f0z,
+                                                 const float* fmx, const float* fmz, const float* fpx,
+                                                 const float* fpz, const float* ox, const float* oz, float* gl,
+                                                 float* gm, float* gr, float* g3, float* g4)
+{
+    image_stiff<VTI>(a, g, z0, axx, azx, axz, azz, t0x, t1z, t2x, t3z, gl, gm, gr, g3);
+    image_rho<VTI>(a, z0, f0x, f0z, fmx, fmz, fpx, fpz, ox, oz, gr, g4);
 }
 
 // Hides a value from the optimiser so that address arithmetic derived from it is
@@ -1031,18 +1097,29 @@ __global__ void __launch_bounds__(kThreads) k_step(const StepArgs a, const int4*
 // group's first shot.  Tiles with every cell at least 4 from the edges take the interior
 // stencils only; the others choose per quad.
 // ---------------------------------------------------------------------------
-template <bool VTI, int TX>
+// 16 zero bytes: the source of direct-to-LDS window loads at positions outside the grid
+__device__ __attribute__((aligned(16))) float kZeroQuad[4] = {0.f, 0.f, 0.f, 0.f};
+
+// One quad of a window straight from global memory into LDS (global_load_lds_dwordx4):
+// lane l of the wavefront writes the 16 bytes at lds_wave + 16 l, lds_wave being the
+// same for all lanes of the wavefront.
+__device__ __forceinline__ void quad_to_lds(const float* src, float* lds_wave)
+{
+    __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void*)lds_wave, 16, 0, 0);
+}
+
+template <bool VTI, int TX, int NBUF>
 __global__ void __launch_bounds__(kThreads) k_img(const StepArgs a)
 {
     constexpr int TZ = kTileZ, HX = 2, HZ = 4;
     constexpr int WX = TX + 2 * HX, WZ = TZ + 2 * HZ, WQ = WZ / 4;
     constexpr int NQ1 = (WX * WQ + kThreads - 1) / kThreads;
     constexpr int QC = TZ / 4, NQ3 = TX * QC / kThreads;
+    constexpr int WS = WX * WZ;  // floats per window array
     static_assert((TX * QC) % kThreads == 0, "tile must give every thread whole quads");
-    __shared__ __attribute__((aligned(16))) float sAx[WX * WZ];
-    __shared__ __attribute__((aligned(16))) float sAz[WX * WZ];
-    __shared__ __attribute__((aligned(16))) float sFx[WX * WZ];
-    __shared__ __attribute__((aligned(16))) float sFz[WX * WZ];
+    // [buffer][Ax, Az, Fx, Fz][window]: window quad i of an array at float offset 4 i, so
+    // that the quads a wavefront loads are consecutive in LDS
+    __shared__ __attribute__((aligned(16))) float sW[NBUF * 4 * WS];
 
     const int lin = xcd_remap(blockIdx.x, gridDim.x);
     const int tile = lin / a.ngroups;
@@ -1056,28 +1133,31 @@ __global__ void __launch_bounds__(kThreads) k_img(const StepArgs a)
     const bool inner = sx >= 4 && ex <= nx - 4 && sz >= 4 && ez <= nz - 4;
     const long long nxz = (long long)nx * ldz;
     const int tid = threadIdx.x;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int s0 = group * a.spb;
     const int s1 = min(a.nshots, s0 + a.spb);
 
-    float4 rax[NQ1], raz[NQ1], rfx[NQ1], rfz[NQ1];
-    auto load_windows = [&](int shot, int tid, int wsx, int wsz) {
+    // windows of one shot into buffer b: window quads tid + m * 256
+    auto issue_windows = [&](int shot, int b) {
         const float* A = a.curr + shot * a.shot_stride;
         const float* F = a.f0 + shot * a.fshot_stride;
+        float* base = sW + b * 4 * WS;
 #pragma unroll
         for (int m = 0; m < NQ1; m++) {
             const int i = tid + m * kThreads;
-            const int lx = i / WQ, lq = i - lx * WQ;
-            const int gx = wsx + lx, gz = wsz + 4 * lq;
-            const bool ok = i < WX * WQ && gx >= 0 && gx < nx && gz >= 0 && gz < ldz;
-            const long long g = ok ? (long long)gx * ldz + gz : 0;
-            const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
-            rax[m] = ok ? ld4(A + g) : z4;
-            raz[m] = ok ? ld4(A + nxz + g) : z4;
-            rfx[m] = ok ? ld4(F + g) : z4;
-            rfz[m] = ok ? ld4(F + nxz + g) : z4;
+            if (i < WX * WQ) {
+                const int lx = i / WQ, lq = i - lx * WQ;
+                const int gx = wsx + lx, gz = wsz + 4 * lq;
+                const bool ok = gx >= 0 && gx < nx && gz >= 0 && gz < ldz;
+                const long long g = (long long)gx * ldz + gz;
+                float* dst = base + 4 * (m * kThreads + wave * 64);
+                quad_to_lds(ok ? A + g : kZeroQuad, dst);
+                quad_to_lds(ok ? A + nxz + g : kZeroQuad, dst + WS);
+                quad_to_lds(ok ? F + g : kZeroQuad, dst + 2 * WS);
+                quad_to_lds(ok ? F + nxz + g : kZeroQuad, dst + 3 * WS);
+            }
         }
     };
-    if (s0 < s1) load_windows(s0, tid, wsx, wsz);
 
     float gl[NQ3][4], gm[NQ3][4], gr[NQ3][4], g3[VTI ? NQ3 : 1][4], g4[VTI ? NQ3 : 1][4];
 #pragma unroll
@@ -1088,16 +1168,13 @@ __global__ void __launch_bounds__(kThreads) k_img(const StepArgs a)
             if (VTI) g3[m][k] = g4[m][k] = 0.f;
         }
 
-    const int sx0 = sx, ex0 = ex, sz0 = sz, ez0 = ez, wsx0 = wsx, tid0 = tid;
+    if (NBUF == 2 && s0 < s1) issue_windows(s0, 0);
     for (int shot = s0; shot < s1; shot++) {
-        int sx = sx0, ex = ex0, sz = sz0, ez = ez0, wsx = wsx0, tid = tid0;
-        opaque(sx);
-        opaque(ex);
-        opaque(sz);
-        opaque(ez);
-        opaque(wsx);
-        opaque_v(tid);
-        const int wsz = sz - HZ;
+        const int b = NBUF == 2 ? (shot - s0) & 1 : 0;
+        if (NBUF == 1) {
+            if (shot > s0) __syncthreads();  // every wavefront is done with the windows
+            issue_windows(shot, 0);
+        }
         // cell values of this shot: adjoint density level, snapshots idx-1 and idx+1
         float4 qnx[NQ3], qnz[NQ3], qmx[NQ3], qmz[NQ3], qpx[NQ3], qpz[NQ3];
 #pragma unroll
@@ -1116,27 +1193,35 @@ __global__ void __launch_bounds__(kThreads) k_img(const StepArgs a)
             qpx[m] = ok && a.img_mode != 2 ? ld4(a.fp + fo) : z4;
             qpz[m] = ok && a.img_mode != 2 ? ld4(a.fp + fo + nxz) : z4;
         }
-        __syncthreads();  // the previous shot is done with the windows
-#pragma unroll
-        for (int m = 0; m < NQ1; m++) {
-            const int i = tid + m * kThreads;
-            if (i < WX * WQ) {
-                const int lx = i / WQ, lq = i - lx * WQ;
-                const int o = lx * WZ + 4 * lq;
-                st4(&sAx[o], rax[m]);
-                st4(&sAz[o], raz[m]);
-                st4(&sFx[o], rfx[m]);
-                st4(&sFz[o], rfz[m]);
-            }
-        }
+        __builtin_amdgcn_s_waitcnt(0);  // the windows (and cell values) have landed
         __syncthreads();
-        if (shot + 1 < s1) load_windows(shot + 1, tid, wsx, wsz);
+        // the next shot's windows go into the other buffer while this one is imaged
+        if (NBUF == 2 && shot + 1 < s1) issue_windows(shot + 1, b ^ 1);
+        const float* sAx = sW + b * 4 * WS;
+        const float* sAz = sAx + WS;
+        const float* sFx = sAx + 2 * WS;
+        const float* sFz = sAx + 3 * WS;
 #pragma unroll
         for (int m = 0; m < NQ3; m++) {
             const int i = tid + m * kThreads;
             const int gx = sx + i / QC, z0 = sz + 4 * (i % QC);
             if (gx >= ex || z0 >= ez) continue;
             const int w = (gx - wsx) * WZ + (z0 - wsz);
+            {
+                // density term first, so that the cell values are dead before the
+                // derivative reads
+                float f0x[4], f0z[4], fmx[4], fmz[4], fpx[4], fpz[4], ox[4], oz[4];
+                unpack(ld4(&sFx[w]), f0x);
+                unpack(ld4(&sFz[w]), f0z);
+                unpack(qmx[m], fmx);
+                unpack(qmz[m], fmz);
+                unpack(qpx[m], fpx);
+                unpack(qpz[m], fpz);
+                unpack(qnx[m], ox);
+                unpack(qnz[m], oz);
+                image_rho<VTI>(a, z0, f0x, f0z, fmx, fmz, fpx, fpz, ox, oz, gr[m], g4[VTI ? m : 0]);
+            }
+            __builtin_amdgcn_sched_barrier(0);
             float axx[4], azx[4], axz[4], azz[4], t0x[4], t1z[4], t2x[4], t3z[4];
             if (inner) {
                 quad_derivs<false, WZ>(sAx, sAz, w, gx, z0, nx, nz, a.rdx, a.rdz, axx, azx, axz, azz);
@@ -1145,19 +1230,11 @@ __global__ void __launch_bounds__(kThreads) k_img(const StepArgs a)
                 quad_derivs<true, WZ>(sAx, sAz, w, gx, z0, nx, nz, a.rdx, a.rdz, axx, azx, axz, azz);
                 quad_derivs<true, WZ>(sFx, sFz, w, gx, z0, nx, nz, a.rdx, a.rdz, t0x, t3z, t2x, t1z);
             }
-            float f0x[4], f0z[4], fmx[4], fmz[4], fpx[4], fpz[4], ox[4], oz[4];
-            unpack(ld4(&sFx[w]), f0x);
-            unpack(ld4(&sFz[w]), f0z);
-            unpack(qmx[m], fmx);
-            unpack(qmz[m], fmz);
-            unpack(qpx[m], fpx);
-            unpack(qpz[m], fpz);
-            unpack(qnx[m], ox);
-            unpack(qnz[m], oz);
             const long long g = (long long)gx * ldz + z0;
-            image_accumulate<VTI>(a, g, z0, axx, azx, axz, azz, t0x, t1z, t2x, t3z, f0x, f0z, fmx, fmz, fpx, fpz, ox,
-                                  oz, gl[m], gm[m], gr[m], g3[VTI ? m : 0], g4[VTI ? m : 0]);
+            image_stiff<VTI>(a, g, z0, axx, azx, axz, azz, t0x, t1z, t2x, t3z, gl[m], gm[m], gr[m],
+                             g3[VTI ? m : 0]);
         }
+        if (NBUF == 2) __syncthreads();  // every wavefront is done with buffer b
     }
     if (s0 >= s1) return;
     float* G = a.grad + s0 * a.gslab_stride;
@@ -1518,16 +1595,23 @@ void launch_step(const StepArgs& a, bool vti, BoxClass c, const int4* boxes, con
     else step_class<false>(a, c, boxes, list, n, image, s);
 }
 
-void launch_img(const StepArgs& a, bool vti, int tile_x, hipStream_t s)
+template <bool VTI, int NBUF>
+static void img_dispatch(const StepArgs& a, int tile_x, int n, hipStream_t s)
+{
+    if (tile_x == 32) k_img<VTI, 32, NBUF><<<n, kThreads, 0, s>>>(a);
+    else k_img<VTI, 16, NBUF><<<n, kThreads, 0, s>>>(a);
+}
+
+void launch_img(const StepArgs& a, bool vti, int tile_x, int nbuf, hipStream_t s)
 {
     const int n = a.ntx * a.ntz * a.ngroups;
     if (n <= 0) return;
     if (vti) {
-        if (tile_x == 32) k_img<true, 32><<<n, kThreads, 0, s>>>(a);
-        else k_img<true, 16><<<n, kThreads, 0, s>>>(a);
+        if (nbuf == 2) img_dispatch<true, 2>(a, tile_x, n, s);
+        else img_dispatch<true, 1>(a, tile_x, n, s);
     } else {
-        if (tile_x == 32) k_img<false, 32><<<n, kThreads, 0, s>>>(a);
-        else k_img<false, 16><<<n, kThreads, 0, s>>>(a);
+        if (nbuf == 2) img_dispatch<false, 2>(a, tile_x, n, s);
+        else img_dispatch<false, 1>(a, tile_x, n, s);
     }
 }
 

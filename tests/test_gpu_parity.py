@@ -273,10 +273,11 @@ def test_raw_gradient_matches_oracle(lib, orc64, vti):
     g1 = eng.adjoint(adj)
     for c in range(3):
         assert rel(g1[c], gk[c].astype(np.float64)) < 1e-6
-    # imaging fused into the step kernels: the same gradient up to the order of the sums;
-    # 32-wide tiles (step and imaging): the forward wavefield itself differs in rounding
+    # imaging fused into the step kernels (the same gradient up to the order of the sums,
+    # which the cancellations of the VTI lambda term amplify) and 32-wide tiles (step and
+    # imaging; the forward wavefield itself differs in rounding): within the FP32 gap
     eng.set_option("shots_per_block", 16)
-    for opt, val, tol in (("image_fused", 1, 1e-6), ("tile_x", 32, 1e-5)):
+    for opt, val, tol in (("image_fused", 1, 1e-5), ("tile_x", 32, 1e-5)):
         eng.set_option(opt, val)
         eng.forward(src, keep_checkpoints=True)
         g2 = eng.adjoint(adj)

@@ -66,19 +66,23 @@ def tiles(n, T):
     return [tile_range(k, nt, T, n) for k in range(nt)]
 
 
+def pitch(n):
+    """Device z pitch of the engine: n rounded up to 32 cells (128-byte columns)."""
+    return (n + 31) // 32 * 32
+
+
 def tile_range_quads(k, ntile, T, n):
-    nq, tq = (n + 3) // 4, T // 4
+    ldz = pitch(n)
     if ntile == 1:
         return 0, n
     if k == ntile - 1:
-        return 4 * (nq - tq), n
+        return ldz - T, n
     s = k * T
-    return s, min(s + T, 4 * (nq - tq))
+    return s, min(s + T, ldz - T)
 
 
 def tiles_quads(n, T):
-    nq, tq = (n + 3) // 4, T // 4
-    nt = (nq + tq - 1) // tq
+    nt = (pitch(n) + T - 1) // T
     return [tile_range_quads(k, nt, T, n) for k in range(nt)]
 
 
@@ -139,7 +143,7 @@ def check_x(sx, ex, TX, n, edge):
 
 
 def check_z(sz, ez, TZb, n, edge):
-    ldz = (n + 3) // 4 * 4
+    ldz = pitch(n)
     assert sz % 4 == 0 and 0 <= sz < ez <= n and ez - sz <= TZb
     ws, WZ, pz0, PZ = sz - 8, TZb + 16, sz - 4, TZb + 8
     loaded = set(p for p in range(ws, ws + WZ) if 0 <= p < n)  # padding rows are not data
@@ -235,7 +239,7 @@ def check_img_tiles(nx, nz, T):
     """k_img: TX x 64 tiles of the tile grid, windows with halo 2 in x and 4 in z (quad
     reads of rows z0-4 .. z0+7); a tile with every cell 4 or more from the edges takes
     the interior stencils only."""
-    ldz = (nz + 3) // 4 * 4
+    ldz = pitch(nz)
     for sx, ex in tiles(nx, T):
         wsx, WX = window_start(sx, 2), T + 4
         lx = set(p for p in range(wsx, wsx + WX) if 0 <= p < nx)
