@@ -138,9 +138,15 @@ struct Engine {
     int opt_spb_img = 0;   // shots per imaging workgroup (0: 4, or opt_spb when fused)
     int opt_image_fused = 0;  // 1: imaging condition fused into the adjoint step kernels
     int opt_img_buffers = 1;  // LDS window buffers of the imaging kernel (1 or 2)
+    int opt_img_stream = 1;   // 1: imaging kernel on its own stream beside the next step
     int opt_edge_stream = 1;  // edge-band launches on their own stream
     hipStream_t st_edge = nullptr;       // edge-tile launches run beside the interior ones
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    // imaging of adjoint step it runs on st_img beside step it+1; step it+2 (which
+    // overwrites level it) and any recompute (which overwrites snapshot slots) wait for it
+    hipStream_t st_img = nullptr;
+    hipEvent_t ev_img_in = nullptr, ev_img_done[2] = {nullptr, nullptr};
+    bool img_out[2] = {false, false};
     double last_fwd_loop_ms = 0.0;  // HIP-event time of the last forward step loop (timing option)
     int last_fwd_loop_steps = 0;
     bool vti = false;
@@ -187,6 +193,9 @@ struct Engine {
         HIPCHECK(hipStreamCreateWithFlags(&st_edge, hipStreamNonBlocking));
         HIPCHECK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
         HIPCHECK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+        HIPCHECK(hipStreamCreateWithFlags(&st_img, hipStreamNonBlocking));
+        HIPCHECK(hipEventCreateWithFlags(&ev_img_in, hipEventDisableTiming));
+        for (auto& e : ev_img_done) HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         upload_constants(st);
         set_tile(opt_tile_x);
         build_tapers();
@@ -201,6 +210,13 @@ struct Engine {
             (void)hipStreamSynchronize(st_edge);
             (void)hipStreamDestroy(st_edge);
         }
+        if (st_img) {
+            (void)hipStreamSynchronize(st_img);
+            (void)hipStreamDestroy(st_img);
+        }
+        if (ev_img_in) (void)hipEventDestroy(ev_img_in);
+        for (auto& e : ev_img_done)
+            if (e) (void)hipEventDestroy(e);
         graph_fwd.reset();
         graph_adj.reset();
         if (ev_fork) (void)hipEventDestroy(ev_fork);
@@ -553,6 +569,10 @@ struct Engine {
         a.rdz = (float)(1.0 / g.dz);
         a.rdx2 = (float)(1.0 / (g.dx * g.dx));
         a.rdz2 = (float)(1.0 / (g.dz * g.dz));
+        a.c1x = (float)(kD1Interior[0] / g.dx);
+        a.c2x = (float)(kD1Interior[1] / g.dx);
+        a.c1z = (float)(kD1Interior[0] / g.dz);
+        a.c2z = (float)(kD1Interior[1] / g.dz);
         a.dt = (float)g.dt;
         a.dt2 = (float)g.dt * (float)g.dt;
         a.sat_x = (float)(1.0 / (g.dx * kH0));
@@ -649,6 +669,11 @@ struct Engine {
         a.prev = L.P();
         a.curr = L.C();
         a.next = L.N();
+        // the new level overwrites level it-2, which the imaging of step it-2 reads
+        if (img_out[it & 1]) {
+            HIPCHECK(hipStreamWaitEvent(st, ev_img_done[it & 1], 0));
+            img_out[it & 1] = false;
+        }
         // extraction of the current level (forward)
         if (rcv)
             launch_extract(L.C(), 2 * nxz, nxz, ldz, d_rcv_ext_ents.as<PointEntry>(), nr, rcv, (long long)nr * g.nt, it,
@@ -695,7 +720,15 @@ struct Engine {
         if (img_idx >= 0 && !fuse_img) {
             ia.curr = L.C();
             ia.next = L.N();
-            launch_img(ia, vti, tile_x, opt_img_buffers, st);
+            if (opt_img_stream) {
+                HIPCHECK(hipEventRecord(ev_img_in, st));
+                HIPCHECK(hipStreamWaitEvent(st_img, ev_img_in, 0));
+                launch_img(ia, vti, tile_x, opt_img_buffers, st_img);
+                HIPCHECK(hipEventRecord(ev_img_done[it & 1], st_img));
+                img_out[it & 1] = true;
+            } else {
+                launch_img(ia, vti, tile_x, opt_img_buffers, st);
+            }
         }
         L.rotate();
         stat_steps++;
@@ -757,7 +790,7 @@ struct Engine {
     {
         auto P = [](const void* p) { return (long long)(uintptr_t)p; };
         return {kind, P(p0), P(p1), flag, g.nt, ns, nr, tile_x, opt_spb, opt_spb_edge, opt_lag, seg_K, nseg,
-                opt_bf16, opt_stiff, opt_edge_stream, opt_spb_img, opt_image_fused, opt_img_buffers, P(d_ckpt16.p), P(d_ckpt_scale.p), P(d_amax.p),
+                opt_bf16, opt_stiff, opt_edge_stream, opt_spb_img, opt_image_fused, opt_img_buffers, opt_img_stream, P(d_ckpt16.p), P(d_ckpt_scale.p), P(d_amax.p),
                 P(d_fwd.p), P(d_adj.p), P(d_ckpt.p), P(d_seg.p), P(d_gshot.p), P(d_model.p), P(d_deriv.p),
                 P(d_boxes.p), P(d_list_core.p), P(d_list_bz.p), P(d_list_bz8.p), P(d_list_bx.p), P(inj_src.d_cells.p), P(inj_src.d_ptr.p),
                 P(inj_src.d_ctrace.p), P(inj_src.d_cw.p), P(inj_rcv.d_cells.p), P(inj_rcv.d_ptr.p), P(inj_rcv.d_ctrace.p),
@@ -960,9 +993,20 @@ struct Engine {
     }
 
     // recompute the snapshots of segment j from its checkpoint
+    // the main stream waits for every imaging launch still running on st_img
+    void img_join()
+    {
+        for (int p = 0; p < 2; p++)
+            if (img_out[p]) {
+                HIPCHECK(hipStreamWaitEvent(st, ev_img_done[p], 0));
+                img_out[p] = false;
+            }
+    }
+
     void load_segment(int j, const float* dsrc)
     {
         if (seg_loaded == j) return;
+        img_join();  // the recompute overwrites snapshot slots the imaging reads
         const long long ss = state_stride();
         Levels F = levels_in(d_fwd);
         if (g.sub == 1) {
@@ -1108,6 +1152,7 @@ struct Engine {
             const int seg0 = seg_loaded;
             run_loop(graph_adj, loop_key(1, dadj, dsrc, seg0), [&] {
                 Levels Ac = A;
+                img_out[0] = img_out[1] = false;
                 for (int it = 0; it < nt - 1; it++) {
                     int idx = -1;
                     if ((nt - 2 - it) % sub == 0) {
@@ -1117,6 +1162,7 @@ struct Engine {
                     }
                     step(Ac, it, true, dadj, nullptr, idx);
                 }
+                img_join();
             });
         } else {
             for (int it = 0; it < nt - 1; it++) {
@@ -1274,6 +1320,8 @@ int fwi2d_engine_set_option(fwi2d_engine* e, const char* key, long long value)
             x->opt_spb_edge = (int)value;
         } else if (k == "image_fused") {
             x->opt_image_fused = value ? 1 : 0;
+        } else if (k == "image_stream") {
+            x->opt_img_stream = value ? 1 : 0;
         } else if (k == "image_buffers") {
             if (value != 1 && value != 2) throw std::runtime_error("image_buffers must be 1 or 2");
             x->opt_img_buffers = (int)value;
@@ -1300,6 +1348,7 @@ long long fwi2d_engine_get_stat(fwi2d_engine* e, const char* key)
     if (k == "edge_stream") return x->opt_edge_stream;
     if (k == "image_fused") return x->opt_image_fused;
     if (k == "image_buffers") return x->opt_img_buffers;
+    if (k == "image_stream") return x->opt_img_stream;
     if (k == "ntiles") return x->ntiles();
     if (k == "interior_tiles") return (long long)x->list_core.size();
     if (k == "graph_captures") return x->graph_captures;

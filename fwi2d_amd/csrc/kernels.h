@@ -33,6 +33,7 @@ struct StepArgs {
     int nx, nz, ldz, ntx, ntz, nshots, nt, it, taper_prev;
     int spb, ngroups;      // shots per workgroup, shot groups (ceil(nshots / spb))
     float rdx, rdz, rdx2, rdz2, dt, dt2, sat_x, sat_z;
+    float c1x, c2x, c1z, c2z;  // interior first-derivative coefficients divided by dx, dz
     int bc_top, bc_bottom, bc_left, bc_right;
     const float *lam, *mu, *rho, *c13, *eps;
     const float* irho;     // dt^2 / rho

@@ -168,6 +168,16 @@ struct QuadW {
     W5 x[4], z[4];
 };
 
+__device__ __forceinline__ W5 d2_scaled(W5 w, float f)
+{
+    w.m2 *= f;
+    w.m1 *= f;
+    w.c *= f;
+    w.p1 *= f;
+    w.p2 *= f;
+    return w;
+}
+
 __device__ __forceinline__ float d2_apply(const W5& w, const float* pu, int s)
 {
     return w.c * pu[0] + w.m1 * pu[-s] + w.p1 * pu[s] + w.m2 * pu[-2 * s] + w.p2 * pu[2 * s];
@@ -326,11 +336,10 @@ struct StepSmem {
 };
 
 // D1 of a 12-cell column (cells z0-4 .. z0+7) at the 4 quad cells
-__device__ __forceinline__ void d1_quad_z(const float* c, float* out)
+__device__ __forceinline__ void d1_quad_z(const float* c, float* out, float c1, float c2)
 {
 #pragma unroll
-    for (int k = 0; k < 4; k++)
-        out[k] = cD1I[0] * (c[5 + k] - c[3 + k]) + cD1I[1] * (c[6 + k] - c[2 + k]);
+    for (int k = 0; k < 4; k++) out[k] = c1 * (c[5 + k] - c[3 + k]) + c2 * (c[6 + k] - c[2 + k]);
 }
 
 // D2(c) of a 12-cell column at the 4 quad cells (2 fields sharing the coefficient)
@@ -389,8 +398,8 @@ __device__ __forceinline__ void product_quad(const StepArgs& a, const StepSmem& 
         }
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            dxux[k] = (cD1I[0] * (X[2][k] - X[1][k]) + cD1I[1] * (X[3][k] - X[0][k])) * a.rdx;
-            dxuz[k] = (cD1I[0] * (Z[2][k] - Z[1][k]) + cD1I[1] * (Z[3][k] - Z[0][k])) * a.rdx;
+            dxux[k] = a.c1x * (X[2][k] - X[1][k]) + a.c2x * (X[3][k] - X[0][k]);
+            dxuz[k] = a.c1x * (Z[2][k] - Z[1][k]) + a.c2x * (Z[3][k] - Z[0][k]);
         }
     } else if constexpr (EDGE) {
 #pragma unroll 1
@@ -403,13 +412,8 @@ __device__ __forceinline__ void product_quad(const StepArgs& a, const StepSmem& 
         float Uc[12], Zc[12];
         load_col12(sm.ux, w, Uc);
         load_col12(sm.uz, w, Zc);
-        d1_quad_z(Uc, dzux);
-        d1_quad_z(Zc, dzuz);
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            dzux[k] *= a.rdz;
-            dzuz[k] *= a.rdz;
-        }
+        d1_quad_z(Uc, dzux, a.c1z, a.c2z);
+        d1_quad_z(Zc, dzuz, a.c1z, a.c2z);
     } else if constexpr (EDGE) {
 #pragma unroll 1
         for (int k = 0; k < 4; k++) {
@@ -523,10 +527,15 @@ __device__ __forceinline__ void rhs_quad(const StepArgs& a, const StepSmem& sm, 
             } else {
                 d2u = d2_apply(wx, &uk[2], 1);
             }
-            rx[k] = 2.f * d2u * a.rdx2;
-            rz[k] = d2_apply(wx, &zk[2], 1) * a.rdx2;
-            rx[k] += (cD1I[0] * (P[2][k] - P[1][k]) + cD1I[1] * (P[3][k] - P[0][k])) * a.rdx;
-            rz[k] += (cD1I[0] * (Q[2][k] - Q[1][k]) + cD1I[1] * (Q[3][k] - Q[0][k])) * a.rdx;
+            if constexpr (CW) {  // cached weights already carry 1/dx^2
+                rx[k] = 2.f * d2u;
+                rz[k] = d2_apply(wx, &zk[2], 1);
+            } else {
+                rx[k] = 2.f * d2u * a.rdx2;
+                rz[k] = d2_apply(wx, &zk[2], 1) * a.rdx2;
+            }
+            rx[k] += a.c1x * (P[2][k] - P[1][k]) + a.c2x * (P[3][k] - P[0][k]);
+            rz[k] += a.c1x * (Q[2][k] - Q[1][k]) + a.c2x * (Q[3][k] - Q[0][k]);
         }
     } else if constexpr (EDGE) {
 #pragma unroll 1
@@ -563,14 +572,19 @@ __device__ __forceinline__ void rhs_quad(const StepArgs& a, const StepSmem& sm, 
         float Pc[12], Qc[12], dpz[4], dqz[4];
         load_col12(VTI ? sm.pz : sm.px, q, Pc);
         load_col12(sm.qx, q, Qc);
-        d1_quad_z(Pc, dpz);
-        d1_quad_z(Qc, dqz);
+        d1_quad_z(Pc, dpz, a.c1z, a.c2z);
+        d1_quad_z(Qc, dqz, a.c1z, a.c2z);
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            rz[k] += 2.f * d2zz[k] * a.rdz2;
-            rx[k] += d2zx[k] * a.rdz2;
-            rz[k] += dpz[k] * a.rdz;
-            rx[k] += dqz[k] * a.rdz;
+            if constexpr (CW) {  // cached weights already carry 1/dz^2
+                rz[k] += 2.f * d2zz[k];
+                rx[k] += d2zx[k];
+            } else {
+                rz[k] += 2.f * d2zz[k] * a.rdz2;
+                rx[k] += d2zx[k] * a.rdz2;
+            }
+            rz[k] += dpz[k];
+            rx[k] += dqz[k];
         }
     } else if constexpr (EDGE) {
 #pragma unroll 1
@@ -600,9 +614,11 @@ __device__ __forceinline__ void rhs_quad(const StepArgs& a, const StepSmem& sm, 
 // First derivatives of a pair of fields (u, v) at the 4 cells of a quad from an LDS
 // array with column stride CS: (dx u, dz u, dx v, dz v), scaled by 1/h.
 template <bool EDGE, int CS>
-__device__ __forceinline__ void quad_derivs(const float* u, const float* v, int w, int gx, int z0, int nx, int nz,
-                                            float rdx, float rdz, float* xu, float* zu, float* xv, float* zv)
+__device__ __forceinline__ void quad_derivs(const StepArgs& a, const float* u, const float* v, int w, int gx, int z0,
+                                            float* xu, float* zu, float* xv, float* zv)
 {
+    const int nx = a.nx, nz = a.nz;
+    const float rdx = a.rdx, rdz = a.rdz;
     if (!EDGE || (gx >= 4 && gx < nx - 4)) {
         float U[4][4], V[4][4];
         const int off[4] = {-2, -1, 1, 2};
@@ -613,8 +629,8 @@ __device__ __forceinline__ void quad_derivs(const float* u, const float* v, int 
         }
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            xu[k] = (cD1I[0] * (U[2][k] - U[1][k]) + cD1I[1] * (U[3][k] - U[0][k])) * rdx;
-            xv[k] = (cD1I[0] * (V[2][k] - V[1][k]) + cD1I[1] * (V[3][k] - V[0][k])) * rdx;
+            xu[k] = a.c1x * (U[2][k] - U[1][k]) + a.c2x * (U[3][k] - U[0][k]);
+            xv[k] = a.c1x * (V[2][k] - V[1][k]) + a.c2x * (V[3][k] - V[0][k]);
         }
     } else if constexpr (EDGE) {
 #pragma unroll 1
@@ -627,13 +643,8 @@ __device__ __forceinline__ void quad_derivs(const float* u, const float* v, int 
         float Uc[12], Vc[12];
         load_col12(u, w, Uc);
         load_col12(v, w, Vc);
-        d1_quad_z(Uc, zu);
-        d1_quad_z(Vc, zv);
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            zu[k] *= rdz;
-            zv[k] *= rdz;
-        }
+        d1_quad_z(Uc, zu, a.c1z, a.c2z);
+        d1_quad_z(Vc, zv, a.c1z, a.c2z);
     } else if constexpr (EDGE) {
 #pragma unroll 1
         for (int k = 0; k < 4; k++) {
@@ -869,8 +880,8 @@ __global__ void __launch_bounds__(kThreads) k_step(const StepArgs a, const int4*
 #pragma unroll
             for (int k = 0; k < 4; k++) {
                 const float mk[5] = {M[0][k], M[1][k], M[2][k], M[3][k], M[4][k]};
-                qw[m].x[k] = d2_weights(&mk[2], 1);
-                qw[m].z[k] = d2_weights(&Mc[4 + k], 1);
+                qw[m].x[k] = d2_scaled(d2_weights(&mk[2], 1), a.rdx2);
+                qw[m].z[k] = d2_scaled(d2_weights(&Mc[4 + k], 1), a.rdz2);
             }
         }
     }
@@ -1038,26 +1049,35 @@ __global__ void __launch_bounds__(kThreads) k_step(const StepArgs a, const int4*
             st4(nux + g, make_float4(ox[0], ox[1], ox[2], ox[3]));
             st4(nuz + g, make_float4(oz[0], oz[1], oz[2], oz[3]));
             if constexpr (IMG) {
+                // the step's temporaries are dead here: keep the scheduler from mixing the
+                // imaging reads into the step's arithmetic (register pressure)
+                __builtin_amdgcn_sched_barrier(0);
+                const int q = (gx - t.plx) * S::PZ + (z0 - t.pz0);
+                {
+                    // density term: the new level against the second time difference of
+                    // snapshots idx-1, idx, idx+1
+                    float f0x[4], f0z[4], fmx[4], fmz[4], fpx[4], fpz[4];
+                    unpack(ld4(&sFx[q]), f0x);
+                    unpack(ld4(&sFz[q]), f0z);
+                    const long long fo = shot * a.fshot_stride + g;
+                    if (a.img_mode != 1) {
+                        unpack(ld4(a.fm + fo), fmx);
+                        unpack(ld4(a.fm + fo + nxz), fmz);
+                    }
+                    if (a.img_mode != 2) {
+                        unpack(ld4(a.fp + fo), fpx);
+                        unpack(ld4(a.fp + fo + nxz), fpz);
+                    }
+                    image_rho<VTI>(a, z0, f0x, f0z, fmx, fmz, fpx, fpz, ox, oz, gr[m], g4[VTI ? m : 0]);
+                }
+                __builtin_amdgcn_sched_barrier(0);
                 // adjoint derivatives of the step's current level (the reference's lagged
                 // u_x, u_z), forward derivatives of snapshot idx (we_op.cpp:519-541)
                 float axx[4], azx[4], axz[4], azz[4], t0x[4], t1z[4], t2x[4], t3z[4];
-                quad_derivs<EDGE, S::WZ>(sUx, sUz, wq, gx, z0, nx, nz, a.rdx, a.rdz, axx, azx, axz, azz);
-                const int q = (gx - t.plx) * S::PZ + (z0 - t.pz0);
-                quad_derivs<EDGE, S::PZ>(sFx, sFz, q, gx, z0, nx, nz, a.rdx, a.rdz, t0x, t3z, t2x, t1z);
-                float f0x[4], f0z[4], fmx[4], fmz[4], fpx[4], fpz[4];
-                unpack(ld4(&sFx[q]), f0x);
-                unpack(ld4(&sFz[q]), f0z);
-                const long long fo = shot * a.fshot_stride + g;
-                if (a.img_mode != 1) {
-                    unpack(ld4(a.fm + fo), fmx);
-                    unpack(ld4(a.fm + fo + nxz), fmz);
-                }
-                if (a.img_mode != 2) {
-                    unpack(ld4(a.fp + fo), fpx);
-                    unpack(ld4(a.fp + fo + nxz), fpz);
-                }
-                image_accumulate<VTI>(a, g, z0, axx, azx, axz, azz, t0x, t1z, t2x, t3z, f0x, f0z, fmx, fmz, fpx, fpz,
-                                      ox, oz, gl[m], gm[m], gr[m], g3[VTI ? m : 0], g4[VTI ? m : 0]);
+                quad_derivs<EDGE, S::WZ>(a, sUx, sUz, wq, gx, z0, axx, azx, axz, azz);
+                quad_derivs<EDGE, S::PZ>(a, sFx, sFz, q, gx, z0, t0x, t3z, t2x, t1z);
+                image_stiff<VTI>(a, g, z0, axx, azx, axz, azz, t0x, t1z, t2x, t3z, gl[m], gm[m], gr[m],
+                                 g3[VTI ? m : 0]);
             }
         }
     }
@@ -1121,9 +1141,12 @@ __global__ void __launch_bounds__(kThreads) k_img(const StepArgs a)
     // that the quads a wavefront loads are consecutive in LDS
     __shared__ __attribute__((aligned(16))) float sW[NBUF * 4 * WS];
 
+    // group-major order: the tiles of one shot group follow each other (z first), so that
+    // the window halos a tile shares with its neighbours are still in the XCD's L2
     const int lin = xcd_remap(blockIdx.x, gridDim.x);
-    const int tile = lin / a.ngroups;
-    const int group = lin - tile * a.ngroups;
+    const int ntiles = a.ntx * a.ntz;
+    const int group = lin / ntiles;
+    const int tile = lin - group * ntiles;
     const int tix = tile / a.ntz, tiz = tile - tix * a.ntz;
     const int nx = a.nx, nz = a.nz, ldz = a.ldz;
     int sx, ex, sz, ez;
@@ -1224,11 +1247,11 @@ __global__ void __launch_bounds__(kThreads) k_img(const StepArgs a)
             __builtin_amdgcn_sched_barrier(0);
             float axx[4], azx[4], axz[4], azz[4], t0x[4], t1z[4], t2x[4], t3z[4];
             if (inner) {
-                quad_derivs<false, WZ>(sAx, sAz, w, gx, z0, nx, nz, a.rdx, a.rdz, axx, azx, axz, azz);
-                quad_derivs<false, WZ>(sFx, sFz, w, gx, z0, nx, nz, a.rdx, a.rdz, t0x, t3z, t2x, t1z);
+                quad_derivs<false, WZ>(a, sAx, sAz, w, gx, z0, axx, azx, axz, azz);
+                quad_derivs<false, WZ>(a, sFx, sFz, w, gx, z0, t0x, t3z, t2x, t1z);
             } else {
-                quad_derivs<true, WZ>(sAx, sAz, w, gx, z0, nx, nz, a.rdx, a.rdz, axx, azx, axz, azz);
-                quad_derivs<true, WZ>(sFx, sFz, w, gx, z0, nx, nz, a.rdx, a.rdz, t0x, t3z, t2x, t1z);
+                quad_derivs<true, WZ>(a, sAx, sAz, w, gx, z0, axx, azx, axz, azz);
+                quad_derivs<true, WZ>(a, sFx, sFz, w, gx, z0, t0x, t3z, t2x, t1z);
             }
             const long long g = (long long)gx * ldz + z0;
             image_stiff<VTI>(a, g, z0, axx, azx, axz, azz, t0x, t1z, t2x, t3z, gl[m], gm[m], gr[m],

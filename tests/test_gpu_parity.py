@@ -273,6 +273,12 @@ def test_raw_gradient_matches_oracle(lib, orc64, vti):
     g1 = eng.adjoint(adj)
     for c in range(3):
         assert rel(g1[c], gk[c].astype(np.float64)) < 1e-6
+    # the imaging kernel on the engine stream instead of its own: the same launches in the
+    # same order, bit for bit
+    eng.set_option("image_stream", 0)
+    eng.forward(src, keep_checkpoints=True)
+    assert np.array_equal(eng.adjoint(adj), g1)
+    eng.set_option("image_stream", 1)
     # imaging fused into the step kernels (the same gradient up to the order of the sums,
     # which the cancellations of the VTI lambda term amplify) and 32-wide tiles (step and
     # imaging; the forward wavefield itself differs in rounding): within the FP32 gap
