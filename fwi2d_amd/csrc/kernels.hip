@@ -771,6 +771,17 @@ __device__ __forceinline__ void opaque(TileInfo& t)
     opaque(t.pz0);
 }
 
+// 16 zero bytes: the source of direct-to-LDS window loads at positions outside the grid
+__device__ __attribute__((aligned(16))) float kZeroQuad[4] = {0.f, 0.f, 0.f, 0.f};
+
+// One quad of a window straight from global memory into LDS (global_load_lds_dwordx4):
+// lane l of the wavefront writes the 16 bytes at lds_wave + 16 l, lds_wave being the
+// same for all lanes of the wavefront.
+__device__ __forceinline__ void quad_to_lds(const float* src, float* lds_wave)
+{
+    __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void*)lds_wave, 16, 0, 0);
+}
+
 // A workgroup advances the shots of one group through one box of the grid.  Boxes of the
 // core (EDGE = false) hold only cells whose stencils are all interior, so no closure or
 // SAT code is compiled in; boxes of the edge bands (EDGE = true) take the closure and
@@ -902,7 +913,12 @@ __global__ void __launch_bounds__(kThreads) k_step(const StepArgs a, const int4*
             wuz[m] = ok ? ld4(cux + nxz + g) : z4;
         }
     };
-    if (s0 < s1) load_window(s0, tid, t);
+    // core boxes: the state window goes straight from global memory into LDS at the start
+    // of each shot (no register staging); other boxes prefetch the next shot's window
+    // into registers during this shot
+    constexpr bool DMA = !EDGE && !IMG && S::WZ == 4 * S::WQ;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    if (!DMA && s0 < s1) load_window(s0, tid, t);
 
     float gl[IMG ? S::NQ3 : 1][4], gm[IMG ? S::NQ3 : 1][4], gr[IMG ? S::NQ3 : 1][4];
     float g3[IMG && VTI ? S::NQ3 : 1][4], g4[IMG && VTI ? S::NQ3 : 1][4];  // VTI stiffness mode
@@ -938,14 +954,32 @@ __global__ void __launch_bounds__(kThreads) k_step(const StepArgs a, const int4*
             qprz[m] = ok ? ld4(pux + nxz + g) : z4;
         }
         __syncthreads();  // the previous shot is done with the LDS state and products
+        if constexpr (DMA) {
+            const float* cux = a.curr + shot * a.shot_stride;
 #pragma unroll
-        for (int m = 0; m < S::NQ1; m++) {
-            const int i = tid + m * kThreads;
-            if (i < S::WX * S::WQ) {
-                const int lx = i / S::WQ, lq = i - lx * S::WQ;
-                const int o = lx * S::WZ + 4 * lq;
-                st4(&sUx[o], wux[m]);
-                st4(&sUz[o], wuz[m]);
+            for (int m = 0; m < S::NQ1; m++) {
+                const int i = tid + m * kThreads;
+                if (i < S::WX * S::WQ) {
+                    const int lx = i / S::WQ, lq = i - lx * S::WQ;
+                    const int gx = t.wsx + lx, gz = t.wsz + 4 * lq;
+                    const bool ok = gx >= 0 && gx < nx && gz >= 0 && gz < ldz;
+                    const int g = gx * ldz + gz;
+                    const int o = 4 * (m * kThreads + wave * 64);  // window quad i at 4 i
+                    quad_to_lds(ok ? cux + g : kZeroQuad, &sUx[o]);
+                    quad_to_lds(ok ? cux + nxz + g : kZeroQuad, &sUz[o]);
+                }
+            }
+            __builtin_amdgcn_s_waitcnt(0);
+        } else {
+#pragma unroll
+            for (int m = 0; m < S::NQ1; m++) {
+                const int i = tid + m * kThreads;
+                if (i < S::WX * S::WQ) {
+                    const int lx = i / S::WQ, lq = i - lx * S::WQ;
+                    const int o = lx * S::WZ + 4 * lq;
+                    st4(&sUx[o], wux[m]);
+                    st4(&sUz[o], wuz[m]);
+                }
             }
         }
         if constexpr (IMG) {
@@ -965,7 +999,7 @@ __global__ void __launch_bounds__(kThreads) k_step(const StepArgs a, const int4*
             }
         }
         __syncthreads();
-        if (shot + 1 < s1) load_window(shot + 1, tid, t);
+        if (!DMA && shot + 1 < s1) load_window(shot + 1, tid, t);
 
         // products of first derivatives
 #pragma unroll
@@ -1117,17 +1151,6 @@ __global__ void __launch_bounds__(kThreads) k_step(const StepArgs a, const int4*
 // group's first shot.  Tiles with every cell at least 4 from the edges take the interior
 // stencils only; the others choose per quad.
 // ---------------------------------------------------------------------------
-// 16 zero bytes: the source of direct-to-LDS window loads at positions outside the grid
-__device__ __attribute__((aligned(16))) float kZeroQuad[4] = {0.f, 0.f, 0.f, 0.f};
-
-// One quad of a window straight from global memory into LDS (global_load_lds_dwordx4):
-// lane l of the wavefront writes the 16 bytes at lds_wave + 16 l, lds_wave being the
-// same for all lanes of the wavefront.
-__device__ __forceinline__ void quad_to_lds(const float* src, float* lds_wave)
-{
-    __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void*)lds_wave, 16, 0, 0);
-}
-
 template <bool VTI, int TX, int NBUF>
 __global__ void __launch_bounds__(kThreads) k_img(const StepArgs a)
 {
