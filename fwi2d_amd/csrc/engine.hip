@@ -234,7 +234,7 @@ struct Engine {
 
     // Step decomposition: the core [6, nx-6) x [8, core_z_end) where every stencil is
     // interior, cut into tile_x x 64 boxes (the tile grid clipped to the core), and the
-    // edge bands around it: 64 x 16 boxes along the top and bottom (corners included),
+    // edge bands around it: 120 x 8 / 64 x 16 boxes along the top and bottom (corners included),
     // 8 x 128 boxes along the left and right sides.  Boxes partition the grid.
     void build_boxes()
     {
@@ -260,11 +260,11 @@ struct Engine {
                 add(list_core, std::max(sx, cx0), std::min(ex, cx1), std::max(sz, cz0), std::min(ez, cz1));
             }
         }
-        // top and bottom bands: 8-row bands in 128 x 8 boxes, taller ones (the bottom band
+        // top and bottom bands: 8-row bands in 120 x 8 boxes, taller ones (the bottom band
         // holds 8 to 11 rows) in 64 x 16 boxes
         auto zband = [&](int sz, int ez) {
             const bool thin = ez - sz <= 8;
-            const int w = thin ? 128 : 64;
+            const int w = thin ? kBandZ8Width : 64;
             const int nb = (nx + w - 1) / w;
             for (int k = 0; k < nb; k++) {
                 int sx, ex;

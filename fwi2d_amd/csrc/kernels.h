@@ -83,9 +83,11 @@ void tile_range_x(int k, int ntile, int T, int n, int& s, int& e);
 void tile_range_z(int k, int ntile, int nz, int ldz, int& s, int& e);
 
 // Box shapes of the step decomposition: core boxes (TX x 64, every stencil interior) and
-// edge-band boxes (128 x 8 for 8-row bands and 64 x 16 for taller ones along the top and
-// bottom, 8 x 128 along the sides).
+// edge-band boxes (120 x 8 for 8-row bands and 64 x 16 for taller ones along the top and
+// bottom, 8 x 128 along the sides).  The 8-row boxes are 120 wide so that their state
+// window is 128 columns: two window quads and two product quads per thread.
 enum class BoxClass { core16, core32, band_z, band_x, band_z8 };
+constexpr int kBandZ8Width = 120;
 constexpr int kCoreX0 = 6;   // core: x in [6, nx - 6), z in [8, core_z_end(nz))
 constexpr int kCoreZ0 = 8;
 inline int core_z_end(int nz) { return ((nz - 8) / 4) * 4; }

@@ -275,24 +275,29 @@ __device__ __forceinline__ float d1_row0(const float* p, int s) { return d1_clos
 // consecutive columns, so the z closure row of every lane of a wave is the same and the
 // closure coefficients are read with scalar loads.  Their LDS columns then hold an odd
 // number of quads, which keeps the column-strided 16-byte reads free of bank conflicts.
+// 8-row bands (TZ = 8) hold compact windows: 16 state rows and 12 product rows, starting
+// at row 0 for the top band; elsewhere the state rows start 8 and the product rows 4
+// above the box (the interior product quads read 4 state rows above themselves, the
+// bottom closure rows the 8 rows above the last one).
 template <int TX, int TZ_>
 struct StepShape {
-    static constexpr int TZ = TZ_, BX = TX;
+    static constexpr int TZ = TZ_;
     static constexpr bool ROWMAJ = TZ <= 16;
+    static constexpr int BX = ROWMAJ ? (TX + 63) / 64 * 64 : TX;  // lanes per quad row (row-major)
+    static constexpr bool COMPACT = TZ == 8;
     static constexpr int HX = 4, HZ = 8;
     static constexpr int WX = TX + 2 * HX;                     // state window columns
-    static constexpr int WQ = (TZ + 2 * HZ) / 4;               // state window quads per column
+    static constexpr int WQ = COMPACT ? 4 : (TZ + 2 * HZ) / 4; // state window quads per column
     static constexpr int WZ = 4 * (ROWMAJ ? (WQ | 1) : WQ);    // LDS column stride of the window
     static constexpr int IX = TX + 4;                          // product columns
-    static constexpr int PQ = (TZ + 8) / 4;                    // product quads: rows [sz-4, sz+TZ+4)
+    static constexpr int PQ = COMPACT ? 3 : (TZ + 8) / 4;      // product quads: rows [sz-4, sz+TZ+4)
     static constexpr int PZ = 4 * (ROWMAJ ? (PQ | 1) : PQ);    // LDS column stride of the products
     static constexpr int IXP = ROWMAJ ? (IX + 63) / 64 * 64 : IX;  // product slots per quad row
     static constexpr int QC = TZ / 4;                          // new-level quads per column
     static constexpr int NQ1 = (WX * WQ + kThreads - 1) / kThreads;  // window quads per thread
     static constexpr int NQ2 = (IXP * PQ + kThreads - 1) / kThreads; // product quads per thread
-    static constexpr int NQ3 = (TX * QC) / kThreads;                  // new-level quads per thread
-    static_assert((TX * QC) % kThreads == 0, "box shape must give every thread whole quads");
-    static_assert(!ROWMAJ || TX % 64 == 0, "row-major boxes need whole wavefronts per quad row");
+    static constexpr int NQ3 = (BX * QC + kThreads - 1) / kThreads;  // new-level quads per thread
+    static_assert(ROWMAJ || (TX * QC) % kThreads == 0, "box shape must give every thread whole quads");
 };
 
 struct TileInfo {
@@ -790,7 +795,8 @@ __device__ __forceinline__ void quad_to_lds(const float* src, float* lds_wave)
 // the gradient contributions of the workgroup's shots are summed in registers and added
 // once per launch to the gradient slab of the group's first shot.
 template <bool VTI, int TX, int TZ, bool EDGE, bool IMG>
-__global__ void __launch_bounds__(kThreads) k_step(const StepArgs a, const int4* boxes, const int* list)
+__global__ void __launch_bounds__(kThreads)
+k_step(const StepArgs a, const int4* boxes, const int* list)
 {
     using S = StepShape<TX, TZ>;
     __shared__ __attribute__((aligned(16))) float sUx[S::WX * S::WZ];
@@ -817,8 +823,13 @@ __global__ void __launch_bounds__(kThreads) k_step(const StepArgs a, const int4*
     t.sz = box.z;
     t.ez = box.w;
     t.wsx = window_start(t.sx, S::HX);
-    t.wsz = t.sz - S::HZ;
-    t.pz0 = t.sz - 4;
+    if constexpr (S::COMPACT) {
+        t.wsz = t.sz == 0 ? 0 : t.sz - 8;
+        t.pz0 = t.sz == 0 ? 0 : t.sz - 4;
+    } else {
+        t.wsz = t.sz - S::HZ;
+        t.pz0 = t.sz - 4;
+    }
     product_range(t.sx, t.ex, nx, t.plx, t.phx);
     product_range(t.sz, t.ez, nz, t.plz, t.phz);
     const long long nxz = (long long)nx * ldz;
@@ -1629,7 +1640,7 @@ static void step_class(const StepArgs& a, BoxClass c, const int4* boxes, const i
     case BoxClass::core16: step_launch<VTI, 16, 64, false>(a, boxes, list, n, image, s); break;
     case BoxClass::core32: step_launch<VTI, 32, 64, false>(a, boxes, list, n, image, s); break;
     case BoxClass::band_z: step_launch<VTI, 64, 16, true>(a, boxes, list, n, image, s); break;
-    case BoxClass::band_z8: step_launch<VTI, 128, 8, true>(a, boxes, list, n, image, s); break;
+    case BoxClass::band_z8: step_launch<VTI, kBandZ8Width, 8, true>(a, boxes, list, n, image, s); break;
     case BoxClass::band_x: step_launch<VTI, 8, 128, true>(a, boxes, list, n, image, s); break;
     }
 }

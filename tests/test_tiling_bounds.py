@@ -17,7 +17,7 @@ import pytest
 TZ, H_IMG = 64, 2
 CORE_X0, CORE_Z0 = 6, 8
 SHAPES = {"core16": (16, 64, False), "core32": (32, 64, False), "band_z": (64, 16, True),
-          "band_z8": (128, 8, True), "band_x": (8, 128, True)}
+          "band_z8": (120, 8, True), "band_x": (8, 128, True)}
 
 
 def tile_range(k, ntile, T, n):
@@ -105,7 +105,7 @@ def build_boxes(nx, nz, tile_x):
             add(core, max(sx, cx0), min(ex, cx1), max(sz, cz0), min(ez, cz1))
     for sz, ez in ((0, cz0), (cz1, nz)):
         thin = ez - sz <= 8
-        for sx, ex in tiles(nx, 128 if thin else 64):
+        for sx, ex in tiles(nx, 120 if thin else 64):
             add("band_z8" if thin else "band_z", sx, ex, sz, ez)
     for sz in range(cz0, cz1, 128):
         ez = min(sz + 128, cz1)
@@ -142,10 +142,17 @@ def check_x(sx, ex, TX, n, edge):
             assert all((g + o) in loaded and 0 <= g + o - ws < W for o in (-1, 0, 1))
 
 
+def z_windows(sz, TZb):
+    """(state window start, rows, product start, rows) of a box (k_step's TileInfo)."""
+    if TZb == 8:  # compact windows of the 8-row bands
+        return (0, 16, 0, 12) if sz == 0 else (sz - 8, 16, sz - 4, 12)
+    return sz - 8, TZb + 16, sz - 4, TZb + 8
+
+
 def check_z(sz, ez, TZb, n, edge):
     ldz = pitch(n)
     assert sz % 4 == 0 and 0 <= sz < ez <= n and ez - sz <= TZb
-    ws, WZ, pz0, PZ = sz - 8, TZb + 16, sz - 4, TZb + 8
+    ws, WZ, pz0, PZ = z_windows(sz, TZb)
     loaded = set(p for p in range(ws, ws + WZ) if 0 <= p < n)  # padding rows are not data
     plo, phi = needed_products(sz, ez, n)
     need = set(range(plo, phi))
@@ -290,18 +297,22 @@ def shape_consts(TX, TZb):
     """StepShape of kernels.hip: lane mapping constants of one box class."""
     rowmaj = TZb <= 16
     WQ, PQ, IX = (TZb + 16) // 4, (TZb + 8) // 4, TX + 4
+    if TZb == 8:
+        WQ, PQ = 4, 3
     WZ = 4 * ((WQ | 1) if rowmaj else WQ)
     PZ = 4 * ((PQ | 1) if rowmaj else PQ)
     IXP = (IX + 63) // 64 * 64 if rowmaj else IX
+    BX = (TX + 63) // 64 * 64 if rowmaj else TX
     QC = TZb // 4
-    return dict(rowmaj=rowmaj, WQ=WQ, PQ=PQ, IX=IX, WZ=WZ, PZ=PZ, IXP=IXP, QC=QC, WX=TX + 8,
-                NQ2=(IXP * PQ + THREADS - 1) // THREADS, NQ3=TX * QC // THREADS)
+    return dict(rowmaj=rowmaj, WQ=WQ, PQ=PQ, IX=IX, WZ=WZ, PZ=PZ, IXP=IXP, QC=QC, WX=TX + 8, BX=BX,
+                NQ2=(IXP * PQ + THREADS - 1) // THREADS, NQ3=(BX * QC + THREADS - 1) // THREADS)
 
 
 def out_quad(c, TX, i):
-    """(column, quad row) of new-level slot i (out_quad of kernels.hip)."""
+    """(column, quad row) of new-level slot i (out_quad of kernels.hip); a column at or
+    past the box width is an idle lane."""
     if c["rowmaj"]:
-        return i % TX, i // TX
+        return i % c["BX"], i // c["BX"]
     return i // c["QC"], i % c["QC"]
 
 
@@ -320,11 +331,14 @@ def test_lane_mappings(cls):
     same on all 64 lanes of its wavefront; LDS column strides hold the windows."""
     TX, TZb, _ = SHAPES[cls]
     c = shape_consts(TX, TZb)
-    assert (TX * c["QC"]) % THREADS == 0
+    assert c["rowmaj"] or (TX * c["QC"]) % THREADS == 0
     seen = set()
     for i in range(c["NQ3"] * THREADS):
         col, row = out_quad(c, TX, i)
-        assert 0 <= col < TX and 0 <= row < c["QC"]
+        assert 0 <= col < c["BX"] and 0 <= row < c["QC"]
+        if col >= TX:
+            continue
+        assert (col, row) not in seen
         seen.add((col, row))
         if c["rowmaj"]:
             assert out_quad(c, TX, i - i % WAVE)[1] == row
@@ -338,7 +352,8 @@ def test_lane_mappings(cls):
             assert (lc, lq) not in seen
             seen.add((lc, lq))
     assert len(seen) == c["IX"] * c["PQ"]
-    assert c["WZ"] >= TZb + 16 and c["PZ"] >= TZb + 8 and c["WZ"] % 4 == 0 and c["PZ"] % 4 == 0
+    _, wrows, _, prows = z_windows(8, TZb)
+    assert c["WZ"] >= wrows and c["PZ"] >= prows and c["WZ"] % 4 == 0 and c["PZ"] % 4 == 0
     if c["rowmaj"]:
         # column-strided 16-byte LDS reads of consecutive lanes: odd quad strides
         assert (c["WZ"] // 4) % 2 == 1 and (c["PZ"] // 4) % 2 == 1
