@@ -139,6 +139,7 @@ struct Engine {
     int opt_image_fused = 0;  // 1: imaging condition fused into the adjoint step kernels
     int opt_img_buffers = 1;  // LDS window buffers of the imaging kernel (1 or 2)
     int opt_img_stream = 1;   // 1: imaging kernel on its own stream beside the next step
+    int opt_img_sbp = 1;      // 1: density term by parts in time (sub = 1, separate imaging)
     int opt_edge_stream = 1;  // edge-band launches on their own stream
     hipStream_t st_edge = nullptr;       // edge-tile launches run beside the interior ones
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
@@ -602,27 +603,31 @@ struct Engine {
     }
 
     // prev/curr/next of one propagation (forward or adjoint)
+    // prev/curr/next of one propagation (forward or adjoint), rotating over n = 3 levels,
+    // or 4 when the imaging of a step still reads the step's previous level while the
+    // next step runs
     struct Levels {
-        float* b[3];
+        float* b[4] = {nullptr, nullptr, nullptr, nullptr};
+        int n = 3;
         int prev = 0, curr = 1, next = 2;
         void rotate()
         {
-            int t = prev;
             prev = curr;
             curr = next;
-            next = t;
+            next = (next + 1) % n;
         }
         float* P() const { return b[prev]; }
         float* C() const { return b[curr]; }
         float* N() const { return b[next]; }
     };
 
-    Levels levels_in(DevMem& m)
+    Levels levels_in(DevMem& m, int n = 3)
     {
         const long long ss = state_stride();
-        m.reserve((size_t)3 * ss * sizeof(float));
+        m.reserve((size_t)n * ss * sizeof(float));
         Levels L;
-        for (int i = 0; i < 3; i++) L.b[i] = m.as<float>() + i * ss;
+        L.n = n;
+        for (int i = 0; i < n; i++) L.b[i] = m.as<float>() + i * ss;
         return L;
     }
 
@@ -630,8 +635,7 @@ struct Engine {
     void init_levels(Levels& L, bool adjoint, const float* traces)
     {
         const long long ss = state_stride();
-        launch_fill(L.b[0], 3 * ss, 0.f, st);
-        (void)ss;
+        launch_fill(L.b[0], L.n * ss, 0.f, st);
         L.prev = 0;
         L.curr = 1;
         L.next = 2;
@@ -647,15 +651,18 @@ struct Engine {
     // img_idx >= 0: image forward snapshot img_idx in this adjoint step (its segment must
     // be loaded): the step's current level is the lagged adjoint level and its new level
     // the adjoint level of the reference's compute_gradients call
-    void step(Levels& L, int it, bool adjoint, const float* traces, float* rcv, int img_idx = -1)
+    // sbp: img_idx is snapshot j of the by-parts imaging (imaged after this step with the
+    // step's previous, current and new levels, see sbp_image_args)
+    void step(Levels& L, int it, bool adjoint, const float* traces, float* rcv, int img_idx = -1, bool sbp = false)
     {
         StepArgs a = base_args();
         // imaging fused into the step kernels, or (default) a separate k_img pass after the
         // step and its point sources
-        const bool fuse_img = img_idx >= 0 && opt_image_fused;
+        const bool fuse_img = img_idx >= 0 && opt_image_fused && !sbp;
         StepArgs ia{};
         if (img_idx >= 0) {
-            set_image_args(a, img_idx);
+            if (sbp) sbp_image_args(a, img_idx);
+            else set_image_args(a, img_idx);
             ia = a;
             // shots per imaging workgroup: the separate kernel is fastest with small
             // groups (more workgroups in flight; measured 4 < 16 < 64 at 64 shots)
@@ -718,17 +725,10 @@ struct Engine {
                             d_deriv.as<float>(), d_tx.as<float>(), d_tz.as<float>(), d_div_tb.as<float>(),
                             d_div_lr.as<float>(), im, st);
         if (img_idx >= 0 && !fuse_img) {
+            ia.aprev = L.P();
             ia.curr = L.C();
             ia.next = L.N();
-            if (opt_img_stream) {
-                HIPCHECK(hipEventRecord(ev_img_in, st));
-                HIPCHECK(hipStreamWaitEvent(st_img, ev_img_in, 0));
-                launch_img(ia, vti, tile_x, opt_img_buffers, st_img);
-                HIPCHECK(hipEventRecord(ev_img_done[it & 1], st_img));
-                img_out[it & 1] = true;
-            } else {
-                launch_img(ia, vti, tile_x, opt_img_buffers, st);
-            }
+            run_img(ia, it, sbp);
         }
         L.rotate();
         stat_steps++;
@@ -790,7 +790,7 @@ struct Engine {
     {
         auto P = [](const void* p) { return (long long)(uintptr_t)p; };
         return {kind, P(p0), P(p1), flag, g.nt, ns, nr, tile_x, opt_spb, opt_spb_edge, opt_lag, seg_K, nseg,
-                opt_bf16, opt_stiff, opt_edge_stream, opt_spb_img, opt_image_fused, opt_img_buffers, opt_img_stream, P(d_ckpt16.p), P(d_ckpt_scale.p), P(d_amax.p),
+                opt_bf16, opt_stiff, opt_edge_stream, opt_spb_img, opt_image_fused, opt_img_buffers, opt_img_stream, opt_img_sbp, P(d_ckpt16.p), P(d_ckpt_scale.p), P(d_amax.p),
                 P(d_fwd.p), P(d_adj.p), P(d_ckpt.p), P(d_seg.p), P(d_gshot.p), P(d_model.p), P(d_deriv.p),
                 P(d_boxes.p), P(d_list_core.p), P(d_list_bz.p), P(d_list_bz8.p), P(d_list_bx.p), P(inj_src.d_cells.p), P(inj_src.d_ptr.p),
                 P(inj_src.d_ctrace.p), P(inj_src.d_cw.p), P(inj_rcv.d_cells.p), P(inj_rcv.d_ptr.p), P(inj_rcv.d_ctrace.p),
@@ -993,6 +993,61 @@ struct Engine {
     }
 
     // recompute the snapshots of segment j from its checkpoint
+    // imaging launch of adjoint step it, on st_img beside the next step (or on st)
+    void run_img(const StepArgs& ia, int it, bool sbp)
+    {
+        if (opt_img_stream) {
+            HIPCHECK(hipEventRecord(ev_img_in, st));
+            HIPCHECK(hipStreamWaitEvent(st_img, ev_img_in, 0));
+            launch_img(ia, vti, tile_x, opt_img_buffers, sbp, st_img);
+            HIPCHECK(hipEventRecord(ev_img_done[it & 1], st_img));
+            img_out[it & 1] = true;
+        } else {
+            launch_img(ia, vti, tile_x, opt_img_buffers, sbp, st);
+        }
+    }
+
+    // The density term by parts in time (sub = 1).  The reference's density gradient is
+    // sum over imaged snapshots idx = 0 .. nt-2 of wrho(idx) <a_idx, D2 F_idx>, with a_idx
+    // the adjoint level nt-1-idx and D2 F_idx = F_idx+1 - 2 F_idx + F_idx-1 (F_1 - F_0 at
+    // idx = 0).  Collected per snapshot j it is <F_j, c_j-1 - 2 c_j + c_j+1>, c = wrho a,
+    // where only imaged idx contribute (c_j-1 for j >= 1, -2 c_j for 1 <= j <= nt-2 and
+    // -c_0 for j = 0, c_j+1 for j <= nt-3).  Snapshot j = nt-1-it is imaged after adjoint
+    // step it: a_j-1, a_j, a_j+1 are its new, current and previous levels, and its
+    // stiffness terms pair the previous level's derivatives with F_j's (the reference's
+    // lag), for j <= nt-2.  Snapshot 0 is imaged after the last step.
+    void sbp_image_args(StepArgs& a, int j)
+    {
+        const int sj = j / seg_K;
+        if (seg_loaded != sj) throw std::logic_error("imaging segment not loaded");
+        const int last = g.nt - 2;  // last imaged snapshot
+        auto wr = [&](int i) {
+            int mode;
+            float wlm, wrho;
+            image_weights(i, mode, wlm, wrho);
+            return wrho;
+        };
+        int mode;
+        float wrho;
+        image_weights(j, mode, a.wlm, wrho);
+        a.img_mode = mode;
+        a.wrho = wrho;
+        a.wn = (j >= 1 && j - 1 <= last) ? wr(j - 1) : 0.f;
+        a.wc = (j >= 1 && j <= last) ? -2.f * wr(j) : (j == 0 ? -wr(0) : 0.f);
+        a.wp = (j + 1 <= last) ? wr(j + 1) : 0.f;
+        a.img_deriv = j <= last ? 1 : 0;
+        a.f0 = seg_slot(sj, j);
+        a.fm = a.fp = nullptr;
+        a.fshot_stride = 2 * nxz;
+        a.grad = d_gshot.as<float>();
+        a.gslab_stride = (long long)np * nxz;
+        const float* dv = d_deriv.as<float>();
+        a.ge = vti ? dv + 3 * nxz : nullptr;
+        a.gv2 = vti ? dv + 4 * nxz : nullptr;
+        a.gv3 = vti ? dv + 5 * nxz : nullptr;
+        a.stiff = vti && opt_stiff;
+    }
+
     // the main stream waits for every imaging launch still running on st_img
     void img_join()
     {
@@ -1142,10 +1197,37 @@ struct Engine {
         const int nslab = ns;
         d_gshot.reserve((size_t)nslab * np * nxz * sizeof(float));
         launch_fill(d_gshot.as<float>(), (long long)nslab * np * nxz, 0.f, st);
-        Levels A = levels_in(d_adj);
-        init_levels(A, true, dadj);
         const int nt = g.nt, sub = g.sub;
-        if (fused) {
+        // the density term by parts needs each step's previous level while the next step
+        // runs: four adjoint levels
+        const bool sbp = fused && opt_img_sbp && sub == 1 && !opt_image_fused;
+        Levels A = levels_in(d_adj, sbp ? 4 : 3);
+        init_levels(A, true, dadj);
+        if (sbp) {
+            const int seg0 = seg_loaded;
+            run_loop(graph_adj, loop_key(2, dadj, dsrc, seg0), [&] {
+                Levels Ac = A;
+                img_out[0] = img_out[1] = false;
+                for (int it = 0; it < nt - 1; it++) {
+                    const int j = nt - 1 - it;
+                    load_segment(j / seg_K, dsrc);
+                    stat_image_calls++;
+                    step(Ac, it, true, dadj, nullptr, j, true);
+                }
+                // snapshot 0: the last level and the one before it
+                load_segment(0, dsrc);
+                StepArgs ia = base_args();
+                sbp_image_args(ia, 0);
+                ia.spb = std::max(1, std::min(opt_spb_img > 0 ? opt_spb_img : 4, ns));
+                ia.ngroups = (ns + ia.spb - 1) / ia.spb;
+                ia.aprev = Ac.P();
+                ia.curr = Ac.C();
+                ia.next = Ac.C();  // weight 0
+                stat_image_calls++;
+                run_img(ia, nt - 1, true);
+                img_join();
+            });
+        } else if (fused) {
             // step it takes the adjoint from level it to it+1; the pair (it, it+1) images
             // snapshot (nt-2-it)/sub when that divides (the reference images at its
             // iteration it+1, after the step, with the derivatives of level it)
@@ -1320,6 +1402,8 @@ int fwi2d_engine_set_option(fwi2d_engine* e, const char* key, long long value)
             x->opt_spb_edge = (int)value;
         } else if (k == "image_fused") {
             x->opt_image_fused = value ? 1 : 0;
+        } else if (k == "image_sbp") {
+            x->opt_img_sbp = value ? 1 : 0;
         } else if (k == "image_stream") {
             x->opt_img_stream = value ? 1 : 0;
         } else if (k == "image_buffers") {
@@ -1349,6 +1433,7 @@ long long fwi2d_engine_get_stat(fwi2d_engine* e, const char* key)
     if (k == "image_fused") return x->opt_image_fused;
     if (k == "image_buffers") return x->opt_img_buffers;
     if (k == "image_stream") return x->opt_img_stream;
+    if (k == "image_sbp") return x->opt_img_sbp;
     if (k == "ntiles") return x->ntiles();
     if (k == "interior_tiles") return (long long)x->list_core.size();
     if (k == "graph_captures") return x->graph_captures;

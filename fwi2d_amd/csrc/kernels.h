@@ -56,6 +56,12 @@ struct StepArgs {
     long long gslab_stride;    // shots' sum to the slab of its first shot
     const float *ge, *gv2, *gv3;  // VTI: 1 + 2 eps, d c13 / d lambda, d c13 / d mu
     int stiff;                 // VTI: image c11, c13, c33, c55, rho instead of lambda, mu, rho
+    // density term by parts in time (k_img, one snapshot per step): f0 is snapshot j, its
+    // derivatives meet those of aprev; the density term is f0 . (wn next + wc curr + wp
+    // aprev) with the second-difference weights of the neighbouring snapshots
+    const float* aprev;
+    float wn, wc, wp;
+    int img_deriv;             // 0: snapshot j has no stiffness terms (only density terms)
 };
 
 struct ImageArgs {
@@ -119,7 +125,7 @@ void launch_image(const ImageArgs& a, bool vti, int tile_x, hipStream_t s);
 // imaging condition after an adjoint step over the tile_x x 64 tile grid (a.ntx x a.ntz
 // tiles, a.ngroups groups of a.spb shots); a.curr lagged adjoint level, a.next the new
 // one; nbuf 2: the next shot's windows load into a second LDS buffer during this shot
-void launch_img(const StepArgs& a, bool vti, int tile_x, int nbuf, hipStream_t s);
+void launch_img(const StepArgs& a, bool vti, int tile_x, int nbuf, bool sbp, hipStream_t s);
 void launch_sum_shots(const float* per_shot, long long shot_stride, int nshots, long long n, float* out,
                       int accumulate, hipStream_t s);
 void launch_fill(float* p, long long n, float v, hipStream_t s);

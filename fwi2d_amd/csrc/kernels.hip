@@ -1162,7 +1162,9 @@ k_step(const StepArgs a, const int4* boxes, const int* list)
 // group's first shot.  Tiles with every cell at least 4 from the edges take the interior
 // stencils only; the others choose per quad.
 // ---------------------------------------------------------------------------
-template <bool VTI, int TX, int NBUF>
+// SBP = true: the density term by parts in time (a.aprev is the derivative level, the
+// cell values are the adjoint levels it and it+1, see StepArgs).
+template <bool VTI, int TX, int NBUF, bool SBP>
 __global__ void __launch_bounds__(kThreads) k_img(const StepArgs a)
 {
     constexpr int TZ = kTileZ, HX = 2, HZ = 4;
@@ -1196,7 +1198,7 @@ __global__ void __launch_bounds__(kThreads) k_img(const StepArgs a)
 
     // windows of one shot into buffer b: window quads tid + m * 256
     auto issue_windows = [&](int shot, int b) {
-        const float* A = a.curr + shot * a.shot_stride;
+        const float* A = (SBP ? a.aprev : a.curr) + shot * a.shot_stride;
         const float* F = a.f0 + shot * a.fshot_stride;
         float* base = sW + b * 4 * WS;
 #pragma unroll
@@ -1232,23 +1234,30 @@ __global__ void __launch_bounds__(kThreads) k_img(const StepArgs a)
             if (shot > s0) __syncthreads();  // every wavefront is done with the windows
             issue_windows(shot, 0);
         }
-        // cell values of this shot: adjoint density level, snapshots idx-1 and idx+1
-        float4 qnx[NQ3], qnz[NQ3], qmx[NQ3], qmz[NQ3], qpx[NQ3], qpz[NQ3];
+        // cell values of this shot: the adjoint density level and snapshots idx-1, idx+1
+        // (by parts: the adjoint levels it+1 and it)
+        float4 qnx[NQ3], qnz[NQ3], qmx[NQ3], qmz[NQ3], qpx[NQ3], qpz[SBP ? 1 : NQ3];
 #pragma unroll
         for (int m = 0; m < NQ3; m++) {
             const int i = tid + m * kThreads;
             const int gx = sx + i / QC, z0 = sz + 4 * (i % QC);
             const bool ok = gx < ex && z0 < ez;
             const long long g = ok ? (long long)gx * ldz + z0 : 0;
-            const long long fo = shot * a.fshot_stride + g;
             const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
             const float* N = a.next + shot * a.shot_stride;
             qnx[m] = ok ? ld4(N + g) : z4;
             qnz[m] = ok ? ld4(N + nxz + g) : z4;
-            qmx[m] = ok && a.img_mode != 1 ? ld4(a.fm + fo) : z4;
-            qmz[m] = ok && a.img_mode != 1 ? ld4(a.fm + fo + nxz) : z4;
-            qpx[m] = ok && a.img_mode != 2 ? ld4(a.fp + fo) : z4;
-            qpz[m] = ok && a.img_mode != 2 ? ld4(a.fp + fo + nxz) : z4;
+            if constexpr (SBP) {
+                const float* C = a.curr + shot * a.shot_stride;
+                qmx[m] = ok ? ld4(C + g) : z4;
+                qmz[m] = ok ? ld4(C + nxz + g) : z4;
+            } else {
+                const long long fo = shot * a.fshot_stride + g;
+                qmx[m] = ok && a.img_mode != 1 ? ld4(a.fm + fo) : z4;
+                qmz[m] = ok && a.img_mode != 1 ? ld4(a.fm + fo + nxz) : z4;
+                qpx[m] = ok && a.img_mode != 2 ? ld4(a.fp + fo) : z4;
+                qpz[m] = ok && a.img_mode != 2 ? ld4(a.fp + fo + nxz) : z4;
+            }
         }
         __builtin_amdgcn_s_waitcnt(0);  // the windows (and cell values) have landed
         __syncthreads();
@@ -1264,7 +1273,32 @@ __global__ void __launch_bounds__(kThreads) k_img(const StepArgs a)
             const int gx = sx + i / QC, z0 = sz + 4 * (i % QC);
             if (gx >= ex || z0 >= ez) continue;
             const int w = (gx - wsx) * WZ + (z0 - wsz);
-            {
+            if constexpr (SBP) {
+                // density term by parts: snapshot j against the weighted adjoint levels
+                float f0x[4], f0z[4], px[4], pz[4], cx[4], cz[4], nx4[4], nz4[4];
+                unpack(ld4(&sFx[w]), f0x);
+                unpack(ld4(&sFz[w]), f0z);
+                unpack(ld4(&sAx[w]), px);
+                unpack(ld4(&sAz[w]), pz);
+                unpack(qmx[m], cx);
+                unpack(qmz[m], cz);
+                unpack(qnx[m], nx4);
+                unpack(qnz[m], nz4);
+                const bool st = VTI && a.stiff;
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    if (z0 + k >= nz) continue;
+                    const float ax = a.wn * nx4[k] + a.wc * cx[k] + a.wp * px[k];
+                    const float az = a.wn * nz4[k] + a.wc * cz[k] + a.wp * pz[k];
+                    const float r = f0x[k] * ax + f0z[k] * az;
+                    if (VTI) {
+                        gr[m][k] += st ? 0.f : r;
+                        g4[VTI ? m : 0][k] += st ? r : 0.f;
+                    } else {
+                        gr[m][k] += r;
+                    }
+                }
+            } else {
                 // density term first, so that the cell values are dead before the
                 // derivative reads
                 float f0x[4], f0z[4], fmx[4], fmz[4], fpx[4], fpz[4], ox[4], oz[4];
@@ -1278,6 +1312,7 @@ __global__ void __launch_bounds__(kThreads) k_img(const StepArgs a)
                 unpack(qnz[m], oz);
                 image_rho<VTI>(a, z0, f0x, f0z, fmx, fmz, fpx, fpz, ox, oz, gr[m], g4[VTI ? m : 0]);
             }
+            if (SBP && !a.img_deriv) continue;
             __builtin_amdgcn_sched_barrier(0);
             float axx[4], azx[4], axz[4], azz[4], t0x[4], t1z[4], t2x[4], t3z[4];
             if (inner) {
@@ -1652,23 +1687,30 @@ void launch_step(const StepArgs& a, bool vti, BoxClass c, const int4* boxes, con
     else step_class<false>(a, c, boxes, list, n, image, s);
 }
 
-template <bool VTI, int NBUF>
+template <bool VTI, int NBUF, bool SBP>
 static void img_dispatch(const StepArgs& a, int tile_x, int n, hipStream_t s)
 {
-    if (tile_x == 32) k_img<VTI, 32, NBUF><<<n, kThreads, 0, s>>>(a);
-    else k_img<VTI, 16, NBUF><<<n, kThreads, 0, s>>>(a);
+    if (tile_x == 32) k_img<VTI, 32, NBUF, SBP><<<n, kThreads, 0, s>>>(a);
+    else k_img<VTI, 16, NBUF, SBP><<<n, kThreads, 0, s>>>(a);
 }
 
-void launch_img(const StepArgs& a, bool vti, int tile_x, int nbuf, hipStream_t s)
+template <bool VTI, bool SBP>
+static void img_bufs(const StepArgs& a, int tile_x, int nbuf, int n, hipStream_t s)
+{
+    if (nbuf == 2) img_dispatch<VTI, 2, SBP>(a, tile_x, n, s);
+    else img_dispatch<VTI, 1, SBP>(a, tile_x, n, s);
+}
+
+void launch_img(const StepArgs& a, bool vti, int tile_x, int nbuf, bool sbp, hipStream_t s)
 {
     const int n = a.ntx * a.ntz * a.ngroups;
     if (n <= 0) return;
     if (vti) {
-        if (nbuf == 2) img_dispatch<true, 2>(a, tile_x, n, s);
-        else img_dispatch<true, 1>(a, tile_x, n, s);
+        if (sbp) img_bufs<true, true>(a, tile_x, nbuf, n, s);
+        else img_bufs<true, false>(a, tile_x, nbuf, n, s);
     } else {
-        if (nbuf == 2) img_dispatch<false, 2>(a, tile_x, n, s);
-        else img_dispatch<false, 1>(a, tile_x, n, s);
+        if (sbp) img_bufs<false, true>(a, tile_x, nbuf, n, s);
+        else img_bufs<false, false>(a, tile_x, nbuf, n, s);
     }
 }
 
