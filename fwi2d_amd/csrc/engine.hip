@@ -140,7 +140,7 @@ struct Engine {
     int opt_img_buffers = 1;  // LDS window buffers of the imaging kernel (1 or 2)
     int opt_img_stream = 1;   // 1: imaging kernel on its own stream beside the next step
     int opt_img_sbp = 1;      // 1: density term by parts in time (sub = 1, separate imaging)
-    int opt_edge_stream = 1;  // edge-band launches on their own stream
+    int opt_edge_stream = 1;  // edge-band launches on their own stream (1: before the core launch, 2: after)
     hipStream_t st_edge = nullptr;       // edge-tile launches run beside the interior ones
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     // imaging of adjoint step it runs on st_img beside step it+1; step it+2 (which
@@ -692,17 +692,21 @@ struct Engine {
         e.ngroups = (ns + e.spb - 1) / e.spb;
         const bool img = fuse_img;
         const int4* bx = d_boxes.as<int4>();
+        // edge_stream 1: bands enqueued before the core launch, 2: after it (they then fill
+        // the core launch's last, partial round of workgroups)
         hipStream_t se = opt_edge_stream ? st_edge : st;
-        if (opt_edge_stream) {
-            HIPCHECK(hipEventRecord(ev_fork, st));
-            HIPCHECK(hipStreamWaitEvent(st_edge, ev_fork, 0));
-        }
+        if (opt_edge_stream) HIPCHECK(hipEventRecord(ev_fork, st));
+        auto core = [&] {
+            launch_step(a, vti, tile_x == 32 ? BoxClass::core32 : BoxClass::core16, bx, d_list_core.as<int>(),
+                        (int)list_core.size(), img, st);
+        };
+        if (opt_edge_stream == 2) core();
+        if (opt_edge_stream) HIPCHECK(hipStreamWaitEvent(st_edge, ev_fork, 0));
         launch_step(e, vti, BoxClass::band_z8, bx, d_list_bz8.as<int>(), (int)list_bz8.size(), img, se);
         launch_step(e, vti, BoxClass::band_z, bx, d_list_bz.as<int>(), (int)list_bz.size(), img, se);
         launch_step(e, vti, BoxClass::band_x, bx, d_list_bx.as<int>(), (int)list_bx.size(), img, se);
         if (opt_edge_stream) HIPCHECK(hipEventRecord(ev_join, st_edge));
-        launch_step(a, vti, tile_x == 32 ? BoxClass::core32 : BoxClass::core16, bx, d_list_core.as<int>(),
-                    (int)list_core.size(), img, st);
+        if (opt_edge_stream != 2) core();
         if (opt_edge_stream) HIPCHECK(hipStreamWaitEvent(st, ev_join, 0));
         // point sources of this step into the new level
         const InjList& inj = adjoint ? inj_rcv : inj_src;
@@ -865,7 +869,7 @@ struct Engine {
             for (int spb : {1, 2, 4, 8, 16}) {
                 if (spb > 1 && spb > ns) continue;
                 for (int spe : {1, 2, 4})
-                    for (int es : {1, 0}) {
+                    for (int es : {1, 2, 0}) {
                         if (spe > 1 && spe > ns) continue;
                         set_tile(tx);
                         opt_spb = spb;
@@ -1390,7 +1394,8 @@ int fwi2d_engine_set_option(fwi2d_engine* e, const char* key, long long value)
             x->opt_bf16 = (int)value;
             x->ckpt_epoch = -1;
         } else if (k == "edge_stream") {
-            x->opt_edge_stream = value ? 1 : 0;
+            if (value < 0 || value > 2) throw std::runtime_error("edge_stream must be 0, 1 or 2");
+            x->opt_edge_stream = (int)value;
         } else if (k == "autotune") {
             x->opt_autotune = value ? 1 : 0;
             x->tuned = false;

@@ -87,8 +87,13 @@ __device__ __forceinline__ int xcd_remap(int bid, int nb)
     return (xcd < r) ? xcd * (q + 1) + local : r * (q + 1) + (xcd - r) * q + local;
 }
 
-__device__ __forceinline__ float4 ld4(const float* p) { return *reinterpret_cast<const float4*>(p); }
-__device__ __forceinline__ void st4(float* p, float4 v) { *reinterpret_cast<float4*>(p) = v; }
+// every quad access is 16-byte aligned (quads start at multiples of 4 cells of 16-byte
+// aligned arrays); saying so lets LDS quads load with one 128-bit read
+__device__ __forceinline__ float4 ld4(const float* p)
+{
+    return *reinterpret_cast<const float4*>(__builtin_assume_aligned(p, 16));
+}
+__device__ __forceinline__ void st4(float* p, float4 v) { *reinterpret_cast<float4*>(__builtin_assume_aligned(p, 16)) = v; }
 __device__ __forceinline__ void unpack(float4 v, float* d)
 {
     d[0] = v.x;
@@ -358,11 +363,24 @@ __device__ __forceinline__ void d2_quad_z(const float* m, const float* u1, const
     }
 }
 
+// i is a multiple of 4: the reads are indexed in quads, so that the compiler knows their
+// 16-byte alignment and emits 128-bit LDS reads (not pairs of 64-bit ones, which conflict
+// on banks for lanes 16 bytes apart)
+typedef float v4f __attribute__((ext_vector_type(4)));
+
 __device__ __forceinline__ void load_col12(const float* base, int i, float* c)
 {
-    unpack(ld4(base + i - 4), c);
-    unpack(ld4(base + i), c + 4);
-    unpack(ld4(base + i + 4), c + 8);
+    const v4f* q = reinterpret_cast<const v4f*>(base) + (i >> 2);
+    v4f a = q[-1], b = q[0], d = q[1];
+    // whole quads: the compiler would otherwise read only the 8 rows a first derivative
+    // uses, as 64-bit LDS reads 8 bytes off the 16-byte grid (2-way bank conflicts)
+    __asm__ volatile("" : "+v"(a), "+v"(d));
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        c[k] = a[k];
+        c[4 + k] = b[k];
+        c[8 + k] = d[k];
+    }
 }
 
 // D2(c) u at global index g of n along stride s: interior weights or closure rows

@@ -88,7 +88,8 @@ int fwi2d_engine_dot_f64(fwi2d_engine* e, const float* a, const float* b, const 
 /* Tuning and test options (no reference counterpart): "tile_x" (16 | 32 cells per tile
    along x), "shots_per_block" (1..64 shots advanced in turn by one step workgroup of an
    interior tile), "shots_per_block_edge" (the same for tiles at the grid edge),
-   "edge_stream" (1: edge-band launches run on a second stream beside the core launch),
+   "edge_stream" (1: edge-band launches run on a second stream beside the core launch,
+   enqueued before it; 2: the same, enqueued after it; 0: after it on the same stream),
    "shots_per_block_img" (shots per imaging workgroup; 0: 4, or "shots_per_block" when
    fused),
    "image_fused" (1: the imaging condition runs inside the adjoint step kernels; 0, the
