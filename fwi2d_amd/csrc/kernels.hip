@@ -1,15 +1,17 @@
 // Device kernels of the elastic SBP-SAT propagation engine, written for gfx950.
 //
 // Layout: every field is [x][ldz] with z contiguous (index ix*ldz + iz), the reference's
-// [x][z] order with z padded to a multiple of 4.  A workgroup of 256 threads owns a tile
-// of 64 (z) x TX (x) cells and a group of shots; lanes run along z and each moves 4
-// consecutive cells, so a wavefront's global accesses are contiguous 256-byte columns.
+// [x][z] order with z padded to a multiple of 32 cells (128-byte columns).  Workgroups
+// of 256 threads own a box of cells and a group of shots; a thread moves 4 consecutive
+// z cells (a quad) with 128-bit accesses.  The time step runs over a box decomposition
+// (interior core boxes and edge bands, built by the engine); the imaging kernel runs
+// over the tile grid below.
 //
-// Tiling: tiles start at multiples of the tile size except the last one in each
+// Tile grid: tiles start at multiples of the tile size except the last one in each
 // direction, which is anchored to end at the grid edge (start = n - T).  Each tile
 // writes the cells in [start, next tile start) (the last tile writes to the edge), so
-// writes never overlap, the SBP closure rows near the far edge are always written by
-// the anchored tile, and a halo of 4 cells holds every stencil input.
+// writes never overlap and the SBP closure rows near the far edge are always written by
+// the anchored tile.
 #include "kernels.h"
 #include "sbp_tables.h"
 #include <hip/hip_fp16.h>
@@ -809,9 +811,10 @@ __device__ __forceinline__ void quad_to_lds(const float* src, float* lds_wave)
 // core (EDGE = false) hold only cells whose stencils are all interior, so no closure or
 // SAT code is compiled in; boxes of the edge bands (EDGE = true) take the closure and
 // SAT paths where a quad needs them.
-// IMG: adjoint step with the imaging condition of forward snapshot a.img_mode/idx fused in;
-// the gradient contributions of the workgroup's shots are summed in registers and added
-// once per launch to the gradient slab of the group's first shot.
+// IMG (the image_fused option; by default the imaging runs in k_img): adjoint step with
+// the imaging condition of forward snapshot idx fused in; the gradient contributions of
+// the workgroup's shots are summed in registers and added once per launch to the
+// gradient slab of the group's first shot.
 template <bool VTI, int TX, int TZ, bool EDGE, bool IMG>
 __global__ void __launch_bounds__(kThreads)
 k_step(const StepArgs a, const int4* boxes, const int* list)
