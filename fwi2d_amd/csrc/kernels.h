@@ -46,8 +46,9 @@ struct StepArgs {
     const float* curr;
     float* next;
     long long shot_stride;
-    // imaging fused into an adjoint step: the derivatives of the step's current level
-    // and the new level image forward snapshot idx (compute_gradients, we_op.cpp:505-562)
+    // imaging (k_img after an adjoint step, or fused into it): the derivatives of the
+    // step's current level and its new level image forward snapshot idx
+    // (compute_gradients, we_op.cpp:505-562)
     int img_mode;              // 0 interior idx, 1 first (idx 0), 2 last
     float wlm, wrho;           // lambda/mu and rho weights of this snapshot
     const float *f0, *fm, *fp; // forward snapshots idx, idx-1, idx+1 (shot 0 base)
