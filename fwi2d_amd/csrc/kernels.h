@@ -125,7 +125,8 @@ void launch_extract(const float* state, long long shot_stride, long long nxz, in
 void launch_image(const ImageArgs& a, bool vti, int tile_x, hipStream_t s);
 // imaging condition after an adjoint step over the tile_x x 64 tile grid (a.ntx x a.ntz
 // tiles, a.ngroups groups of a.spb shots); a.curr lagged adjoint level, a.next the new
-// one; nbuf 2: the next shot's windows load into a second LDS buffer during this shot
+// one (sbp: the density term by parts, a.aprev the lagged level, see StepArgs); nbuf 2:
+// the next shot's windows load into a second LDS buffer during this shot
 void launch_img(const StepArgs& a, bool vti, int tile_x, int nbuf, bool sbp, hipStream_t s);
 void launch_sum_shots(const float* per_shot, long long shot_stride, int nshots, long long n, float* out,
                       int accumulate, hipStream_t s);
