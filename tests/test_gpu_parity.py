@@ -74,11 +74,13 @@ def oracle_forward(orc, prob, model, sxz, rxz, src, nt, dt, sub=0, full=None):
     return np.concatenate(out, axis=1)
 
 
-def run_engine(lib, prob, model, sxz, rxz, src, nt, dt, sub=0, tile_x=16, keep=False, spb=None):
+def run_engine(lib, prob, model, sxz, rxz, src, nt, dt, sub=0, tile_x=16, keep=False, spb=None, options=()):
     eng = lib.Engine(lib.grid_from(prob, nt, dt, sub))
     eng.set_option("tile_x", tile_x)
     if spb is not None:
         eng.set_option("shots_per_block", spb)
+    for k, v in options:
+        eng.set_option(k, v)
     eng.set_model(model)
     eng.set_shots(sxz, rxz)
     return eng, eng.forward(src, keep_checkpoints=keep)
@@ -120,6 +122,10 @@ def test_tile_width_batch_and_shot_groups(lib):
     for spb in (1, 2, 3):
         _, c = run_engine(lib, *case, spb=spb)
         assert np.array_equal(a, c), spb
+    # edge-band workgroup groups and launch orders (same stream, before or after the core)
+    for opts in ((("shots_per_block_edge", 2),), (("edge_stream", 0),), (("edge_stream", 2),)):
+        _, c = run_engine(lib, *case, options=opts)
+        assert np.array_equal(a, c), opts
     # each shot alone gives the same traces as the batch
     r0 = 0
     for s in range(3):
