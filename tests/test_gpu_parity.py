@@ -377,6 +377,21 @@ def test_operator_level_forward_and_gradient(lib, orc64):
     assert rel(g2, 2 * gk.astype(np.float64)) < 1e-6
 
 
+def test_config0_full_size_forward(lib, orc64, orc32):
+    """The reference's CPU-runnable case at full size: 1 shot on 201 x 601, nt = 2000,
+    15 Hz Ricker at 20 m depth, 150 surface receivers, free surface and 30-cell tapers,
+    particle-velocity seismograms (apply_forward), against the FP64 oracle within the
+    FP32 gap of the oracle's own FP32 build."""
+    prob, model, src = W.config0()
+    ref, _ = orc64.apply_forward(orc64.make_op(prob), model, src)
+    gap = rel(orc32.apply_forward(orc32.make_op(prob), model, src)[0], ref)
+    got = lib.WeOp(prob, src).apply_forward(model)
+    err = rel(got, ref)
+    tol = max(1e-5, 2 * gap)
+    print("config 0: engine-vs-fp64 %.3e  oracle-fp32-vs-fp64 %.3e  tol %.1e" % (err, gap, tol))
+    assert np.all(np.isfinite(got)) and err <= tol
+
+
 def test_errors_are_reported(lib):
     prob, model, sxz, rxz, src, nt, dt = engine_case(nt=50)
     eng = lib.Engine(lib.grid_from(prob, nt, dt, 1))
