@@ -49,6 +49,10 @@ def parse():
                          "counters per dispatch) and exit")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device-to-device copy 5 times (HBM counter calibration) and exit")
+    ap.add_argument("--step-only", type=int, metavar="EDGE_STREAM", default=None,
+                    help="only autotune and time the forward time step of the workload with this edge_stream "
+                         "setting (0: every launch of a step on one stream, for per-launch kernel traces), "
+                         "print the step time and exit")
     return ap.parse_args()
 
 
@@ -274,6 +278,27 @@ def cpu_baseline(nt_sample=400, kind="c2"):
                 % (nz, nx, " (VTI)" if vti else "", nt_sample, nt_sample))
 
 
+def step_only(args):
+    """The forward time step alone: autotuned as in the bench, then 40 eager steps with the
+    given edge_stream setting, timed with HIP events on the engine stream."""
+    import torch
+    import fwi2d_amd
+    fwi2d_amd.load()
+    wl = WORKLOADS[args.workload]
+    ns, nt = args.shots or wl[4], 41
+    prob, m_true, m0, sxz, rxz, src, dt = workload(args.workload, ns, nt)
+    B = Batch(fwi2d_amd, torch, prob, m_true, m0, sxz, rxz, src, nt, dt, args.tile_x)
+    B.eng.set_option("graph", 0)
+    B.eng.set_option("edge_stream", args.step_only)
+    B.eng.set_option("timing", 1)
+    for _ in range(2):
+        B.eng.forward_device(B.src.data_ptr(), B.rcv.data_ptr(), keep_checkpoints=False)
+    us = B.eng.stat("fwd_loop_ns") / 1e3 / B.eng.stat("fwd_loop_steps")
+    print(json.dumps({"workload": args.workload, "edge_stream": args.step_only, "step_us": round(us, 2),
+                      "tile_x": B.eng.stat("tile_x"), "shots_per_block": B.eng.stat("shots_per_block"),
+                      "shots_per_block_edge": B.eng.stat("shots_per_block_edge")}))
+
+
 def main():
     args = parse()
     if args.summarize:
@@ -281,6 +306,9 @@ def main():
         return
     if args.calibrate_copy:
         calibrate_copy()
+        return
+    if args.step_only is not None:
+        step_only(args)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build runs on a single MI355X; see DESIGN.md for "
