@@ -311,8 +311,9 @@ def main():
         step_only(args)
         return
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build runs on a single MI355X; see DESIGN.md for "
-              "how the batched engine shards shots over GPUs.", file=sys.stderr)
+        print("bench.py measures one GPU only: the engine batches the shots of one MI355X; across GPUs "
+              "the shots are split, one engine per GPU, and only the raw gradient is all-reduced.",
+              file=sys.stderr)
         sys.exit(2)
     import torch
     import fwi2d_amd
