@@ -98,8 +98,13 @@ int fwi2d_engine_dot_f64(fwi2d_engine* e, const float* a, const float* b, const 
    2 the next shot's windows load during this shot's imaging),
    "image_stream" (1, the default: the imaging kernel of adjoint step it runs on a second
    stream beside step it+1; 0: on the engine stream; results are identical),
-   "autotune" (1: at the next forward run, time a few steps of every tile_x and
-   shots-per-workgroup choice and keep the fastest; forward results are unchanged),
+   "image_sbp" (1, the default for sub = 1: the density term of the imaging condition is
+   summed by parts in time, one snapshot per imaging launch and four rotating adjoint
+   levels; 0: the reference's direct form with three snapshots per launch; the two agree
+   to rounding),
+   "autotune" (1: at the next forward run, time a few steps of every tile_x,
+   shots-per-workgroup and edge_stream choice and keep the fastest; forward results are
+   unchanged),
    "checkpoint_format" (0 FP32; 1 BF16, 2 FP16: checkpoint levels stored in 16 bits with
    a max-abs scale per level and shot, which halves their memory; the gradient then
    carries the error of the recomputation from rounded levels),
@@ -110,7 +115,7 @@ int fwi2d_engine_dot_f64(fwi2d_engine* e, const float* a, const float* b, const 
    HIP events), "graph" (1: capture each forward / adjoint time loop in a hipGraph the
    first time it runs in a configuration and replay it afterwards).  Statistics: "steps", "recompute_steps",
    "image_calls", "segment", "nseg", "tile_x", "shots_per_block", "shots_per_block_edge", "edge_stream",
-   "image_fused", "image_buffers", "image_stream", "ntiles", "interior_tiles", "graph_captures", "graph_replays", "nrec",
+   "image_fused", "image_buffers", "image_stream", "image_sbp", "ntiles", "interior_tiles", "graph_captures", "graph_replays", "nrec",
    "nshots", "fwd_loop_ns", "fwd_loop_steps"; -1 for an unknown key. */
 int fwi2d_engine_set_option(fwi2d_engine* e, const char* key, long long value);
 long long fwi2d_engine_get_stat(fwi2d_engine* e, const char* key);
