@@ -307,6 +307,59 @@ struct fwi2d_op {
     }
 };
 
+// Least-squares FWI objective on the operator: nlls_fwi (optimization.hpp:295-455) and
+// nlls_fwi_eco (optimization.hpp:459-951) at their default data processing (no
+// integration, trace normalisation, envelope, filter, interferometry, double difference
+// or source rescaling).  Both shot loops become two batched phases: one apply_forward over
+// every shot, the residuals, one apply_jacobianT(add=false) over every shot.  That is the
+// sum of nlls_fwi_eco's per-shot apply_jacobianT(add=true) calls (optimization.hpp:890),
+// without the CPU build's re-transform of the running sum (we_op.cpp:1080-1084, 1192-1236).
+// Reductions accumulate in double precision (the reference's run in data_t).
+struct fwi2d_nlls {
+    fwi2d_op* op = nullptr;
+    std::vector<float> d;      // observed data times the weights (optimization.hpp:336, 531)
+    std::vector<float> w;      // residual weights, empty: none
+    std::vector<float> gmask;  // gradient mask, empty: none
+    std::vector<float> r;      // residual of the last evaluation
+    double dnorm = 1.0;        // nlls_fwi_eco's data norm (optimization.hpp:569-574), else 1
+    size_t ndata = 0, nmodel = 0;
+
+    void evaluate(const float* m, double* func, float* g)
+    {
+        const int nt = op->d.nt_data;
+        const double dt = op->d.dt_data;
+        const size_t ntr = ndata / nt;
+        // res (optimization.hpp:346-374, 660-776): r = W L(m) - W d
+        r.assign(ndata, 0.f);
+        op->apply_forward(false, m, r.data());
+        for (size_t i = 0; i < ndata; i++) r[i] = (w.empty() ? r[i] : r[i] * w[i]) - d[i];
+        // getFunc (optimization.hpp:375-385, 929-942): 1/2 dt (|r|^2 - 1/2 sum of the
+        // trace end samples squared) / dnorm
+        if (func) {
+            double f = 0, ends = 0;
+            for (size_t i = 0; i < ndata; i++) f += (double)r[i] * r[i];
+            for (size_t x = 0; x < ntr; x++) {
+                const double a = r[x * nt], b = r[x * nt + nt - 1];
+                ends += a * a + b * b;
+            }
+            *func = 0.5 * dt * (f - 0.5 * ends) / dnorm;
+        }
+        if (!g) return;
+        // grad (optimization.hpp:390-455, 780-905): J^T W Ht r / dnorm, masked
+        std::vector<float> a(ndata);
+        for (size_t x = 0; x < ntr; x++)
+            for (int t = 0; t < nt; t++) {
+                const size_t i = x * nt + t;
+                const double h = (t == 0 || t == nt - 1) ? 0.5 * dt : dt;
+                a[i] = (float)((w.empty() ? 1.0 : (double)w[i]) * (h * r[i]));
+            }
+        std::vector<float> gr(nmodel);
+        op->apply_jacobianT(false, gr.data(), m, a.data());
+        for (size_t i = 0; i < nmodel; i++)
+            g[i] = (float)((gmask.empty() ? 1.0 : (double)gmask[i]) * (gr[i] / dnorm));
+    }
+};
+
 template <class F>
 static int op_guard(F&& f)
 {
@@ -392,5 +445,59 @@ int fwi2d_op_time_axis(fwi2d_op* op, int* nt, double* dt, int* sub)
 }
 
 void fwi2d_op_destroy(fwi2d_op* op) { delete op; }
+
+int fwi2d_nlls_create(fwi2d_op* op, const float* d, const float* w, const float* gmask, int normalize,
+                      fwi2d_nlls** out)
+{
+    return op_guard([&] {
+        if (!op || !d || !out) throw std::runtime_error("null argument");
+        fwi2d_nlls* f = new fwi2d_nlls;
+        f->op = op;
+        f->ndata = (size_t)2 * op->nr * op->d.nt_data;
+        f->nmodel = (size_t)op->np() * op->nxz();
+        f->d.assign(d, d + f->ndata);
+        auto nonneg = [](const float* p, size_t n) {
+            for (size_t i = 0; i < n; i++)
+                if (!(p[i] >= 0)) return false;
+            return true;
+        };
+        if (w) {
+            if (!nonneg(w, f->ndata)) {
+                delete f;
+                throw std::runtime_error("Data weights  must be non-negative");
+            }
+            f->w.assign(w, w + f->ndata);
+            for (size_t i = 0; i < f->ndata; i++) f->d[i] *= f->w[i];
+        }
+        if (gmask) {
+            if (!nonneg(gmask, f->nmodel)) {
+                delete f;
+                throw std::runtime_error("Gradient mask must be non-negative");
+            }
+            f->gmask.assign(gmask, gmask + f->nmodel);
+        }
+        if (normalize) {
+            double s = 0;
+            for (float v : f->d) s += (double)v * v;
+            s *= op->d.dt_data;
+            f->dnorm = s < 1e-16 ? 1.0 : s;
+        }
+        *out = f;
+    });
+}
+
+int fwi2d_nlls_evaluate(fwi2d_nlls* f, const float* m, double* func, float* g)
+{
+    return op_guard([&] {
+        if (!f || !m) throw std::runtime_error("null argument");
+        f->evaluate(m, func, g);
+    });
+}
+
+const float* fwi2d_nlls_residual(fwi2d_nlls* f) { return f && !f->r.empty() ? f->r.data() : nullptr; }
+
+double fwi2d_nlls_dnorm(fwi2d_nlls* f) { return f ? f->dnorm : 0.0; }
+
+void fwi2d_nlls_destroy(fwi2d_nlls* f) { delete f; }
 
 }  // extern "C"

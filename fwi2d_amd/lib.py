@@ -45,6 +45,7 @@ EXPORTS = [
     "fwi2d_engine_forward", "fwi2d_engine_adjoint", "fwi2d_engine_dot_f64", "fwi2d_engine_set_option",
     "fwi2d_engine_get_stat", "fwi2d_engine_tune_report", "fwi2d_engine_destroy", "fwi2d_op_create", "fwi2d_op_apply_forward",
     "fwi2d_op_apply_jacobianT", "fwi2d_op_engine", "fwi2d_op_time_axis", "fwi2d_op_destroy",
+    "fwi2d_nlls_create", "fwi2d_nlls_evaluate", "fwi2d_nlls_residual", "fwi2d_nlls_dnorm", "fwi2d_nlls_destroy",
 ]
 
 
@@ -81,6 +82,13 @@ def load():
     lib.fwi2d_op_engine.restype = P
     lib.fwi2d_op_time_axis.argtypes = [P, C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_int)]
     lib.fwi2d_op_destroy.argtypes = [P]
+    lib.fwi2d_nlls_create.argtypes = [P, P, P, P, C.c_int, C.POINTER(P)]
+    lib.fwi2d_nlls_evaluate.argtypes = [P, P, C.POINTER(C.c_double), P]
+    lib.fwi2d_nlls_residual.argtypes = [P]
+    lib.fwi2d_nlls_residual.restype = C.POINTER(C.c_float)
+    lib.fwi2d_nlls_dnorm.argtypes = [P]
+    lib.fwi2d_nlls_dnorm.restype = C.c_double
+    lib.fwi2d_nlls_destroy.argtypes = [P]
     _lib = lib
     return lib
 
@@ -247,3 +255,41 @@ class WeOp:
 
     def engine_handle(self):
         return load().fwi2d_op_engine(self._h)
+
+
+class Nlls:
+    """nlls_fwi (normalize=False) / nlls_fwi_eco (normalize=True) on a WeOp, default data
+    processing: evaluate(m) returns the functional (FP64) and the gradient."""
+
+    def __init__(self, op, d, w=None, gmask=None, normalize=False):
+        self.op = op  # the objective uses the operator handle
+        self._d = _f32(d)
+        assert self._d.shape == (2, op.nr, op.nt_data), self._d.shape
+        w = None if w is None else _f32(w)
+        gmask = None if gmask is None else _f32(gmask)
+        self._h = C.c_void_p()
+        _check(load().fwi2d_nlls_create(op._h, _ptr(self._d), _ptr(w), _ptr(gmask), int(bool(normalize)),
+                                        C.byref(self._h)))
+
+    def close(self):
+        if self._h:
+            load().fwi2d_nlls_destroy(self._h)
+            self._h = C.c_void_p()
+
+    __del__ = close
+
+    def evaluate(self, model, gradient=True):
+        m = _f32(model)
+        f = C.c_double()
+        g = np.zeros((self.op.np,) + m.shape[1:], np.float32) if gradient else None
+        _check(load().fwi2d_nlls_evaluate(self._h, _ptr(m), C.byref(f), _ptr(g)))
+        return f.value, g
+
+    def residual(self):
+        p = load().fwi2d_nlls_residual(self._h)
+        if not p:
+            return None
+        return np.ctypeslib.as_array(p, shape=self._d.shape).copy()
+
+    def dnorm(self):
+        return load().fwi2d_nlls_dnorm(self._h)

@@ -178,6 +178,40 @@ int fwi2d_op_time_axis(fwi2d_op* op, int* nt, double* dt, int* sub);
 
 void fwi2d_op_destroy(fwi2d_op* op);
 
+/* ------------------------------------------------------------------------- */
+/* Least-squares objective on the operator (nlls_fwi / nlls_fwi_eco)           */
+/* ------------------------------------------------------------------------- */
+
+typedef struct fwi2d_nlls fwi2d_nlls;
+
+/* Constructors of nlls_fwi (src/include/optimization.hpp:309-344) and nlls_fwi_eco
+   (optimization.hpp:482-590) at their default data processing (no integration, trace
+   normalisation, envelope, filter, interferometry, double difference or source
+   rescaling).  d: observed data [2][nrec_total][nt_data]; w: residual weights of the
+   same shape, or NULL; gmask: gradient mask [nparam][nx][nz], or NULL; normalize: 0 for
+   nlls_fwi, 1 for nlls_fwi_eco with par.normalize_obj_func (dnorm = dt_data * |W d|^2,
+   1 below 1e-16).  Negative weights or mask values are rejected as in the reference.
+   The objective keeps a copy of d, w and gmask, and uses op until it is destroyed. */
+int fwi2d_nlls_create(fwi2d_op* op, const float* d, const float* w, const float* gmask, int normalize,
+                      fwi2d_nlls** out);
+
+/* res + getFunc + grad at the physical model m (nlls_fwi: optimization.hpp:346-455;
+   nlls_fwi_eco: compute_res_and_grad, getFunc, grad, optimization.hpp:592-951), over all
+   shots in two batched phases (forward, then adjoint):
+     r = W F(m) - W d,
+     func = 1/2 dt (|r|^2 - 1/2 sum over traces of r[0]^2 + r[nt-1]^2) / dnorm  (FP64),
+     g = gmask * J(m)^T (W Ht r) / dnorm     [nparam][nx][nz].
+   func or g may be NULL (g NULL: residual and functional only, no adjoint). */
+int fwi2d_nlls_evaluate(fwi2d_nlls* f, const float* m, double* func, float* g);
+
+/* Residual r of the last evaluation ([2][nrec_total][nt_data], owned by f), or NULL. */
+const float* fwi2d_nlls_residual(fwi2d_nlls* f);
+
+/* The normalisation dnorm of the functional and gradient (1 for nlls_fwi). */
+double fwi2d_nlls_dnorm(fwi2d_nlls* f);
+
+void fwi2d_nlls_destroy(fwi2d_nlls* f);
+
 #ifdef __cplusplus
 }
 #endif

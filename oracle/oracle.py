@@ -272,6 +272,44 @@ class Oracle:
         assert rc == 0, rc
         return grad
 
+    # ---- least-squares objective -----------------------------------------
+    @staticmethod
+    def nlls_functional(r, dt, dnorm=1.0):
+        """getFunc of nlls_fwi / nlls_fwi_eco (optimization.hpp:375-385, 929-942):
+        1/2 dt (sum r^2 - 1/2 sum over traces of the first and last samples squared) / dnorm,
+        i.e. the trapezoid rule in time; accumulated in double precision."""
+        tr = np.asarray(r, np.float64).reshape(-1, np.shape(r)[-1])
+        ends = np.sum(tr[:, 0] ** 2 + tr[:, -1] ** 2)
+        return 0.5 * dt * (np.sum(tr * tr) - 0.5 * ends) / dnorm
+
+    def nlls_evaluate(self, prob, model, allsrc, d, w=None, gmask=None, normalize=False, gradient=True):
+        """nlls_fwi (optimization.hpp:309-455) or, with normalize, nlls_fwi_eco with
+        normalize_obj_func (optimization.hpp:482-951), default data processing:
+        r = W F(m) - W d; f = nlls_functional(r, dt, dnorm); g = gmask J^T(W Ht r) / dnorm,
+        with dnorm = dt |W d|^2 (1 below 1e-16) for nlls_fwi_eco and 1 for nlls_fwi.  The
+        shot sum is one multi-shot apply_jacobianT(add=false), i.e. the per-shot calls of
+        nlls_fwi_eco (optimization.hpp:890) summed.  Returns (f, g, r)."""
+        dt = float(prob["mp"]["dt_data"])
+        wd = np.asarray(d, np.float64) if w is None else np.asarray(d, np.float64) * w
+        dnorm = 1.0
+        if normalize:
+            dnorm = float(np.sum(wd * wd)) * dt
+            if dnorm < 1e-16:
+                dnorm = 1.0
+        u, full = self.apply_forward(self.make_op(prob), model, allsrc, keep_full=gradient)
+        u = np.asarray(u, np.float64)
+        r = (u if w is None else u * w) - wd
+        f = self.nlls_functional(r, dt, dnorm)
+        if not gradient:
+            return f, None, r
+        a = self.Ht(False, np.array(r, self.dtype), dt)  # applyHt, then the weights
+        if w is not None:  # (optimization.hpp:399, 435, 780, 879-893)
+            a = a * w
+        g = self.apply_jacobianT(self.make_op(prob), model, a, full) / dnorm
+        if gmask is not None:
+            g = g * gmask
+        return f, g, r
+
 
 class RefSBP:
     """The reference's own SBP templates and spatial_operators.cpp, compiled from source

@@ -377,6 +377,46 @@ def test_operator_level_forward_and_gradient(lib, orc64):
     assert rel(g2, 2 * gk.astype(np.float64)) < 1e-6
 
 
+@pytest.mark.parametrize("normalize", [False, True])
+def test_objective_matches_oracle(lib, orc64, normalize):
+    """nlls_fwi (normalize False) and nlls_fwi_eco (True) on the batched operator, with
+    residual weights and a gradient mask: functional, residual and gradient against the
+    oracle's objective on the same model, each <= 1e-4 (the residual relative to the
+    weighted data, as the operator-level seismograms)."""
+    nx, nz, h = 110, 70, 0.01
+    nt_data, dt_data = 251, 0.002
+    true = W.add_ellipses(W.gradient_model(nx, nz, h, h, seed=3), h, h, seed=4)
+    start = W.smooth_model(true, h, h, 0.1)
+    rx = np.stack([np.arange(2, nx - 2, 3) * h, np.full(len(range(2, nx - 2, 3)), 0.02)], 1)
+    prob = W.base_problem(nx, nz, h, h, nt_data, dt_data, [[0.3, 0.02], [0.8, 0.05], [0.55, 0.3]],
+                          [rx, rx[::2], rx[5:20]], taper=15, dt=0.001, sub=-1)
+    src = W.vertical_force(W.ricker(nt_data, dt_data, 12.0), 3)
+    dobs, _ = orc64.apply_forward(orc64.make_op(prob), true, src)
+    rng = np.random.default_rng(11)
+    w = rng.uniform(0.5, 1.5, dobs.shape).astype(np.float32)
+    gmask = np.clip(1.5 * W._smooth_field((nx, nz), 4, rng) + 0.5, 0, 1)[None].repeat(3, 0).astype(np.float32)
+    fref, gref, rref = orc64.nlls_evaluate(prob, start, src, dobs.astype(np.float32), w=w, gmask=gmask,
+                                           normalize=normalize)
+    op = lib.WeOp(prob, src)
+    obj = lib.Nlls(op, dobs, w=w, gmask=gmask, normalize=normalize)
+    f, g = obj.evaluate(start)
+    err_f = abs(f / fref - 1)
+    wd = w.astype(np.float64) * dobs
+    err_r = np.linalg.norm(obj.residual() - rref) / np.linalg.norm(wd)
+    print("normalize=%s functional %.9e (oracle %.9e) rel err %.3e, residual err %.3e, |r|/|Wd| %.3f"
+          % (normalize, f, fref, err_f, err_r, np.linalg.norm(rref) / np.linalg.norm(wd)))
+    assert err_f <= 1e-4 and err_r <= 1e-4
+    for c in range(3):
+        err = rel(g[c], gref[c])
+        print("objective gradient component %d rel err %.3e" % (c, err))
+        assert err <= 1e-4
+    # functional only: no adjoint, same value
+    f2, g2 = obj.evaluate(start, gradient=False)
+    assert g2 is None and f2 == f
+    obj.close()
+    op.close()
+
+
 def test_config0_full_size_forward(lib, orc64, orc32):
     """The reference's CPU-runnable case at full size: 1 shot on 201 x 601, nt = 2000,
     15 Hz Ricker at 20 m depth, 150 surface receivers, free surface and 30-cell tapers,

@@ -1,0 +1,64 @@
+"""Host logic of the least-squares objective (fwi2d_nlls_*), without a GPU: the
+constructor's checks and data norm (nlls_fwi / nlls_fwi_eco constructors,
+src/include/optimization.hpp:309-344, 482-590).  Creating the operator and the objective
+touches no device; evaluation needs the GPU and is covered by the GPU parity tests."""
+import os
+
+import numpy as np
+import pytest
+
+from fwi2d_amd import workloads as W
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "fwi2d_amd", "libfwi2d_hip.so")
+pytestmark = pytest.mark.skipif(not os.path.exists(LIB), reason="library not built")
+
+
+@pytest.fixture(scope="module")
+def op():
+    from fwi2d_amd import lib
+    nx, nz, h = 40, 30, 0.01
+    rx = np.stack([np.arange(2, nx - 2, 4) * h, np.full(len(range(2, nx - 2, 4)), 0.02)], 1)
+    prob = W.base_problem(nx, nz, h, h, 101, 0.002, [[0.1, 0.02], [0.3, 0.05]], [rx, rx[::2]], taper=5,
+                          dt=0.001, sub=-1)
+    src = W.vertical_force(W.ricker(101, 0.002, 12.0), 2)
+    o = lib.WeOp(prob, src)
+    yield o
+    o.close()
+
+
+def test_data_norm(op):
+    from fwi2d_amd import lib
+    rng = np.random.default_rng(0)
+    d = rng.standard_normal((2, op.nr, op.nt_data)).astype(np.float32)
+    w = rng.uniform(0.5, 2.0, d.shape).astype(np.float32)
+    wd = d.astype(np.float64) * w.astype(np.float32)
+    assert lib.Nlls(op, d).dnorm() == 1.0
+    assert lib.Nlls(op, d, normalize=True).dnorm() == pytest.approx(0.002 * np.sum(d.astype(np.float64) ** 2),
+                                                                    rel=1e-12)
+    # the weights multiply the data in single precision before the norm
+    wd32 = (d * w).astype(np.float64)
+    assert lib.Nlls(op, d, w=w, normalize=True).dnorm() == pytest.approx(0.002 * np.sum(wd32 ** 2), rel=1e-12)
+    assert abs(np.sum(wd32 ** 2) / np.sum(wd ** 2) - 1) < 1e-6
+    # all-zero data: dnorm falls back to 1
+    assert lib.Nlls(op, np.zeros_like(d), normalize=True).dnorm() == 1.0
+    assert lib.Nlls(op, d).residual() is None
+
+
+def test_argument_checks(op):
+    from fwi2d_amd import lib
+    d = np.zeros((2, op.nr, op.nt_data), np.float32)
+    w = np.ones_like(d)
+    w[1, 3, 7] = -1e-3
+    with pytest.raises(lib.Fwi2dError, match="Data weights  must be non-negative"):
+        lib.Nlls(op, d, w=w)
+    mask = np.ones((3, 40, 30), np.float32)
+    mask[2, 5, 5] = -1.0
+    with pytest.raises(lib.Fwi2dError, match="Gradient mask must be non-negative"):
+        lib.Nlls(op, d, gmask=mask)
+    mask[2, 5, 5] = np.nan
+    with pytest.raises(lib.Fwi2dError, match="Gradient mask must be non-negative"):
+        lib.Nlls(op, d, gmask=mask)
+    L = lib.load()
+    assert L.fwi2d_nlls_create(None, None, None, None, 0, None) != 0
+    assert b"null argument" in L.fwi2d_last_error()
