@@ -16,6 +16,7 @@
 #include "fwi2d_hip.h"
 #include "kernels.h"
 #include "sbp_tables.h"
+#include "trace_matrix.h"
 
 namespace fwi2d {
 
@@ -166,6 +167,7 @@ struct Engine {
     // state and checkpoints
     DevMem d_fwd, d_adj, d_ckpt, d_seg, d_gshot, d_gout, d_src, d_rcv, d_adjsrc, d_partial, d_scalar;
     DevMem d_ckpt16, d_ckpt_scale, d_amax;  // 16-bit checkpoint levels, their per-shot scales
+    DevMem d_tm_ptr, d_tm_col, d_tm_w, d_tdata;  // operator-level trace matrix and data traces
     int opt_bf16 = 0;                       // checkpoint levels: 0 FP32, 1 BF16, 2 FP16 (max-abs scaled)
     int opt_stiff = 0;                      // VTI raw gradients w.r.t. c11, c13, c33, c55, rho
     int seg_K = 0, nseg = 0, nsnap = 0, seg_loaded = -1;
@@ -776,6 +778,54 @@ struct Engine {
             launch_copy(u, d_ckpt.as<float>() + k * ss, ss, st);
     }
 
+    // the trace matrix m on the device, its shape and indices checked against the traces
+    void upload_matrix(const TraceMatrix& m, int ncol_expected, int nrow_expected)
+    {
+        if ((ncol_expected >= 0 && m.ncol != ncol_expected) || (nrow_expected >= 0 && m.nrow != nrow_expected) ||
+            m.nrow < 0 || m.ncol < 1 || (int)m.ptr.size() != m.nrow + 1 || m.ptr[0] != 0 ||
+            (size_t)m.ptr[m.nrow] != m.col.size() || m.col.size() != m.w.size())
+            throw std::runtime_error("trace matrix does not match the traces");
+        for (int i = 0; i < m.nrow; i++)
+            if (m.ptr[i + 1] < m.ptr[i]) throw std::runtime_error("trace matrix rows out of order");
+        for (int c : m.col)
+            if (c < 0 || c >= m.ncol) throw std::runtime_error("trace matrix column out of range");
+        auto up = [&](DevMem& d, const void* h, size_t b) {
+            d.reserve(std::max<size_t>(b, 16));
+            if (b) HIPCHECK(hipMemcpyAsync(d.p, h, b, hipMemcpyHostToDevice, st));
+        };
+        up(d_tm_ptr, m.ptr.data(), m.ptr.size() * sizeof(int));
+        up(d_tm_col, m.col.data(), m.col.size() * sizeof(int));
+        up(d_tm_w, m.w.data(), m.w.size() * sizeof(double));
+    }
+
+    void forward_traces(const float* src, bool keep, const TraceMatrix& m, float* out)
+    {
+        require_ready();
+        const long long ntr = 2LL * nr;
+        upload_matrix(m, g.nt, -1);
+        d_rcv.reserve((size_t)std::max<long long>(ntr * g.nt, 1) * sizeof(float));
+        forward(src, d_rcv.as<float>(), keep, false, true);
+        d_tdata.reserve((size_t)std::max<long long>(ntr * m.nrow, 1) * sizeof(float));
+        launch_trace_matvec(d_tm_ptr.as<int>(), d_tm_col.as<int>(), d_tm_w.as<double>(), m.nrow, m.ncol,
+                            d_rcv.as<float>(), d_tdata.as<float>(), ntr, st);
+        if (ntr * m.nrow > 0)
+            HIPCHECK(hipMemcpyAsync(out, d_tdata.p, (size_t)(ntr * m.nrow) * sizeof(float), hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+        HIPCHECK(hipGetLastError());
+    }
+
+    void adjoint_traces(const float* data, const TraceMatrix& m, float* grad)
+    {
+        require_ready();
+        const long long ntr = 2LL * nr;
+        upload_matrix(m, -1, g.nt);
+        const float* ddata = upload_traces(d_tdata, data, std::max<long long>(ntr * m.ncol, 1));
+        d_adjsrc.reserve((size_t)std::max<long long>(ntr * g.nt, 1) * sizeof(float));
+        launch_trace_matvec(d_tm_ptr.as<int>(), d_tm_col.as<int>(), d_tm_w.as<double>(), m.nrow, m.ncol, ddata,
+                            d_adjsrc.as<float>(), ntr, st);
+        adjoint(d_adjsrc.as<float>(), grad, true, false);
+    }
+
     const float* upload_traces(DevMem& d, const float* h, long long n)
     {
         d.reserve((size_t)n * sizeof(float));
@@ -910,16 +960,16 @@ struct Engine {
         tuned = true;
     }
 
-    void forward(const float* src, float* rcv, bool keep, bool on_device)
+    void forward(const float* src, float* rcv, bool keep, bool src_dev, bool rcv_dev)
     {
         require_ready();
         if (keep && g.sub <= 0) throw std::runtime_error("checkpoints need sub > 0");
         ckpt_epoch = -1;  // any forward run replaces the recomputation sources
         const long long ntr_s = (long long)2 * ns * g.nt, ntr_r = (long long)2 * nr * g.nt;
-        const float* dsrc = on_device ? src : upload_traces(d_src, src, ntr_s);
+        const float* dsrc = src_dev ? src : upload_traces(d_src, src, ntr_s);
         if (opt_autotune && !tuned) autotune(dsrc);
         float* drcv = rcv;
-        if (!on_device) {
+        if (!rcv_dev) {
             d_rcv.reserve((size_t)std::max<long long>(ntr_r, 1) * sizeof(float));
             drcv = d_rcv.as<float>();
         }
@@ -970,7 +1020,7 @@ struct Engine {
         if (nr > 0)
             launch_extract(L.C(), 2 * nxz, nxz, ldz, d_rcv_ext_ents.as<PointEntry>(), nr, drcv, (long long)nr * nt,
                            nt - 1, st);
-        if (!on_device && nr > 0)
+        if (!rcv_dev && nr > 0)
             HIPCHECK(hipMemcpyAsync(rcv, drcv, (size_t)ntr_r * sizeof(float), hipMemcpyDeviceToHost, st));
         HIPCHECK(hipStreamSynchronize(st));
         HIPCHECK(hipGetLastError());
@@ -1182,7 +1232,7 @@ struct Engine {
         stat_image_calls++;
     }
 
-    void adjoint(const float* adj, float* grad, bool on_device)
+    void adjoint(const float* adj, float* grad, bool adj_dev, bool grad_dev)
     {
         require_ready();
         if (g.sub <= 0) throw std::runtime_error("the adjoint needs sub > 0");
@@ -1190,7 +1240,7 @@ struct Engine {
             throw std::runtime_error("no checkpoints of a forward run on the current model: run fwi2d_engine_forward "
                                      "with keep_checkpoints first");
         const long long ntr_r = (long long)2 * nr * g.nt;
-        const float* dadj = on_device ? adj : upload_traces(d_adjsrc, adj, std::max<long long>(ntr_r, 1));
+        const float* dadj = adj_dev ? adj : upload_traces(d_adjsrc, adj, std::max<long long>(ntr_r, 1));
         const float* dsrc = fwd_src;
         // with the reference's lag the imaging is fused into the adjoint steps (each
         // workgroup adds its shots' sum to the slab of its first shot); without it (a test
@@ -1263,7 +1313,7 @@ struct Engine {
         // [np][nx][ldz] -> [np][nx][nz]
         HIPCHECK(hipMemcpy2DAsync(grad, (size_t)g.nz * sizeof(float), out, (size_t)ldz * sizeof(float),
                                   (size_t)g.nz * sizeof(float), (size_t)np * g.nx,
-                                  on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, st));
+                                  grad_dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, st));
         HIPCHECK(hipStreamSynchronize(st));
         HIPCHECK(hipGetLastError());
     }
@@ -1343,7 +1393,7 @@ int fwi2d_engine_forward(fwi2d_engine* e, const float* src, float* rcv, int keep
     return guarded([&] {
         if (!e || !src) throw std::runtime_error("null argument");
         HIPCHECK(hipSetDevice(e->impl->device));
-        e->impl->forward(src, rcv, keep_checkpoints != 0, on_device != 0);
+        e->impl->forward(src, rcv, keep_checkpoints != 0, on_device != 0, on_device != 0);
     });
 }
 
@@ -1352,7 +1402,7 @@ int fwi2d_engine_adjoint(fwi2d_engine* e, const float* adj, float* grad, int on_
     return guarded([&] {
         if (!e || !adj || !grad) throw std::runtime_error("null argument");
         HIPCHECK(hipSetDevice(e->impl->device));
-        e->impl->adjoint(adj, grad, on_device != 0);
+        e->impl->adjoint(adj, grad, on_device != 0, on_device != 0);
     });
 }
 
@@ -1465,3 +1515,21 @@ void fwi2d_engine_destroy(fwi2d_engine* e)
 }
 
 }  // extern "C"
+
+namespace fwi2d {
+
+void engine_forward_traces(fwi2d_engine* e, const float* src, bool keep, const TraceMatrix& m, float* out)
+{
+    if (!e || !src || !out) throw std::runtime_error("null argument");
+    HIPCHECK(hipSetDevice(e->impl->device));
+    e->impl->forward_traces(src, keep, m, out);
+}
+
+void engine_adjoint_traces(fwi2d_engine* e, const float* data, const TraceMatrix& m, float* grad)
+{
+    if (!e || !data || !grad) throw std::runtime_error("null argument");
+    HIPCHECK(hipSetDevice(e->impl->device));
+    e->impl->adjoint_traces(data, m, grad);
+}
+
+}  // namespace fwi2d

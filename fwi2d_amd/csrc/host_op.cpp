@@ -3,8 +3,11 @@
 // apply_forward (src/cpp/we_op.cpp:904-1055) and apply_jacobianT
 // (src/cpp/we_op.cpp:1057-1239) around the batched device engine.
 //
-// Trace and gradient post-processing runs on the host in double precision; the
-// time stepping runs on the device in single precision.
+// The per-trace time processing (resampling, Dt, Ht^-1, reversal, dt ratio) is composed
+// here in double precision into one sparse matrix per direction, which the device applies
+// to all traces with double accumulation; gradient post-processing (H quadrature, chain
+// rule) runs here in double precision; the time stepping runs on the device in single
+// precision.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -13,6 +16,7 @@
 #include <vector>
 
 #include "fwi2d_hip.h"
+#include "trace_matrix.h"
 
 namespace fwi2d {
 extern thread_local std::string g_last_error;
@@ -130,23 +134,96 @@ void resample(bool adjoint, bool linear, int hl, int ntr, int ntm, double dtm, i
     }
 }
 
-// Dt (misc.cpp:71-95): forward centred derivative with one-sided ends, or the
-// reference's reverse-time adjoint form
-void time_derivative(bool adj, int ntr, int nt, double dt, const double* in, double* out)
+// Per-trace linear maps in time as sparse rows (row i: (column, weight) pairs), composed
+// on the host in double precision into one TraceMatrix that the device applies.
+using Rows = std::vector<std::vector<std::pair<int, double>>>;
+
+// the resampler as a matrix [ntd][ntm]: dat = S mod (resample above, forward direction)
+Rows resample_rows(bool linear, int hl, int ntm, double dtm, int ntd, double dtd)
 {
-    for (int x = 0; x < ntr; x++) {
-        const double* a = in + (size_t)x * nt;
-        double* o = out + (size_t)x * nt;
-        if (!adj) {
-            o[0] = (a[1] - a[0]) / dt;
-            for (int t = 1; t < nt - 1; t++) o[t] = (a[t + 1] - a[t - 1]) / (2 * dt);
-            o[nt - 1] = (a[nt - 1] - a[nt - 2]) / dt;
-        } else {
-            o[0] = (a[1] + a[0]) / dt;
-            for (int t = 1; t < nt - 1; t++) o[t] = (a[t + 1] - a[t - 1]) / (2 * dt);
-            o[nt - 1] = (-a[nt - 1] - a[nt - 2]) / dt;
+    Rows r(ntd);
+    for (int itd = 0; itd < ntd; itd++) {
+        const double t = itd * dtd;
+        if (linear) {
+            const int i = std::min(ntm - 1, (int)std::floor(t / dtm));
+            const int j = std::min(ntm - 1, (int)std::ceil(t / dtm));
+            const double wj = (t - i * dtm) / dtm, wi = 1 - wj;
+            if (i == j) r[itd] = {{i, wi + wj}};
+            else r[itd] = {{i, wi}, {j, wj}};
+            continue;
+        }
+        const int itm0 = (int)std::floor(t / dtm);
+        const int lo = std::max(0, itm0 - hl + 1), hi = std::min(ntm - 1, itm0 + hl);
+        for (int itm = lo; itm <= hi; itm++) {
+            const double arg = kPi * (t - itm * dtm) / dtm;
+            const double sc = arg == 0.0 ? 1.0 : std::sin(arg) / arg;
+            r[itd].push_back({itm, sc * (0.5 + 0.5 * std::cos(kPi * (itm - t / dtm) / hl))});
         }
     }
+    return r;
+}
+
+// Dt (misc.cpp:71-95) as a matrix [nt][nt]: the centred derivative with one-sided ends,
+// or the reference's reverse-time adjoint form
+Rows dt_rows(bool adj, int nt, double dt)
+{
+    Rows r(nt);
+    const double a = 1 / dt, b = 1 / (2 * dt);
+    r[0] = adj ? Rows::value_type{{0, a}, {1, a}} : Rows::value_type{{0, -a}, {1, a}};
+    for (int t = 1; t < nt - 1; t++) r[t] = {{t - 1, -b}, {t + 1, b}};
+    r[nt - 1] = adj ? Rows::value_type{{nt - 2, -a}, {nt - 1, -a}} : Rows::value_type{{nt - 2, -a}, {nt - 1, a}};
+    return r;
+}
+
+Rows transpose(const Rows& a, int ncol)
+{
+    Rows t(ncol);
+    for (int i = 0; i < (int)a.size(); i++)
+        for (const auto& e : a[i]) t[e.first].push_back({i, e.second});
+    return t;
+}
+
+// a b, with b of bcols columns
+Rows multiply(const Rows& a, const Rows& b, int bcols)
+{
+    Rows c(a.size());
+    std::vector<double> acc(bcols, 0.0);
+    std::vector<char> used(bcols, 0);
+    std::vector<int> cols;
+    for (size_t i = 0; i < a.size(); i++) {
+        cols.clear();
+        for (const auto& e : a[i])
+            for (const auto& f : b[e.first]) {
+                if (!used[f.first]) {
+                    used[f.first] = 1;
+                    cols.push_back(f.first);
+                }
+                acc[f.first] += e.second * f.second;
+            }
+        std::sort(cols.begin(), cols.end());
+        for (int j : cols) {
+            c[i].push_back({j, acc[j]});
+            acc[j] = 0.0;
+            used[j] = 0;
+        }
+    }
+    return c;
+}
+
+fwi2d::TraceMatrix to_matrix(const Rows& a, int ncol)
+{
+    fwi2d::TraceMatrix m;
+    m.nrow = (int)a.size();
+    m.ncol = ncol;
+    m.ptr.assign(1, 0);
+    for (const auto& row : a) {
+        for (const auto& e : row) {
+            m.col.push_back(e.first);
+            m.w.push_back(e.second);
+        }
+        m.ptr.push_back((int)m.col.size());
+    }
+    return m;
 }
 
 double h_weight(int n, int i)
@@ -170,6 +247,8 @@ struct fwi2d_op {
     TimeAxis axis{0, 0, 0};
     std::vector<float> ckpt_model;  // converted model of the last checkpointed forward
     std::vector<float> src_prop;    // source traces at the propagation step
+    fwi2d::TraceMatrix tm_fwd;      // receiver traces -> data [nt_data][nt]
+    fwi2d::TraceMatrix tm_adj;      // data -> adjoint sources [nt][nt_data]
     bool shots_set = false;
 
     int np() const { return d.nparam; }
@@ -224,6 +303,32 @@ struct fwi2d_op {
         resample(false, d.resampling == 1, d.sinc_half_length, 2 * d.ns, d.nt_data, d.dt_data, t.nt, t.dt,
                  allsrc.data(), sp.data());
         src_prop.assign(sp.begin(), sp.end());
+        build_trace_maps();
+    }
+
+    // The trace maps, applied on the device to every trace: apply_forward's (we_op.cpp:
+    // 1000-1030) alpha Dt (particle velocity) then the resampler's adjoint onto the data
+    // axis, and the adjoint sources of apply_jacobianT (we_op.cpp:1112-1127): resampling
+    // onto the propagation axis, Ht^-1, time reversal, -alpha, then Dt's adjoint form.
+    void build_trace_maps()
+    {
+        const int nt = axis.nt, ntd = d.nt_data;
+        const double dt = axis.dt, alpha = dt / d.dt_data;
+        const Rows S = resample_rows(d.resampling == 1, d.sinc_half_length, ntd, d.dt_data, nt, dt);  // [nt][ntd]
+        Rows F = transpose(S, ntd);
+        if (d.seismotype == 1) F = multiply(F, dt_rows(false, nt, dt), nt);
+        for (auto& row : F)
+            for (auto& e : row) e.second *= alpha;
+        tm_fwd = to_matrix(F, nt);
+        Rows A(nt);
+        for (int t = 0; t < nt; t++) {
+            const int k = nt - 1 - t;  // sample k of the resampled trace lands at t
+            const double h = (k == 0 || k == nt - 1) ? 0.5 * dt : dt;
+            A[t] = S[k];
+            for (auto& e : A[t]) e.second *= -alpha / h;
+        }
+        if (d.seismotype == 1) A = multiply(dt_rows(true, nt, dt), A, ntd);
+        tm_adj = to_matrix(A, ntd);
     }
 
     void run_forward(const std::vector<float>& m, std::vector<float>& rcv)
@@ -239,51 +344,34 @@ struct fwi2d_op {
     void apply_forward(bool add, const float* pmod, float* pdat)
     {
         std::vector<float> m = prepare(pmod);
-        std::vector<float> rcv;
-        run_forward(m, rcv);
-        const int nt = axis.nt, ntr = 2 * nr;
-        std::vector<double> r(rcv.begin(), rcv.end());
-        if (d.seismotype == 1) {
-            std::vector<double> v(r.size());
-            time_derivative(false, ntr, nt, axis.dt, r.data(), v.data());
-            r.swap(v);
+        if (nr == 0) {  // no traces: the run only leaves checkpoints
+            std::vector<float> rcv;
+            run_forward(m, rcv);
+            return;
         }
-        const double alpha = axis.dt / d.dt_data;
-        for (double& x : r) x *= alpha;
-        std::vector<double> out((size_t)ntr * d.nt_data);
-        resample(true, d.resampling == 1, d.sinc_half_length, ntr, d.nt_data, d.dt_data, nt, axis.dt, out.data(), r.data());
-        for (size_t i = 0; i < out.size(); i++) pdat[i] = (float)((add ? (double)pdat[i] : 0.0) + out[i]);
+        check(fwi2d_engine_set_model(eng, m.data()));
+        ckpt_model.clear();
+        const size_t n = (size_t)2 * nr * d.nt_data;
+        std::vector<float> out(n);
+        const bool keep = axis.sub > 0;
+        fwi2d::engine_forward_traces(eng, src_prop.data(), keep, tm_fwd, out.data());
+        if (keep) ckpt_model = m;
+        for (size_t i = 0; i < n; i++) pdat[i] = add ? pdat[i] + out[i] : out[i];
     }
 
     void apply_jacobianT(bool add, float* pmod, const float* pmod0, const float* pdat)
     {
         std::vector<float> m = prepare(pmod0);
         if (axis.sub <= 0) throw std::runtime_error("apply_jacobianT needs sub != 0");
-        if (ckpt_model != m) {  // no checkpoints of this model: run the forward first
-            std::vector<float> rcv;
-            run_forward(m, rcv);
-        }
-        const int nt = axis.nt, ntr = 2 * nr;
-        const double dt = axis.dt, alpha = dt / d.dt_data;
-        // adjoint sources (we_op.cpp:1112-1127): resample, Ht^-1, reverse, -alpha, Dt^T
-        std::vector<double> data(pdat, pdat + (size_t)ntr * d.nt_data);
-        std::vector<double> a((size_t)ntr * nt);
-        resample(false, d.resampling == 1, d.sinc_half_length, ntr, d.nt_data, d.dt_data, nt, dt, data.data(), a.data());
-        for (int x = 0; x < ntr; x++) {
-            double* p = a.data() + (size_t)x * nt;
-            for (int t = 0; t < nt; t++) p[t] /= (t == 0 || t == nt - 1) ? 0.5 * dt : dt;
-            std::reverse(p, p + nt);
-            for (int t = 0; t < nt; t++) p[t] *= -alpha;
-        }
-        if (d.seismotype == 1) {
-            std::vector<double> v(a.size());
-            time_derivative(true, ntr, nt, dt, a.data(), v.data());
-            a.swap(v);
-        }
-        std::vector<float> af(a.begin(), a.end());
         const long long n = nxz();
-        std::vector<float> graw((size_t)np() * n);
-        check(fwi2d_engine_adjoint(eng, af.data(), graw.data(), 0));
+        std::vector<float> graw((size_t)np() * n, 0.f);
+        if (nr > 0) {  // no traces: no adjoint sources, a zero gradient
+            if (ckpt_model != m) {  // no checkpoints of this model: run the forward first
+                std::vector<float> rcv;
+                run_forward(m, rcv);
+            }
+            fwi2d::engine_adjoint_traces(eng, pdat, tm_adj, graw.data());
+        }
         // H quadrature (we_op.cpp:1192-1199) and chain rule (we_op.cpp:1208-1219)
         std::vector<double> gd(graw.begin(), graw.end());
         for (int c = 0; c < 3; c++)

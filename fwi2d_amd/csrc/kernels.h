@@ -131,6 +131,9 @@ void launch_img(const StepArgs& a, bool vti, int tile_x, int nbuf, bool sbp, hip
 void launch_sum_shots(const float* per_shot, long long shot_stride, int nshots, long long n, float* out,
                       int accumulate, hipStream_t s);
 void launch_fill(float* p, long long n, float v, hipStream_t s);
+// out [ntr][nrow] = per-trace row-compressed matrix (ptr [nrow + 1], col, w) times in [ntr][ncol]
+void launch_trace_matvec(const int* ptr, const int* col, const double* w, int nrow, int ncol, const float* in,
+                         float* out, long long ntr, hipStream_t s);
 void launch_copy(float* dst, const float* src, long long n, hipStream_t s);
 // per-shot max-abs scaled 16-bit packing of a time level, FP16 (f16) or BF16 (amax:
 // nshots words of scratch)

@@ -1580,6 +1580,24 @@ __global__ void k_fill(float* p, long long n, float v)
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) p[i] = v;
 }
 
+// The per-trace time map of the operator level (resampling, time derivative, quadrature
+// weights, reversal, composed on the host): out[x][i] = sum_k w[k] in[x][col[k]] over the
+// row-compressed entries k of row i, accumulated in double precision; one output sample
+// per thread, consecutive threads on consecutive samples of one trace.
+__global__ void k_trace_matvec(const int* __restrict__ ptr, const int* __restrict__ col, const double* __restrict__ w,
+                               int nrow, int ncol, const float* __restrict__ in, float* __restrict__ out, long long n)
+{
+    for (long long gid = blockIdx.x * (long long)blockDim.x + threadIdx.x; gid < n;
+         gid += (long long)gridDim.x * blockDim.x) {
+        const long long x = gid / nrow;
+        const int i = (int)(gid - x * nrow);
+        const float* a = in + x * ncol;
+        double acc = 0.0;
+        for (int k = ptr[i]; k < ptr[i + 1]; k++) acc += w[k] * (double)a[col[k]];
+        out[gid] = (float)acc;
+    }
+}
+
 // 16-bit checkpoint levels (BF16 or FP16) with a max-abs scale per (level, shot):
 // x = round16(u / s), u = s x.
 // Scales are kept as the bit patterns of non-negative floats, whose integer order is the
@@ -1791,6 +1809,14 @@ void launch_sum_shots(const float* per_shot, long long shot_stride, int nshots, 
 void launch_fill(float* p, long long n, float v, hipStream_t s)
 {
     hipLaunchKernelGGL(k_fill, dim3(grid_for(n)), dim3(kThreads), 0, s, p, n, v);
+}
+
+void launch_trace_matvec(const int* ptr, const int* col, const double* w, int nrow, int ncol, const float* in,
+                         float* out, long long ntr, hipStream_t s)
+{
+    const long long n = ntr * nrow;
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_trace_matvec, dim3(grid_for(n)), dim3(kThreads), 0, s, ptr, col, w, nrow, ncol, in, out, n);
 }
 
 void launch_pack16(bool f16, const float* u, unsigned short* q, float* scale, unsigned* amax, long long per_shot,

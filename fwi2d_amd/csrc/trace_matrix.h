@@ -1,0 +1,31 @@
+// Trace processing of the operator level on the device.
+//
+// Every trace goes through the same linear map in time: the data <-> propagation
+// resampling (sinc_resampler / linear_resampler, operator.cpp:760-856), the time
+// derivative of particle-velocity data (Dt, misc.cpp:71-95), the dt ratio, and for the
+// adjoint sources the inverse trapezoid weights (applyHt, misc.cpp:45-69) and the time
+// reversal (we_op.cpp:1112-1127).  The host composes that map into one row-compressed
+// matrix in double precision; the device applies it to all traces at once.
+#pragma once
+#include <vector>
+
+#include "fwi2d_hip.h"
+
+namespace fwi2d {
+
+// out[x][i] = sum over k in [ptr[i], ptr[i+1]) of w[k] * in[x][col[k]], for every trace x
+struct TraceMatrix {
+    int nrow = 0, ncol = 0;
+    std::vector<int> ptr, col;
+    std::vector<double> w;
+};
+
+// Forward run of every shot (sources at the propagation step, host memory) whose receiver
+// traces [2 nr][nt] are mapped by m (ncol = nt) on the device into out [2 nr][m.nrow] (host).
+void engine_forward_traces(fwi2d_engine* e, const float* src, bool keep, const TraceMatrix& m, float* out);
+
+// Adjoint run whose adjoint sources are m applied on the device to data [2 nr][m.ncol]
+// (host, m.nrow = nt); the raw gradient [np][nx][nz] goes to grad (host).
+void engine_adjoint_traces(fwi2d_engine* e, const float* data, const TraceMatrix& m, float* grad);
+
+}  // namespace fwi2d

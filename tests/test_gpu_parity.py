@@ -377,6 +377,34 @@ def test_operator_level_forward_and_gradient(lib, orc64):
     assert rel(g2, 2 * gk.astype(np.float64)) < 1e-6
 
 
+@pytest.mark.parametrize("resampling,seismotype,dt_data", [(1, 0, 0.002), (0, 0, 0.0023), (1, 1, 0.0023)])
+def test_operator_level_trace_maps(lib, orc64, resampling, seismotype, dt_data):
+    """The device trace maps of the operator level for linear resampling, displacement
+    data and a non-integer ratio of data and propagation steps."""
+    nx, nz, h = 90, 60, 0.01
+    nt_data = 201
+    true = W.add_ellipses(W.gradient_model(nx, nz, h, h, seed=5), h, h, seed=6)
+    start = W.smooth_model(true, h, h, 0.1)
+    rx = np.stack([np.arange(3, nx - 3, 4) * h, np.full(len(range(3, nx - 3, 4)), 0.03)], 1)
+    prob = W.base_problem(nx, nz, h, h, nt_data, dt_data, [[0.25, 0.02], [0.6, 0.1]], [rx, rx[1::2]], taper=12,
+                          dt=0.001, sub=-1, seismotype=seismotype)
+    prob["resampling"] = resampling
+    src = W.vertical_force(W.ricker(nt_data, dt_data, 12.0), 2)
+    dobs, _ = orc64.apply_forward(orc64.make_op(prob), true, src)
+    d0, full = orc64.apply_forward(orc64.make_op(prob), start, src, keep_full=True)
+    op = lib.WeOp(prob, src)
+    d0k = op.apply_forward(start)
+    err_d = rel(d0k, d0)
+    r = (d0 - dobs) * dt_data
+    gref = orc64.apply_jacobianT(orc64.make_op(prob), start, r, full)
+    gk = op.apply_jacobianT(start, r)
+    errs = [rel(gk[c], gref[c]) for c in range(3)]
+    print("resampling=%d seismotype=%d dt_data=%g time axis %s: forward %.3e, gradient %s"
+          % (resampling, seismotype, dt_data, op.time_axis(), err_d, ["%.3e" % e for e in errs]))
+    assert err_d <= 1e-4 and max(errs) <= 1e-4
+    op.close()
+
+
 @pytest.mark.parametrize("normalize", [False, True])
 def test_objective_matches_oracle(lib, orc64, normalize):
     """nlls_fwi (normalize False) and nlls_fwi_eco (True) on the batched operator, with
