@@ -396,53 +396,123 @@ struct fwi2d_op {
 };
 
 // Least-squares FWI objective on the operator: nlls_fwi (optimization.hpp:295-455) and
-// nlls_fwi_eco (optimization.hpp:459-951) at their default data processing (no
-// integration, trace normalisation, envelope, filter, interferometry, double difference
-// or source rescaling).  Both shot loops become two batched phases: one apply_forward over
-// every shot, the residuals, one apply_jacobianT(add=false) over every shot.  That is the
-// sum of nlls_fwi_eco's per-shot apply_jacobianT(add=true) calls (optimization.hpp:890),
-// without the CPU build's re-transform of the running sum (we_op.cpp:1080-1084, 1192-1236).
-// Reductions accumulate in double precision (the reference's run in data_t).
+// nlls_fwi_eco (optimization.hpp:459-951) with residual weights, time integration, trace
+// normalisation, a gradient mask and nlls_fwi_eco's data normalisation (no envelope,
+// filter, interferometry, double difference or source rescaling).  Both shot loops
+// become two batched phases: one apply_forward over every shot, the residuals, one
+// apply_jacobianT(add=false) over every shot.  That is the sum of nlls_fwi_eco's per-shot
+// apply_jacobianT(add=true) calls (optimization.hpp:890), without the CPU build's
+// re-transform of the running sum (we_op.cpp:1080-1084, 1192-1236).  The trace
+// processing and the reductions run in double precision (the reference's in data_t).
 struct fwi2d_nlls {
     fwi2d_op* op = nullptr;
-    std::vector<float> d;      // observed data times the weights (optimization.hpp:336, 531)
+    bool integrate = false, trace_norm = false;
+    std::vector<double> d;     // processed observed data: integrated, weighted, normalised
     std::vector<float> w;      // residual weights, empty: none
     std::vector<float> gmask;  // gradient mask, empty: none
     std::vector<float> r;      // residual of the last evaluation
+    std::vector<double> syn;   // processed synthetics before trace normalisation
+    std::vector<double> norms; // their trace norms
     double dnorm = 1.0;        // nlls_fwi_eco's data norm (optimization.hpp:569-574), else 1
     size_t ndata = 0, nmodel = 0;
 
+    int nt() const { return op->d.nt_data; }
+    size_t ntr() const { return ndata / nt(); }
+
+    // integral (operator.hpp:650-689): trapezoid running sum, and its adjoint
+    void integral(std::vector<double>& u, bool adjoint) const
+    {
+        const int n = nt();
+        const double dt = op->d.dt_data;
+        for (size_t x = 0; x < ntr(); x++) {
+            double* p = u.data() + x * n;
+            if (!adjoint) {
+                p[0] = 0.5 * p[0] * dt;
+                for (int t = 1; t < n - 1; t++) p[t] = p[t - 1] + p[t] * dt;
+                p[n - 1] = p[n - 2] + 0.5 * p[n - 1] * dt;
+            } else {
+                p[n - 1] = p[n - 1] * dt;
+                for (int t = n - 2; t >= 0; t--) p[t] = p[t + 1] + p[t] * dt;
+                p[n - 1] *= 0.5;
+                p[0] *= 0.5;
+            }
+        }
+    }
+
+    // ttnormalize (misc.cpp:97-110): each trace divided by its L2 norm (1 below 1e-16)
+    void normalize(std::vector<double>& u, std::vector<double>* out_norms) const
+    {
+        const int n = nt();
+        if (out_norms) out_norms->assign(ntr(), 1.0);
+        for (size_t x = 0; x < ntr(); x++) {
+            double* p = u.data() + x * n;
+            double s = 0;
+            for (int t = 0; t < n; t++) s += p[t] * p[t];
+            if (s <= 1e-16) s = 1;
+            s = std::sqrt(s);
+            for (int t = 0; t < n; t++) p[t] /= s;
+            if (out_norms) (*out_norms)[x] = s;
+        }
+    }
+
+    // the data processing of the constructors and of res: integrate, weight, normalise
+    void process(std::vector<double>& u, bool keep_syn)
+    {
+        if (integrate) integral(u, false);
+        if (!w.empty())
+            for (size_t i = 0; i < ndata; i++) u[i] *= w[i];
+        if (trace_norm) {
+            if (keep_syn) syn = u;
+            normalize(u, keep_syn ? &norms : nullptr);
+        }
+    }
+
     void evaluate(const float* m, double* func, float* g)
     {
-        const int nt = op->d.nt_data;
+        const int n = nt();
         const double dt = op->d.dt_data;
-        const size_t ntr = ndata / nt;
-        // res (optimization.hpp:346-374, 660-776): r = W L(m) - W d
-        r.assign(ndata, 0.f);
-        op->apply_forward(false, m, r.data());
-        for (size_t i = 0; i < ndata; i++) r[i] = (w.empty() ? r[i] : r[i] * w[i]) - d[i];
+        // res (optimization.hpp:346-374, 660-776): r = P(F(m)) - P(d), P the processing
+        std::vector<float> u(ndata);
+        op->apply_forward(false, m, u.data());
+        std::vector<double> rd(u.begin(), u.end());
+        process(rd, true);
+        for (size_t i = 0; i < ndata; i++) rd[i] -= d[i];
+        r.assign(rd.begin(), rd.end());
         // getFunc (optimization.hpp:375-385, 929-942): 1/2 dt (|r|^2 - 1/2 sum of the
         // trace end samples squared) / dnorm
         if (func) {
             double f = 0, ends = 0;
             for (size_t i = 0; i < ndata; i++) f += (double)r[i] * r[i];
-            for (size_t x = 0; x < ntr; x++) {
-                const double a = r[x * nt], b = r[x * nt + nt - 1];
+            for (size_t x = 0; x < ntr(); x++) {
+                const double a = r[x * n], b = r[x * n + n - 1];
                 ends += a * a + b * b;
             }
             *func = 0.5 * dt * (f - 0.5 * ends) / dnorm;
         }
         if (!g) return;
-        // grad (optimization.hpp:390-455, 780-905): J^T W Ht r / dnorm, masked
-        std::vector<float> a(ndata);
-        for (size_t x = 0; x < ntr; x++)
-            for (int t = 0; t < nt; t++) {
-                const size_t i = x * nt + t;
-                const double h = (t == 0 || t == nt - 1) ? 0.5 * dt : dt;
-                a[i] = (float)((w.empty() ? 1.0 : (double)w[i]) * (h * r[i]));
+        // grad (optimization.hpp:390-455, 780-905): J^T I^T W N'^T Ht r / dnorm, masked,
+        // with N' = I/|u| - u u^T/|u|^3 the Jacobian of the trace normalisation at u = syn
+        std::vector<double> a(r.begin(), r.end());
+        for (size_t x = 0; x < ntr(); x++) {
+            a[x * n] *= 0.5 * dt;
+            for (int t = 1; t < n - 1; t++) a[x * n + t] *= dt;
+            a[x * n + n - 1] *= 0.5 * dt;
+        }
+        if (trace_norm)
+            for (size_t x = 0; x < ntr(); x++) {
+                double* p = a.data() + x * n;
+                const double* q = syn.data() + x * n;
+                const double nx = norms[x];
+                double v = 0;
+                for (int t = 0; t < n; t++) v += p[t] * q[t];
+                for (int t = 0; t < n; t++) p[t] = p[t] / nx - v / (nx * nx * nx) * q[t];
             }
+        if (!w.empty())
+            for (size_t i = 0; i < ndata; i++) a[i] *= w[i];
+        if (integrate) integral(a, true);
+        std::vector<float> af(a.begin(), a.end());
         std::vector<float> gr(nmodel);
-        op->apply_jacobianT(false, gr.data(), m, a.data());
+        op->apply_jacobianT(false, gr.data(), m, af.data());
         for (size_t i = 0; i < nmodel; i++)
             g[i] = (float)((gmask.empty() ? 1.0 : (double)gmask[i]) * (gr[i] / dnorm));
     }
@@ -534,39 +604,34 @@ int fwi2d_op_time_axis(fwi2d_op* op, int* nt, double* dt, int* sub)
 
 void fwi2d_op_destroy(fwi2d_op* op) { delete op; }
 
-int fwi2d_nlls_create(fwi2d_op* op, const float* d, const float* w, const float* gmask, int normalize,
+int fwi2d_nlls_create(fwi2d_op* op, const float* d, const float* w, const float* gmask, int flags,
                       fwi2d_nlls** out)
 {
     return op_guard([&] {
         if (!op || !d || !out) throw std::runtime_error("null argument");
-        fwi2d_nlls* f = new fwi2d_nlls;
-        f->op = op;
-        f->ndata = (size_t)2 * op->nr * op->d.nt_data;
-        f->nmodel = (size_t)op->np() * op->nxz();
-        f->d.assign(d, d + f->ndata);
+        if (flags & ~(FWI2D_NLLS_NORMALIZE_OBJ | FWI2D_NLLS_INTEGRATE | FWI2D_NLLS_TRACE_NORMALIZE))
+            throw std::runtime_error("unknown objective flags");
+        const size_t ndata = (size_t)2 * op->nr * op->d.nt_data, nmodel = (size_t)op->np() * op->nxz();
         auto nonneg = [](const float* p, size_t n) {
             for (size_t i = 0; i < n; i++)
                 if (!(p[i] >= 0)) return false;
             return true;
         };
-        if (w) {
-            if (!nonneg(w, f->ndata)) {
-                delete f;
-                throw std::runtime_error("Data weights  must be non-negative");
-            }
-            f->w.assign(w, w + f->ndata);
-            for (size_t i = 0; i < f->ndata; i++) f->d[i] *= f->w[i];
-        }
-        if (gmask) {
-            if (!nonneg(gmask, f->nmodel)) {
-                delete f;
-                throw std::runtime_error("Gradient mask must be non-negative");
-            }
-            f->gmask.assign(gmask, gmask + f->nmodel);
-        }
-        if (normalize) {
+        if (w && !nonneg(w, ndata)) throw std::runtime_error("Data weights  must be non-negative");
+        if (gmask && !nonneg(gmask, nmodel)) throw std::runtime_error("Gradient mask must be non-negative");
+        fwi2d_nlls* f = new fwi2d_nlls;
+        f->op = op;
+        f->ndata = ndata;
+        f->nmodel = nmodel;
+        f->integrate = (flags & FWI2D_NLLS_INTEGRATE) != 0;
+        f->trace_norm = (flags & FWI2D_NLLS_TRACE_NORMALIZE) != 0;
+        if (w) f->w.assign(w, w + ndata);
+        if (gmask) f->gmask.assign(gmask, gmask + nmodel);
+        f->d.assign(d, d + ndata);
+        f->process(f->d, false);
+        if (flags & FWI2D_NLLS_NORMALIZE_OBJ) {
             double s = 0;
-            for (float v : f->d) s += (double)v * v;
+            for (double v : f->d) s += v * v;
             s *= op->d.dt_data;
             f->dnorm = s < 1e-16 ? 1.0 : s;
         }

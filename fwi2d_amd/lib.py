@@ -257,19 +257,24 @@ class WeOp:
         return load().fwi2d_op_engine(self._h)
 
 
-class Nlls:
-    """nlls_fwi (normalize=False) / nlls_fwi_eco (normalize=True) on a WeOp, default data
-    processing: evaluate(m) returns the functional (FP64) and the gradient."""
+NLLS_NORMALIZE_OBJ, NLLS_INTEGRATE, NLLS_TRACE_NORMALIZE = 1, 2, 4
 
-    def __init__(self, op, d, w=None, gmask=None, normalize=False):
+
+class Nlls:
+    """nlls_fwi (normalize=False) / nlls_fwi_eco (normalize=True: the functional divided
+    by dt |P d|^2) on a WeOp, with residual weights, time integration and trace
+    normalisation as options: evaluate(m) returns the functional (FP64) and the gradient."""
+
+    def __init__(self, op, d, w=None, gmask=None, normalize=False, integrate=False, trace_normalize=False):
         self.op = op  # the objective uses the operator handle
         self._d = _f32(d)
         assert self._d.shape == (2, op.nr, op.nt_data), self._d.shape
         w = None if w is None else _f32(w)
         gmask = None if gmask is None else _f32(gmask)
         self._h = C.c_void_p()
-        _check(load().fwi2d_nlls_create(op._h, _ptr(self._d), _ptr(w), _ptr(gmask), int(bool(normalize)),
-                                        C.byref(self._h)))
+        flags = (NLLS_NORMALIZE_OBJ * bool(normalize) | NLLS_INTEGRATE * bool(integrate)
+                 | NLLS_TRACE_NORMALIZE * bool(trace_normalize))
+        _check(load().fwi2d_nlls_create(op._h, _ptr(self._d), _ptr(w), _ptr(gmask), flags, C.byref(self._h)))
 
     def close(self):
         if self._h:

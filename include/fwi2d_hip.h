@@ -184,23 +184,35 @@ void fwi2d_op_destroy(fwi2d_op* op);
 
 typedef struct fwi2d_nlls fwi2d_nlls;
 
+/* Options of the objective (bit flags):
+   FWI2D_NLLS_NORMALIZE_OBJ: nlls_fwi_eco's par.normalize_obj_func: functional and gradient
+     divided by dnorm = dt_data * |P d|^2 (1 below 1e-16); without it dnorm = 1 (nlls_fwi);
+   FWI2D_NLLS_INTEGRATE: par.integrate / nlls_fwi's integrate: data and synthetics
+     integrated in time (integral, src/include/operator.hpp:650-689);
+   FWI2D_NLLS_TRACE_NORMALIZE: par.normalize / nlls_fwi's normalize: every trace divided
+     by its L2 norm (ttnormalize, src/cpp/misc.cpp:97-110). */
+#define FWI2D_NLLS_NORMALIZE_OBJ 1
+#define FWI2D_NLLS_INTEGRATE 2
+#define FWI2D_NLLS_TRACE_NORMALIZE 4
+
 /* Constructors of nlls_fwi (src/include/optimization.hpp:309-344) and nlls_fwi_eco
-   (optimization.hpp:482-590) at their default data processing (no integration, trace
-   normalisation, envelope, filter, interferometry, double difference or source
-   rescaling).  d: observed data [2][nrec_total][nt_data]; w: residual weights of the
-   same shape, or NULL; gmask: gradient mask [nparam][nx][nz], or NULL; normalize: 0 for
-   nlls_fwi, 1 for nlls_fwi_eco with par.normalize_obj_func (dnorm = dt_data * |W d|^2,
-   1 below 1e-16).  Negative weights or mask values are rejected as in the reference.
+   (optimization.hpp:482-590) without envelope, filter, interferometry, double difference
+   or source rescaling.  d: observed data [2][nrec_total][nt_data]; w: residual weights
+   of the same shape, or NULL; gmask: gradient mask [nparam][nx][nz], or NULL; flags: the
+   options above.  The data are processed once as the constructors do: P d = normalise(W
+   integrate(d)).  Negative weights or mask values are rejected as in the reference.
    The objective keeps a copy of d, w and gmask, and uses op until it is destroyed. */
-int fwi2d_nlls_create(fwi2d_op* op, const float* d, const float* w, const float* gmask, int normalize,
+int fwi2d_nlls_create(fwi2d_op* op, const float* d, const float* w, const float* gmask, int flags,
                       fwi2d_nlls** out);
 
 /* res + getFunc + grad at the physical model m (nlls_fwi: optimization.hpp:346-455;
    nlls_fwi_eco: compute_res_and_grad, getFunc, grad, optimization.hpp:592-951), over all
    shots in two batched phases (forward, then adjoint):
-     r = W F(m) - W d,
+     r = P F(m) - P d,
      func = 1/2 dt (|r|^2 - 1/2 sum over traces of r[0]^2 + r[nt-1]^2) / dnorm  (FP64),
-     g = gmask * J(m)^T (W Ht r) / dnorm     [nparam][nx][nz].
+     g = gmask * J(m)^T P'(F(m))^T Ht r / dnorm     [nparam][nx][nz],
+   with P' the Jacobian of the processing (integral adjoint, weights, the trace
+   normalisation's I/|u| - u u^T/|u|^3). 
    func or g may be NULL (g NULL: residual and functional only, no adjoint). */
 int fwi2d_nlls_evaluate(fwi2d_nlls* f, const float* m, double* func, float* g);
 

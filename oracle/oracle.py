@@ -282,29 +282,84 @@ class Oracle:
         ends = np.sum(tr[:, 0] ** 2 + tr[:, -1] ** 2)
         return 0.5 * dt * (np.sum(tr * tr) - 0.5 * ends) / dnorm
 
-    def nlls_evaluate(self, prob, model, allsrc, d, w=None, gmask=None, normalize=False, gradient=True):
+    @staticmethod
+    def integral(u, dt, adjoint=False):
+        """integral (operator.hpp:650-689): per trace y0 = dt/2 x0, y_t = y_t-1 + dt x_t,
+        y_n-1 = y_n-2 + dt/2 x_n-1; the adjoint is the transposed sum."""
+        u = np.array(u, np.float64)
+        tr = u.reshape(-1, u.shape[-1])
+        wts = np.full(tr.shape[1], dt)
+        wts[0] = wts[-1] = 0.5 * dt
+        if not adjoint:
+            out = np.cumsum(tr * wts, axis=1)
+            out[:, -1] = out[:, -2] + 0.5 * dt * tr[:, -1]
+        else:
+            out = np.cumsum((tr * dt)[:, ::-1], axis=1)[:, ::-1]
+            out[:, -1] = 0.5 * dt * tr[:, -1]
+            out[:, 0] *= 0.5
+        return out.reshape(u.shape)
+
+    @staticmethod
+    def ttnormalize(u):
+        """ttnormalize (misc.cpp:97-110): each trace over its L2 norm (1 below 1e-16).
+        Returns (normalised, norms)."""
+        tr = np.array(u, np.float64).reshape(-1, np.shape(u)[-1])
+        n2 = np.sum(tr * tr, axis=1)
+        n2[n2 <= 1e-16] = 1.0
+        norms = np.sqrt(n2)
+        return (tr / norms[:, None]).reshape(np.shape(u)), norms
+
+    def nlls_data_terms(self, u, d, dt, w=None, integrate=False, trace_normalize=False):
+        """Data side of nlls_fwi / nlls_fwi_eco (optimization.hpp:330-455, 512-556, 660-905):
+        with P(u) = ttnormalize(W integral(u)) (each step optional), returns the residual
+        r = P(u) - P(d), the adjoint source a = P'(u)^T Ht r, i.e. the gradient in u of
+        1/2 r^T Ht r, and P(d).  P'^T = integral^T W (I/|s| - s s^T/|s|^3) per trace, with
+        s the weighted integrated synthetics."""
+        def process(v):
+            v = np.asarray(v, np.float64)
+            if integrate:
+                v = self.integral(v, dt)
+            if w is not None:
+                v = v * w
+            syn, norms = v, None
+            if trace_normalize:
+                v, norms = self.ttnormalize(v)
+            return v, syn, norms
+        pd = process(d)[0]
+        pu, syn, norms = process(u)
+        r = pu - pd
+        a = self.Ht(False, np.array(r, self.dtype), dt)
+        if trace_normalize:
+            at = a.reshape(-1, a.shape[-1])
+            st = syn.reshape(at.shape)
+            v = np.sum(at * st, axis=1)
+            a = (at / norms[:, None] - (v / norms ** 3)[:, None] * st).reshape(a.shape)
+        if w is not None:
+            a = a * w
+        if integrate:
+            a = self.integral(a, dt, adjoint=True)
+        return r, a, pd
+
+    def nlls_evaluate(self, prob, model, allsrc, d, w=None, gmask=None, normalize=False, gradient=True,
+                      integrate=False, trace_normalize=False):
         """nlls_fwi (optimization.hpp:309-455) or, with normalize, nlls_fwi_eco with
-        normalize_obj_func (optimization.hpp:482-951), default data processing:
-        r = W F(m) - W d; f = nlls_functional(r, dt, dnorm); g = gmask J^T(W Ht r) / dnorm,
-        with dnorm = dt |W d|^2 (1 below 1e-16) for nlls_fwi_eco and 1 for nlls_fwi.  The
-        shot sum is one multi-shot apply_jacobianT(add=false), i.e. the per-shot calls of
+        normalize_obj_func (optimization.hpp:482-951); integrate and trace_normalize are
+        par.integrate / par.normalize: r, a from nlls_data_terms at u = F(m);
+        f = nlls_functional(r, dt, dnorm); g = gmask J^T a / dnorm, with
+        dnorm = dt |P d|^2 (1 below 1e-16) for nlls_fwi_eco and 1 for nlls_fwi.  The shot
+        sum is one multi-shot apply_jacobianT(add=false), i.e. the per-shot calls of
         nlls_fwi_eco (optimization.hpp:890) summed.  Returns (f, g, r)."""
         dt = float(prob["mp"]["dt_data"])
-        wd = np.asarray(d, np.float64) if w is None else np.asarray(d, np.float64) * w
+        u, full = self.apply_forward(self.make_op(prob), model, allsrc, keep_full=gradient)
+        r, a, pd = self.nlls_data_terms(u, d, dt, w, integrate, trace_normalize)
         dnorm = 1.0
         if normalize:
-            dnorm = float(np.sum(wd * wd)) * dt
+            dnorm = float(np.sum(pd * pd)) * dt
             if dnorm < 1e-16:
                 dnorm = 1.0
-        u, full = self.apply_forward(self.make_op(prob), model, allsrc, keep_full=gradient)
-        u = np.asarray(u, np.float64)
-        r = (u if w is None else u * w) - wd
         f = self.nlls_functional(r, dt, dnorm)
         if not gradient:
             return f, None, r
-        a = self.Ht(False, np.array(r, self.dtype), dt)  # applyHt, then the weights
-        if w is not None:  # (optimization.hpp:399, 435, 780, 879-893)
-            a = a * w
         g = self.apply_jacobianT(self.make_op(prob), model, a, full) / dnorm
         if gmask is not None:
             g = g * gmask

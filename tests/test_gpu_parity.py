@@ -405,12 +405,16 @@ def test_operator_level_trace_maps(lib, orc64, resampling, seismotype, dt_data):
     op.close()
 
 
-@pytest.mark.parametrize("normalize", [False, True])
-def test_objective_matches_oracle(lib, orc64, normalize):
+@pytest.mark.parametrize("normalize,integrate,trace_normalize", [(False, False, False), (True, False, False),
+                                                                  (True, True, True)])
+def test_objective_matches_oracle(lib, orc64, orc32, normalize, integrate, trace_normalize):
     """nlls_fwi (normalize False) and nlls_fwi_eco (True) on the batched operator, with
-    residual weights and a gradient mask: functional, residual and gradient against the
-    oracle's objective on the same model, each <= 1e-4 (the residual relative to the
-    weighted data, as the operator-level seismograms)."""
+    residual weights, a gradient mask, and time integration and trace normalisation in
+    the last case: functional, residual (relative to the processed data) and gradient
+    against the oracle's objective on the same model, each within max(1e-4, 2 x the gap
+    of the reference algorithm's own FP32 build).  Trace normalisation scales every trace
+    to unit norm, weak ones included, which magnifies the FP32 propagation error: the
+    FP32 build is then 3e-4..1e-3 away from FP64 in the gradient."""
     nx, nz, h = 110, 70, 0.01
     nt_data, dt_data = 251, 0.002
     true = W.add_ellipses(W.gradient_model(nx, nz, h, h, seed=3), h, h, seed=4)
@@ -423,21 +427,27 @@ def test_objective_matches_oracle(lib, orc64, normalize):
     rng = np.random.default_rng(11)
     w = rng.uniform(0.5, 1.5, dobs.shape).astype(np.float32)
     gmask = np.clip(1.5 * W._smooth_field((nx, nz), 4, rng) + 0.5, 0, 1)[None].repeat(3, 0).astype(np.float32)
-    fref, gref, rref = orc64.nlls_evaluate(prob, start, src, dobs.astype(np.float32), w=w, gmask=gmask,
-                                           normalize=normalize)
+    opts = dict(normalize=normalize, integrate=integrate, trace_normalize=trace_normalize)
+    fref, gref, rref = orc64.nlls_evaluate(prob, start, src, dobs.astype(np.float32), w=w, gmask=gmask, **opts)
+    pd = orc64.nlls_data_terms(dobs, dobs.astype(np.float32), dt_data, w=w, integrate=integrate,
+                               trace_normalize=trace_normalize)[2]
     op = lib.WeOp(prob, src)
-    obj = lib.Nlls(op, dobs, w=w, gmask=gmask, normalize=normalize)
+    obj = lib.Nlls(op, dobs, w=w, gmask=gmask, **opts)
     f, g = obj.evaluate(start)
+    f32, g32, r32 = orc32.nlls_evaluate(prob, start, src, dobs.astype(np.float32), w=w, gmask=gmask, **opts)
     err_f = abs(f / fref - 1)
-    wd = w.astype(np.float64) * dobs
-    err_r = np.linalg.norm(obj.residual() - rref) / np.linalg.norm(wd)
-    print("normalize=%s functional %.9e (oracle %.9e) rel err %.3e, residual err %.3e, |r|/|Wd| %.3f"
-          % (normalize, f, fref, err_f, err_r, np.linalg.norm(rref) / np.linalg.norm(wd)))
-    assert err_f <= 1e-4 and err_r <= 1e-4
+    err_r = np.linalg.norm(obj.residual() - rref) / np.linalg.norm(pd)
+    tol_f = max(1e-4, 2 * abs(f32 / fref - 1))
+    tol_r = max(1e-4, 2 * np.linalg.norm(r32 - rref) / np.linalg.norm(pd))
+    print("%s functional %.9e (oracle %.9e) rel err %.3e (tol %.1e), residual err %.3e (tol %.1e), |r|/|Pd| %.3f"
+          % (opts, f, fref, err_f, tol_f, err_r, tol_r, np.linalg.norm(rref) / np.linalg.norm(pd)))
+    assert err_f <= tol_f and err_r <= tol_r
     for c in range(3):
         err = rel(g[c], gref[c])
-        print("objective gradient component %d rel err %.3e" % (c, err))
-        assert err <= 1e-4
+        tol = max(1e-4, 2 * rel(g32[c], gref[c]))
+        print("objective gradient component %d rel err %.3e (reference FP32 gap %.3e, tol %.1e)"
+              % (c, err, rel(g32[c], gref[c]), tol))
+        assert err <= tol
     # functional only: no adjoint, same value
     f2, g2 = obj.evaluate(start, gradient=False)
     assert g2 is None and f2 == f

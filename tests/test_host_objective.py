@@ -36,10 +36,18 @@ def test_data_norm(op):
     assert lib.Nlls(op, d).dnorm() == 1.0
     assert lib.Nlls(op, d, normalize=True).dnorm() == pytest.approx(0.002 * np.sum(d.astype(np.float64) ** 2),
                                                                     rel=1e-12)
-    # the weights multiply the data in single precision before the norm
-    wd32 = (d * w).astype(np.float64)
-    assert lib.Nlls(op, d, w=w, normalize=True).dnorm() == pytest.approx(0.002 * np.sum(wd32 ** 2), rel=1e-12)
-    assert abs(np.sum(wd32 ** 2) / np.sum(wd ** 2) - 1) < 1e-6
+    # the weights multiply the data before the norm
+    assert lib.Nlls(op, d, w=w, normalize=True).dnorm() == pytest.approx(0.002 * np.sum(wd ** 2), rel=1e-12)
+    # trace normalisation: every trace has unit norm, dnorm = dt * ntraces
+    assert lib.Nlls(op, d, normalize=True, trace_normalize=True).dnorm() == pytest.approx(0.002 * 2 * op.nr,
+                                                                                          rel=1e-12)
+    # integration: the trapezoid running sum of the data
+    tw = np.full(op.nt_data, 0.002)
+    tw[0] = tw[-1] = 0.001
+    di = np.cumsum(d.astype(np.float64) * tw, axis=-1)
+    di[..., -1] = di[..., -2] + 0.001 * d[..., -1]
+    assert lib.Nlls(op, d, normalize=True, integrate=True).dnorm() == pytest.approx(0.002 * np.sum(di ** 2),
+                                                                                    rel=1e-12)
     # all-zero data: dnorm falls back to 1
     assert lib.Nlls(op, np.zeros_like(d), normalize=True).dnorm() == 1.0
     assert lib.Nlls(op, d).residual() is None
@@ -62,3 +70,6 @@ def test_argument_checks(op):
     L = lib.load()
     assert L.fwi2d_nlls_create(None, None, None, None, 0, None) != 0
     assert b"null argument" in L.fwi2d_last_error()
+    h = lib.C.c_void_p()
+    assert L.fwi2d_nlls_create(op._h, d.ctypes.data_as(lib.C.c_void_p), None, None, 8, lib.C.byref(h)) != 0
+    assert b"unknown objective flags" in L.fwi2d_last_error()

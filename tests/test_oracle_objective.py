@@ -8,6 +8,13 @@ src/include/optimization.hpp:295-951), on the CPU.
     the functional and the gradient into two exact parts (so the shot sum is a sum).
   * A constant weight a scales nlls_fwi's functional and gradient by a^2 and leaves
     nlls_fwi_eco's (normalised by dt |W d|^2) unchanged; the mask multiplies the gradient.
+  * Time integration: closed form for a constant trace, and its adjoint passes the
+    dot-product test; trace normalisation gives unit traces and leaves zero traces zero.
+  * Data side with integration, weights and trace normalisation: the adjoint source is the
+    gradient of 1/2 r^T Ht r in the synthetics, checked against central differences to
+    1e-7 (a dropped term of the normalisation's Jacobian or a wrong order of the steps is
+    off by far more); with trace normalisation the functional does not change when the
+    source amplitude does.
   * The gradient is the derivative of the functional along a smooth model perturbation,
     to the accuracy of the reference's imaging condition (one-step lag of the adjoint
     derivatives, snapshots every sub steps, wide-stencil products): within 15 % here,
@@ -108,3 +115,64 @@ def test_gradient_is_the_functional_derivative(orc64, small_problem, normalize):
     print("normalize=%s directional derivative: finite difference %.6e, gradient %.6e (ratio - 1 = %.3e)"
           % (normalize, fd, gp, fd / gp - 1))
     assert abs(fd / gp - 1) < 0.15
+
+
+def test_integral_and_trace_normalisation(orc64):
+    dt, nt = 0.004, 50
+    y = orc64.integral(np.full((2, nt), 3.0), dt)
+    k = np.arange(nt) + 0.5
+    k[-1] = nt - 1
+    assert np.allclose(y, 3.0 * dt * k[None, :], rtol=1e-14, atol=0)
+    rng = np.random.default_rng(2)
+    x, v = rng.standard_normal((2, 3, nt))
+    lhs = np.sum(orc64.integral(x, dt) * v)
+    rhs = np.sum(x * orc64.integral(v, dt, adjoint=True))
+    assert lhs == pytest.approx(rhs, rel=1e-13)
+    u = rng.standard_normal((2, 4, nt))
+    u[1, 2] = 0.0
+    un, norms = orc64.ttnormalize(u)
+    tn = np.sqrt(np.sum(un.reshape(-1, nt) ** 2, axis=1))
+    assert np.allclose(np.delete(tn, 6), 1.0, rtol=1e-14) and tn[6] == 0.0 and norms[6] == 1.0
+
+
+@pytest.mark.parametrize("integrate,trace_normalize,weights", [(True, False, False), (False, True, True),
+                                                               (True, True, True)])
+def test_data_side_gradient(orc64, integrate, trace_normalize, weights):
+    rng = np.random.default_rng(7)
+    nt, dt = 80, 0.003
+    t = np.arange(nt) * dt
+    u = np.sin(40 * t)[None, None, :] * rng.uniform(0.5, 2, (2, 5, 1)) + 0.3 * rng.standard_normal((2, 5, nt))
+    d = u + 0.5 * rng.standard_normal(u.shape)
+    w = rng.uniform(0.2, 1.5, u.shape) if weights else None
+    args = dict(w=w, integrate=integrate, trace_normalize=trace_normalize)
+    r, a, _ = orc64.nlls_data_terms(u, d, dt, **args)
+    v = rng.standard_normal(u.shape)
+    eps = 1e-5
+    fp = orc64.nlls_functional(orc64.nlls_data_terms(u + eps * v, d, dt, **args)[0], dt)
+    fm = orc64.nlls_functional(orc64.nlls_data_terms(u - eps * v, d, dt, **args)[0], dt)
+    fd = (fp - fm) / (2 * eps)
+    assert fd == pytest.approx(float(np.sum(a * v)), rel=1e-7)
+    if trace_normalize:
+        r3 = orc64.nlls_data_terms(3.0 * u, d, dt, **args)[0]
+        assert np.abs(r3 - r).max() <= 1e-12 * np.abs(r).max()
+
+
+def test_options_gradient_is_the_functional_derivative(orc64, small_problem):
+    """Integration, trace normalisation and nlls_fwi_eco's normalisation together."""
+    P = small_problem
+    opts = dict(normalize=True, integrate=True, trace_normalize=True)
+    f, g, _ = orc64.nlls_evaluate(P["prob"], P["start"], P["src"], P["dobs"], **opts)
+    nx, nz = P["start"].shape[1:]
+    rng = np.random.default_rng(5)
+    p = np.stack([W._smooth_field((nx, nz), 5, rng) for _ in range(3)])
+    p *= np.array([0.05, 0.03, 0.05])[:, None, None]
+    eps = 3e-3
+    fp = orc64.nlls_evaluate(P["prob"], P["start"] + eps * p, P["src"], P["dobs"], gradient=False, **opts)[0]
+    fm = orc64.nlls_evaluate(P["prob"], P["start"] - eps * p, P["src"], P["dobs"], gradient=False, **opts)[0]
+    fd = (fp - fm) / (2 * eps)
+    gp = float(np.sum(g * p))
+    print("options: finite difference %.6e, gradient %.6e (ratio - 1 = %.3e)" % (fd, gp, fd / gp - 1))
+    assert abs(fd / gp - 1) < 0.15
+    # trace normalisation: a louder source leaves the functional unchanged
+    f3 = orc64.nlls_evaluate(P["prob"], P["start"], 3.0 * P["src"], P["dobs"], gradient=False, **opts)[0]
+    assert f3 == pytest.approx(f, rel=1e-9)
