@@ -455,6 +455,33 @@ def test_objective_matches_oracle(lib, orc64, orc32, normalize, integrate, trace
     op.close()
 
 
+def test_operator_level_vti_objective(lib, orc64):
+    """VTI through the operator level and nlls_fwi_eco: physical (vp, vs, rho, delta, eps)
+    models with the delta admissibility clip and c13 conversion, the 5-component gradient
+    (delta and epsilon components zero, as in the reference), against the oracle."""
+    nx, nz, h = 90, 60, 0.01
+    nt_data, dt_data = 201, 0.002
+    m = W.add_ellipses(W.gradient_model(nx, nz, h, h, seed=8), h, h, seed=9)
+    de = W.vti_fields(nx, nz, h, h, seed=4)
+    true = np.concatenate([m, de], 0)
+    start = np.concatenate([W.smooth_model(m, h, h, 0.1), de], 0)
+    rx = np.stack([np.arange(3, nx - 3, 4) * h, np.full(len(range(3, nx - 3, 4)), 0.03)], 1)
+    prob = W.base_problem(nx, nz, h, h, nt_data, dt_data, [[0.25, 0.02], [0.6, 0.1]], [rx, rx[1::2]], vti=1,
+                          taper=12, dt=0.001, sub=-1)
+    src = W.vertical_force(W.ricker(nt_data, dt_data, 12.0), 2)
+    dobs, _ = orc64.apply_forward(orc64.make_op(prob), true, src)
+    fref, gref, _ = orc64.nlls_evaluate(prob, start, src, dobs.astype(np.float32), normalize=True)
+    op = lib.WeOp(prob, src)
+    obj = lib.Nlls(op, dobs, normalize=True)
+    f, g = obj.evaluate(start)
+    errs = [rel(g[c], gref[c]) for c in range(3)]
+    print("VTI objective: functional rel err %.3e, gradient %s" % (abs(f / fref - 1), ["%.3e" % e for e in errs]))
+    assert g.shape == (5, nx, nz) and not g[3:].any()
+    assert abs(f / fref - 1) <= 1e-4 and max(errs) <= 1e-4
+    obj.close()
+    op.close()
+
+
 def test_config0_full_size_forward(lib, orc64, orc32):
     """The reference's CPU-runnable case at full size: 1 shot on 201 x 601, nt = 2000,
     15 Hz Ricker at 20 m depth, 150 surface receivers, free surface and 30-cell tapers,
