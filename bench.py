@@ -49,6 +49,9 @@ def parse():
                          "counters per dispatch) and exit")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device-to-device copy 5 times (HBM counter calibration) and exit")
+    ap.add_argument("--operator", action="store_true",
+                    help="time the operator level instead: nlls_fwi_eco evaluations (forward, residual, "
+                         "adjoint) through fwi2d_op / fwi2d_nlls from host arrays, one JSON line")
     ap.add_argument("--step-only", type=int, metavar="EDGE_STREAM", default=None,
                     help="only autotune and time the forward time step of the workload with this edge_stream "
                          "setting (0: every launch of a step on one stream, for per-launch kernel traces), "
@@ -299,6 +302,45 @@ def step_only(args):
                       "shots_per_block_edge": B.eng.stat("shots_per_block_edge")}))
 
 
+def operator_level(args):
+    """The reference's call path on the same workload: physical models and host arrays
+    through fwi2d_op (analyzeModel, source resampling, device trace maps, H quadrature
+    and chain rule) and the nlls_fwi_eco objective (FP64 functional) -- `steps` timed
+    evaluations of functional and gradient after `warmup` untimed ones."""
+    import fwi2d_amd
+    from fwi2d_amd import workloads as W
+    fwi2d_amd.load()
+    wl = WORKLOADS[args.workload]
+    ns = args.shots or wl[4]
+    nt = args.nt or wl[5]
+    nz, nx, h, dt, _, _, vti, gseed, _ = wl
+    true = W.gradient_model(nx, nz, h, h, seed=0)
+    if vti:
+        true = np.concatenate([true, W.vti_fields(nx, nz, h, h)], 0)
+    start = W.smooth_model(true, h, h, 0.3)
+    sxz, rxz, freqs = W.config2_geometry(ns, nx, h, seed=gseed)
+    wav = np.stack([W.ricker(nt, dt, f) for f in freqs])
+    src = np.stack([wav * np.cos(1.5708), wav * np.sin(1.5708)]).astype(np.float32)
+    prob = W.base_problem(nx, nz, h, h, nt, dt, sxz, rxz, vti=int(vti), dt=dt, sub=1)
+    op = fwi2d_amd.WeOp(prob, src)
+    dobs = op.apply_forward(true)
+    obj = fwi2d_amd.Nlls(op, dobs, normalize=True)
+    log("operator and objective ready: %d shots, %dx%d, nt=%d" % (ns, nz, nx, nt))
+    for _ in range(args.warmup):
+        obj.evaluate(start)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        f, g = obj.evaluate(start)
+    el = time.perf_counter() - t0
+    log("timed evaluations done: %.3f s" % el)
+    print(json.dumps({"metric": "operator-level FWI gradient shots/s (nlls_fwi_eco, host arrays)",
+                      "value": round(ns * args.steps / el, 4), "unit": "shots/s", "steps": args.steps,
+                      "warmup": args.warmup, "s_per_evaluation": round(el / args.steps, 4),
+                      "functional_f64": f, "gradient_finite": bool(np.isfinite(g).all()),
+                      "config": {"workload": "%s: %d shots, %dx%d, nt=%d%s" % (args.workload, ns, nz, nx, nt,
+                                                                             ", VTI" if vti else "")}}))
+
+
 def main():
     args = parse()
     if args.summarize:
@@ -309,6 +351,9 @@ def main():
         return
     if args.step_only is not None:
         step_only(args)
+        return
+    if args.operator:
+        operator_level(args)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU only: the engine batches the shots of one MI355X; across GPUs "
