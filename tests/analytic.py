@@ -66,3 +66,49 @@ def homogeneous_case(vti=False, n=301, h=0.01, nt=1001, dt=0.001, f0=4.0):
     src[1, 0] = wav
     ref = greens_traces(wav, dt, vp, vs, rho, offsets)
     return prob, model, np.array([[xs, zs]]), [rxz], src, nt, dt, ref
+
+
+def rayleigh_speed(vp, vs):
+    """Root of the Rayleigh equation (2 - c^2/vs^2)^2 = 4 sqrt(1 - c^2/vp^2) sqrt(1 - c^2/vs^2)."""
+    from scipy.optimize import brentq
+    f = lambda c: (2 - c ** 2 / vs ** 2) ** 2 - 4 * np.sqrt(1 - c ** 2 / vp ** 2) * np.sqrt(1 - c ** 2 / vs ** 2)
+    return brentq(f, 0.5 * vs, 0.999 * vs)
+
+
+def surface_case(n_x=401, n_z=301, h=0.01, nt=2901, dt=0.001, f0=4.0):
+    """A vertical 4 Hz Ricker force on a free surface (absorbing elsewhere) and two surface
+    receivers 1 and 2 km away; the Rayleigh pulse reaches both before any boundary
+    reflection.  Returns prob, converted model, sxz, rxz, src, nt, dt, (vp, vs)."""
+    vp, vs, rho = 2.0, 1.0, 2.0
+    lam, mu = rho * (vp ** 2 - 2 * vs ** 2), rho * vs ** 2
+    model = np.stack([np.full((n_x, n_z), v) for v in (lam, mu, rho)])
+    prob = dict(nx=n_x, nz=n_z, dx=h, dz=h, bc_top=1, bc_bottom=2, bc_left=2, bc_right=2, taper_top=0,
+                taper_bottom=30, taper_left=30, taper_right=30, taper_strength=0.05, vti=0,
+                gdamp_tmin=0.0, gdamp_tmax=0.0, gdamp_power=1.0)
+    t = np.arange(nt) * dt
+    a = (np.pi * f0 * (t - 1.2 / f0)) ** 2
+    src = np.zeros((2, 1, nt))
+    src[1, 0] = (1.0 - 2.0 * a) * np.exp(-a)
+    xs = 0.5
+    rxz = np.array([[xs + 1.0, 0.0], [xs + 2.0, 0.0]])
+    return prob, model, np.array([[xs, 0.0]]), [rxz], src, nt, dt, (vp, vs, f0)
+
+
+def surface_wave_speed(uz, dt, c_expected, f0, offsets=(1.0, 2.0)):
+    """Speed of the pulse between two surface traces: each trace windowed (Hann, 0.9 s)
+    around the expected arrival, the lag of the cross-correlation peak refined by a
+    parabola.  Returns (speed, peak amplitude ratio far/near)."""
+    nt = uz.shape[-1]
+    t = np.arange(nt) * dt
+
+    def win(tr, x):
+        m = np.abs(t - (x / c_expected + 1.2 / f0)) < 0.45
+        out = np.zeros(nt)
+        out[m] = tr[m] * np.hanning(m.sum())
+        return out
+    a1, a2 = win(uz[0], offsets[0]), win(uz[1], offsets[1])
+    cc = np.correlate(a2, a1, "full")
+    k = int(np.argmax(cc))
+    y0, y1, y2 = cc[k - 1], cc[k], cc[k + 1]
+    lag = (k - (nt - 1) + 0.5 * (y0 - y2) / (y0 - 2 * y1 + y2)) * dt
+    return (offsets[1] - offsets[0]) / lag, np.abs(a2).max() / np.abs(a1).max()

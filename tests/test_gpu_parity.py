@@ -497,6 +497,19 @@ def test_engine_matches_greens_function(lib, vti):
         assert err < 2e-3
 
 
+def test_engine_free_surface_rayleigh_wave(lib):
+    """Physics pin of the engine's free surface: the surface pulse of a vertical surface
+    force travels at the root of the Rayleigh equation (to 3e-3) without spreading."""
+    from analytic import rayleigh_speed, surface_case, surface_wave_speed
+    prob, model, sxz, rxz, src, nt, dt, (vp, vs, f0) = surface_case()
+    _, out = run_engine(lib, prob, model, sxz, rxz, src.astype(np.float32), nt, dt)
+    cR = rayleigh_speed(vp, vs)
+    c, amp = surface_wave_speed(out[1].astype(np.float64), dt, cR, f0)
+    print("surface pulse speed %.5f vs, Rayleigh root %.5f vs (rel err %.2e), amplitude ratio %.3f"
+          % (c / vs, cR / vs, c / cR - 1, amp))
+    assert abs(c / cR - 1) < 3e-3 and 0.9 < amp < 1.2
+
+
 def test_config0_full_size_forward(lib, orc64, orc32):
     """The reference's CPU-runnable case at full size: 1 shot on 201 x 601, nt = 2000,
     15 Hz Ricker at 20 m depth, 150 surface receivers, free surface and 30-cell tapers,

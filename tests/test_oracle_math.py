@@ -14,7 +14,9 @@ coefficient, a dropped term, a sign or index error) fails it:
   * cos^2 taper values at the band ends.
   * Physics: a point force in a homogeneous full space matches the closed-form
     plane-strain Green's tensor (P and S radiation, near field, off-grid receivers) to
-    2e-3, isotropic and VTI with eps = delta = 0 (where c13 = lambda).
+    2e-3, isotropic and VTI with eps = delta = 0 (where c13 = lambda); a vertical force on
+    the free surface launches a pulse that travels between two surface receivers at the
+    root of the Rayleigh equation (to 3e-3) without geometric spreading.
   * Imaging condition: along rho (lambda, mu fixed) the gradient equals the finite-
     difference derivative of the L2 misfit; along lambda it does too when the
     reference's one-step lag of the adjoint derivatives is switched off, and with the
@@ -298,3 +300,17 @@ def test_point_force_matches_greens_function(orc64, vti):
         err = np.linalg.norm(out[:, k] - ref[:, k]) / np.linalg.norm(ref[:, k])
         print("vti=%s receiver %d: rel err vs Green's function %.3e" % (vti, k, err))
         assert err < 2e-3
+
+
+def test_free_surface_rayleigh_wave(orc64):
+    from analytic import rayleigh_speed, surface_case, surface_wave_speed
+    prob, model, sxz, rxz, src, nt, dt, (vp, vs, f0) = surface_case()
+    p = dict(prob, nt=nt, dt=dt, sub=0)
+    inj = orc64.delta_m3(p["nx"], p["nz"], p["dx"], p["dz"], 0, 0, sxz[:, 0], sxz[:, 1])
+    ext = orc64.delta_m3(p["nx"], p["nz"], p["dx"], p["dz"], 0, 0, rxz[0][:, 0], rxz[0][:, 1])
+    out = orc64.propagate(grid_struct(p), model, inj, src, ext=ext)
+    cR = rayleigh_speed(vp, vs)
+    c, amp = surface_wave_speed(out[1], dt, cR, f0)
+    print("surface pulse speed %.5f vs, Rayleigh root %.5f vs (rel err %.2e), amplitude ratio %.3f"
+          % (c / vs, cR / vs, c / cR - 1, amp))
+    assert abs(c / cR - 1) < 3e-3 and 0.9 < amp < 1.2
