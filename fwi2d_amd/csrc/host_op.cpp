@@ -653,4 +653,9 @@ double fwi2d_nlls_dnorm(fwi2d_nlls* f) { return f ? f->dnorm : 0.0; }
 
 void fwi2d_nlls_destroy(fwi2d_nlls* f) { delete f; }
 
+int fwi2d_nlls_objective(void* ctx, const float* m, double* f, float* g)
+{
+    return fwi2d_nlls_evaluate(static_cast<fwi2d_nlls*>(ctx), m, f, g);
+}
+
 }  // extern "C"

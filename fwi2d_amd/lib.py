@@ -46,7 +46,16 @@ EXPORTS = [
     "fwi2d_engine_get_stat", "fwi2d_engine_tune_report", "fwi2d_engine_destroy", "fwi2d_op_create", "fwi2d_op_apply_forward",
     "fwi2d_op_apply_jacobianT", "fwi2d_op_engine", "fwi2d_op_time_axis", "fwi2d_op_destroy",
     "fwi2d_nlls_create", "fwi2d_nlls_evaluate", "fwi2d_nlls_residual", "fwi2d_nlls_dnorm", "fwi2d_nlls_destroy",
+    "fwi2d_nlls_objective", "fwi2d_lbfgs",
 ]
+
+OBJECTIVE_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_double), C.POINTER(C.c_float))
+
+
+class LbfgsOptions(C.Structure):
+    _fields_ = [("niter", C.c_int), ("max_trial", C.c_int), ("threshold", C.c_double), ("memory", C.c_int),
+                ("c1", C.c_double), ("c2", C.c_double), ("a0", C.c_double), ("a1", C.c_double),
+                ("step_max", C.c_double), ("version", C.c_int), ("H0", C.c_void_p)]
 
 
 class Fwi2dError(RuntimeError):
@@ -89,6 +98,7 @@ def load():
     lib.fwi2d_nlls_dnorm.argtypes = [P]
     lib.fwi2d_nlls_dnorm.restype = C.c_double
     lib.fwi2d_nlls_destroy.argtypes = [P]
+    lib.fwi2d_lbfgs.argtypes = [C.c_void_p, P, C.c_longlong, P, C.POINTER(LbfgsOptions), P, C.POINTER(C.c_int)]
     _lib = lib
     return lib
 
@@ -298,3 +308,36 @@ class Nlls:
 
     def dnorm(self):
         return load().fwi2d_nlls_dnorm(self._h)
+
+
+def lbfgs(objective, m0, niter=10, max_trial=10, threshold=0.0, memory=5, c1=1e-4, c2=0.9, a0=1.0, a1=0.0,
+          step_max=1e6, version=0, H0=None):
+    """The reference's l-BFGS with the regular Wolfe line search (fwi2d_lbfgs).  objective:
+    an Nlls (run through fwi2d_nlls_objective) or a Python callable m -> (f, g).
+    Returns (model, functional history, iterations)."""
+    lib = load()
+    m = _f32(m0).copy()
+    n = m.size
+    h0 = None if H0 is None else _f32(H0).ravel()
+    opt = LbfgsOptions(int(niter), int(max_trial), float(threshold), int(memory), float(c1), float(c2), float(a0),
+                       float(a1), float(step_max), int(version), None if h0 is None else h0.ctypes.data)
+    hist = np.zeros(niter + 1)
+    iters = C.c_int()
+    if isinstance(objective, Nlls):
+        fn = C.cast(lib.fwi2d_nlls_objective, C.c_void_p)
+        ctx = objective._h
+    else:
+        def call(_ctx, pm, pf, pg):
+            try:
+                f, g = objective(np.ctypeslib.as_array(pm, shape=(n,)).copy().reshape(m.shape))
+                pf[0] = float(f)
+                np.ctypeslib.as_array(pg, shape=(n,))[:] = np.asarray(g, np.float32).ravel()
+                return 0
+            except Exception:  # reported as a failed evaluation
+                return 1
+        cb = OBJECTIVE_FN(call)
+        fn = C.cast(cb, C.c_void_p)
+        ctx = None
+    _check(lib.fwi2d_lbfgs(fn, ctx, n, _ptr(m), C.byref(opt), _ptr(hist), C.byref(iters)))
+    k = iters.value
+    return m, hist[:k + 1], k

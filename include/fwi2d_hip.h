@@ -224,6 +224,42 @@ double fwi2d_nlls_dnorm(fwi2d_nlls* f);
 
 void fwi2d_nlls_destroy(fwi2d_nlls* f);
 
+/* ------------------------------------------------------------------------- */
+/* Solver: l-BFGS with the regular Wolfe line search                         */
+/* ------------------------------------------------------------------------- */
+
+/* Objective callback: the functional *f and the gradient g (n values) at model m;
+   returns 0 on success.  The least-squares objective itself is one:
+   fwi2d_nlls_objective with ctx = the fwi2d_nlls handle. */
+typedef int (*fwi2d_objective_fn)(void* ctx, const float* m, double* f, float* g);
+int fwi2d_nlls_objective(void* ctx, const float* m, double* f, float* g);
+
+/* Parameters of lbfgs (src/include/nlsolver.hpp:308-324) and regular_wolfe
+   (nlsolver.hpp:85-114); the reference's defaults are memory 5, c1 1e-4, c2 0.9, a0 1,
+   a1 0, step_max 1e6, version 0. */
+typedef struct {
+    int niter;          /* iterations */
+    int max_trial;      /* trial steps per line search */
+    double threshold;   /* stop when the relative decrease of the functional falls below it */
+    int memory;         /* stored (s, y) pairs */
+    double c1, c2;      /* sufficient decrease and curvature constants, 0 < c1 < c2 < 1 */
+    double a0, a1;      /* first step a0 / |g|, or a1 |m| / |g| if larger */
+    double step_max;    /* largest step length */
+    int version;        /* 0: next first step stp * df_prev / df; 1: 2 (f - f_prev) / df, at most 1 */
+    const float* H0;    /* initial diagonal inverse Hessian (n values > 0), or NULL for 1 */
+} fwi2d_lbfgs_options;
+
+/* lbfgs::run (src/cpp/nlsolver.cpp:1207-1415) with regular_wolfe::lineSearch / zoom
+   (nlsolver.cpp:184-421) on model m (n values, updated in place).  Every evaluation
+   returns functional and gradient together, as nlls_fwi_eco's res() does.
+   func_history (niter + 1 values, or NULL) receives the functional before the first
+   iteration and after each accepted step; *iterations the accepted steps.  Stops, as
+   the reference, after niter iterations, a relative decrease below threshold, a
+   functional or gradient norm below 1e-10, a failed line search (the model is then the
+   last trial) or a search direction that is not a descent direction. */
+int fwi2d_lbfgs(fwi2d_objective_fn fn, void* ctx, long long n, float* m, const fwi2d_lbfgs_options* opt,
+                double* func_history, int* iterations);
+
 #ifdef __cplusplus
 }
 #endif

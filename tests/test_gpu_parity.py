@@ -510,6 +510,33 @@ def test_engine_free_surface_rayleigh_wave(lib):
     assert abs(c / cR - 1) < 3e-3 and 0.9 < amp < 1.2
 
 
+def test_lbfgs_fwi_on_the_batched_objective(lib, orc64):
+    """End-to-end FWI on the device: the reference's l-BFGS with the regular Wolfe line
+    search driving nlls_fwi_eco through fwi2d_nlls_objective.  The first functional
+    equals the objective's own evaluation, the functional history decreases at every
+    accepted step, and the model moves towards the true one."""
+    nx, nz, h = 110, 70, 0.01
+    nt_data, dt_data = 251, 0.002
+    true = W.add_ellipses(W.gradient_model(nx, nz, h, h, seed=3), h, h, seed=4)
+    start = W.smooth_model(true, h, h, 0.1)
+    rx = np.stack([np.arange(2, nx - 2, 3) * h, np.full(len(range(2, nx - 2, 3)), 0.02)], 1)
+    prob = W.base_problem(nx, nz, h, h, nt_data, dt_data, [[0.15, 0.02], [0.55, 0.02], [0.95, 0.02]],
+                          [rx, rx, rx], taper=15, dt=0.001, sub=-1)
+    src = W.vertical_force(W.ricker(nt_data, dt_data, 12.0), 3)
+    dobs, _ = orc64.apply_forward(orc64.make_op(prob), true, src)
+    op = lib.WeOp(prob, src)
+    obj = lib.Nlls(op, dobs, normalize=True)
+    f0, _ = obj.evaluate(start)
+    m, hist, k = lib.lbfgs(obj, start, niter=5, max_trial=10)
+    e0 = np.linalg.norm(start - true) / np.linalg.norm(true)
+    e1 = np.linalg.norm(m - true) / np.linalg.norm(true)
+    print("l-BFGS: %d iterations, functional %s, model rel err %.4e -> %.4e" % (k, ["%.4e" % v for v in hist], e0, e1))
+    assert k >= 3 and hist[0] == pytest.approx(f0, rel=1e-6)
+    assert np.all(np.diff(hist) < 0) and hist[-1] < 0.7 * hist[0] and e1 < e0
+    obj.close()
+    op.close()
+
+
 def test_config0_full_size_forward(lib, orc64, orc32):
     """The reference's CPU-runnable case at full size: 1 shot on 201 x 601, nt = 2000,
     15 Hz Ricker at 20 m depth, 150 surface receivers, free surface and 30-cell tapers,
