@@ -194,3 +194,20 @@ def config3(nshots=16, seed=0):
     wav = np.stack([ricker(nt, dt, f) for f in freqs])
     src = np.stack([wav * np.cos(1.5708), wav * np.sin(1.5708)])
     return prob, model, src
+
+
+def config4(nshots=48, f0=5.0, seed=0):
+    """End-to-end FWI: 48 shots, 600 x 2000, h = 10 m, nt = 4000 (dt = 1 ms); true = depth
+    gradient + seeded lens anomalies, start = the true model smoothed over 500 m; Ricker
+    sources of peak frequency f0 (one band of the multiscale schedule) on the 1 ms data
+    axis.  Physical models (vp, vs, rho) and operator-level problem."""
+    nz, nx, h = 600, 2000, 0.01
+    nt, dt = 4000, 0.001
+    true = add_ellipses(gradient_model(nx, nz, h, h, seed=seed), h, h, n=4, amp=0.08, seed=seed + 7)
+    start = smooth_model(true, h, h, 0.5)
+    sxz, rxz, _ = config2_geometry(nshots, nx, h, seed=seed + 4)
+    # propagation step and snapshot interval from the model (courant 0.6), as in FWI runs
+    # where the line search may try models with larger velocities
+    prob = base_problem(nx, nz, h, h, nt, dt, sxz, rxz, dt=-1.0, sub=-1)
+    src = vertical_force(ricker(nt, dt, f0), nshots)
+    return prob, true, start, src
