@@ -323,6 +323,7 @@ def lbfgs(objective, m0, niter=10, max_trial=10, threshold=0.0, memory=5, c1=1e-
                        float(a1), float(step_max), int(version), None if h0 is None else h0.ctypes.data)
     hist = np.zeros(niter + 1)
     iters = C.c_int()
+    errors = []
     if isinstance(objective, Nlls):
         fn = C.cast(lib.fwi2d_nlls_objective, C.c_void_p)
         ctx = objective._h
@@ -333,11 +334,15 @@ def lbfgs(objective, m0, niter=10, max_trial=10, threshold=0.0, memory=5, c1=1e-
                 pf[0] = float(f)
                 np.ctypeslib.as_array(pg, shape=(n,))[:] = np.asarray(g, np.float32).ravel()
                 return 0
-            except Exception:  # reported as a failed evaluation
+            except Exception as ex:  # the solver stops with an error; re-raised below
+                errors.append(ex)
                 return 1
         cb = OBJECTIVE_FN(call)
         fn = C.cast(cb, C.c_void_p)
         ctx = None
-    _check(lib.fwi2d_lbfgs(fn, ctx, n, _ptr(m), C.byref(opt), _ptr(hist), C.byref(iters)))
+    rc = lib.fwi2d_lbfgs(fn, ctx, n, _ptr(m), C.byref(opt), _ptr(hist), C.byref(iters))
+    if rc != 0 and errors:
+        raise Fwi2dError(lib.fwi2d_last_error().decode()) from errors[0]
+    _check(rc)
     k = iters.value
     return m, hist[:k + 1], k
