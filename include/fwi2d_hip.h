@@ -1,13 +1,17 @@
 /*
  * fwi2d_hip.h -- C ABI of the MI355X (gfx950) elastic propagation engine.
  *
- * Two levels, both plain C (pointers, sizes, status codes):
+ * Four levels, all plain C (pointers, sizes, status codes):
  *
  *  1. Operator level: a drop-in for the reference's non-linear elastic operators
  *     nl_we_op_e / nl_we_op_vti (src/include/we_op.hpp:54-86, 150-176): the
  *     apply_forward / apply_jacobianT pair that the reference's objectives
  *     (nlls_fwi, nlls_fwi_eco: src/include/optimization.hpp:351, 452, 656, 890)
  *     and WE_MODELING.x (src/main/we_modeling.cpp:73) call.
+ *
+ *  1a. Objective and solver on top of it: nlls_fwi / nlls_fwi_eco over all shots in
+ *     two batched phases (fwi2d_nlls_*), and the reference's l-BFGS with the regular
+ *     Wolfe line search over an objective callback (fwi2d_lbfgs).
  *
  *  2. Engine level: batched device propagation that replaces the per-shot device
  *     hook nl_we_op_e::propagate_gpu (src/include/we_op.hpp:82, 174; body in
