@@ -57,6 +57,8 @@ def parse():
                          "nlls_fwi_eco objective, 48 shots on 600x2000 with 5 Hz sources, one JSON line")
     ap.add_argument("--bands", default="5", help="with --fwi: comma-separated source peak frequencies (Hz), "
                                                  "one l-BFGS run of ITERS iterations per band, low to high")
+    ap.add_argument("--fwi-params", choices=["vp", "all"], default="vp",
+                    help="with --fwi: invert vp only (gradient mask zero on vs and rho, the default) or all three")
     ap.add_argument("--step-only", type=int, metavar="EDGE_STREAM", default=None,
                     help="only autotune and time the forward time step of the workload with this edge_stream "
                          "setting (0: every launch of a step on one stream, for per-launch kernel traces), "
@@ -368,7 +370,11 @@ def fwi_run(args):
         ns = len(prob["sxz"])
         op = fwi2d_amd.WeOp(prob, src)
         dobs = op.apply_forward(true)
-        obj = fwi2d_amd.Nlls(op, dobs, normalize=True)
+        gmask = None
+        if args.fwi_params == "vp":
+            gmask = np.zeros_like(true, dtype=np.float32)
+            gmask[0] = 1.0
+        obj = fwi2d_amd.Nlls(op, dobs, gmask=gmask, normalize=True)
         count = [0]
 
         def evaluate(x):
@@ -392,7 +398,9 @@ def fwi_run(args):
                       "seconds": round(time.perf_counter() - t_all, 1),
                       "model_rel_err": [err(start), err(m)], "vp_rel_err": [vp_err(start), vp_err(m)],
                       "config": {"workload": "%d shots, 600x2000 h=10 m, 4000 samples of 1 ms, Ricker bands %s Hz, "
-                                             "%d l-BFGS iterations per band" % (ns, args.bands, args.fwi)}}))
+                                             "%d l-BFGS iterations per band, inverting %s"
+                                             % (ns, args.bands, args.fwi, "vp" if args.fwi_params == "vp"
+                                                else "vp, vs, rho")}}))
 
 
 def main():
