@@ -353,8 +353,9 @@ def fwi_run(args):
     band (Ricker source peak frequency, low to high), observed data from the true model,
     then `--fwi` l-BFGS iterations (regular Wolfe line search, first step 1% of the model
     norm) through the operator level and the nlls_fwi_eco objective, starting from the
-    previous band's result (the first from the 500 m smoothed model).  One progress line
-    per objective evaluation, one JSON line at the end."""
+    previous band's result (the first from the 500 m smoothed model).  By default only vp
+    is inverted (a gradient mask holds vs and rho).  One progress line per objective
+    evaluation, one JSON line at the end."""
     import fwi2d_amd
     from fwi2d_amd import workloads as W
     fwi2d_amd.load()
