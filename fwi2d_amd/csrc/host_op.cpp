@@ -9,10 +9,12 @@
 // rule) runs here in double precision; the time stepping runs on the device in single
 // precision.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "fwi2d_hip.h"
@@ -234,6 +236,26 @@ double h_weight(int n, int i)
     return 1.0;
 }
 
+// f(x) for x in [0, n) on up to 16 host threads, in chunks of 64 taken in turn; every x
+// is computed the same way whatever thread takes it
+template <class F>
+void parallel_traces(size_t n, F&& f)
+{
+    const unsigned nth = std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
+    if (n < 256 || nth == 1) {
+        for (size_t x = 0; x < n; x++) f(x);
+        return;
+    }
+    std::atomic<size_t> next{0};
+    std::vector<std::thread> pool;
+    for (unsigned t = 0; t < nth; t++)
+        pool.emplace_back([&] {
+            for (size_t b; (b = next.fetch_add(64)) < n;)
+                for (size_t x = b; x < std::min(n, b + 64); x++) f(x);
+        });
+    for (auto& t : pool) t.join();
+}
+
 }  // namespace
 
 struct fwi2d_op {
@@ -411,59 +433,52 @@ struct fwi2d_nlls {
     std::vector<float> w;      // residual weights, empty: none
     std::vector<float> gmask;  // gradient mask, empty: none
     std::vector<float> r;      // residual of the last evaluation
-    std::vector<double> syn;   // processed synthetics before trace normalisation
-    std::vector<double> norms; // their trace norms
     double dnorm = 1.0;        // nlls_fwi_eco's data norm (optimization.hpp:569-574), else 1
     size_t ndata = 0, nmodel = 0;
 
     int nt() const { return op->d.nt_data; }
     size_t ntr() const { return ndata / nt(); }
 
-    // integral (operator.hpp:650-689): trapezoid running sum, and its adjoint
-    void integral(std::vector<double>& u, bool adjoint) const
+    // integral (operator.hpp:650-689) of one trace: trapezoid running sum, or its adjoint
+    void integral_trace(double* p, bool adjoint) const
     {
         const int n = nt();
         const double dt = op->d.dt_data;
-        for (size_t x = 0; x < ntr(); x++) {
-            double* p = u.data() + x * n;
-            if (!adjoint) {
-                p[0] = 0.5 * p[0] * dt;
-                for (int t = 1; t < n - 1; t++) p[t] = p[t - 1] + p[t] * dt;
-                p[n - 1] = p[n - 2] + 0.5 * p[n - 1] * dt;
-            } else {
-                p[n - 1] = p[n - 1] * dt;
-                for (int t = n - 2; t >= 0; t--) p[t] = p[t + 1] + p[t] * dt;
-                p[n - 1] *= 0.5;
-                p[0] *= 0.5;
-            }
+        if (!adjoint) {
+            p[0] = 0.5 * p[0] * dt;
+            for (int t = 1; t < n - 1; t++) p[t] = p[t - 1] + p[t] * dt;
+            p[n - 1] = p[n - 2] + 0.5 * p[n - 1] * dt;
+        } else {
+            p[n - 1] = p[n - 1] * dt;
+            for (int t = n - 2; t >= 0; t--) p[t] = p[t + 1] + p[t] * dt;
+            p[n - 1] *= 0.5;
+            p[0] *= 0.5;
         }
     }
 
-    // ttnormalize (misc.cpp:97-110): each trace divided by its L2 norm (1 below 1e-16)
-    void normalize(std::vector<double>& u, std::vector<double>* out_norms) const
+    // ttnormalize (misc.cpp:97-110) of one trace: divided by its L2 norm (1 below 1e-16),
+    // which is returned
+    double normalize_trace(double* p) const
     {
         const int n = nt();
-        if (out_norms) out_norms->assign(ntr(), 1.0);
-        for (size_t x = 0; x < ntr(); x++) {
-            double* p = u.data() + x * n;
-            double s = 0;
-            for (int t = 0; t < n; t++) s += p[t] * p[t];
-            if (s <= 1e-16) s = 1;
-            s = std::sqrt(s);
-            for (int t = 0; t < n; t++) p[t] /= s;
-            if (out_norms) (*out_norms)[x] = s;
-        }
+        double s = 0;
+        for (int t = 0; t < n; t++) s += p[t] * p[t];
+        if (s <= 1e-16) s = 1;
+        s = std::sqrt(s);
+        for (int t = 0; t < n; t++) p[t] /= s;
+        return s;
     }
 
-    // the data processing of the constructors and of res: integrate, weight, normalise
-    void process(std::vector<double>& u, bool keep_syn)
+    // the data processing of the constructors: integrate, weight, normalise
+    void process(std::vector<double>& u) const
     {
-        if (integrate) integral(u, false);
-        if (!w.empty())
-            for (size_t i = 0; i < ndata; i++) u[i] *= w[i];
-        if (trace_norm) {
-            if (keep_syn) syn = u;
-            normalize(u, keep_syn ? &norms : nullptr);
+        const int n = nt();
+        for (size_t x = 0; x < ntr(); x++) {
+            double* p = u.data() + x * n;
+            if (integrate) integral_trace(p, false);
+            if (!w.empty())
+                for (int t = 0; t < n; t++) p[t] *= w[x * n + t];
+            if (trace_norm) normalize_trace(p);
         }
     }
 
@@ -471,46 +486,55 @@ struct fwi2d_nlls {
     {
         const int n = nt();
         const double dt = op->d.dt_data;
-        // res (optimization.hpp:346-374, 660-776): r = P(F(m)) - P(d), P the processing
+        const size_t T = ntr();
         std::vector<float> u(ndata);
         op->apply_forward(false, m, u.data());
-        std::vector<double> rd(u.begin(), u.end());
-        process(rd, true);
-        for (size_t i = 0; i < ndata; i++) rd[i] -= d[i];
-        r.assign(rd.begin(), rd.end());
-        // getFunc (optimization.hpp:375-385, 929-942): 1/2 dt (|r|^2 - 1/2 sum of the
-        // trace end samples squared) / dnorm
-        if (func) {
-            double f = 0, ends = 0;
-            for (size_t i = 0; i < ndata; i++) f += (double)r[i] * r[i];
-            for (size_t x = 0; x < ntr(); x++) {
-                const double a = r[x * n], b = r[x * n + n - 1];
-                ends += a * a + b * b;
+        r.resize(ndata);
+        std::vector<double> fpart(T);
+        std::vector<float> af(g ? ndata : 0);
+        // trace by trace (each independent, so the result does not depend on the threads):
+        // res (optimization.hpp:346-374, 660-776): r = P(F(m)) - P(d), P the processing;
+        // the trace's share of getFunc (optimization.hpp:375-385, 929-942):
+        // |r|^2 - 1/2 (r[0]^2 + r[n-1]^2); and the adjoint source of grad
+        // (optimization.hpp:390-455, 780-905): I^T W N'^T Ht r, with
+        // N' = I/|s| - s s^T/|s|^3 the Jacobian of the trace normalisation at s
+        parallel_traces(T, [&](size_t x) {
+            std::vector<double> v(u.begin() + x * n, u.begin() + (x + 1) * n), s;
+            const float* wx = w.empty() ? nullptr : w.data() + x * n;
+            if (integrate) integral_trace(v.data(), false);
+            if (wx)
+                for (int t = 0; t < n; t++) v[t] *= wx[t];
+            double nrm = 1.0;
+            if (trace_norm) {
+                s = v;
+                nrm = normalize_trace(v.data());
             }
-            *func = 0.5 * dt * (f - 0.5 * ends) / dnorm;
+            float* rx = r.data() + x * n;
+            const double* dx = d.data() + x * n;
+            double f = 0;
+            for (int t = 0; t < n; t++) {
+                rx[t] = (float)(v[t] - dx[t]);
+                f += (double)rx[t] * rx[t];
+            }
+            fpart[x] = f - 0.5 * ((double)rx[0] * rx[0] + (double)rx[n - 1] * rx[n - 1]);
+            if (!g) return;
+            for (int t = 0; t < n; t++) v[t] = (double)rx[t] * ((t == 0 || t == n - 1) ? 0.5 * dt : dt);
+            if (trace_norm) {
+                double vs = 0;
+                for (int t = 0; t < n; t++) vs += v[t] * s[t];
+                for (int t = 0; t < n; t++) v[t] = v[t] / nrm - vs / (nrm * nrm * nrm) * s[t];
+            }
+            if (wx)
+                for (int t = 0; t < n; t++) v[t] *= wx[t];
+            if (integrate) integral_trace(v.data(), true);
+            for (int t = 0; t < n; t++) af[x * n + t] = (float)v[t];
+        });
+        if (func) {
+            double f = 0;
+            for (double p : fpart) f += p;
+            *func = 0.5 * dt * f / dnorm;
         }
         if (!g) return;
-        // grad (optimization.hpp:390-455, 780-905): J^T I^T W N'^T Ht r / dnorm, masked,
-        // with N' = I/|u| - u u^T/|u|^3 the Jacobian of the trace normalisation at u = syn
-        std::vector<double> a(r.begin(), r.end());
-        for (size_t x = 0; x < ntr(); x++) {
-            a[x * n] *= 0.5 * dt;
-            for (int t = 1; t < n - 1; t++) a[x * n + t] *= dt;
-            a[x * n + n - 1] *= 0.5 * dt;
-        }
-        if (trace_norm)
-            for (size_t x = 0; x < ntr(); x++) {
-                double* p = a.data() + x * n;
-                const double* q = syn.data() + x * n;
-                const double nx = norms[x];
-                double v = 0;
-                for (int t = 0; t < n; t++) v += p[t] * q[t];
-                for (int t = 0; t < n; t++) p[t] = p[t] / nx - v / (nx * nx * nx) * q[t];
-            }
-        if (!w.empty())
-            for (size_t i = 0; i < ndata; i++) a[i] *= w[i];
-        if (integrate) integral(a, true);
-        std::vector<float> af(a.begin(), a.end());
         std::vector<float> gr(nmodel);
         op->apply_jacobianT(false, gr.data(), m, af.data());
         for (size_t i = 0; i < nmodel; i++)
@@ -628,7 +652,7 @@ int fwi2d_nlls_create(fwi2d_op* op, const float* d, const float* w, const float*
         if (w) f->w.assign(w, w + ndata);
         if (gmask) f->gmask.assign(gmask, gmask + nmodel);
         f->d.assign(d, d + ndata);
-        f->process(f->d, false);
+        f->process(f->d);
         if (flags & FWI2D_NLLS_NORMALIZE_OBJ) {
             double s = 0;
             for (double v : f->d) s += v * v;
