@@ -12,6 +12,8 @@
 #include <atomic>
 #include <cmath>
 #include <cstring>
+#include <exception>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <thread>
@@ -236,25 +238,52 @@ double h_weight(int n, int i)
     return 1.0;
 }
 
-// f(x) for x in [0, n) on up to 16 host threads, in chunks of 64 taken in turn; every x
-// is computed the same way whatever thread takes it
-template <class F>
+// f(x, ws) for x in [0, n) on up to 16 host threads, in chunks of 64 taken in turn; ws is
+// a workspace of type W that each thread constructs once and reuses.  Every x is computed
+// the same way whatever thread takes it.  An exception thrown by f (or by a thread's
+// start) stops the hand-out of further chunks and is rethrown here once every started
+// thread has joined, so it reaches the caller's error handling instead of terminating
+// the process.
+template <class W, class F>
 void parallel_traces(size_t n, F&& f)
 {
     const unsigned nth = std::min(16u, std::max(1u, std::thread::hardware_concurrency()));
     if (n < 256 || nth == 1) {
-        for (size_t x = 0; x < n; x++) f(x);
+        W ws;
+        for (size_t x = 0; x < n; x++) f(x, ws);
         return;
     }
     std::atomic<size_t> next{0};
-    std::vector<std::thread> pool;
-    for (unsigned t = 0; t < nth; t++)
-        pool.emplace_back([&] {
+    std::mutex mu;
+    std::exception_ptr err;
+    auto work = [&] {
+        try {
+            W ws;
             for (size_t b; (b = next.fetch_add(64)) < n;)
-                for (size_t x = b; x < std::min(n, b + 64); x++) f(x);
-        });
+                for (size_t x = b; x < std::min(n, b + 64); x++) f(x, ws);
+        } catch (...) {
+            std::lock_guard<std::mutex> lk(mu);
+            if (!err) err = std::current_exception();
+            next.store(n);
+        }
+    };
+    std::vector<std::thread> pool;
+    try {
+        pool.reserve(nth);
+        for (unsigned t = 0; t < nth; t++) pool.emplace_back(work);
+    } catch (...) {
+        next.store(n);
+        for (auto& t : pool) t.join();
+        throw;
+    }
     for (auto& t : pool) t.join();
+    if (err) std::rethrow_exception(err);
 }
+
+// per-thread scratch of the objective's trace loop
+struct TraceWork {
+    std::vector<double> v, s;
+};
 
 }  // namespace
 
@@ -498,8 +527,10 @@ struct fwi2d_nlls {
         // |r|^2 - 1/2 (r[0]^2 + r[n-1]^2); and the adjoint source of grad
         // (optimization.hpp:390-455, 780-905): I^T W N'^T Ht r, with
         // N' = I/|s| - s s^T/|s|^3 the Jacobian of the trace normalisation at s
-        parallel_traces(T, [&](size_t x) {
-            std::vector<double> v(u.begin() + x * n, u.begin() + (x + 1) * n), s;
+        parallel_traces<TraceWork>(T, [&](size_t x, TraceWork& ws) {
+            std::vector<double>& v = ws.v;
+            std::vector<double>& s = ws.s;
+            v.assign(u.begin() + x * n, u.begin() + (x + 1) * n);
             const float* wx = w.empty() ? nullptr : w.data() + x * n;
             if (integrate) integral_trace(v.data(), false);
             if (wx)

@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <limits>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -23,6 +24,7 @@ namespace {
 
 const double kZero = 1e-10;  // nlsolver.cpp:6
 const double kStol = 1e-3;   // nlsolver.cpp:7
+const double kNaN = std::numeric_limits<double>::quiet_NaN();
 
 double dot(const std::vector<float>& a, const std::vector<float>& b)
 {
@@ -59,13 +61,32 @@ struct Problem {
     void* ctx;
     std::vector<float> m, g;  // current model and the gradient of the last evaluation
     double f = 0;
-    int feval = 0, geval = 0;
+    int feval = 0, geval = 0;  // counted as lbfgs::run counts them (nlsolver.cpp:1239, 1268; 242, 275)
+    fwi2d_lbfgs_report* report = nullptr;
+    int iter = 0, trial = 0;  // iteration and trial of the line search in progress
+    // res() + getFunc(): the objective returns the gradient with the functional, as
+    // nlls_fwi_eco's compute_res_and_grad does; it is counted when the search reads it
     void eval()
     {
-        if (fn(ctx, m.data(), &f, g.data()) != 0) throw std::runtime_error("objective evaluation failed");
+        if (fn(ctx, m.data(), &f, g.data()) != 0)
+            throw std::runtime_error("objective evaluation failed: " + fwi2d::g_last_error);
         if (f != f) throw std::runtime_error("the objective function is NaN");
         feval++;
-        geval++;
+    }
+    // one trial of a line search: its step length and functional, and the gradient norm
+    // if the search reads the gradient (the reference's verbose trial block)
+    void record(double alpha, double gnorm)
+    {
+        if (!report) return;
+        const int k = report->ntrials++;
+        if (report->trials && k < report->trials_cap) {
+            double* row = report->trials + 5 * (size_t)k;
+            row[0] = iter;
+            row[1] = trial;
+            row[2] = alpha;
+            row[3] = f;
+            row[4] = gnorm;
+        }
     }
 };
 
@@ -109,15 +130,22 @@ struct RegularWolfe {
             if (n_trial >= max_trial) return false;
             const double fa = trial(P, m0, p, alpha);
             n_trial++;
+            P.trial = n_trial;
             f1 = fa;
             if (fa <= kZero) {
+                P.record(alpha, kNaN);
                 stp = alpha;
                 f0 = f;
                 f = fa;
                 return false;
             }
-            if (f1 > f + c1 * alpha * df || (f1 >= f0 && n_trial > 1)) return zoom(false, P, m0, p, n_trial, max_trial, gnorm);
+            if (f1 > f + c1 * alpha * df || (f1 >= f0 && n_trial > 1)) {
+                P.record(alpha, kNaN);
+                return zoom(false, P, m0, p, n_trial, max_trial, gnorm);
+            }
+            P.geval++;  // phi'(alpha): the gradient is read
             gnorm = std::sqrt(dot(P.g, P.g));
+            P.record(alpha, gnorm);
             const double df1 = dot(P.g, p);
             if (df1 >= c2 * df) {  // curvature condition
                 stp = alpha;
@@ -152,19 +180,24 @@ struct RegularWolfe {
             else alpha = cubic(f, df, alo, flo, ahi, fhi);
             const double fa = trial(P, m0, p, alpha);
             n_trial++;
+            P.trial = n_trial;
             f0 = f1;
             f1 = fa;
             if (fa <= kZero) {
+                P.record(alpha, kNaN);
                 stp = alpha;
                 f0 = f;
                 f = fa;
                 return false;
             }
             if (f1 > f + c1 * alpha * df || f1 >= flo) {
+                P.record(alpha, kNaN);
                 ahi = alpha;
                 fhi = f1;
             } else {
+                P.geval++;
                 gnorm = std::sqrt(dot(P.g, P.g));
+                P.record(alpha, gnorm);
                 const double df1 = dot(P.g, p);
                 if (df1 >= c2 * df) {
                     stp = alpha;
@@ -188,9 +221,13 @@ struct RegularWolfe {
 extern "C" {
 
 int fwi2d_lbfgs(fwi2d_objective_fn fn, void* ctx, long long n, float* m, const fwi2d_lbfgs_options* opt,
-                double* func_history, int* iterations)
+                double* func_history, int* iterations, fwi2d_lbfgs_report* report)
 {
     try {
+        if (report) {
+            if (report->trials_cap < 0) throw std::runtime_error("negative trials_cap");
+            report->feval = report->geval = report->ntrials = 0;
+        }
         if (!fn || !m || !opt || n < 1) throw std::runtime_error("null argument");
         const fwi2d_lbfgs_options& o = *opt;
         if (!(o.c1 > 0 && o.c1 < 1 && o.c2 > o.c1 && o.c2 < 1))
@@ -203,6 +240,7 @@ int fwi2d_lbfgs(fwi2d_objective_fn fn, void* ctx, long long n, float* m, const f
             H0.assign(o.H0, o.H0 + n);
         }
         Problem P{fn, ctx, std::vector<float>(m, m + n), std::vector<float>((size_t)n, 0.f)};
+        P.report = report;
         RegularWolfe ls{o.c1, o.c2, o.a0, o.a1, o.step_max, o.version};
         std::vector<double> func;
         auto finish = [&](int k) {
@@ -210,9 +248,14 @@ int fwi2d_lbfgs(fwi2d_objective_fn fn, void* ctx, long long n, float* m, const f
             if (func_history)
                 for (size_t i = 0; i < func.size(); i++) func_history[i] = func[i];
             if (iterations) *iterations = k;
+            if (report) {
+                report->feval = P.feval;
+                report->geval = P.geval;
+            }
         };
         // lbfgs::run (nlsolver.cpp:1207-1346)
         P.eval();
+        P.geval++;  // the gradient of the starting model (nlsolver.cpp:1266-1269)
         func.push_back(P.f);
         if (func[0] <= kZero) {
             finish(0);
@@ -261,6 +304,7 @@ int fwi2d_lbfgs(fwi2d_objective_fn fn, void* ctx, long long n, float* m, const f
             if (dot(p, P.g) >= 0) break;  // not a descent direction
             if (k > 0) m0 = P.m;
             g0 = P.g;
+            P.iter = k;
             success = ls.search(P, m0, p, k, o.max_trial, gnorm);
             if (!success) break;
             func.push_back(P.f);

@@ -58,6 +58,11 @@ class LbfgsOptions(C.Structure):
                 ("step_max", C.c_double), ("version", C.c_int), ("H0", C.c_void_p)]
 
 
+class LbfgsReport(C.Structure):
+    _fields_ = [("feval", C.c_int), ("geval", C.c_int), ("ntrials", C.c_int), ("trials_cap", C.c_int),
+                ("trials", C.c_void_p)]
+
+
 class Fwi2dError(RuntimeError):
     pass
 
@@ -98,7 +103,8 @@ def load():
     lib.fwi2d_nlls_dnorm.argtypes = [P]
     lib.fwi2d_nlls_dnorm.restype = C.c_double
     lib.fwi2d_nlls_destroy.argtypes = [P]
-    lib.fwi2d_lbfgs.argtypes = [C.c_void_p, P, C.c_longlong, P, C.POINTER(LbfgsOptions), P, C.POINTER(C.c_int)]
+    lib.fwi2d_lbfgs.argtypes = [C.c_void_p, P, C.c_longlong, P, C.POINTER(LbfgsOptions), P, C.POINTER(C.c_int),
+                                C.POINTER(LbfgsReport)]
     _lib = lib
     return lib
 
@@ -311,10 +317,12 @@ class Nlls:
 
 
 def lbfgs(objective, m0, niter=10, max_trial=10, threshold=0.0, memory=5, c1=1e-4, c2=0.9, a0=1.0, a1=0.0,
-          step_max=1e6, version=0, H0=None):
+          step_max=1e6, version=0, H0=None, report=False):
     """The reference's l-BFGS with the regular Wolfe line search (fwi2d_lbfgs).  objective:
     an Nlls (run through fwi2d_nlls_objective) or a Python callable m -> (f, g).
-    Returns (model, functional history, iterations)."""
+    Returns (model, functional history, iterations), and with report=True also a dict
+    with the evaluation counts ("feval", "geval") and the line-search trials ("trials":
+    rows of iteration, trial, step length, function value, gradient norm or NaN)."""
     lib = load()
     m = _f32(m0).copy()
     n = m.size
@@ -323,6 +331,9 @@ def lbfgs(objective, m0, niter=10, max_trial=10, threshold=0.0, memory=5, c1=1e-
                        float(a1), float(step_max), int(version), None if h0 is None else h0.ctypes.data)
     hist = np.zeros(niter + 1)
     iters = C.c_int()
+    cap = max(1, niter * max_trial)
+    rows = np.zeros((cap, 5))
+    rep = LbfgsReport(0, 0, 0, cap, rows.ctypes.data)
     errors = []
     if isinstance(objective, Nlls):
         fn = C.cast(lib.fwi2d_nlls_objective, C.c_void_p)
@@ -340,9 +351,12 @@ def lbfgs(objective, m0, niter=10, max_trial=10, threshold=0.0, memory=5, c1=1e-
         cb = OBJECTIVE_FN(call)
         fn = C.cast(cb, C.c_void_p)
         ctx = None
-    rc = lib.fwi2d_lbfgs(fn, ctx, n, _ptr(m), C.byref(opt), _ptr(hist), C.byref(iters))
+    rc = lib.fwi2d_lbfgs(fn, ctx, n, _ptr(m), C.byref(opt), _ptr(hist), C.byref(iters), C.byref(rep))
     if rc != 0 and errors:
         raise Fwi2dError(lib.fwi2d_last_error().decode()) from errors[0]
     _check(rc)
     k = iters.value
+    if report:
+        info = dict(feval=rep.feval, geval=rep.geval, trials=rows[:min(rep.ntrials, cap)].copy())
+        return m, hist[:k + 1], k, info
     return m, hist[:k + 1], k

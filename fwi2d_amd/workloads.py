@@ -196,6 +196,47 @@ def config3(nshots=16, seed=0):
     return prob, model, src
 
 
+def notebook_elastic_fwi():
+    """The published elastic FWI example of the reference (examples/elastic_fwi.ipynb
+    cells 3, 5, 8 and 11), as the reference programs see it.
+
+    - Wavelet (cell 3): 1001 samples of 4 ms, w = (1 - (tau/sig)^2) exp(-(tau/sig)^2 / 2),
+      tau = t - 50 dt, sig = sqrt(2)/(2 pi f0), f0 = 10 Hz, stored as float32.
+    - Models (cell 5): 801 x 201 cells of 10 m; two layers (vp, vs, rho) = (2.5, 1.5, 1.0)
+      above z index 100 and (4.0, 2.0, 2.0) below; the true model adds (3.5, 1.75, 1.75)
+      in x indices 375-425, z indices 70-79.
+    - Geometry (cells 8, 11): ns=10 sx0=2 sz0=0.1 sxinc=0.4 nr=401 rx0=2 rz0=0.1
+      rxinc=0.01, built as readParameters does (src/cpp/param.cpp:226-244) in the
+      reference's single-precision data_t: x = x0 + float(i) * inc in float32.
+    - Operator: bc_top=1, other sides 2 with 50-cell tapers of strength 0.08, dt=-1
+      (courant 0.6), seismotype=1, fangle=1.5708 (analyzeWavelet, src/cpp/we_op.cpp:
+      196-217, scaling in data_t), sub=-1 (analyzeNLInversion, src/cpp/param.cpp:253).
+
+    Returns (prob, true_model, start_model, allsrc [2][10][1001] float32)."""
+    f32 = np.float32
+    nt, dt, f0, shift = 1001, 0.004, 10.0, 50
+    t = np.linspace(0, (nt - 1) * dt, nt)
+    sig = np.sqrt(2) / (2 * f0 * np.pi)
+    w = ((1.0 - ((t - shift * dt) / sig) ** 2) * np.exp(-0.5 * ((t - shift * dt) / sig) ** 2)).astype(f32)
+    nz, nx, h = 201, 801, 0.01
+    m0 = np.zeros((3, nx, nz))
+    m0[0, :, :100], m0[0, :, 100:] = 2.5, 4.0
+    m0[1, :, :100], m0[1, :, 100:] = 1.5, 2.0
+    m0[2, :, :100], m0[2, :, 100:] = 1.0, 2.0
+    true = m0.copy()
+    true[0, 375:426, 70:80], true[1, 375:426, 70:80], true[2, 375:426, 70:80] = 3.5, 1.75, 1.75
+    ns, nr = 10, 401
+    sx = f32(2) + np.arange(ns, dtype=f32) * f32(0.4)
+    sxz = np.stack([sx, np.full(ns, f32(0.1), f32)], 1).astype(np.float64)
+    rx = f32(2) + np.arange(nr, dtype=f32) * f32(0.01)
+    rxz = np.stack([rx, np.full(nr, f32(0.1), f32)], 1).astype(np.float64)
+    prob = base_problem(nx, nz, float(f32(h)), float(f32(h)), nt, float(f32(dt)), sxz, [rxz] * ns,
+                        taper=50, taper_strength=float(f32(0.08)), dt=-1.0, sub=-1)
+    fa = f32(1.5708)
+    allsrc = np.stack([np.tile(w * f32(np.cos(fa)), (ns, 1)), np.tile(w * f32(np.sin(fa)), (ns, 1))])
+    return prob, true.astype(f32), m0.astype(f32), allsrc.astype(f32)
+
+
 def config4(nshots=48, f0=5.0, seed=0):
     """End-to-end FWI: 48 shots, 600 x 2000, h = 10 m, nt = 4000 (dt = 1 ms); true = depth
     gradient + seeded lens anomalies, start = the true model smoothed over 500 m; Ricker

@@ -253,16 +253,31 @@ typedef struct {
     const float* H0;    /* initial diagonal inverse Hessian (n values > 0), or NULL for 1 */
 } fwi2d_lbfgs_options;
 
+/* What the reference prints about a run at verbose=1 (nlsolver.cpp:1262-1346 and the
+   trial blocks of regular_wolfe::lineSearch / zoom, nlsolver.cpp:236-311, 350-420):
+   the evaluation counts, as lbfgs::run counts them (a function evaluation per objective
+   call, a gradient evaluation for the starting model and each trial whose gradient the
+   search reads), and one row per line-search trial. */
+typedef struct {
+    int feval, geval;   /* output: function / gradient evaluations */
+    int ntrials;        /* output: trials made (rows beyond trials_cap are not stored) */
+    int trials_cap;     /* input: rows available in trials */
+    double* trials;     /* input: [trials_cap][5] or NULL; each row: iteration (from 0), trial
+                           (from 1), step length, function value, gradient norm (NaN when the
+                           trial's gradient was not read: "Last gradient norm" in the reference) */
+} fwi2d_lbfgs_report;
+
 /* lbfgs::run (src/cpp/nlsolver.cpp:1207-1415) with regular_wolfe::lineSearch / zoom
    (nlsolver.cpp:184-421) on model m (n values, updated in place).  Every evaluation
    returns functional and gradient together, as nlls_fwi_eco's res() does.
    func_history (niter + 1 values, or NULL) receives the functional before the first
-   iteration and after each accepted step; *iterations the accepted steps.  Stops, as
-   the reference, after niter iterations, a relative decrease below threshold, a
-   functional or gradient norm below 1e-10, a failed line search (the model is then the
-   last trial) or a search direction that is not a descent direction. */
+   iteration and after each accepted step; *iterations the accepted steps; report (or
+   NULL) the evaluation counts and the trials.  Stops, as the reference, after niter
+   iterations, a relative decrease below threshold, a functional or gradient norm below
+   1e-10, a failed line search (the model is then the last trial) or a search direction
+   that is not a descent direction. */
 int fwi2d_lbfgs(fwi2d_objective_fn fn, void* ctx, long long n, float* m, const fwi2d_lbfgs_options* opt,
-                double* func_history, int* iterations);
+                double* func_history, int* iterations, fwi2d_lbfgs_report* report);
 
 #ifdef __cplusplus
 }

@@ -67,6 +67,30 @@ int orc_num_threads(void) { return omp_get_max_threads(); }
 int orc_num_threads(void) { return 1; }
 #endif
 
+/* Subnormal numbers flushed to zero (x86 MXCSR FTZ and DAZ) on the calling thread and
+   every thread of the OpenMP pool.  In single precision the decaying tails of long
+   runs become subnormal and slow the CPU many times over; flushing changes values only
+   below 1.2e-38.  Returns the number of threads set. */
+#if defined(__x86_64__) || defined(__i386__)
+#include <xmmintrin.h>
+int orc_flush_subnormals(void)
+{
+    int n = 1;
+    _mm_setcsr(_mm_getcsr() | 0x8040);
+#ifdef _OPENMP
+#pragma omp parallel
+    {
+        _mm_setcsr(_mm_getcsr() | 0x8040);
+#pragma omp single
+        n = omp_get_num_threads();
+    }
+#endif
+    return n;
+}
+#else
+int orc_flush_subnormals(void) { return 0; }
+#endif
+
 /* ------------------------------------------------------------------------- */
 /* SBP coefficient tables                                                      */
 /* ------------------------------------------------------------------------- */
@@ -725,10 +749,12 @@ int orc_rhs(const orc_grid* g, const real* model, const real* curr, const real* 
 
 /* The reference images with the adjoint derivatives of the previous adjoint step
    (we_op.cpp:680 runs before they are recomputed at 697-717).  Tests may switch this
-   lag off to check that the lambda and rho gradients are then the exact discrete
-   gradient; the default (1) is the reference behaviour. */
+   lag off (0) to check that the lambda and rho gradients are then the exact discrete
+   gradient, or reverse it (-1: the derivatives of the next adjoint level, the adjoint
+   value of the current one) to check that the lag's error changes sign with its
+   direction; the default (1) is the reference behaviour. */
 static int imaging_lag = 1;
-void orc_set_imaging_lag(int on) { imaging_lag = on; }
+void orc_set_imaging_lag(int lag) { imaging_lag = lag; }
 
 static void adjoint_derivatives(const orc_grid* g, const real* curr, real* ux, real* uzd)
 {
@@ -757,7 +783,7 @@ int orc_propagate(const orc_grid* g, const real* model,
     if (nx < 12 || nz < 12 || nt < 2) return 1;
     if (grad && (sub <= 0 || !full)) return 2;
 
-    real* work = (real*)calloc((size_t)nxz * 24, sizeof(real));
+    real* work = (real*)calloc((size_t)nxz * 26, sizeof(real));
     if (!work) return 3;
     real* prev = work;              /* prev, curr, next: 3 x [2][nxz] */
     real* curr = work + 2 * nxz;
@@ -768,6 +794,7 @@ int orc_propagate(const orc_grid* g, const real* model,
     real* gtmp = work + 11 * nxz;   /* 4 fields for the imaging condition */
     coef_fields c;
     coef_init(g, model, work + 15 * nxz, &c);   /* 9 fields */
+    real* lead_curr = work + 24 * nxz;          /* 2 fields: adjoint level imaged after the step (lag -1) */
     const real* rho = model + 2 * nxz;
 
     /* initial condition: prev = 1/2 dt^2/rho * inject(src[0])  (we_op.cpp:655-663) */
@@ -785,10 +812,12 @@ int orc_propagate(const orc_grid* g, const real* model,
             extract(curr, outx, nt, it, next_, exi, ezi, exw, ezw, nz);
             extract(curr + nxz, outz, nt, it, next_, exi, ezi, exw, ezw, nz);
         }
-        if (grad && (nt - 1 - it) % sub == 0 && it != 0) {
+        const int image_now = grad && (nt - 1 - it) % sub == 0 && it != 0;
+        if (image_now && imaging_lag >= 0) {
             if (!imaging_lag) adjoint_derivatives(g, curr, ux, uzd);
             compute_gradients(g, model, full, curr, ux, uzd, gtmp, grad, (nt - 1 - it) / sub, sub * dt);
         }
+        if (image_now && imaging_lag < 0) memcpy(lead_curr, curr, 2 * nxz * sizeof(real));
 
         real* cx = curr;
         real* cz = curr + nxz;
@@ -815,6 +844,10 @@ int orc_propagate(const orc_grid* g, const real* model,
         orc_taper(cz, nx, nz, g->taper_top, g->taper_bottom, g->taper_left, g->taper_right, g->taper_strength);
         orc_taper(nxt, nx, nz, g->taper_top, g->taper_bottom, g->taper_left, g->taper_right, g->taper_strength);
         orc_taper(nzt, nx, nz, g->taper_top, g->taper_bottom, g->taper_left, g->taper_right, g->taper_strength);
+        if (image_now && imaging_lag < 0) {
+            adjoint_derivatives(g, next, ux, uzd);
+            compute_gradients(g, model, full, lead_curr, ux, uzd, gtmp, grad, (nt - 1 - it) / sub, sub * dt);
+        }
 
         real* b = prev; prev = curr; curr = next; next = b;
     }
@@ -826,7 +859,7 @@ int orc_propagate(const orc_grid* g, const real* model,
         extract(curr + nxz, outz, nt, nt - 1, next_, exi, ezi, exw, ezw, nz);
     }
     if (grad) {
-        if (!imaging_lag) adjoint_derivatives(g, curr, ux, uzd);
+        if (imaging_lag <= 0) adjoint_derivatives(g, curr, ux, uzd);
         compute_gradients(g, model, full, curr, ux, uzd, gtmp, grad, 0, sub * dt);
     }
     free(work);

@@ -112,3 +112,73 @@ def surface_wave_speed(uz, dt, c_expected, f0, offsets=(1.0, 2.0)):
     y0, y1, y2 = cc[k - 1], cc[k], cc[k + 1]
     lag = (k - (nt - 1) + 0.5 * (y0 - y2) / (y0 - 2 * y1 + y2)) * dt
     return (offsets[1] - offsets[0]) / lag, np.abs(a2).max() / np.abs(a1).max()
+
+
+def christoffel_vti(c11, c13, c33, c55, rho, theta):
+    """Phase velocities (qP, qSV) of a plane wave in a VTI medium whose normal makes the
+    angle theta (radians) with the symmetry axis z: rho v^2 are the eigenvalues of the
+    Christoffel matrix [[c11 s^2 + c55 c^2, (c13 + c55) s c], [(c13 + c55) s c, c55 s^2 + c33 c^2]]
+    with s = sin(theta), c = cos(theta)."""
+    s, c = np.sin(theta), np.cos(theta)
+    G = np.array([[c11 * s * s + c55 * c * c, (c13 + c55) * s * c], [(c13 + c55) * s * c, c55 * s * s + c33 * c * c]])
+    ev = np.linalg.eigvalsh(G)
+    return np.sqrt(ev[1] / rho), np.sqrt(ev[0] / rho)
+
+
+VTI_PLANE = dict(vp0=2.0, vs0=1.0, rho=2.0, eps=0.2, delta=0.1)
+
+
+def plane_wave_case(theta_deg, mode, n=481, h=0.005, nt=1000, dt=0.0008, f0=10.0, d=(0.2, 0.5), vti=VTI_PLANE):
+    """A plane wave from a line of point forces through the centre of a homogeneous VTI
+    square (absorbing sides), its normal n at theta_deg from the z axis (towards +x),
+    forces along n (mode "P") or along the line (mode "SV"); two receivers on the normal
+    d km from the line.  Returns prob, converted model (lambda, mu, rho, c13, eps), source
+    positions [k][2], receivers [2][2], source traces [2][k][nt], the polarisation vector
+    to project on, and the expected phase velocity of the mode (Christoffel)."""
+    p = vti
+    lam, mu = p["rho"] * (p["vp0"] ** 2 - 2 * p["vs0"] ** 2), p["rho"] * p["vs0"] ** 2
+    c13 = np.sqrt(2 * (lam + 2 * mu) * (lam + mu) * p["delta"] + (lam + mu) ** 2) - mu
+    c11, c33, c55 = (1 + 2 * p["eps"]) * (lam + 2 * mu), lam + 2 * mu, mu
+    th = np.radians(theta_deg)
+    vP, vS = christoffel_vti(c11, c13, c33, c55, p["rho"], th)
+    model = np.stack([np.full((n, n), v) for v in (lam, mu, p["rho"], c13, p["eps"])])
+    taper = 40
+    prob = dict(nx=n, nz=n, dx=h, dz=h, bc_top=2, bc_bottom=2, bc_left=2, bc_right=2, taper_top=taper,
+                taper_bottom=taper, taper_left=taper, taper_right=taper, taper_strength=0.05, vti=1,
+                gdamp_tmin=0.0, gdamp_tmax=0.0, gdamp_power=1.0)
+    L = (n - 1) * h
+    c = np.array([L / 2, L / 2])
+    nv = np.array([np.sin(th), np.cos(th)])
+    tv = np.array([np.cos(th), -np.sin(th)])
+    lo, hi = 3 * h, L - 3 * h
+    pts = [c + s * tv for s in np.arange(-L, L, h)]
+    pts = np.array([q for q in pts if lo <= q[0] <= hi and lo <= q[1] <= hi])
+    t = np.arange(nt) * dt
+    a = (np.pi * f0 * (t - 1.2 / f0)) ** 2
+    wav = (1.0 - 2.0 * a) * np.exp(-a)
+    fdir = nv if mode == "P" else tv
+    src = np.stack([np.tile(wav * fdir[0], (len(pts), 1)), np.tile(wav * fdir[1], (len(pts), 1))])
+    rxz = np.array([c + di * nv for di in d])
+    v = vP if mode == "P" else vS
+    return prob, model, pts, rxz, src, fdir, v
+
+
+def plane_wave_speed(traces, pol, dt, v_expected, f0=10.0, d=(0.2, 0.5)):
+    """Phase velocity between the two receivers of plane_wave_case: the displacement
+    projected on pol, each trace windowed (Hann, +/- 1.1/f0) around the expected arrival
+    of the mode, the lag of the cross-correlation peak refined by a parabola."""
+    u = pol[0] * traces[0] + pol[1] * traces[1]
+    nt = u.shape[-1]
+    t = np.arange(nt) * dt
+
+    def win(tr, x):
+        m = np.abs(t - (x / v_expected + 1.2 / f0)) < 1.1 / f0
+        out = np.zeros(nt)
+        out[m] = tr[m] * np.hanning(m.sum())
+        return out
+    a1, a2 = win(u[0], d[0]), win(u[1], d[1])
+    cc = np.correlate(a2, a1, "full")
+    k = int(np.argmax(cc))
+    y0, y1, y2 = cc[k - 1], cc[k], cc[k + 1]
+    lag = (k - (nt - 1) + 0.5 * (y0 - y2) / (y0 - 2 * y1 + y2)) * dt
+    return (d[1] - d[0]) / lag

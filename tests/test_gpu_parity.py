@@ -497,6 +497,24 @@ def test_engine_matches_greens_function(lib, vti):
         assert err < 2e-3
 
 
+@pytest.mark.parametrize("theta,mode", [(0, "P"), (90, "P"), (45, "P"), (45, "SV")])
+def test_engine_vti_plane_wave_phase_velocity(lib, theta, mode):
+    """Physics pin of the engine's VTI terms with eps, delta != 0: a plane wave from a line
+    of point forces travels at the Christoffel phase velocity of its direction (c33 along
+    z, c11 along x, c13 at 45 degrees), within 2e-3.  Every point of the line is a shot of
+    one batch and the receiver traces are summed over shots (superposition)."""
+    from analytic import plane_wave_case, plane_wave_speed
+    prob, model, pts, rxz, src, pol, v = plane_wave_case(theta, mode)
+    nt, dt = src.shape[-1], 0.0008
+    k = len(pts)
+    _, out = run_engine(lib, prob, model, pts, [rxz] * k, src.astype(np.float32), nt, dt)
+    traces = out.astype(np.float64).reshape(2, k, 2, nt).sum(axis=1)
+    vm = plane_wave_speed(traces, pol, dt, v)
+    print("theta %d %s (%d shots): phase velocity %.5f, Christoffel %.5f (rel %.2e)"
+          % (theta, mode, k, vm, v, vm / v - 1))
+    assert abs(vm / v - 1) < 2e-3
+
+
 def test_engine_free_surface_rayleigh_wave(lib):
     """Physics pin of the engine's free surface: the surface pulse of a vertical surface
     force travels at the root of the Rayleigh equation (to 3e-3) without spreading."""
