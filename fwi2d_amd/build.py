@@ -17,7 +17,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libfwi2d_hip.so")
 
 HIP_SOURCES = ["kernels.hip", "engine.hip"]
-CXX_SOURCES = ["host_op.cpp", "solver.cpp"]
+CXX_SOURCES = ["host_op.cpp", "solver.cpp", "io.cpp"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
 HIP_FLAGS = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstring>
 #include <exception>
+#include <memory>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -282,8 +283,104 @@ void parallel_traces(size_t n, F&& f)
 
 // per-thread scratch of the objective's trace loop
 struct TraceWork {
-    std::vector<double> v, s;
+    std::vector<double> v, s, c;
 };
+
+// conv1dnd (operator.hpp:1035-1057; operator.cpp:2092-2137) of one trace of n samples
+// with the filter f: y[t] = sum_j f[j] x[t - j + sh] over the j that keep t - j + sh in
+// the trace (sh = floor((nf + 1)/2) - 1 centred, else 0); the adjoint scatters
+// y[t] f[j] to x[t - j + sh].  x and y must not overlap.
+void conv_trace(const std::vector<double>& f, bool centered, bool adjoint, const double* x, double* y, int n)
+{
+    const int nf = (int)f.size();
+    const int sh = centered ? (nf + 1) / 2 - 1 : 0;
+    std::fill(y, y + n, 0.0);
+    for (int j = 0; j < nf; j++) {
+        const int k = sh - j;  // forward: y[t] takes x[t + k]
+        const int lo = std::max(0, -k), hi = std::min(n, n - k);
+        const double fj = f[j];
+        if (!adjoint)
+            for (int t = lo; t < hi; t++) y[t] += fj * x[t + k];
+        else
+            for (int t = lo; t < hi; t++) y[t + k] += fj * x[t];
+    }
+}
+
+// Real-input discrete Fourier transform of x (n values) onto its n/2 + 1 non-negative
+// frequencies, and the inverse from those frequencies (with 1/n, the imaginary part of
+// frequency 0 ignored, as FFTW's r2c / c2r pair that fxTransform uses, operator.cpp:
+// 858-977).  Plain O(n^2) sums in double precision: they run once per filter.
+void dft_r2c(const std::vector<double>& x, std::vector<double>& re, std::vector<double>& im)
+{
+    const int n = (int)x.size(), nf = n / 2 + 1;
+    re.assign(nf, 0.0);
+    im.assign(nf, 0.0);
+    for (int k = 0; k < nf; k++) {
+        double a = 0, b = 0;
+        for (int t = 0; t < n; t++) {
+            const double ph = 2 * kPi * (double)(((long long)k * t) % n) / n;
+            a += x[t] * std::cos(ph);
+            b -= x[t] * std::sin(ph);
+        }
+        re[k] = a;
+        im[k] = b;
+    }
+}
+
+std::vector<double> dft_c2r(const std::vector<double>& re, const std::vector<double>& im, int n)
+{
+    std::vector<double> x(n);
+    const int nf = n / 2 + 1;
+    for (int t = 0; t < n; t++) {
+        double s = re[0];
+        for (int k = 1; k < nf; k++) {
+            // the conjugate-symmetric upper half counts twice, except a Nyquist frequency
+            const double m = (2 * k == n) ? 1.0 : 2.0;
+            const double ph = 2 * kPi * (double)(((long long)k * t) % n) / n;
+            s += m * (re[k] * std::cos(ph) - im[k] * std::sin(ph));
+        }
+        x[t] = s / n;
+    }
+    return x;
+}
+
+// The filter as nlls_fwi_eco's constructor prepares it (optimization.hpp:526-534), padded
+// to odd length for the zero and minimum phases: zero_phase (operator.cpp:2197-2250),
+// the amplitude spectrum with the linear phase exp(-i w (n-1)/2); minimum_phase
+// (operator.cpp:2252-2320), exp(FT(causal fold of the real cepstrum of log(|FT f| + eps)))
+// with eps = 1e-7.
+std::vector<double> prepare_filter(const float* f, int nf, int phase)
+{
+    std::vector<double> x(f, f + nf);
+    if (phase == FWI2D_FILTER_DEFAULT) return x;
+    if (nf % 2 == 0) x.push_back(0.0);
+    const int n = (int)x.size(), nh = n / 2 + 1;
+    std::vector<double> re, im;
+    dft_r2c(x, re, im);
+    if (phase == FWI2D_FILTER_ZERO) {
+        for (int k = 0; k < nh; k++) {
+            const double r = std::hypot(re[k], im[k]);
+            const double ph = -k * (2 * kPi / n) * (n - 1) / 2;
+            re[k] = r * std::cos(ph);
+            im[k] = r * std::sin(ph);
+        }
+        return dft_c2r(re, im, n);
+    }
+    for (int k = 0; k < nh; k++) {
+        re[k] = std::log(std::hypot(re[k], im[k]) + 1e-7);
+        im[k] = 0.0;
+    }
+    std::vector<double> c = dft_c2r(re, im, n);
+    for (int k = 1; k < nh; k++) c[k] *= 2;
+    for (int k = nh; k < n; k++) c[k] = 0.0;
+    dft_r2c(c, re, im);
+    for (int k = 0; k < nh; k++) {
+        const double a = std::exp(re[k]);
+        re[k] = a * std::cos(im[k]);
+        im[k] = a * std::sin(im[k]);
+    }
+    return dft_c2r(re, im, n);
+}
 
 }  // namespace
 
@@ -447,9 +544,9 @@ struct fwi2d_op {
 };
 
 // Least-squares FWI objective on the operator: nlls_fwi (optimization.hpp:295-455) and
-// nlls_fwi_eco (optimization.hpp:459-951) with residual weights, time integration, trace
-// normalisation, a gradient mask and nlls_fwi_eco's data normalisation (no envelope,
-// filter, interferometry, double difference or source rescaling).  Both shot loops
+// nlls_fwi_eco (optimization.hpp:459-951) with residual weights, time integration, a 1-D
+// filter, trace normalisation, a gradient mask and nlls_fwi_eco's data normalisation (no
+// envelope, interferometry, double difference or source rescaling).  Both shot loops
 // become two batched phases: one apply_forward over every shot, the residuals, one
 // apply_jacobianT(add=false) over every shot.  That is the sum of nlls_fwi_eco's per-shot
 // apply_jacobianT(add=true) calls (optimization.hpp:890), without the CPU build's
@@ -461,6 +558,8 @@ struct fwi2d_nlls {
     std::vector<double> d;     // processed observed data: integrated, weighted, normalised
     std::vector<float> w;      // residual weights, empty: none
     std::vector<float> gmask;  // gradient mask, empty: none
+    std::vector<double> filt;  // prepared filter, empty: none
+    bool filt_centered = true;
     std::vector<float> r;      // residual of the last evaluation
     double dnorm = 1.0;        // nlls_fwi_eco's data norm (optimization.hpp:569-574), else 1
     size_t ndata = 0, nmodel = 0;
@@ -498,15 +597,21 @@ struct fwi2d_nlls {
         return s;
     }
 
-    // the data processing of the constructors: integrate, weight, normalise
+    // the data processing of the constructors (optimization.hpp:512-542): integrate,
+    // weight, filter, normalise
     void process(std::vector<double>& u) const
     {
         const int n = nt();
+        std::vector<double> c(filt.empty() ? 0 : n);
         for (size_t x = 0; x < ntr(); x++) {
             double* p = u.data() + x * n;
             if (integrate) integral_trace(p, false);
             if (!w.empty())
                 for (int t = 0; t < n; t++) p[t] *= w[x * n + t];
+            if (!filt.empty()) {
+                conv_trace(filt, filt_centered, false, p, c.data(), n);
+                std::copy(c.begin(), c.end(), p);
+            }
             if (trace_norm) normalize_trace(p);
         }
     }
@@ -525,7 +630,7 @@ struct fwi2d_nlls {
         // res (optimization.hpp:346-374, 660-776): r = P(F(m)) - P(d), P the processing;
         // the trace's share of getFunc (optimization.hpp:375-385, 929-942):
         // |r|^2 - 1/2 (r[0]^2 + r[n-1]^2); and the adjoint source of grad
-        // (optimization.hpp:390-455, 780-905): I^T W N'^T Ht r, with
+        // (optimization.hpp:390-455, 780-905): I^T W C^T N'^T Ht r, with
         // N' = I/|s| - s s^T/|s|^3 the Jacobian of the trace normalisation at s
         parallel_traces<TraceWork>(T, [&](size_t x, TraceWork& ws) {
             std::vector<double>& v = ws.v;
@@ -535,6 +640,11 @@ struct fwi2d_nlls {
             if (integrate) integral_trace(v.data(), false);
             if (wx)
                 for (int t = 0; t < n; t++) v[t] *= wx[t];
+            if (!filt.empty()) {
+                ws.c.resize(n);
+                conv_trace(filt, filt_centered, false, v.data(), ws.c.data(), n);
+                v.swap(ws.c);
+            }
             double nrm = 1.0;
             if (trace_norm) {
                 s = v;
@@ -554,6 +664,11 @@ struct fwi2d_nlls {
                 double vs = 0;
                 for (int t = 0; t < n; t++) vs += v[t] * s[t];
                 for (int t = 0; t < n; t++) v[t] = v[t] / nrm - vs / (nrm * nrm * nrm) * s[t];
+            }
+            if (!filt.empty()) {
+                ws.c.resize(n);
+                conv_trace(filt, filt_centered, true, v.data(), ws.c.data(), n);
+                v.swap(ws.c);
             }
             if (wx)
                 for (int t = 0; t < n; t++) v[t] *= wx[t];
@@ -662,10 +777,19 @@ void fwi2d_op_destroy(fwi2d_op* op) { delete op; }
 int fwi2d_nlls_create(fwi2d_op* op, const float* d, const float* w, const float* gmask, int flags,
                       fwi2d_nlls** out)
 {
+    return fwi2d_nlls_create_filtered(op, d, w, gmask, flags, nullptr, 0, FWI2D_FILTER_DEFAULT, out);
+}
+
+int fwi2d_nlls_create_filtered(fwi2d_op* op, const float* d, const float* w, const float* gmask, int flags,
+                               const float* filter, int nf, int filter_phase, fwi2d_nlls** out)
+{
     return op_guard([&] {
         if (!op || !d || !out) throw std::runtime_error("null argument");
         if (flags & ~(FWI2D_NLLS_NORMALIZE_OBJ | FWI2D_NLLS_INTEGRATE | FWI2D_NLLS_TRACE_NORMALIZE))
             throw std::runtime_error("unknown objective flags");
+        if (filter && nf < 1) throw std::runtime_error("the filter needs at least one sample");
+        if (filter_phase < FWI2D_FILTER_DEFAULT || filter_phase > FWI2D_FILTER_MINIMUM)
+            throw std::runtime_error("unknown filter phase");
         const size_t ndata = (size_t)2 * op->nr * op->d.nt_data, nmodel = (size_t)op->np() * op->nxz();
         auto nonneg = [](const float* p, size_t n) {
             for (size_t i = 0; i < n; i++)
@@ -674,7 +798,7 @@ int fwi2d_nlls_create(fwi2d_op* op, const float* d, const float* w, const float*
         };
         if (w && !nonneg(w, ndata)) throw std::runtime_error("Data weights  must be non-negative");
         if (gmask && !nonneg(gmask, nmodel)) throw std::runtime_error("Gradient mask must be non-negative");
-        fwi2d_nlls* f = new fwi2d_nlls;
+        std::unique_ptr<fwi2d_nlls> f(new fwi2d_nlls);
         f->op = op;
         f->ndata = ndata;
         f->nmodel = nmodel;
@@ -682,6 +806,12 @@ int fwi2d_nlls_create(fwi2d_op* op, const float* d, const float* w, const float*
         f->trace_norm = (flags & FWI2D_NLLS_TRACE_NORMALIZE) != 0;
         if (w) f->w.assign(w, w + ndata);
         if (gmask) f->gmask.assign(gmask, gmask + nmodel);
+        if (filter) {
+            for (int i = 0; i < nf; i++)
+                if (!std::isfinite(filter[i])) throw std::runtime_error("the filter contains a non-finite value");
+            f->filt = prepare_filter(filter, nf, filter_phase);
+            f->filt_centered = filter_phase != FWI2D_FILTER_MINIMUM;
+        }
         f->d.assign(d, d + ndata);
         f->process(f->d);
         if (flags & FWI2D_NLLS_NORMALIZE_OBJ) {
@@ -690,7 +820,31 @@ int fwi2d_nlls_create(fwi2d_op* op, const float* d, const float* w, const float*
             s *= op->d.dt_data;
             f->dnorm = s < 1e-16 ? 1.0 : s;
         }
-        *out = f;
+        *out = f.release();
+    });
+}
+
+int fwi2d_filter_prepare(const float* filter, int nf, int filter_phase, double* out, int* nout)
+{
+    return op_guard([&] {
+        if (!filter || !out || !nout) throw std::runtime_error("null argument");
+        if (nf < 1) throw std::runtime_error("the filter needs at least one sample");
+        if (filter_phase < FWI2D_FILTER_DEFAULT || filter_phase > FWI2D_FILTER_MINIMUM)
+            throw std::runtime_error("unknown filter phase");
+        const std::vector<double> f = prepare_filter(filter, nf, filter_phase);
+        std::copy(f.begin(), f.end(), out);
+        *nout = (int)f.size();
+    });
+}
+
+int fwi2d_filter_apply(const double* filter, int nf, int centered, int adjoint, long long ntr, int nt,
+                       const double* in, double* out)
+{
+    return op_guard([&] {
+        if (!filter || !in || !out) throw std::runtime_error("null argument");
+        if (nf < 1 || nt < 1 || ntr < 0) throw std::runtime_error("invalid sizes");
+        const std::vector<double> f(filter, filter + nf);
+        for (long long x = 0; x < ntr; x++) conv_trace(f, centered != 0, adjoint != 0, in + x * nt, out + x * nt, nt);
     });
 }
 

@@ -46,8 +46,14 @@ EXPORTS = [
     "fwi2d_engine_get_stat", "fwi2d_engine_tune_report", "fwi2d_engine_destroy", "fwi2d_op_create", "fwi2d_op_apply_forward",
     "fwi2d_op_apply_jacobianT", "fwi2d_op_engine", "fwi2d_op_time_axis", "fwi2d_op_destroy",
     "fwi2d_nlls_create", "fwi2d_nlls_evaluate", "fwi2d_nlls_residual", "fwi2d_nlls_dnorm", "fwi2d_nlls_destroy",
-    "fwi2d_nlls_objective", "fwi2d_lbfgs",
+    "fwi2d_nlls_objective", "fwi2d_lbfgs", "fwi2d_nlls_create_filtered", "fwi2d_filter_prepare",
+    "fwi2d_filter_apply", "fwi2d_read_header", "fwi2d_read_data", "fwi2d_write",
+    "fwi2d_read_coord", "fwi2d_regular_geometry",
 ]
+
+FORMAT_SEP, FORMAT_BINARY = 0, 1
+
+FILTER_PHASES = {"default": 0, "zero": 1, "minimum": 2}
 
 OBJECTIVE_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_double), C.POINTER(C.c_float))
 
@@ -61,6 +67,11 @@ class LbfgsOptions(C.Structure):
 class LbfgsReport(C.Structure):
     _fields_ = [("feval", C.c_int), ("geval", C.c_int), ("ntrials", C.c_int), ("trials_cap", C.c_int),
                 ("trials", C.c_void_p)]
+
+
+class Hypercube(C.Structure):
+    _fields_ = [("ndim", C.c_int), ("n", C.c_longlong * 6), ("o", C.c_double * 6), ("d", C.c_double * 6),
+                ("label", (C.c_char * 64) * 6), ("esize", C.c_int), ("xdr", C.c_int), ("in_", C.c_char * 4096)]
 
 
 class Fwi2dError(RuntimeError):
@@ -97,12 +108,20 @@ def load():
     lib.fwi2d_op_time_axis.argtypes = [P, C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_int)]
     lib.fwi2d_op_destroy.argtypes = [P]
     lib.fwi2d_nlls_create.argtypes = [P, P, P, P, C.c_int, C.POINTER(P)]
+    lib.fwi2d_nlls_create_filtered.argtypes = [P, P, P, P, C.c_int, P, C.c_int, C.c_int, C.POINTER(P)]
+    lib.fwi2d_filter_prepare.argtypes = [P, C.c_int, C.c_int, P, C.POINTER(C.c_int)]
+    lib.fwi2d_filter_apply.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_longlong, C.c_int, P, P]
     lib.fwi2d_nlls_evaluate.argtypes = [P, P, C.POINTER(C.c_double), P]
     lib.fwi2d_nlls_residual.argtypes = [P]
     lib.fwi2d_nlls_residual.restype = C.POINTER(C.c_float)
     lib.fwi2d_nlls_dnorm.argtypes = [P]
     lib.fwi2d_nlls_dnorm.restype = C.c_double
     lib.fwi2d_nlls_destroy.argtypes = [P]
+    lib.fwi2d_read_header.argtypes = [C.c_char_p, C.c_int, C.c_int, C.POINTER(Hypercube)]
+    lib.fwi2d_read_data.argtypes = [C.POINTER(Hypercube), P]
+    lib.fwi2d_write.argtypes = [C.c_char_p, C.c_int, C.POINTER(Hypercube), P, C.c_char_p]
+    lib.fwi2d_read_coord.argtypes = [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_longlong), P, P, P]
+    lib.fwi2d_regular_geometry.argtypes = [C.c_int] + [C.c_float] * 4 + [C.c_int] + [C.c_float] * 6 + [P, P]
     lib.fwi2d_lbfgs.argtypes = [C.c_void_p, P, C.c_longlong, P, C.POINTER(LbfgsOptions), P, C.POINTER(C.c_int),
                                 C.POINTER(LbfgsReport)]
     _lib = lib
@@ -278,19 +297,24 @@ NLLS_NORMALIZE_OBJ, NLLS_INTEGRATE, NLLS_TRACE_NORMALIZE = 1, 2, 4
 
 class Nlls:
     """nlls_fwi (normalize=False) / nlls_fwi_eco (normalize=True: the functional divided
-    by dt |P d|^2) on a WeOp, with residual weights, time integration and trace
+    by dt |P d|^2) on a WeOp, with residual weights, time integration, a 1-D filter
+    (filter_phase "default", "zero" or "minimum", as par.filter_phase) and trace
     normalisation as options: evaluate(m) returns the functional (FP64) and the gradient."""
 
-    def __init__(self, op, d, w=None, gmask=None, normalize=False, integrate=False, trace_normalize=False):
+    def __init__(self, op, d, w=None, gmask=None, normalize=False, integrate=False, trace_normalize=False,
+                 filter=None, filter_phase="default"):
         self.op = op  # the objective uses the operator handle
         self._d = _f32(d)
         assert self._d.shape == (2, op.nr, op.nt_data), self._d.shape
         w = None if w is None else _f32(w)
         gmask = None if gmask is None else _f32(gmask)
+        filt = None if filter is None else _f32(filter).ravel()
         self._h = C.c_void_p()
         flags = (NLLS_NORMALIZE_OBJ * bool(normalize) | NLLS_INTEGRATE * bool(integrate)
                  | NLLS_TRACE_NORMALIZE * bool(trace_normalize))
-        _check(load().fwi2d_nlls_create(op._h, _ptr(self._d), _ptr(w), _ptr(gmask), flags, C.byref(self._h)))
+        _check(load().fwi2d_nlls_create_filtered(op._h, _ptr(self._d), _ptr(w), _ptr(gmask), flags, _ptr(filt),
+                                                 0 if filt is None else filt.size, FILTER_PHASES[filter_phase],
+                                                 C.byref(self._h)))
 
     def close(self):
         if self._h:
@@ -314,6 +338,26 @@ class Nlls:
 
     def dnorm(self):
         return load().fwi2d_nlls_dnorm(self._h)
+
+
+def filter_prepare(f, phase="default"):
+    """The objective's filter preparation (zero_phase / minimum_phase): (filter, centred)."""
+    f = _f32(f).ravel()
+    out = np.zeros(f.size + 1)
+    n = C.c_int()
+    _check(load().fwi2d_filter_prepare(_ptr(f), f.size, FILTER_PHASES[phase], _ptr(out), C.byref(n)))
+    return out[:n.value].copy(), phase != "minimum"
+
+
+def filter_apply(f, traces, centered=True, adjoint=False):
+    """conv1dnd forward (or adjoint) of every trace (last axis) with the filter f, in double precision."""
+    f = np.ascontiguousarray(f, np.float64).ravel()
+    x = np.ascontiguousarray(traces, np.float64)
+    out = np.zeros_like(x)
+    nt = x.shape[-1]
+    _check(load().fwi2d_filter_apply(_ptr(f), f.size, int(centered), int(adjoint), x.size // nt, nt, _ptr(x),
+                                     _ptr(out)))
+    return out
 
 
 def lbfgs(objective, m0, niter=10, max_trial=10, threshold=0.0, memory=5, c1=1e-4, c2=0.9, a0=1.0, a1=0.0,
@@ -360,3 +404,70 @@ def lbfgs(objective, m0, niter=10, max_trial=10, threshold=0.0, memory=5, c1=1e-
         info = dict(feval=rep.feval, geval=rep.geval, trials=rows[:min(rep.ntrials, cap)].copy())
         return m, hist[:k + 1], k, info
     return m, hist[:k + 1], k
+
+
+# ---------------------------------------------------------------------------
+# On-disk formats (host only)
+# ---------------------------------------------------------------------------
+def read_header(path, format=FORMAT_SEP, ndim0=1):
+    """The header of a SEPlib (format 0) or binary-descriptor (format 1) file, as a dict
+    with n, o, d, label (one entry per kept axis, axis 1 first), esize, xdr and in."""
+    h = Hypercube()
+    _check(load().fwi2d_read_header(os.fsencode(path), int(format), int(ndim0), C.byref(h)))
+    nd = h.ndim
+    return dict(n=[h.n[i] for i in range(nd)], o=[h.o[i] for i in range(nd)], d=[h.d[i] for i in range(nd)],
+                label=[h.label[i].value.decode() for i in range(nd)], esize=h.esize, xdr=h.xdr,
+                in_=h.in_.decode(), _h=h)
+
+
+def read(path, format=FORMAT_SEP, ndim0=1):
+    """read<data_t>(file, format) of the reference: (array, header) with the array in
+    C order [nN]...[n1] (axis 1 fastest) and single precision."""
+    hd = read_header(path, format, ndim0)
+    h = hd.pop("_h")
+    n = [h.n[i] for i in range(6)]
+    out = np.zeros(int(np.prod(n)), np.float32)
+    _check(load().fwi2d_read_data(C.byref(h), _ptr(out)))
+    return out.reshape(hd["n"][::-1]), hd
+
+
+def write(path, data, o=None, d=None, label=None, format=FORMAT_SEP, datapath=None):
+    """write<data_t>(vec, file, format) of the reference: data in C order [nN]...[n1]
+    (axis 1 fastest), o, d, label per axis from axis 1 (defaults 0, 1, none)."""
+    a = _f32(data)
+    n = list(a.shape[::-1]) or [1]
+    if len(n) > 6:
+        raise ValueError("at most 6 axes")
+    h = Hypercube()
+    h.ndim = len(n)
+    for i, ni in enumerate(n):
+        h.n[i] = ni
+        h.o[i] = 0.0 if o is None else float(o[i])
+        h.d[i] = 1.0 if d is None else float(d[i])
+        if label is not None and label[i]:
+            h.label[i].value = str(label[i]).encode()[:63]
+    _check(load().fwi2d_write(os.fsencode(path), int(format), C.byref(h), _ptr(a),
+                              None if datapath is None else os.fsencode(datapath)))
+
+
+def read_coord(path):
+    """readCoord: (sxz [ns][2], rxz list of [nrec][2] arrays, dips list of [nrec] arrays)."""
+    lib = load()
+    ns, nr = C.c_int(0), C.c_longlong(0)
+    _check(lib.fwi2d_read_coord(os.fsencode(path), C.byref(ns), C.byref(nr), None, None, None))
+    sxz = np.zeros((ns.value, 2))
+    nrec = np.zeros(ns.value, np.int32)
+    rxz = np.zeros((nr.value, 3))
+    _check(lib.fwi2d_read_coord(os.fsencode(path), C.byref(ns), C.byref(nr), _ptr(sxz), _ptr(nrec), _ptr(rxz)))
+    off = np.concatenate([[0], np.cumsum(nrec)])
+    return sxz, [rxz[off[i]:off[i + 1], :2].copy() for i in range(ns.value)], \
+        [rxz[off[i]:off[i + 1], 2].copy() for i in range(ns.value)]
+
+
+def regular_geometry(ns, sx0, sz0, sxinc, szinc, nr, rx0, rz0, rxinc, rzinc, rdip0=0.0, rdipinc=0.0):
+    """The regular acquisition of the shot parameters: (sxz [ns][2], rxz [ns][nr][3])."""
+    sxz = np.zeros((ns, 2))
+    rxz = np.zeros((ns, nr, 3))
+    _check(load().fwi2d_regular_geometry(ns, sx0, sz0, sxinc, szinc, nr, rx0, rz0, rxinc, rzinc, rdip0, rdipinc,
+                                         _ptr(sxz), _ptr(rxz)))
+    return sxz, rxz

@@ -209,6 +209,34 @@ typedef struct fwi2d_nlls fwi2d_nlls;
 int fwi2d_nlls_create(fwi2d_op* op, const float* d, const float* w, const float* gmask, int flags,
                       fwi2d_nlls** out);
 
+/* par.filter_phase (src/include/param.hpp:58): how the objective's 1-D filter is prepared
+   before use (optimization.hpp:526-534).  DEFAULT: as given, convolved centred; ZERO:
+   zero_phase (src/cpp/operator.cpp:2197-2250), centred; MINIMUM: minimum_phase
+   (operator.cpp:2252-2320, eps 1e-7), convolved causally (not centred).  The zero and
+   minimum phases pad an even-length filter with one zero sample. */
+#define FWI2D_FILTER_DEFAULT 0
+#define FWI2D_FILTER_ZERO 1
+#define FWI2D_FILTER_MINIMUM 2
+
+/* nlls_fwi_eco's constructor with its filter argument (optimization.hpp:483, 526-539):
+   as fwi2d_nlls_create, plus a 1-D filter of nf samples on the data time axis (or NULL)
+   prepared by filter_phase.  Observed data and synthetics are convolved with it in time
+   (conv1dnd, src/include/operator.hpp:1035-1057; src/cpp/operator.cpp:2092-2137) after
+   the weights and before the trace normalisation, and the adjoint source with its
+   adjoint (optimization.hpp:688-696, 846-854). */
+int fwi2d_nlls_create_filtered(fwi2d_op* op, const float* d, const float* w, const float* gmask, int flags,
+                               const float* filter, int nf, int filter_phase, fwi2d_nlls** out);
+
+/* The filter preparation alone (host only, no device): out receives the prepared filter
+   in double precision (room for nf + 1 values), *nout its length. */
+int fwi2d_filter_prepare(const float* filter, int nf, int filter_phase, double* out, int* nout);
+
+/* conv1dnd apply_forward / apply_adjoint (operator.cpp:2092-2137) alone, host only, in
+   double precision: out = C in (adjoint: C^T in) for ntr traces of nt samples, with the
+   filter's centred (shift floor((nf + 1)/2) - 1) or causal convolution. */
+int fwi2d_filter_apply(const double* filter, int nf, int centered, int adjoint, long long ntr, int nt,
+                       const double* in, double* out);
+
 /* res + getFunc + grad at the physical model m (nlls_fwi: optimization.hpp:346-455;
    nlls_fwi_eco: compute_res_and_grad, getFunc, grad, optimization.hpp:592-951), over all
    shots in two batched phases (forward, then adjoint):
@@ -278,6 +306,60 @@ typedef struct {
    that is not a descent direction. */
 int fwi2d_lbfgs(fwi2d_objective_fn fn, void* ctx, long long n, float* m, const fwi2d_lbfgs_options* opt,
                 double* func_history, int* iterations, fwi2d_lbfgs_report* report);
+
+/* ------------------------------------------------------------------------- */
+/* On-disk formats (host only; no device)                                     */
+/* ------------------------------------------------------------------------- */
+
+/* read<T>(file, format) / write<T>(vec, file, format) (src/include/IO.hpp:432-443):
+   FWI2D_FORMAT_SEP is a SEPlib header with its binary (sepRead / sepWrite, IO.hpp:14-176),
+   FWI2D_FORMAT_BINARY the plain descriptor of binRead / binWrite (IO.hpp:180-430). */
+#define FWI2D_FORMAT_SEP 0
+#define FWI2D_FORMAT_BINARY 1
+
+/* A regular hypercube of up to 6 axes (n, o, d, label; axis 1 fastest) and the binary it
+   describes. */
+typedef struct {
+    int ndim;            /* axes kept: trailing unit axes are dropped, down to ndim0 */
+    long long n[6];
+    double o[6], d[6];
+    char label[6][64];
+    int esize;           /* bytes per element in the file: 4 or 8 */
+    int xdr;             /* 1: SEPlib data_format "xdr_*" (big-endian 4-byte words); 0: native */
+    char in[4096];       /* path of the binary */
+} fwi2d_hypercube;
+
+/* The header of `path`: n1..n6, o1..o6, d1..d6, label1..label6 (defaults 1, 0, 1, ""),
+   esize (default 4), in=, and for SEPlib data_format (default native).  A later
+   definition of a key overrides an earlier one (SEPlib header history); the binary
+   format skips lines starting with '#'.  A relative in= path that does not exist from the
+   working directory is looked up next to the header. */
+int fwi2d_read_header(const char* path, int format, int ndim0, fwi2d_hypercube* h);
+
+/* The n1 * ... * n6 values of the binary of h, converted to single precision (data_t);
+   the binary's size must equal n1 * ... * n6 * esize (IO.hpp:404-406). */
+int fwi2d_read_data(const fwi2d_hypercube* h, float* data);
+
+/* Writes the header `path` and its binary <datapath>/<name of path>@ (datapath NULL or
+   "": $DATAPATH, else the header's directory) with the ndim axes of h and n1 * ... * nN
+   single-precision values: SEPlib as data_format="xdr_float" (srite's default), the
+   binary format in native byte order (binWrite).  h->esize, h->xdr and h->in are ignored. */
+int fwi2d_write(const char* path, int format, const fwi2d_hypercube* h, const float* data, const char* datapath);
+
+/* readCoord (src/cpp/param.cpp:4-77): the source-receiver file (lines "id sx sz rx rz
+   [dip]", '#' comments; consecutive lines with the same integer id form one shot).
+   Call with sxz, nrec, rxz NULL to get *ns and *nrec_total; then with arrays of those
+   sizes (sxz [ns][2], nrec [ns], rxz [nrec_total][3] = x, z, dip) and *ns, *nrec_total
+   holding their capacities.  Values are read in single precision as the reference's
+   data_t.  Blank lines are skipped (the reference would repeat the previous line's id). */
+int fwi2d_read_coord(const char* path, int* ns, long long* nrec_total, double* sxz, int* nrec, double* rxz);
+
+/* The regular acquisition of readParameters (param.cpp:221-244) in the reference's
+   single-precision arithmetic: shot s at (sx0 + s sxinc, sz0 + s szinc), receiver r of
+   every shot at (rx0 + r rxinc, rz0 + r rzinc) with dip rdip0 + r rdipinc.
+   sxz [ns][2], rxz [ns][nr][3]. */
+int fwi2d_regular_geometry(int ns, float sx0, float sz0, float sxinc, float szinc, int nr, float rx0, float rz0,
+                           float rxinc, float rzinc, float rdip0, float rdipinc, double* sxz, double* rxz);
 
 #ifdef __cplusplus
 }

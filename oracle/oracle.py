@@ -309,18 +309,84 @@ class Oracle:
         norms = np.sqrt(n2)
         return (tr / norms[:, None]).reshape(np.shape(u)), norms
 
-    def nlls_data_terms(self, u, d, dt, w=None, integrate=False, trace_normalize=False):
+    @staticmethod
+    def conv1dnd(u, f, centered=True, adjoint=False):
+        """conv1dnd (operator.hpp:1035-1057; operator.cpp:2092-2137): every trace convolved
+        in time with the 1-D filter f, y[t] = sum_j f[j] x[t - j + sh], the terms with
+        t - j + sh outside the trace left out; sh = floor((nf + 1)/2) - 1 when centred,
+        else 0.  The adjoint scatters y[t] f[j] back to x[t - j + sh]."""
+        u = np.asarray(u, np.float64)
+        f = np.asarray(f, np.float64).ravel()
+        tr = u.reshape(-1, u.shape[-1])
+        nt, nf = tr.shape[1], f.size
+        sh = (nf + 1) // 2 - 1 if centered else 0
+        out = np.zeros_like(tr)
+        for j in range(nf):
+            k = sh - j  # y[t] takes x[t + k]
+            lo, hi = max(0, -k), min(nt, nt - k)
+            if lo >= hi:
+                continue
+            if not adjoint:
+                out[:, lo:hi] += f[j] * tr[:, lo + k:hi + k]
+            else:
+                out[:, lo + k:hi + k] += f[j] * tr[:, lo:hi]
+        return out.reshape(u.shape)
+
+    @staticmethod
+    def zero_phase(f):
+        """zero_phase (operator.cpp:2197-2250): the filter padded to odd length n, its
+        spectrum replaced by the amplitude spectrum times the linear phase
+        exp(-i w (n-1)/2), w = 2 pi k/n; a symmetric filter of the same amplitude spectrum
+        (centred by conv1dnd's shift)."""
+        f = np.asarray(f, np.float64).ravel()
+        if f.size % 2 == 0:
+            f = np.concatenate([f, [0.0]])
+        n = f.size
+        k = np.arange(n // 2 + 1)
+        X = np.abs(np.fft.rfft(f)) * np.exp(-1j * k * (2 * np.pi / n) * (n - 1) / 2)
+        return np.fft.irfft(X, n)
+
+    @staticmethod
+    def minimum_phase(f, eps=1e-7):
+        """minimum_phase (operator.cpp:2252-2320), the homomorphic construction: the filter
+        padded to odd length n; the real cepstrum c = IFT(log(|FT f| + eps)); c folded onto
+        its causal part (c_k doubled for 1 <= k <= n/2, zero above); the filter
+        IFT(exp(FT c))."""
+        f = np.asarray(f, np.float64).ravel()
+        if f.size % 2 == 0:
+            f = np.concatenate([f, [0.0]])
+        n = f.size
+        c = np.fft.irfft(np.log(np.abs(np.fft.rfft(f)) + eps) + 0j, n)
+        c[1:n // 2 + 1] *= 2
+        c[n // 2 + 1:] = 0
+        return np.fft.irfft(np.exp(np.fft.rfft(c)), n)
+
+    @classmethod
+    def prepare_filter(cls, f, phase="default"):
+        """The filter as nlls_fwi_eco's constructor prepares it (optimization.hpp:526-534):
+        (filter, centred) for par.filter_phase "default" (as given), "zero" or "minimum"
+        (not centred)."""
+        if phase == "zero":
+            return cls.zero_phase(f), True
+        if phase == "minimum":
+            return cls.minimum_phase(f), False
+        return np.asarray(f, np.float64).ravel(), True
+
+    def nlls_data_terms(self, u, d, dt, w=None, integrate=False, trace_normalize=False, filt=None, centered=True):
         """Data side of nlls_fwi / nlls_fwi_eco (optimization.hpp:330-455, 512-556, 660-905):
-        with P(u) = ttnormalize(W integral(u)) (each step optional), returns the residual
-        r = P(u) - P(d), the adjoint source a = P'(u)^T Ht r, i.e. the gradient in u of
-        1/2 r^T Ht r, and P(d).  P'^T = integral^T W (I/|s| - s s^T/|s|^3) per trace, with
-        s the weighted integrated synthetics."""
+        with P(u) = ttnormalize(C W integral(u)) (each step optional; C the filter's
+        conv1dnd), returns the residual r = P(u) - P(d), the adjoint source
+        a = P'(u)^T Ht r, i.e. the gradient in u of 1/2 r^T Ht r, and P(d).
+        P'^T = integral^T W C^T (I/|s| - s s^T/|s|^3) per trace, with s the filtered,
+        weighted, integrated synthetics."""
         def process(v):
             v = np.asarray(v, np.float64)
             if integrate:
                 v = self.integral(v, dt)
             if w is not None:
                 v = v * w
+            if filt is not None:
+                v = self.conv1dnd(v, filt, centered)
             syn, norms = v, None
             if trace_normalize:
                 v, norms = self.ttnormalize(v)
@@ -334,6 +400,8 @@ class Oracle:
             st = syn.reshape(at.shape)
             v = np.sum(at * st, axis=1)
             a = (at / norms[:, None] - (v / norms ** 3)[:, None] * st).reshape(a.shape)
+        if filt is not None:
+            a = self.conv1dnd(a, filt, centered, adjoint=True)
         if w is not None:
             a = a * w
         if integrate:
@@ -341,17 +409,21 @@ class Oracle:
         return r, a, pd
 
     def nlls_evaluate(self, prob, model, allsrc, d, w=None, gmask=None, normalize=False, gradient=True,
-                      integrate=False, trace_normalize=False):
+                      integrate=False, trace_normalize=False, filt=None, filter_phase="default"):
         """nlls_fwi (optimization.hpp:309-455) or, with normalize, nlls_fwi_eco with
         normalize_obj_func (optimization.hpp:482-951); integrate and trace_normalize are
-        par.integrate / par.normalize: r, a from nlls_data_terms at u = F(m);
+        par.integrate / par.normalize, filt and filter_phase nlls_fwi_eco's filter and
+        par.filter_phase (prepared by prepare_filter): r, a from nlls_data_terms at u = F(m);
         f = nlls_functional(r, dt, dnorm); g = gmask J^T a / dnorm, with
         dnorm = dt |P d|^2 (1 below 1e-16) for nlls_fwi_eco and 1 for nlls_fwi.  The shot
         sum is one multi-shot apply_jacobianT(add=false), i.e. the per-shot calls of
         nlls_fwi_eco (optimization.hpp:890) summed.  Returns (f, g, r)."""
         dt = float(prob["mp"]["dt_data"])
         u, full = self.apply_forward(self.make_op(prob), model, allsrc, keep_full=gradient)
-        r, a, pd = self.nlls_data_terms(u, d, dt, w, integrate, trace_normalize)
+        centered = True
+        if filt is not None:
+            filt, centered = self.prepare_filter(filt, filter_phase)
+        r, a, pd = self.nlls_data_terms(u, d, dt, w, integrate, trace_normalize, filt, centered)
         dnorm = 1.0
         if normalize:
             dnorm = float(np.sum(pd * pd)) * dt

@@ -10,11 +10,11 @@ src/include/optimization.hpp:295-951), on the CPU.
     nlls_fwi_eco's (normalised by dt |W d|^2) unchanged; the mask multiplies the gradient.
   * Time integration: closed form for a constant trace, and its adjoint passes the
     dot-product test; trace normalisation gives unit traces and leaves zero traces zero.
-  * Data side with integration, weights and trace normalisation: the adjoint source is the
-    gradient of 1/2 r^T Ht r in the synthetics, checked against central differences to
-    1e-7 (a dropped term of the normalisation's Jacobian or a wrong order of the steps is
-    off by far more); with trace normalisation the functional does not change when the
-    source amplitude does.
+  * Data side with integration, weights, a filter in each phase and trace normalisation:
+    the adjoint source is the gradient of 1/2 r^T Ht r in the synthetics, checked against
+    central differences to 1e-7 (a dropped term of the normalisation's Jacobian or a wrong
+    order of the steps is off by far more); with trace normalisation the functional does
+    not change when the source amplitude does.
   * The gradient is the derivative of the functional along a smooth model perturbation,
     to the accuracy of the reference's imaging condition (one-step lag of the adjoint
     derivatives, snapshots every sub steps, wide-stencil products): within 15 % here,
@@ -135,9 +135,10 @@ def test_integral_and_trace_normalisation(orc64):
     assert np.allclose(np.delete(tn, 6), 1.0, rtol=1e-14) and tn[6] == 0.0 and norms[6] == 1.0
 
 
-@pytest.mark.parametrize("integrate,trace_normalize,weights", [(True, False, False), (False, True, True),
-                                                               (True, True, True)])
-def test_data_side_gradient(orc64, integrate, trace_normalize, weights):
+@pytest.mark.parametrize("integrate,trace_normalize,weights,phase", [
+    (True, False, False, None), (False, True, True, None), (True, True, True, None),
+    (False, False, False, "default"), (True, True, True, "zero"), (False, True, True, "minimum")])
+def test_data_side_gradient(orc64, integrate, trace_normalize, weights, phase):
     rng = np.random.default_rng(7)
     nt, dt = 80, 0.003
     t = np.arange(nt) * dt
@@ -145,6 +146,9 @@ def test_data_side_gradient(orc64, integrate, trace_normalize, weights):
     d = u + 0.5 * rng.standard_normal(u.shape)
     w = rng.uniform(0.2, 1.5, u.shape) if weights else None
     args = dict(w=w, integrate=integrate, trace_normalize=trace_normalize)
+    if phase is not None:
+        # an asymmetric 12-tap filter, so that a missing flip or shift in the adjoint shows
+        args["filt"], args["centered"] = orc64.prepare_filter(rng.standard_normal(12) + np.linspace(1, 0, 12), phase)
     r, a, _ = orc64.nlls_data_terms(u, d, dt, **args)
     v = rng.standard_normal(u.shape)
     eps = 1e-5
