@@ -9,6 +9,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -21,6 +22,10 @@
 namespace fwi2d {
 
 thread_local std::string g_last_error;
+
+// Block partials of the FP64 dot product (fwi2d_engine_dot_f64 and its allocation-free
+// variant use the same count, so their sums are bit-identical)
+constexpr int kDotBlocks = 1024;
 
 #define HIPCHECK(x)                                                                                   \
     do {                                                                                              \
@@ -1320,7 +1325,7 @@ struct Engine {
 
     double dot_f64(const float* x, const float* y, const float* w, long long n)
     {
-        const int nb = 1024;
+        const int nb = kDotBlocks;
         d_partial.reserve(nb * sizeof(double));
         d_scalar.reserve(sizeof(double));
         launch_dot_f64(x, y, w, n, d_partial.as<double>(), nb, st);
@@ -1338,6 +1343,34 @@ using fwi2d::Engine;
 struct fwi2d_engine {
     Engine* impl;
 };
+
+// Device memory of the allocation-free entry point fwi2d_dot_f64_noalloc: the block
+// partials and the result of one dot product, (kDotBlocks + 1) doubles = 8 KB on every
+// device the library runs on, held until the process ends.  Host code reaches it only
+// through the address that hipGetSymbolAddress returns for the current device
+// (noalloc_scratch); the array's name on the host is not a device address.
+static __device__ __align__(256) double scratch[fwi2d::kDotBlocks + 1];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch[kMaxDevices] = {};
+
+// The device address of `scratch` on `device` (which must be current), resolved once per
+// device on its first use
+double* noalloc_scratch(int device)
+{
+    if (device < 0 || device >= kMaxDevices) throw std::runtime_error("device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[device]) {
+        void* p = nullptr;
+        HIPCHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) throw std::runtime_error("hipGetSymbolAddress returned no address for the scratch array");
+        g_scratch[device] = static_cast<double*>(p);
+    }
+    return g_scratch[device];
+}
+}  // namespace
 
 template <class F>
 static int guarded(F&& f)
@@ -1413,6 +1446,30 @@ int fwi2d_engine_dot_f64(fwi2d_engine* e, const float* a, const float* b, const 
         HIPCHECK(hipSetDevice(e->impl->device));
         *out = e->impl->dot_f64(a, b, w, n);
     });
+}
+
+int fwi2d_dot_f64_noalloc(int device, const float* a, const float* b, const float* w, long long n, double* out)
+{
+    int prev = -1;
+    const int rc = guarded([&] {
+        if (!a || !out) throw std::runtime_error("null argument");
+        if (n < 0 || n > FWI2D_DOT_NOALLOC_MAX_N)
+            throw std::runtime_error("fwi2d_dot_f64_noalloc: n = " + std::to_string(n) +
+                                     " outside [0, FWI2D_DOT_NOALLOC_MAX_N]");
+        int ndev = 0;
+        HIPCHECK(hipGetDeviceCount(&ndev));
+        if (device < 0 || device >= ndev) throw std::runtime_error("no HIP device " + std::to_string(device));
+        HIPCHECK(hipGetDevice(&prev));
+        HIPCHECK(hipSetDevice(device));
+        double* part = noalloc_scratch(device);
+        // the null stream: ordered after earlier work on the library's (blocking) streams
+        fwi2d::launch_dot_f64(a, b, w, n, part, fwi2d::kDotBlocks, 0);
+        fwi2d::launch_reduce_f64(part, fwi2d::kDotBlocks, part + fwi2d::kDotBlocks, 0);
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipMemcpy(out, part + fwi2d::kDotBlocks, sizeof(double), hipMemcpyDeviceToHost));
+    });
+    if (prev >= 0) (void)hipSetDevice(prev);
+    return rc;
 }
 
 int fwi2d_engine_set_option(fwi2d_engine* e, const char* key, long long value)

@@ -94,6 +94,7 @@ def load():
     lib.fwi2d_engine_forward.argtypes = [P, P, P, C.c_int, C.c_int]
     lib.fwi2d_engine_adjoint.argtypes = [P, P, P, C.c_int]
     lib.fwi2d_engine_dot_f64.argtypes = [P, P, P, P, C.c_longlong, C.POINTER(C.c_double)]
+    lib.fwi2d_dot_f64_noalloc.argtypes = [C.c_int, P, P, P, C.c_longlong, C.POINTER(C.c_double)]
     lib.fwi2d_engine_set_option.argtypes = [P, C.c_char_p, C.c_longlong]
     lib.fwi2d_engine_get_stat.argtypes = [P, C.c_char_p]
     lib.fwi2d_engine_get_stat.restype = C.c_longlong

@@ -89,6 +89,22 @@ int fwi2d_engine_adjoint(fwi2d_engine* e, const float* adj, float* grad, int on_
 int fwi2d_engine_dot_f64(fwi2d_engine* e, const float* a, const float* b, const float* w, long long n,
                          double* out);
 
+/* fwi2d_engine_dot_f64 without an engine and without any device allocation.  No
+   counterpart in the reference: for callers that cannot allocate device memory (code
+   under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+   kernels and block count as fwi2d_engine_dot_f64, so the result is bit-identical to it.
+   The block partials and the result live in a static device array of the library
+   ((1024 + 1) doubles, 8 KB per device, held until the process ends), whose address is
+   looked up once per device on the first call there; nothing else is allocated.  Runs
+   on the null stream of `device` (after earlier work on the library's streams) and
+   returns synchronously; the caller's current device is restored.  a, b, w are device
+   pointers (b NULL means b = a, w NULL means w = 1); out is a host pointer.  n must lie
+   in [0, FWI2D_DOT_NOALLOC_MAX_N].  Every call on one GPU uses the same array: calls
+   that overlap on one GPU (from several host threads) race and must be serialised by
+   the caller. */
+#define FWI2D_DOT_NOALLOC_MAX_N (1LL << 40)
+int fwi2d_dot_f64_noalloc(int device, const float* a, const float* b, const float* w, long long n, double* out);
+
 /* Tuning and test options (no reference counterpart): "tile_x" (16 | 32 cells per tile
    along x), "shots_per_block" (1..64 shots advanced in turn by one step workgroup of an
    interior tile), "shots_per_block_edge" (the same for tiles at the grid edge),
