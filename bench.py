@@ -28,6 +28,10 @@ sys.path.insert(0, ROOT)
 METRIC = "FWI gradient shots/s & Gcell-updates/s, batched vs plain, grad err≤1e-4"
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL_UPDATE_FWD = 36.0   # iso forward step: read curr 2 + prev 2 + model 3, write next 2 floats
+# one gradient, per cell and time step: the forward step (36 B iso, 44 B VTI) plus the adjoint
+# step with imaging at sub = 1 (68 B iso, 76 B VTI: + 8 B of forward state, + 24 B read-modify-
+# write of the 3 gradient fields); recomputation of the forward wavefield counts as overhead
+BYTES_PER_GRADIENT_CELL_STEP = {False: 36.0 + 68.0, True: 44.0 + 76.0}
 
 
 def parse():
@@ -41,6 +45,9 @@ def parse():
     ap.add_argument("--nt", type=int, default=None, help="time steps (default 4000 for c2, 6000 for c3)")
     ap.add_argument("--plain-shots", type=int, default=4)
     ap.add_argument("--tile-x", type=int, default=16)
+    ap.add_argument("--checkpoint-budget", type=int, default=0, metavar="MIB",
+                    help="device memory (MiB) for checkpoints and snapshot slots of the batched form; longer "
+                         "checkpoint segments leave fewer steps to recompute (0: the sqrt scheme's minimum)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-check", action="store_true", help="skip the small-case gradient error check")
     ap.add_argument("--cpu-steps", type=int, default=1600, help="time steps of the CPU baseline sample")
@@ -53,12 +60,21 @@ def parse():
                     help="time the operator level instead: nlls_fwi_eco evaluations (forward, residual, "
                          "adjoint) through fwi2d_op / fwi2d_nlls from host arrays, one JSON line")
     ap.add_argument("--fwi", type=int, metavar="ITERS", default=None,
-                    help="end-to-end FWI instead: ITERS l-BFGS iterations (regular Wolfe line search) on the "
-                         "nlls_fwi_eco objective, 48 shots on 600x2000 with 5 Hz sources, one JSON line")
-    ap.add_argument("--bands", default="5", help="with --fwi: comma-separated source peak frequencies (Hz), "
-                                                 "one l-BFGS run of ITERS iterations per band, low to high")
+                    help="end-to-end FWI instead: ITERS l-BFGS iterations (regular Wolfe line search) per band on "
+                         "the nlls_fwi_eco objective, 48 shots on 600x2000, one JSON line")
+    ap.add_argument("--bands", default="5,8,12",
+                    help="with --fwi: comma-separated low-pass cut-off frequencies (Hz) of the objective's filter, "
+                         "one l-BFGS run per band, low to high")
+    ap.add_argument("--fwi-source-hz", type=float, default=10.0,
+                    help="with --fwi: peak frequency of the Ricker source of every band (the data are modelled once)")
     ap.add_argument("--fwi-params", choices=["vp", "all"], default="vp",
                     help="with --fwi: invert vp only (gradient mask zero on vs and rho, the default) or all three")
+    ap.add_argument("--fwi-mute", type=float, default=0.0, metavar="KM",
+                    help="with --fwi: gradient mask zero above this depth, a sin^2 ramp over the next KM (0: none)")
+    ap.add_argument("--fwi-a1", type=float, default=0.0,
+                    help="with --fwi: regular_wolfe's a1, a first step of a1 |m| / |g| if that is larger than "
+                         "a0 / |g| (the reference's default 0)")
+    ap.add_argument("--fwi-trials", type=int, default=10, help="with --fwi: trial steps per line search")
     ap.add_argument("--step-only", type=int, metavar="EDGE_STREAM", default=None,
                     help="only autotune and time the forward time step of the workload with this edge_stream "
                          "setting (0: every launch of a step on one stream, for per-launch kernel traces), "
@@ -174,9 +190,11 @@ def workload(kind, nshots, nt):
 class Batch:
     """One engine holding a shot batch, with device-resident traces (torch tensors)."""
 
-    def __init__(self, fwi2d_amd, torch, prob, model_true, model, sxz, rxz, src, nt, dt, tile_x, tuned=True):
+    def __init__(self, fwi2d_amd, torch, prob, model_true, model, sxz, rxz, src, nt, dt, tile_x, tuned=True,
+                 checkpoint_budget=0):
         self.torch = torch
         self.eng = fwi2d_amd.Engine(fwi2d_amd.grid_from(prob, nt, dt, 1))
+        self.eng.set_option("checkpoint_budget", checkpoint_budget)
         if tuned:
             # batched form: step shapes picked by the autotuner at the first forward run,
             # time loops captured in hipGraphs and replayed
@@ -348,65 +366,84 @@ def operator_level(args):
                                                                              ", VTI" if vti else "")}}))
 
 
+def lowpass_filter(fc, dt):
+    """A zero-phase low-pass FIR: a Hann-windowed sinc of cut-off fc (Hz), 1.5 periods of
+    the cut-off on each side, unit gain at zero frequency."""
+    half = int(np.ceil(1.5 / (fc * dt)))
+    k = np.arange(-half, half + 1)
+    f = 2 * fc * dt * np.sinc(2 * fc * dt * k) * np.hanning(2 * half + 3)[1:-1]
+    return (f / f.sum()).astype(np.float32)
+
+
 def fwi_run(args):
-    """The end-to-end FWI workload (48 shots, 600x2000, 4000 samples of 1 ms): for each
-    band (Ricker source peak frequency, low to high), observed data from the true model,
-    then `--fwi` l-BFGS iterations (regular Wolfe line search, first step 1% of the model
-    norm) through the operator level and the nlls_fwi_eco objective, starting from the
-    previous band's result (the first from the 500 m smoothed model).  The gradient is
-    muted above 100 m (ramp to 200 m) and, by default, only vp is inverted.  One progress
-    line per objective evaluation, one JSON line at the end."""
+    """The end-to-end FWI workload C4 (48 shots, 600x2000, 4000 samples of 1 ms): data
+    modelled once from the true model with a broadband Ricker source, then, band by band
+    (low-pass cut-offs from --bands, low to high), `--fwi` l-BFGS iterations (regular
+    Wolfe line search at the reference defaults unless --fwi-a1) on nlls_fwi_eco with the
+    band's filter (the reference's filter= option: observed data and synthetics convolved
+    with the same zero-phase low-pass), each band starting from the previous band's model
+    (the first from the 500 m smoothed true model).  One progress line per objective
+    evaluation, one JSON line at the end with the functional history and the model
+    errors per band."""
     import fwi2d_amd
     from fwi2d_amd import workloads as W
     fwi2d_amd.load()
     bands = [float(b) for b in args.bands.split(",")]
-    _, true, start, _ = W.config4(nshots=args.shots or 48, f0=bands[0])
+    prob, true, start, src = W.config4(nshots=args.shots or 48, f0=args.fwi_source_hz)
+    ns = len(prob["sxz"])
+    dt = prob["mp"]["dt_data"]
+    h = prob["dz"]
+    depth = np.arange(prob["nz"]) * h
+    deep = depth >= 0.2  # below the near-surface zone of the sources and receivers (20 m)
     err = lambda x: float(np.linalg.norm(x - true) / np.linalg.norm(true))
     vp_err = lambda x: float(np.linalg.norm(x[0] - true[0]) / np.linalg.norm(true[0]))
+    vp_err_deep = lambda x: float(np.linalg.norm((x[0] - true[0])[:, deep]) / np.linalg.norm(true[0][:, deep]))
+    gmask = np.ones_like(true, dtype=np.float32)
+    if args.fwi_mute > 0:
+        ramp = np.clip((depth - args.fwi_mute) / args.fwi_mute, 0.0, 1.0)
+        gmask[:] = (np.sin(0.5 * np.pi * ramp) ** 2)[None, None, :]
+    if args.fwi_params == "vp":
+        gmask[1:] = 0.0
+    op = fwi2d_amd.WeOp(prob, src)
+    dobs = op.apply_forward(true)
+    log("observed data modelled: %d shots, %d traces" % (ns, 2 * op.nr))
     m = start
     out = []
     t_all = time.perf_counter()
-    for f0 in bands:
-        prob, _, _, src = W.config4(nshots=args.shots or 48, f0=f0)
-        ns = len(prob["sxz"])
-        op = fwi2d_amd.WeOp(prob, src)
-        dobs = op.apply_forward(true)
-        # gradient mask: no update above 100 m, a sin^2 ramp to 200 m (the sources and
-        # receivers are at 20 m, where the gradient peaks); vp only unless --fwi-params all
-        h = prob["dz"]
-        depth = np.arange(prob["nz"]) * h
-        ramp = np.clip((depth - 0.1) / 0.1, 0.0, 1.0)
-        gmask = np.zeros_like(true, dtype=np.float32)
-        gmask[:] = (np.sin(0.5 * np.pi * ramp) ** 2)[None, None, :]
-        if args.fwi_params == "vp":
-            gmask[1:] = 0.0
-        obj = fwi2d_amd.Nlls(op, dobs, gmask=gmask, normalize=True)
+    for fc in bands:
+        filt = lowpass_filter(fc, dt)
+        obj = fwi2d_amd.Nlls(op, dobs, gmask=gmask, normalize=True, filter=filt)
         count = [0]
 
         def evaluate(x):
             f, g = obj.evaluate(x)
             count[0] += 1
-            log("band %g Hz evaluation %d: functional %.6e" % (f0, count[0], f))
+            log("band %g Hz evaluation %d: functional %.6e" % (fc, count[0], f))
             return f, g
         t0 = time.perf_counter()
-        # first step: 1% of the model norm (regular_wolfe's a1, nlsolver.hpp:90)
-        m1, hist, k = fwi2d_amd.lbfgs(evaluate, m, niter=args.fwi, max_trial=20, a1=0.01)
+        m1, hist, k, info = fwi2d_amd.lbfgs(evaluate, m, niter=args.fwi, max_trial=args.fwi_trials, a1=args.fwi_a1,
+                                            report=True)
         el = time.perf_counter() - t0
-        out.append({"band_hz": f0, "iterations": k, "evaluations": count[0], "seconds": round(el, 1),
-                    "functional_history": [float(v) for v in hist],
-                    "model_rel_err": [err(m), err(m1)], "vp_rel_err": [vp_err(m), vp_err(m1)]})
-        log("band %g Hz: %d iterations, %d evaluations, %.1f s, functional %.4e -> %.4e"
-            % (f0, k, count[0], el, hist[0], hist[-1]))
+        out.append({"band_hz": fc, "filter_taps": int(filt.size), "iterations": k, "evaluations": count[0],
+                    "seconds": round(el, 1), "functional_history": [float(v) for v in hist],
+                    "model_rel_err": [err(m), err(m1)], "vp_rel_err": [vp_err(m), vp_err(m1)],
+                    "vp_rel_err_below_200m": [vp_err_deep(m), vp_err_deep(m1)],
+                    "trials": [[int(r[0]), int(r[1]), float(r[2]), float(r[3])] for r in info["trials"]]})
+        log("band %g Hz: %d iterations, %d evaluations, %.1f s, functional %.4e -> %.4e, vp error %.4f -> %.4f"
+            % (fc, k, count[0], el, hist[0], hist[k], vp_err(m), vp_err(m1)))
         m = m1
         obj.close()
-        op.close()
-    print(json.dumps({"metric": "end-to-end FWI (l-BFGS, nlls_fwi_eco, multiscale bands)", "bands": out,
+    op.close()
+    print(json.dumps({"metric": "end-to-end FWI (l-BFGS, nlls_fwi_eco with filter= bands)", "bands": out,
                       "seconds": round(time.perf_counter() - t_all, 1),
                       "model_rel_err": [err(start), err(m)], "vp_rel_err": [vp_err(start), vp_err(m)],
-                      "config": {"workload": "%d shots, 600x2000 h=10 m, 4000 samples of 1 ms, Ricker bands %s Hz, "
-                                             "%d l-BFGS iterations per band, inverting %s"
-                                             % (ns, args.bands, args.fwi, "vp" if args.fwi_params == "vp"
-                                                else "vp, vs, rho")}}))
+                      "vp_rel_err_below_200m": [vp_err_deep(start), vp_err_deep(m)],
+                      "config": {"workload": "%d shots, 600x2000 h=10 m, 4000 samples of 1 ms, %g Hz Ricker, "
+                                             "low-pass bands %s Hz, %d l-BFGS iterations per band, inverting %s, "
+                                             "mute %g km, a1 %g"
+                                             % (ns, args.fwi_source_hz, args.bands, args.fwi,
+                                                "vp" if args.fwi_params == "vp" else "vp, vs, rho", args.fwi_mute,
+                                                args.fwi_a1)}}))
 
 
 def main():
@@ -445,7 +482,8 @@ def main():
 
     # ---- batched form -------------------------------------------------------
     log("workload ready: %d shots, %dx%d, nt=%d" % (ns, nz, nx, nt))
-    B = Batch(fwi2d_amd, torch, prob, m_true, m0, sxz, rxz, src, nt, dt, args.tile_x)
+    B = Batch(fwi2d_amd, torch, prob, m_true, m0, sxz, rxz, src, nt, dt, args.tile_x,
+              checkpoint_budget=args.checkpoint_budget)
     log("observed data modelled")
     for _ in range(args.warmup):
         B.gradient()
@@ -461,6 +499,7 @@ def main():
     shots_per_s = ns * args.steps / el
     gcells = shots_per_s * cells_per_grad / 1e9
     grad_norm, misfit = B.grad_norm, B.misfit
+    grad_gbs = BYTES_PER_GRADIENT_CELL_STEP[vti] * ns * nt * nx * nz / (ms_per_step / 1000.0) / 1e9
     traffic = measured_traffic() if (args.workload, ns) == ("c2", 64) else None
 
     # ---- dominant kernel: fused step, timed with HIP events on its stream ----
@@ -479,6 +518,7 @@ def main():
     cells_per_launch = ns * nx * nz
     achieved = bytes_per_cell * cells_per_launch / step_s / 1e9
     recompute = B.eng.stat("recompute_steps")
+    segment = B.eng.stat("segment")
     del B
     torch.cuda.empty_cache()
 
@@ -529,6 +569,8 @@ def main():
         "plain_one_shot_per_launch_shots_per_s": plain_shots_per_s and round(plain_shots_per_s, 4),
         "batched_vs_plain": plain_shots_per_s and round(shots_per_s / plain_shots_per_s, 3),
         "recompute_steps_per_gradient": int(recompute // max(args.steps + args.warmup, 1)),
+        "checkpoint_segment": segment,
+        "checkpoint_budget_mib": args.checkpoint_budget,
         "grad_rel_err_vs_fp64": grad_err,
         "grad_err_sample": "1 shot, 160x100 grid%s, nt=600, raw lambda/mu/rho gradients vs FP64 oracle"
                            % (" (VTI)" if vti else ""),
@@ -538,7 +580,11 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "k_step (fused time step of all shots: core-box launch + edge-band launches)",
                      "avg_launch_us": round(step_s * 1e6, 2),
-                     "bytes_per_cell_update": bytes_per_cell, "cells_per_launch": cells_per_launch},
+                     "bytes_per_cell_update": bytes_per_cell, "cells_per_launch": cells_per_launch,
+                     # the whole gradient (forward + checkpointed adjoint with imaging) against the
+                     # same peak: algorithmic bytes of one gradient of the batch / its wall time
+                     "gradient_achieved": round(grad_gbs, 1), "gradient_frac": round(grad_gbs / HBM_PEAK_GBS, 4),
+                     "gradient_bytes_per_cell_step": BYTES_PER_GRADIENT_CELL_STEP[vti]},
     }
     if grad_err16:
         line["grad_rel_err_vs_fp64_16bit_checkpoints"] = grad_err16

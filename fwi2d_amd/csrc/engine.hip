@@ -120,6 +120,7 @@ struct Engine {
     std::vector<int> list_core, list_bz, list_bz8, list_bx;  // box indices per box class
     DevMem d_boxes, d_list_core, d_list_bz, d_list_bz8, d_list_bx;
     int opt_tile_x = 16, opt_segment = 0, opt_lag = 1, opt_graph = 0, opt_timing = 0;
+    long long opt_ckpt_budget_mb = 0;  // device memory for checkpoints and snapshot slots; 0: sqrt scheme
     // hipGraph replay of whole time loops: a loop is captured once per configuration
     // (every pointer and size it uses) and replayed while that configuration holds
     struct LoopGraph {
@@ -754,6 +755,22 @@ struct Engine {
         nsnap = (g.nt - 1) / g.sub + 1;
         int K = opt_segment > 0 ? opt_segment : (int)std::ceil(std::sqrt(2.0 * nsnap));
         K = std::max(1, std::min(K, nsnap));
+        if (opt_segment == 0 && opt_ckpt_budget_mb > 0) {
+            // the longest segments whose checkpoints (2 levels per segment after the first)
+            // and snapshot slots (K + 2 levels) fit the budget: the last segment is kept
+            // whole by the forward run, so every snapshot beyond the first nsnap - K is
+            // never recomputed
+            const double level = (double)state_stride() * sizeof(float);
+            const double ck_level = (double)state_stride() * (opt_bf16 ? 2 : 4);
+            const double budget = (double)opt_ckpt_budget_mb * 1048576.0;
+            for (int k = nsnap; k > K; k--) {
+                const int nseg_k = (nsnap + k - 1) / k;
+                if (2.0 * (nseg_k - 1) * ck_level + (k + 2.0) * level <= budget) {
+                    K = k;
+                    break;
+                }
+            }
+        }
         seg_K = K;
         nseg = (nsnap + K - 1) / K;
     }
@@ -1484,6 +1501,10 @@ int fwi2d_engine_set_option(fwi2d_engine* e, const char* key, long long value)
         } else if (k == "segment") {
             if (value < 0) throw std::runtime_error("segment must be >= 0");
             x->opt_segment = (int)value;
+            x->ckpt_epoch = -1;
+        } else if (k == "checkpoint_budget") {
+            if (value < 0) throw std::runtime_error("checkpoint_budget must be >= 0");
+            x->opt_ckpt_budget_mb = value;
             x->ckpt_epoch = -1;
         } else if (k == "lag") {
             x->opt_lag = value ? 1 : 0;

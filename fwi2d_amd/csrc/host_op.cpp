@@ -281,11 +281,6 @@ void parallel_traces(size_t n, F&& f)
     if (err) std::rethrow_exception(err);
 }
 
-// per-thread scratch of the objective's trace loop
-struct TraceWork {
-    std::vector<double> v, s, c;
-};
-
 // conv1dnd (operator.hpp:1035-1057; operator.cpp:2092-2137) of one trace of n samples
 // with the filter f: y[t] = sum_j f[j] x[t - j + sh] over the j that keep t - j + sh in
 // the trace (sh = floor((nf + 1)/2) - 1 centred, else 0); the adjoint scatters
@@ -305,6 +300,105 @@ void conv_trace(const std::vector<double>& f, bool centered, bool adjoint, const
             for (int t = lo; t < hi; t++) y[t + k] += fj * x[t];
     }
 }
+
+// In-place iterative radix-2 complex FFT of n = 2^k points (re, im), forward (sign -1)
+// or inverse without the 1/n (sign +1); tw holds exp(-2 pi i j / n), j < n/2.
+void fft_pow2(std::vector<double>& re, std::vector<double>& im, const std::vector<double>& twr,
+              const std::vector<double>& twi, bool inverse)
+{
+    const size_t n = re.size();
+    for (size_t i = 1, j = 0; i < n; i++) {  // bit reversal
+        size_t b = n >> 1;
+        for (; j & b; b >>= 1) j ^= b;
+        j |= b;
+        if (i < j) {
+            std::swap(re[i], re[j]);
+            std::swap(im[i], im[j]);
+        }
+    }
+    for (size_t len = 2; len <= n; len <<= 1) {
+        const size_t half = len >> 1, step = n / len;
+        for (size_t i = 0; i < n; i += len)
+            for (size_t k = 0; k < half; k++) {
+                const double wr = twr[k * step], wi = inverse ? -twi[k * step] : twi[k * step];
+                const size_t a = i + k, b = a + half;
+                const double xr = re[b] * wr - im[b] * wi, xi = re[b] * wi + im[b] * wr;
+                re[b] = re[a] - xr;
+                im[b] = im[a] - xi;
+                re[a] += xr;
+                im[a] += xi;
+            }
+    }
+}
+
+// conv1dnd of traces of n samples with one filter, as conv_trace computes it: directly for
+// short filters, through zero-padded FFTs of 2^k >= n + nf - 1 points for long ones (the
+// circular convolution is then the linear one; the adjoint is the circular correlation
+// with the filter, read at lag k - sh).  Both are exact up to double rounding.
+struct TraceFilter {
+    std::vector<double> f;
+    bool centered = true;
+    int n = 0;
+    size_t N = 0;
+    std::vector<double> twr, twi, fr, fi;  // twiddles; the filter's spectrum
+    struct Work {
+        std::vector<double> re, im;
+    };
+
+    void init(const std::vector<double>& filt, bool cen, int nt)
+    {
+        f = filt;
+        centered = cen;
+        n = nt;
+        N = 0;
+        if (f.size() < 48) return;  // direct sums are cheaper
+        N = 1;
+        while (N < (size_t)n + f.size() - 1) N <<= 1;
+        twr.resize(N / 2);
+        twi.resize(N / 2);
+        for (size_t j = 0; j < N / 2; j++) {
+            twr[j] = std::cos(2 * kPi * (double)j / (double)N);
+            twi[j] = -std::sin(2 * kPi * (double)j / (double)N);
+        }
+        fr.assign(N, 0.0);
+        fi.assign(N, 0.0);
+        std::copy(f.begin(), f.end(), fr.begin());
+        fft_pow2(fr, fi, twr, twi, false);
+    }
+
+    bool empty() const { return f.empty(); }
+
+    // y = C x (adjoint: C^T x); x and y must not overlap
+    void apply(bool adjoint, const double* x, double* y, Work& w) const
+    {
+        if (!N) {
+            conv_trace(f, centered, adjoint, x, y, n);
+            return;
+        }
+        const int sh = centered ? ((int)f.size() + 1) / 2 - 1 : 0;
+        w.re.assign(N, 0.0);
+        w.im.assign(N, 0.0);
+        std::copy(x, x + n, w.re.begin());
+        fft_pow2(w.re, w.im, twr, twi, false);
+        for (size_t k = 0; k < N; k++) {
+            const double a = w.re[k], b = w.im[k], c = fr[k], d = adjoint ? -fi[k] : fi[k];
+            w.re[k] = a * c - b * d;
+            w.im[k] = a * d + b * c;
+        }
+        fft_pow2(w.re, w.im, twr, twi, true);
+        const double s = 1.0 / (double)N;
+        for (int t = 0; t < n; t++) {
+            const long long m = adjoint ? ((long long)t - sh + (long long)N) % (long long)N : (long long)t + sh;
+            y[t] = w.re[(size_t)m] * s;
+        }
+    }
+};
+
+// per-thread scratch of the objective's trace loop
+struct TraceWork {
+    std::vector<double> v, s, c;
+    TraceFilter::Work fw;
+};
 
 // Real-input discrete Fourier transform of x (n values) onto its n/2 + 1 non-negative
 // frequencies, and the inverse from those frequencies (with 1/n, the imaginary part of
@@ -558,8 +652,7 @@ struct fwi2d_nlls {
     std::vector<double> d;     // processed observed data: integrated, weighted, normalised
     std::vector<float> w;      // residual weights, empty: none
     std::vector<float> gmask;  // gradient mask, empty: none
-    std::vector<double> filt;  // prepared filter, empty: none
-    bool filt_centered = true;
+    TraceFilter filt;          // prepared filter, empty: none
     std::vector<float> r;      // residual of the last evaluation
     double dnorm = 1.0;        // nlls_fwi_eco's data norm (optimization.hpp:569-574), else 1
     size_t ndata = 0, nmodel = 0;
@@ -603,13 +696,14 @@ struct fwi2d_nlls {
     {
         const int n = nt();
         std::vector<double> c(filt.empty() ? 0 : n);
+        TraceFilter::Work fw;
         for (size_t x = 0; x < ntr(); x++) {
             double* p = u.data() + x * n;
             if (integrate) integral_trace(p, false);
             if (!w.empty())
                 for (int t = 0; t < n; t++) p[t] *= w[x * n + t];
             if (!filt.empty()) {
-                conv_trace(filt, filt_centered, false, p, c.data(), n);
+                filt.apply(false, p, c.data(), fw);
                 std::copy(c.begin(), c.end(), p);
             }
             if (trace_norm) normalize_trace(p);
@@ -642,7 +736,7 @@ struct fwi2d_nlls {
                 for (int t = 0; t < n; t++) v[t] *= wx[t];
             if (!filt.empty()) {
                 ws.c.resize(n);
-                conv_trace(filt, filt_centered, false, v.data(), ws.c.data(), n);
+                filt.apply(false, v.data(), ws.c.data(), ws.fw);
                 v.swap(ws.c);
             }
             double nrm = 1.0;
@@ -667,7 +761,7 @@ struct fwi2d_nlls {
             }
             if (!filt.empty()) {
                 ws.c.resize(n);
-                conv_trace(filt, filt_centered, true, v.data(), ws.c.data(), n);
+                filt.apply(true, v.data(), ws.c.data(), ws.fw);
                 v.swap(ws.c);
             }
             if (wx)
@@ -809,8 +903,8 @@ int fwi2d_nlls_create_filtered(fwi2d_op* op, const float* d, const float* w, con
         if (filter) {
             for (int i = 0; i < nf; i++)
                 if (!std::isfinite(filter[i])) throw std::runtime_error("the filter contains a non-finite value");
-            f->filt = prepare_filter(filter, nf, filter_phase);
-            f->filt_centered = filter_phase != FWI2D_FILTER_MINIMUM;
+            f->filt.init(prepare_filter(filter, nf, filter_phase), filter_phase != FWI2D_FILTER_MINIMUM,
+                         op->d.nt_data);
         }
         f->d.assign(d, d + ndata);
         f->process(f->d);
@@ -843,8 +937,10 @@ int fwi2d_filter_apply(const double* filter, int nf, int centered, int adjoint, 
     return op_guard([&] {
         if (!filter || !in || !out) throw std::runtime_error("null argument");
         if (nf < 1 || nt < 1 || ntr < 0) throw std::runtime_error("invalid sizes");
-        const std::vector<double> f(filter, filter + nf);
-        for (long long x = 0; x < ntr; x++) conv_trace(f, centered != 0, adjoint != 0, in + x * nt, out + x * nt, nt);
+        TraceFilter f;
+        f.init(std::vector<double>(filter, filter + nf), centered != 0, nt);
+        TraceFilter::Work w;
+        for (long long x = 0; x < ntr; x++) f.apply(adjoint != 0, in + x * nt, out + x * nt, w);
     });
 }
 

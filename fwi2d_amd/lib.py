@@ -47,7 +47,7 @@ EXPORTS = [
     "fwi2d_op_apply_jacobianT", "fwi2d_op_engine", "fwi2d_op_time_axis", "fwi2d_op_destroy",
     "fwi2d_nlls_create", "fwi2d_nlls_evaluate", "fwi2d_nlls_residual", "fwi2d_nlls_dnorm", "fwi2d_nlls_destroy",
     "fwi2d_nlls_objective", "fwi2d_lbfgs", "fwi2d_nlls_create_filtered", "fwi2d_filter_prepare",
-    "fwi2d_filter_apply", "fwi2d_read_header", "fwi2d_read_data", "fwi2d_write",
+    "fwi2d_filter_apply", "fwi2d_dot_f64_noalloc", "fwi2d_read_header", "fwi2d_read_data", "fwi2d_write",
     "fwi2d_read_coord", "fwi2d_regular_geometry",
 ]
 

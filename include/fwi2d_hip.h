@@ -130,7 +130,12 @@ int fwi2d_dot_f64_noalloc(int device, const float* a, const float* b, const floa
    carries the error of the recomputation from rounded levels),
    "stiffness_gradients" (VTI, 1: fwi2d_engine_adjoint returns raw gradients w.r.t.
    c11, c13, c33, c55, rho instead of lambda, mu, rho, 0, 0),
-   "segment" (snapshots per checkpoint segment, 0 = about sqrt(2 nt/sub)), "lag" (1 the
+   "segment" (snapshots per checkpoint segment, 0 = about sqrt(2 nt/sub), or with
+   "checkpoint_budget" the longest that fits it), "checkpoint_budget" (MiB of device
+   memory for the checkpoint levels and the segment's snapshot slots; 0, the default: the
+   sqrt scheme's minimum.  Longer segments cost the same recomputation per step, and the
+   last segment, which the forward run keeps whole, is not recomputed: a budget of about
+   nt/sub / 4 levels saves a quarter of the recomputation), "lag" (1 the
    reference's one-step imaging lag, 0 none), "timing" (1 time the forward step loop with
    HIP events), "graph" (1: capture each forward / adjoint time loop in a hipGraph the
    first time it runs in a configuration and replay it afterwards).  Statistics: "steps", "recompute_steps",

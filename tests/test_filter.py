@@ -84,7 +84,7 @@ def test_minimum_phase(orc64):
 
 @needs_lib
 @pytest.mark.parametrize("phase", ["default", "zero", "minimum"])
-@pytest.mark.parametrize("nf", [1, 24, 25])
+@pytest.mark.parametrize("nf", [1, 24, 25, 60, 301])
 def test_host_filter_matches_oracle(orc64, phase, nf):
     from fwi2d_amd import lib
     rng = np.random.default_rng(nf)
