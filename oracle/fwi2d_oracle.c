@@ -526,30 +526,201 @@ void orc_delta_m3(int nx, int nz, double dx, double dz, double ox, double oz, in
     }
 }
 
-/* out[footprint] += w * trace[itr][it]; add==0 zeroes the footprints first (injector.cpp:66-91) */
-static void inject(int add, const real* tr, int nt, int it, int ntr, const int* xi, const int* zi,
-                   const real* xw, const real* zw, real* out, int nz)
-{
-    if (!add)
-        for (int t = 0; t < ntr; t++)
-            for (int a = 0; a < 3; a++)
-                for (int b = 0; b < 3; b++) out[(long)xi[3 * t + a] * nz + zi[3 * t + b]] = 0;
-    for (int t = 0; t < ntr; t++)
-        for (int a = 0; a < 3; a++)
-            for (int b = 0; b < 3; b++)
-                out[(long)xi[3 * t + a] * nz + zi[3 * t + b]] += xw[6 * t + a] * zw[6 * t + b] * tr[(long)t * nt + it];
-}
-
-/* trace[itr][it] = sum w * in[footprint] (injector.cpp:93-112, add=true on a zeroed trace) */
-static void extract(const real* in, real* tr, int nt, int it, int ntr, const int* xi, const int* zi,
-                    const real* xw, const real* zw, int nz)
+/* ddelta_m3 (injector.cpp:114-239): the delta_m3 footprint of each point with, in
+   columns 3..5, the derivative weights: injection (1-2a)/(2h^2 H), 2a/(h^2 H),
+   (-1-2a)/(2h^2 H); extraction (1-2a)/(2h), 2a/h, (-1-2a)/(2h).
+   xind/zind: [ntr][3]; xw/zw: [ntr][2][6] (row 0 injection, row 1 extraction). */
+void orc_ddelta_m3(int nx, int nz, double dx, double dz, double ox, double oz, int ntr,
+                   const double* locx, const double* locz, int* xind, int* zind, real* xw, real* zw)
 {
     for (int t = 0; t < ntr; t++) {
-        real v = tr[(long)t * nt + it];
-        for (int a = 0; a < 3; a++)
-            for (int b = 0; b < 3; b++) v += xw[6 * t + 3 + a] * zw[6 * t + 3 + b] * in[(long)xi[3 * t + a] * nz + zi[3 * t + b]];
-        tr[(long)t * nt + it] = v;
+        int px = (int)lround((locx[t] - ox) / dx);
+        int pz = (int)lround((locz[t] - oz) / dz);
+        if (px == 0) px = 1;
+        if (px == nx - 1) px = nx - 2;
+        if (pz == 0) pz = 1;
+        if (pz == nz - 1) pz = nz - 2;
+        for (int k = 0; k < 3; k++) { xind[3 * t + k] = px - 1 + k; zind[3 * t + k] = pz - 1 + k; }
+        real ax = (real)((locx[t] - dx * px - ox) / dx);
+        real az = (real)((locz[t] - dz * pz - oz) / dz);
+        real* wx = xw + 12 * t;
+        real* wz = zw + 12 * t;
+        const double hx[3] = {hcoef_at(nx, px - 1), hcoef_at(nx, px), hcoef_at(nx, px + 1)};
+        const double hz[3] = {hcoef_at(nz, pz - 1), hcoef_at(nz, pz), hcoef_at(nz, pz + 1)};
+        wx[0] = (ax * ax - ax) / (real)(2 * dx * hx[0]);
+        wx[1] = (-ax * ax + 1) / (real)(dx * hx[1]);
+        wx[2] = (ax * ax + ax) / (real)(2 * dx * hx[2]);
+        wx[3] = (1 - 2 * ax) / (real)(2 * dx * dx * hx[0]);
+        wx[4] = 2 * ax / (real)(dx * dx * hx[1]);
+        wx[5] = (-1 - 2 * ax) / (real)(2 * dx * dx * hx[2]);
+        wz[0] = (az * az - az) / (real)(2 * dz * hz[0]);
+        wz[1] = (-az * az + 1) / (real)(dz * hz[1]);
+        wz[2] = (az * az + az) / (real)(2 * dz * hz[2]);
+        wz[3] = (1 - 2 * az) / (real)(2 * dz * dz * hz[0]);
+        wz[4] = 2 * az / (real)(dz * dz * hz[1]);
+        wz[5] = (-1 - 2 * az) / (real)(2 * dz * dz * hz[2]);
+        wx[6] = (ax * ax - ax) / 2; wx[7] = -ax * ax + 1; wx[8] = (ax * ax + ax) / 2;
+        wx[9] = (1 - 2 * ax) / (real)(2 * dx); wx[10] = 2 * ax / (real)dx; wx[11] = (-1 - 2 * ax) / (real)(2 * dx);
+        wz[6] = (az * az - az) / 2; wz[7] = -az * az + 1; wz[8] = (az * az + az) / 2;
+        wz[9] = (1 - 2 * az) / (real)(2 * dz); wz[10] = 2 * az / (real)dz; wz[11] = (-1 - 2 * az) / (real)(2 * dz);
     }
+}
+
+/* dipole_m3 (injector.cpp:241-378): two delta_m3 footprints at the ends of a gauge of length
+   gl centred on each point along its dip (x -/+ gl/2 cos(dip), z -/+ gl/2 sin(dip)).
+   xind/zind: [ntr][6] (columns 0..2 first end, 3..5 second); xw/zw: [ntr][2][6]. */
+void orc_dipole_m3(int nx, int nz, double dx, double dz, double ox, double oz, int ntr, const double* locx,
+                   const double* locz, const double* dip, double gl, int* xind, int* zind, real* xw, real* zw)
+{
+    for (int t = 0; t < ntr; t++) {
+        const real c = (real)cos(dip[t]), sn = (real)sin(dip[t]);
+        const real hg = (real)(gl / 2);
+        double ex[2] = {locx[t] - hg * c, locx[t] + hg * c};
+        double ez[2] = {locz[t] - hg * sn, locz[t] + hg * sn};
+        for (int e = 0; e < 2; e++) {
+            int px = (int)lround((ex[e] - ox) / dx);
+            int pz = (int)lround((ez[e] - oz) / dz);
+            if (px == 0) px = 1;
+            if (px == nx - 1) px = nx - 2;
+            if (pz == 0) pz = 1;
+            if (pz == nz - 1) pz = nz - 2;
+            for (int k = 0; k < 3; k++) { xind[6 * t + 3 * e + k] = px - 1 + k; zind[6 * t + 3 * e + k] = pz - 1 + k; }
+            real ax = (real)((ex[e] - ox) / dx - px);
+            real az = (real)((ez[e] - oz) / dz - pz);
+            real* wx = xw + 12 * t + 3 * e;
+            real* wz = zw + 12 * t + 3 * e;
+            wx[0] = (ax * ax - ax) / (real)(2 * dx * hcoef_at(nx, px - 1));
+            wx[1] = (-ax * ax + 1) / (real)(dx * hcoef_at(nx, px));
+            wx[2] = (ax * ax + ax) / (real)(2 * dx * hcoef_at(nx, px + 1));
+            wz[0] = (az * az - az) / (real)(2 * dz * hcoef_at(nz, pz - 1));
+            wz[1] = (-az * az + 1) / (real)(dz * hcoef_at(nz, pz));
+            wz[2] = (az * az + az) / (real)(2 * dz * hcoef_at(nz, pz + 1));
+            wx[6] = (ax * ax - ax) / 2; wx[7] = -ax * ax + 1; wx[8] = (ax * ax + ax) / 2;
+            wz[6] = (az * az - az) / 2; wz[7] = -az * az + 1; wz[8] = (az * az + az) / 2;
+        }
+    }
+}
+
+/* dipole_to_strain (misc.cpp:18-43) on [2][ntr][nt]: forward, p0 = (cos p0 + sin p1)/gl
+   and p1 = 0; adjoint, p1 = sin p0/gl and p0 = cos p0/gl. */
+void orc_dipole_to_strain(int adj, real* p, const double* dip, int ntr, int nt, double gl)
+{
+    for (int x = 0; x < ntr; x++) {
+        const real c = (real)cos(dip[x]), sn = (real)sin(dip[x]);
+        real* p0 = p + (long)x * nt;
+        real* p1 = p + (long)(ntr + x) * nt;
+        for (int t = 0; t < nt; t++) {
+            if (!adj) { p0[t] = (c * p0[t] + sn * p1[t]) / (real)gl; p1[t] = 0; }
+            else { p1[t] = sn * p0[t] / (real)gl; p0[t] *= c / (real)gl; }
+        }
+    }
+}
+
+/* A point injector or extractor as lists of (grid cell, weight, trace row) per field
+   component (0 = x, 1 = z): injection adds w * row[it] to the cell, extraction adds w times
+   the cell to row[it]; rows are offsets from a base pointer per component.  delta_m3,
+   ddelta_m3 and dipole_m3 are all of this form; the entries follow the reference's loop
+   order (trace, then x, then z offset of the footprint). */
+typedef struct {
+    int n[2];
+    long* cell[2];
+    long* off[2];
+    real* w[2];
+} pts;
+
+static void pts_alloc(pts* p, int n)
+{
+    for (int c = 0; c < 2; c++) {
+        p->n[c] = 0;
+        p->cell[c] = (long*)malloc(sizeof(long) * (size_t)(n > 0 ? n : 1));
+        p->off[c] = (long*)malloc(sizeof(long) * (size_t)(n > 0 ? n : 1));
+        p->w[c] = (real*)malloc(sizeof(real) * (size_t)(n > 0 ? n : 1));
+    }
+}
+
+static void pts_free(pts* p)
+{
+    for (int c = 0; c < 2; c++) { free(p->cell[c]); free(p->off[c]); free(p->w[c]); }
+}
+
+static void pts_add(pts* p, int c, long cell, long off, real w)
+{
+    const int k = p->n[c]++;
+    p->cell[c][k] = cell; p->off[c][k] = off; p->w[c][k] = w;
+}
+
+/* delta_m3 footprints, both components reading (writing) trace t at offset t * nt from
+   their base; row = 0 injection weights, 1 extraction */
+static void pts_delta(pts* p, int ntr, const int* xi, const int* zi, const real* xw, const real* zw, int row,
+                      int nt, int nz)
+{
+    pts_alloc(p, 9 * ntr);
+    for (int c = 0; c < 2; c++)
+        for (int t = 0; t < ntr; t++)
+            for (int a = 0; a < 3; a++)
+                for (int b = 0; b < 3; b++)
+                    pts_add(p, c, (long)xi[3 * t + a] * nz + zi[3 * t + b], (long)t * nt,
+                            xw[6 * t + 3 * row + a] * zw[6 * t + 3 * row + b]);
+}
+
+/* ddelta_m3 injection of a moment tensor (we_op.cpp:640-654; injector.cpp:191-217): the
+   x component takes -d/dx of Mxx and -d/dz of Mxz, the z component -d/dx of Mxz and -d/dz
+   of Mzz; offsets of the Mxx, Mzz and Mxz traces of point t are oxx + t nt, ozz + t nt,
+   oxz + t nt from one base */
+static void pts_ddelta(pts* p, int ntr, const int* xi, const int* zi, const real* xw, const real* zw, long oxx,
+                       long ozz, long oxz, int nt, int nz)
+{
+    pts_alloc(p, 18 * ntr);
+    for (int t = 0; t < ntr; t++)
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++) {
+                const long cell = (long)xi[3 * t + a] * nz + zi[3 * t + b];
+                const real wdx = -xw[12 * t + 3 + a] * zw[12 * t + b];
+                const real wdz = -xw[12 * t + a] * zw[12 * t + 3 + b];
+                pts_add(p, 0, cell, oxx + (long)t * nt, wdx);
+                pts_add(p, 0, cell, oxz + (long)t * nt, wdz);
+                pts_add(p, 1, cell, oxz + (long)t * nt, wdx);
+                pts_add(p, 1, cell, ozz + (long)t * nt, wdz);
+            }
+}
+
+/* dipole_m3 (injector.cpp:333-378): -first end + second end, both components on trace t */
+static void pts_dipole(pts* p, int ntr, const int* xi, const int* zi, const real* xw, const real* zw, int row,
+                       int nt, int nz)
+{
+    pts_alloc(p, 18 * ntr);
+    for (int c = 0; c < 2; c++)
+        for (int t = 0; t < ntr; t++)
+            for (int a = 0; a < 3; a++)
+                for (int b = 0; b < 3; b++) {
+                    if (row == 0) {  /* inject: the first end subtracts, the second adds */
+                        pts_add(p, c, (long)xi[6 * t + a] * nz + zi[6 * t + b], (long)t * nt,
+                                -xw[12 * t + a] * zw[12 * t + b]);
+                        pts_add(p, c, (long)xi[6 * t + 3 + a] * nz + zi[6 * t + 3 + b], (long)t * nt,
+                                xw[12 * t + 3 + a] * zw[12 * t + 3 + b]);
+                    } else {  /* extract: second end minus first end */
+                        pts_add(p, c, (long)xi[6 * t + 3 + a] * nz + zi[6 * t + 3 + b], (long)t * nt,
+                                xw[12 * t + 9 + a] * zw[12 * t + 9 + b]);
+                        pts_add(p, c, (long)xi[6 * t + a] * nz + zi[6 * t + b], (long)t * nt,
+                                -xw[12 * t + 6 + a] * zw[12 * t + 6 + b]);
+                    }
+                }
+}
+
+static void pts_inject(const pts* p, const real* base0, const real* base1, int it, real* ux, real* uz)
+{
+    const real* base[2] = {base0, base1};
+    real* u[2] = {ux, uz};
+    for (int c = 0; c < 2; c++)
+        for (int k = 0; k < p->n[c]; k++) u[c][p->cell[c][k]] += p->w[c][k] * base[c][p->off[c][k] + it];
+}
+
+static void pts_extract(const pts* p, real* base0, real* base1, int it, const real* ux, const real* uz)
+{
+    real* base[2] = {base0, base1};
+    const real* u[2] = {ux, uz};
+    for (int c = 0; c < 2; c++)
+        for (int k = 0; k < p->n[c]; k++) base[c][p->off[c][k] + it] += p->w[c][k] * u[c][p->cell[c][k]];
 }
 
 /* ------------------------------------------------------------------------- */
@@ -770,11 +941,8 @@ static void adjoint_derivatives(const orc_grid* g, const real* curr, real* ux, r
    receivers in adjoint mode).  ext*: extractor tables and output traces (forward mode only).
    full: snapshots (forward writes when sub>0, adjoint reads).  grad != NULL selects the
    adjoint/imaging mode.  Returns 0 on success. */
-int orc_propagate(const orc_grid* g, const real* model,
-                  int ninj, const int* ixi, const int* izi, const real* ixw, const real* izw,
-                  const real* inx, const real* inz,
-                  int next_, const int* exi, const int* ezi, const real* exw, const real* ezw,
-                  real* outx, real* outz, real* full, real* grad)
+static int propagate_pts(const orc_grid* g, const real* model, const pts* inj, const real* ib0, const real* ib1,
+                         const pts* ext, real* ob0, real* ob1, real* full, real* grad)
 {
     d2b_init();
     const int nx = g->nx, nz = g->nz, nt = g->nt, sub = g->sub, vti = g->vti;
@@ -798,8 +966,7 @@ int orc_propagate(const orc_grid* g, const real* model,
     const real* rho = model + 2 * nxz;
 
     /* initial condition: prev = 1/2 dt^2/rho * inject(src[0])  (we_op.cpp:655-663) */
-    inject(0, inx, nt, 0, ninj, ixi, izi, ixw, izw, prev, nz);
-    inject(0, inz, nt, 0, ninj, ixi, izi, ixw, izw, prev + nxz, nz);
+    pts_inject(inj, ib0, ib1, 0, prev, prev + nxz);
     for (long i = 0; i < nxz; i++) {
         prev[i] *= (real)(0.5 * dt * dt) / rho[i];
         prev[nxz + i] *= (real)(0.5 * dt * dt) / rho[i];
@@ -808,10 +975,7 @@ int orc_propagate(const orc_grid* g, const real* model,
     for (int it = 0; it < nt - 1; it++) {
         if (!grad && sub > 0 && it % sub == 0 && full)
             memcpy(full + (long)(it / sub) * 2 * nxz, curr, 2 * nxz * sizeof(real));
-        if (!grad && next_ > 0) {
-            extract(curr, outx, nt, it, next_, exi, ezi, exw, ezw, nz);
-            extract(curr + nxz, outz, nt, it, next_, exi, ezi, exw, ezw, nz);
-        }
+        if (!grad && ext) pts_extract(ext, ob0, ob1, it, curr, curr + nxz);
         const int image_now = grad && (nt - 1 - it) % sub == 0 && it != 0;
         if (image_now && imaging_lag >= 0) {
             if (!imaging_lag) adjoint_derivatives(g, curr, ux, uzd);
@@ -825,8 +989,7 @@ int orc_propagate(const orc_grid* g, const real* model,
         real* nzt = next + nxz;
         rhs_volume(g, model, &c, cx, cz, nxt, nzt, ux, uzd, tmp);
         /* sources (we_op.cpp:722-724) */
-        inject(1, inx, nt, it, ninj, ixi, izi, ixw, izw, nxt, nz);
-        inject(1, inz, nt, it, ninj, ixi, izi, ixw, izw, nzt, nz);
+        pts_inject(inj, ib0, ib1, it, nxt, nzt);
         const real* px = prev;
         const real* pz = prev + nxz;
         rhs_sat(g, model, &c, cx, cz, px, pz, nxt, nzt);
@@ -854,16 +1017,31 @@ int orc_propagate(const orc_grid* g, const real* model,
 
     if (!grad && sub > 0 && (nt - 1) % sub == 0 && full)
         memcpy(full + (long)((nt - 1) / sub) * 2 * nxz, curr, 2 * nxz * sizeof(real));
-    if (!grad && next_ > 0) {
-        extract(curr, outx, nt, nt - 1, next_, exi, ezi, exw, ezw, nz);
-        extract(curr + nxz, outz, nt, nt - 1, next_, exi, ezi, exw, ezw, nz);
-    }
+    if (!grad && ext) pts_extract(ext, ob0, ob1, nt - 1, curr, curr + nxz);
     if (grad) {
         if (imaging_lag <= 0) adjoint_derivatives(g, curr, ux, uzd);
         compute_gradients(g, model, full, curr, ux, uzd, gtmp, grad, 0, sub * dt);
     }
     free(work);
     return 0;
+}
+
+/* propagate with delta_m3 injectors and extractors (the point-force, point-receiver case):
+   inx/inz [ninj][nt] source traces per component, outx/outz [next][nt] receiver traces
+   (accumulated), full [nsnap][2][nx][nz] snapshots (forward) or read (adjoint with grad). */
+int orc_propagate(const orc_grid* g, const real* model,
+                  int ninj, const int* ixi, const int* izi, const real* ixw, const real* izw,
+                  const real* inx, const real* inz,
+                  int next_, const int* exi, const int* ezi, const real* exw, const real* ezw,
+                  real* outx, real* outz, real* full, real* grad)
+{
+    pts inj, ext;
+    pts_delta(&inj, ninj, ixi, izi, ixw, izw, 0, g->nt, g->nz);
+    if (next_ > 0) pts_delta(&ext, next_, exi, ezi, exw, ezw, 1, g->nt, g->nz);
+    const int rc = propagate_pts(g, model, &inj, inx, inz, next_ > 0 ? &ext : NULL, outx, outz, full, grad);
+    pts_free(&inj);
+    if (next_ > 0) pts_free(&ext);
+    return rc;
 }
 
 /* ------------------------------------------------------------------------- */
@@ -1043,6 +1221,9 @@ typedef struct {
     int seismotype;        /* 0 displacement, 1 velocity */
     int resampling;        /* 0 sinc, 1 linear */
     int sinc_hl;
+    int mt;                /* 1: moment-tensor sources, allsrc [3][ns][nt] (mxx, mzz, mxz) */
+    double gl;             /* > 0: DAS receivers of this gauge length (dipole_m3) */
+    const double* rdip;    /* [nr_total] receiver dips (rad), used when gl > 0 */
 } orc_op;
 
 static void op_prepare(orc_op* op, const real* pmod, real** model_out)
@@ -1072,8 +1253,33 @@ static void resample(const orc_op* op, int adjoint, int add, int ntr, int ntm, d
     else orc_sinc(adjoint, add, ntr, ntm, dtm, ntd, dtd, op->sinc_hl, mod, dat);
 }
 
-/* allsrc: [2][ns][nt_data]; data: [2][nr_total][nt_data]; full (optional, sub>0):
-   [ns][1+nt/sub][2][nx][nz]. */
+/* The receivers of shot s (first receiver r0, n of them) as an injector (row 0) or
+   extractor (row 1): delta_m3 points, or dipole_m3 gauges when op->gl > 0; traces at
+   offsets r * nt from the component bases. */
+static void receiver_pts(const orc_op* op, int r0, int n, int row, int nt, pts* p)
+{
+    const int nx = op->g.nx, nz = op->g.nz;
+    double* rx = (double*)malloc(sizeof(double) * (n > 0 ? n : 1));
+    double* rz = (double*)malloc(sizeof(double) * (n > 0 ? n : 1));
+    for (int r = 0; r < n; r++) { rx[r] = op->rxz[2 * (r0 + r)]; rz[r] = op->rxz[2 * (r0 + r) + 1]; }
+    const int w = op->gl > 0 ? 6 : 3;
+    int* xi = (int*)malloc(sizeof(int) * w * (n > 0 ? n : 1));
+    int* zi = (int*)malloc(sizeof(int) * w * (n > 0 ? n : 1));
+    real* xw = (real*)malloc(sizeof(real) * 2 * w * (n > 0 ? n : 1));
+    real* zw = (real*)malloc(sizeof(real) * 2 * w * (n > 0 ? n : 1));
+    if (op->gl > 0) {
+        orc_dipole_m3(nx, nz, op->g.dx, op->g.dz, op->g.ox, op->g.oz, n, rx, rz, op->rdip + r0, op->gl, xi, zi, xw, zw);
+        pts_dipole(p, n, xi, zi, xw, zw, row, nt, nz);
+    } else {
+        orc_delta_m3(nx, nz, op->g.dx, op->g.dz, op->g.ox, op->g.oz, n, rx, rz, xi, zi, xw, zw);
+        pts_delta(p, n, xi, zi, xw, zw, row, nt, nz);
+    }
+    free(rx); free(rz); free(xi); free(zi); free(xw); free(zw);
+}
+
+/* allsrc: [2][ns][nt_data] point forces, or [3][ns][nt_data] moment tensors (mxx, mzz,
+   mxz) when op->mt; data: [2][nr_total][nt_data]; full (optional, sub>0):
+   [ns][1+nt/sub][2][nx][nz].  we_op.cpp:904-1055. */
 int orc_apply_forward(orc_op* op, int add, const real* pmod, const real* allsrc, real* data, real* full)
 {
     real* m;
@@ -1082,31 +1288,40 @@ int orc_apply_forward(orc_op* op, int add, const real* pmod, const real* allsrc,
     const double dtd = op->mp.dt_data, dt = op->g.dt;
     const long nxz = (long)nx * nz;
     const int nr = nr_total(op);
-    real* src = (real*)calloc((size_t)2 * ns * nt, sizeof(real));
+    const int nc = op->mt ? 3 : 2;
+    real* src = (real*)calloc((size_t)nc * ns * nt, sizeof(real));
     real* rcv = (real*)calloc((size_t)2 * nr * nt, sizeof(real));
-    resample(op, 0, 0, 2 * ns, ntd, dtd, nt, dt, (real*)allsrc, src);
+    resample(op, 0, 0, nc * ns, ntd, dtd, nt, dt, (real*)allsrc, src);
     const int nsnap = op->g.sub > 0 ? 1 + nt / op->g.sub : 0;
     int r0 = 0, rc = 0;
     for (int s = 0; s < ns && !rc; s++) {
         int ixi[3], izi[3];
-        real ixw[6], izw[6];
-        orc_delta_m3(nx, nz, op->g.dx, op->g.dz, op->g.ox, op->g.oz, 1, &op->sxz[2 * s], &op->sxz[2 * s + 1], ixi, izi, ixw, izw);
-        int n = op->nrec[s];
-        int* exi = (int*)malloc(sizeof(int) * 3 * n);
-        int* ezi = (int*)malloc(sizeof(int) * 3 * n);
-        real* exw = (real*)malloc(sizeof(real) * 6 * n);
-        real* ezw = (real*)malloc(sizeof(real) * 6 * n);
-        double* rx = (double*)malloc(sizeof(double) * n);
-        double* rz = (double*)malloc(sizeof(double) * n);
-        for (int r = 0; r < n; r++) { rx[r] = op->rxz[2 * (r0 + r)]; rz[r] = op->rxz[2 * (r0 + r) + 1]; }
-        orc_delta_m3(nx, nz, op->g.dx, op->g.dz, op->g.ox, op->g.oz, n, rx, rz, exi, ezi, exw, ezw);
+        real ixw[12], izw[12];
+        pts inj, ext;
+        const real *ib0, *ib1;
+        if (op->mt) {
+            orc_ddelta_m3(nx, nz, op->g.dx, op->g.dz, op->g.ox, op->g.oz, 1, &op->sxz[2 * s], &op->sxz[2 * s + 1],
+                          ixi, izi, ixw, izw);
+            pts_ddelta(&inj, 1, ixi, izi, ixw, izw, (long)s * nt, (long)(ns + s) * nt, (long)(2 * ns + s) * nt, nt, nz);
+            ib0 = ib1 = src;
+        } else {
+            orc_delta_m3(nx, nz, op->g.dx, op->g.dz, op->g.ox, op->g.oz, 1, &op->sxz[2 * s], &op->sxz[2 * s + 1],
+                         ixi, izi, ixw, izw);
+            pts_delta(&inj, 1, ixi, izi, ixw, izw, 0, nt, nz);
+            ib0 = src + (long)s * nt;
+            ib1 = src + (long)(ns + s) * nt;
+        }
+        const int n = op->nrec[s];
+        receiver_pts(op, r0, n, 1, nt, &ext);
         real* fs = full && nsnap ? full + (long)s * nsnap * 2 * nxz : NULL;
-        rc = orc_propagate(&op->g, m, 1, ixi, izi, ixw, izw, src + (long)s * nt, src + (long)(ns + s) * nt,
-                           n, exi, ezi, exw, ezw, rcv + (long)r0 * nt, rcv + (long)(nr + r0) * nt, fs, NULL);
-        free(exi); free(ezi); free(exw); free(ezw); free(rx); free(rz);
+        rc = propagate_pts(&op->g, m, &inj, ib0, ib1, n > 0 ? &ext : NULL, rcv + (long)r0 * nt,
+                           rcv + (long)(nr + r0) * nt, fs, NULL);
+        pts_free(&inj);
+        pts_free(&ext);
         r0 += n;
     }
     if (!rc) {
+        if (op->gl > 0) orc_dipole_to_strain(0, rcv, op->rdip, nr, nt, op->gl);
         if (op->seismotype == 1) {
             real* t = (real*)calloc((size_t)2 * nr * nt, sizeof(real));
             orc_Dt(0, 2 * nr, nt, dt, rcv, t);
@@ -1152,20 +1367,15 @@ int orc_apply_jacobianT(orc_op* op, int add, real* grad, const real* pmod0, cons
         free(rcv);
         rcv = t;
     }
+    if (op->gl > 0) orc_dipole_to_strain(1, rcv, op->rdip, nr, nt, op->gl);  /* we_op.cpp:1130-1135 */
     int r0 = 0, rc = 0;
     for (int s = 0; s < ns && !rc; s++) {
-        int n = op->nrec[s];
-        int* ixi = (int*)malloc(sizeof(int) * 3 * n);
-        int* izi = (int*)malloc(sizeof(int) * 3 * n);
-        real* ixw = (real*)malloc(sizeof(real) * 6 * n);
-        real* izw = (real*)malloc(sizeof(real) * 6 * n);
-        double* rx = (double*)malloc(sizeof(double) * n);
-        double* rz = (double*)malloc(sizeof(double) * n);
-        for (int r = 0; r < n; r++) { rx[r] = op->rxz[2 * (r0 + r)]; rz[r] = op->rxz[2 * (r0 + r) + 1]; }
-        orc_delta_m3(nx, nz, op->g.dx, op->g.dz, op->g.ox, op->g.oz, n, rx, rz, ixi, izi, ixw, izw);
-        rc = orc_propagate(&op->g, m, n, ixi, izi, ixw, izw, rcv + (long)r0 * nt, rcv + (long)(nr + r0) * nt,
-                           0, NULL, NULL, NULL, NULL, NULL, NULL, (real*)full + (long)s * nsnap * 2 * nxz, g);
-        free(ixi); free(izi); free(ixw); free(izw); free(rx); free(rz);
+        const int n = op->nrec[s];
+        pts inj;
+        receiver_pts(op, r0, n, 0, nt, &inj);
+        rc = propagate_pts(&op->g, m, &inj, rcv + (long)r0 * nt, rcv + (long)(nr + r0) * nt, NULL, NULL, NULL,
+                           (real*)full + (long)s * nsnap * 2 * nxz, g);
+        pts_free(&inj);
         r0 += n;
     }
     if (!rc) {

@@ -38,7 +38,8 @@ class OrcModelPar(C.Structure):
 class OrcOp(C.Structure):
     _fields_ = [("g", OrcGrid), ("mp", OrcModelPar), ("ns", C.c_int), ("sxz", C.POINTER(C.c_double)),
                 ("nrec", C.POINTER(C.c_int)), ("rxz", C.POINTER(C.c_double)), ("seismotype", C.c_int),
-                ("resampling", C.c_int), ("sinc_hl", C.c_int)]
+                ("resampling", C.c_int), ("sinc_hl", C.c_int), ("mt", C.c_int), ("gl", C.c_double),
+                ("rdip", C.POINTER(C.c_double))]
 
 
 def grid_struct(p):
@@ -243,8 +244,46 @@ class Oracle:
         op.seismotype = prob.get("seismotype", 0)
         op.resampling = prob.get("resampling", 0)
         op.sinc_hl = prob.get("sinc_hl", 11)
-        op._keep = (sxz, nrec, rxz)
+        op.mt = int(prob.get("mt", 0))
+        op.gl = float(prob.get("gl", 0.0))
+        rdip = np.ascontiguousarray(np.concatenate(prob["rdip"]) if "rdip" in prob else np.zeros(len(rxz)),
+                                    dtype=np.float64)
+        op.rdip = _ptr(rdip, C.c_double)
+        op._keep = (sxz, nrec, rxz, rdip)
         return op
+
+    def ddelta_m3(self, nx, nz, dx, dz, ox, oz, locx, locz):
+        """ddelta_m3 footprints: (xind [n][3], zind [n][3], xw [n][2][6], zw [n][2][6])."""
+        locx = np.ascontiguousarray(locx, np.float64)
+        locz = np.ascontiguousarray(locz, np.float64)
+        n = len(locx)
+        xi, zi = np.zeros((n, 3), np.int32), np.zeros((n, 3), np.int32)
+        xw, zw = np.zeros((n, 2, 6), self.dtype), np.zeros((n, 2, 6), self.dtype)
+        self.lib.orc_ddelta_m3(nx, nz, C.c_double(dx), C.c_double(dz), C.c_double(ox), C.c_double(oz), n,
+                               _ptr(locx, C.c_double), _ptr(locz, C.c_double), _ptr(xi, C.c_int), _ptr(zi, C.c_int),
+                               self.p(xw), self.p(zw))
+        return xi, zi, xw, zw
+
+    def dipole_m3(self, nx, nz, dx, dz, ox, oz, locx, locz, dip, gl):
+        """dipole_m3 footprints: (xind [n][6], zind [n][6], xw [n][2][6], zw [n][2][6])."""
+        locx = np.ascontiguousarray(locx, np.float64)
+        locz = np.ascontiguousarray(locz, np.float64)
+        dip = np.ascontiguousarray(dip, np.float64)
+        n = len(locx)
+        xi, zi = np.zeros((n, 6), np.int32), np.zeros((n, 6), np.int32)
+        xw, zw = np.zeros((n, 2, 6), self.dtype), np.zeros((n, 2, 6), self.dtype)
+        self.lib.orc_dipole_m3(nx, nz, C.c_double(dx), C.c_double(dz), C.c_double(ox), C.c_double(oz), n,
+                               _ptr(locx, C.c_double), _ptr(locz, C.c_double), _ptr(dip, C.c_double),
+                               C.c_double(gl), _ptr(xi, C.c_int), _ptr(zi, C.c_int), self.p(xw), self.p(zw))
+        return xi, zi, xw, zw
+
+    def dipole_to_strain(self, adj, traces, dip, gl):
+        """dipole_to_strain on [2][ntr][nt] (a copy)."""
+        t = self.arr(traces).copy()
+        dip = np.ascontiguousarray(dip, np.float64)
+        self.lib.orc_dipole_to_strain(int(adj), self.p(t), _ptr(dip, C.c_double), t.shape[1], t.shape[2],
+                                      C.c_double(gl))
+        return t
 
     def apply_forward(self, op, model, allsrc, keep_full=False):
         model = self.arr(model)
