@@ -85,15 +85,18 @@ def parse():
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_step_iso_64shots_500x1700_traffic.md")
 
 
-def measured_traffic():
-    """HBM bytes per k_step launch from the committed counter summary (FETCH_SIZE and
-    WRITE_SIZE per dispatch, calibrated against a copy kernel of known size), or None."""
+def measured_traffic(spb=None):
+    """HBM bytes per forward step from the committed counter summary (FETCH_SIZE and
+    WRITE_SIZE per dispatch, calibrated against a copy kernel of known size) for the
+    core launch's shots per workgroup spb (the autotuner's choice), or None."""
     try:
-        for line in open(TRAFFIC_FILE):
-            if line.startswith("traffic_bytes_per_launch:"):
-                return float(line.split(":")[1])
+        lines = open(TRAFFIC_FILE).read().splitlines()
     except OSError:
-        pass
+        return None
+    for key in (["traffic_bytes_per_launch_spb%d:" % spb] if spb else []) + ["traffic_bytes_per_launch:"]:
+        for line in lines:
+            if line.startswith(key):
+                return float(line.split(":")[1])
     return None
 
 
@@ -412,6 +415,9 @@ def fwi_run(args):
     t_all = time.perf_counter()
     for fc in bands:
         filt = lowpass_filter(fc, dt)
+        t1 = time.perf_counter()
+        fwi2d_amd.filter_apply(filt, dobs)  # one pass over every trace: an evaluation makes two
+        t_filter = time.perf_counter() - t1
         obj = fwi2d_amd.Nlls(op, dobs, gmask=gmask, normalize=True, filter=filt)
         count = [0]
 
@@ -424,7 +430,8 @@ def fwi_run(args):
         m1, hist, k, info = fwi2d_amd.lbfgs(evaluate, m, niter=args.fwi, max_trial=args.fwi_trials, a1=args.fwi_a1,
                                             report=True)
         el = time.perf_counter() - t0
-        out.append({"band_hz": fc, "filter_taps": int(filt.size), "iterations": k, "evaluations": count[0],
+        out.append({"band_hz": fc, "filter_taps": int(filt.size), "filter_pass_seconds": round(t_filter, 3),
+                    "iterations": k, "evaluations": count[0],
                     "seconds": round(el, 1), "functional_history": [float(v) for v in hist],
                     "model_rel_err": [err(m), err(m1)], "vp_rel_err": [vp_err(m), vp_err(m1)],
                     "vp_rel_err_below_200m": [vp_err_deep(m), vp_err_deep(m1)],
@@ -500,7 +507,7 @@ def main():
     gcells = shots_per_s * cells_per_grad / 1e9
     grad_norm, misfit = B.grad_norm, B.misfit
     grad_gbs = BYTES_PER_GRADIENT_CELL_STEP[vti] * ns * nt * nx * nz / (ms_per_step / 1000.0) / 1e9
-    traffic = measured_traffic() if (args.workload, ns) == ("c2", 64) else None
+    traffic = measured_traffic(B.eng.stat("shots_per_block")) if (args.workload, ns) == ("c2", 64) else None
 
     # ---- dominant kernel: fused step, timed with HIP events on its stream ----
     # (eager launches: one step = the core-box launch plus the edge-band launches beside it)

@@ -59,14 +59,14 @@ struct DevMem {
 struct InjList {
     std::vector<InjCell> cells;
     std::vector<int> ptr;
-    std::vector<long long> ctrace;
+    std::vector<long long> ctx, ctz;  // trace offsets read by the x and z field components
     std::vector<float> cw;
-    DevMem d_cells, d_ptr, d_ctrace, d_cw;
+    DevMem d_cells, d_ptr, d_ctx, d_ctz, d_cw;
     void build(const std::vector<PointEntry>& ents, hipStream_t s)
     {
         struct C {
             int shot, gx, gz, k;
-            long long trace;
+            long long tx, tz;
             float w;
         };
         std::vector<C> all;
@@ -75,7 +75,7 @@ struct InjList {
             const PointEntry& e = ents[k];
             for (int ix = 0; ix < 3; ix++)
                 for (int iz = 0; iz < 3; iz++)
-                    all.push_back({e.shot, e.xi0 + ix, e.zi0 + iz, k, e.trace, e.wx[ix] * e.wz[iz]});
+                    all.push_back({e.shot, e.xi0 + ix, e.zi0 + iz, k, e.trace, e.tz, e.wx[ix] * e.wz[iz]});
         }
         std::stable_sort(all.begin(), all.end(), [](const C& a, const C& b) {
             if (a.shot != b.shot) return a.shot < b.shot;
@@ -85,24 +85,27 @@ struct InjList {
         });
         cells.clear();
         ptr.assign(1, 0);
-        ctrace.clear();
+        ctx.clear();
+        ctz.clear();
         cw.clear();
         for (size_t i = 0; i < all.size(); i++) {
             if (i == 0 || all[i].shot != all[i - 1].shot || all[i].gx != all[i - 1].gx || all[i].gz != all[i - 1].gz) {
-                if (i > 0) ptr.push_back((int)ctrace.size());
+                if (i > 0) ptr.push_back((int)ctx.size());
                 cells.push_back({all[i].shot, all[i].gx, all[i].gz, 0});
             }
-            ctrace.push_back(all[i].trace);
+            ctx.push_back(all[i].tx);
+            ctz.push_back(all[i].tz);
             cw.push_back(all[i].w);
         }
-        if (!all.empty()) ptr.push_back((int)ctrace.size());
+        if (!all.empty()) ptr.push_back((int)ctx.size());
         auto up = [&](DevMem& d, const void* h, size_t b) {
             d.reserve(std::max<size_t>(b, 16));
             if (b) HIPCHECK(hipMemcpyAsync(d.p, h, b, hipMemcpyHostToDevice, s));
         };
         up(d_cells, cells.data(), cells.size() * sizeof(InjCell));
         up(d_ptr, ptr.data(), ptr.size() * sizeof(int));
-        up(d_ctrace, ctrace.data(), ctrace.size() * sizeof(long long));
+        up(d_ctx, ctx.data(), ctx.size() * sizeof(long long));
+        up(d_ctz, ctz.data(), ctz.size() * sizeof(long long));
         up(d_cw, cw.data(), cw.size() * sizeof(float));
     }
 };
@@ -165,6 +168,8 @@ struct Engine {
 
     // acquisition
     int ns = 0, nr = 0;
+    int src_nc = 2;        // source trace components: 2 (force x, z) or 3 (mxx, mzz, mxz)
+    double gauge = 0.0;    // > 0: DAS receivers of this gauge length
     std::vector<PointEntry> src_ents, rcv_ents;  // flat, ordered by shot
     std::vector<int> src_shot_ptr, rcv_shot_ptr;
     DevMem d_src_ents, d_rcv_ents, d_src_shot_ptr, d_rcv_shot_ptr;
@@ -472,77 +477,150 @@ struct Engine {
     }
 
     // ------------------------------------------------------------------
-    // acquisition: delta_m3 weights (injector.cpp:3-64), tile lists
+    // acquisition: the 3x3 footprints of delta_m3 (injector.cpp:3-64), ddelta_m3
+    // (injector.cpp:114-190) and dipole_m3 (injector.cpp:241-331)
     // ------------------------------------------------------------------
-    PointEntry make_point(double x, double z, double ox, double oz, int shot, long long trace) const
+    struct Pivot {
+        int px, pz;     // footprint centre
+        double ax, az;  // position relative to it, in cells
+    };
+    Pivot pivot(double x, double z, double ox, double oz) const
     {
         const int nx = g.nx, nz = g.nz;
-        const double dx = g.dx, dz = g.dz;
-        int px = (int)std::lround((x - ox) / dx);
-        int pz = (int)std::lround((z - oz) / dz);
+        int px = (int)std::lround((x - ox) / g.dx);
+        int pz = (int)std::lround((z - oz) / g.dz);
         if (px == 0) px = 1;
         if (px == nx - 1) px = nx - 2;
         if (pz == 0) pz = 1;
         if (pz == nz - 1) pz = nz - 2;
         if (px < 1 || px > nx - 2 || pz < 1 || pz > nz - 2)
             throw std::runtime_error("source or receiver falls outside the model");
-        const double ax = (x - dx * px - ox) / dx, az = (z - dz * pz - oz) / dz;
+        return {px, pz, (x - g.dx * px - ox) / g.dx, (z - g.dz * pz - oz) / g.dz};
+    }
+    // injection weights (a^2 - a)/(2 h H), (1 - a^2)/(h H), (a^2 + a)/(2 h H) around pivot p;
+    // with deriv, the derivative weights (1 - 2a)/(2 h^2 H), 2a/(h^2 H), (-1 - 2a)/(2 h^2 H)
+    static void inj_weights(double a, int p, int n, double h, bool deriv, float* w)
+    {
+        const double h0 = h_weight(n, p - 1), h1 = h_weight(n, p), h2 = h_weight(n, p + 1);
+        if (!deriv) {
+            w[0] = (float)((a * a - a) / (2 * h * h0));
+            w[1] = (float)((1 - a * a) / (h * h1));
+            w[2] = (float)((a * a + a) / (2 * h * h2));
+        } else {
+            w[0] = (float)((1 - 2 * a) / (2 * h * h * h0));
+            w[1] = (float)(2 * a / (h * h * h1));
+            w[2] = (float)((-1 - 2 * a) / (2 * h * h * h2));
+        }
+    }
+    // extraction (Lagrange) weights (a^2 - a)/2, 1 - a^2, (a^2 + a)/2
+    static void ext_weights(double a, float* w)
+    {
+        w[0] = (float)((a * a - a) / 2);
+        w[1] = (float)(1 - a * a);
+        w[2] = (float)((a * a + a) / 2);
+    }
+    PointEntry footprint(const Pivot& q, int shot, long long tx, long long tz) const
+    {
         PointEntry e{};
-        e.xi0 = px - 1;
-        e.zi0 = pz - 1;
-        e.wx[0] = (float)((ax * ax - ax) / (2 * dx * h_weight(nx, px - 1)));
-        e.wx[1] = (float)((1 - ax * ax) / (dx * h_weight(nx, px)));
-        e.wx[2] = (float)((ax * ax + ax) / (2 * dx * h_weight(nx, px + 1)));
-        e.wz[0] = (float)((az * az - az) / (2 * dz * h_weight(nz, pz - 1)));
-        e.wz[1] = (float)((1 - az * az) / (dz * h_weight(nz, pz)));
-        e.wz[2] = (float)((az * az + az) / (2 * dz * h_weight(nz, pz + 1)));
-        e.trace = trace;
+        e.xi0 = q.px - 1;
+        e.zi0 = q.pz - 1;
+        e.trace = tx;
+        e.tz = tz;
         e.shot = shot;
         return e;
     }
 
-    // extraction weights (Lagrange, injector.cpp:55-61) of the same point
-    static PointEntry extraction_weights(PointEntry e, double x, double z, double ox, double oz, double dx, double dz)
-    {
-        const double ax = (x - dx * (e.xi0 + 1) - ox) / dx, az = (z - dz * (e.zi0 + 1) - oz) / dz;
-        e.wx[0] = (float)((ax * ax - ax) / 2);
-        e.wx[1] = (float)(1 - ax * ax);
-        e.wx[2] = (float)((ax * ax + ax) / 2);
-        e.wz[0] = (float)((az * az - az) / 2);
-        e.wz[1] = (float)(1 - az * az);
-        e.wz[2] = (float)((az * az + az) / 2);
-        return e;
-    }
+    std::vector<PointEntry> rcv_ext_ents;  // receiver footprints with extraction weights
+    std::vector<int> rcv_ext_ptr;          // footprints of receiver r: rcv_ext_ptr[r] .. [r+1]
+    DevMem d_rcv_ext_ents, d_rcv_ext_ptr;
 
-    std::vector<PointEntry> rcv_ext_ents;  // receivers with extraction weights
-    DevMem d_rcv_ext_ents;
-
-    void set_shots(int nshots, const double* sxz, const int* nrec, const double* rxz, double ox, double oz)
+    void set_shots(int nshots, const double* sxz, const int* nrec, const double* rxz, double ox, double oz, int mt = 0,
+                   double gl = 0.0, const double* rdip = nullptr)
     {
         if (nshots < 1) throw std::runtime_error("need at least one shot");
-        ns = nshots;
-        nr = 0;
-        for (int s = 0; s < ns; s++) {
+        if (gl > 0 && !rdip) throw std::runtime_error("DAS receivers need their dips");
+        int nrt = 0;
+        for (int s = 0; s < nshots; s++) {
             if (nrec[s] < 0) throw std::runtime_error("negative receiver count");
-            nr += nrec[s];
+            nrt += nrec[s];
         }
         const int nt = g.nt;
-        src_ents.clear();
-        rcv_ents.clear();
-        rcv_ext_ents.clear();
-        src_shot_ptr.assign(ns + 1, 0);
-        rcv_shot_ptr.assign(ns + 1, 0);
+        std::vector<PointEntry> se, re, xe;
+        std::vector<int> sp(nshots + 1, 0), rp(nshots + 1, 0), xp(1, 0);
         int r = 0;
-        for (int s = 0; s < ns; s++) {
-            src_ents.push_back(make_point(sxz[2 * s], sxz[2 * s + 1], ox, oz, s, (long long)s * nt));
-            src_shot_ptr[s + 1] = s + 1;
-            for (int k = 0; k < nrec[s]; k++, r++) {
-                PointEntry e = make_point(rxz[2 * r], rxz[2 * r + 1], ox, oz, s, (long long)r * nt);
-                rcv_ents.push_back(e);
-                rcv_ext_ents.push_back(extraction_weights(e, rxz[2 * r], rxz[2 * r + 1], ox, oz, g.dx, g.dz));
+        for (int s = 0; s < nshots; s++) {
+            const Pivot q = pivot(sxz[2 * s], sxz[2 * s + 1], ox, oz);
+            if (!mt) {  // point force: x and z traces of shot s
+                PointEntry e = footprint(q, s, (long long)s * nt, (long long)(nshots + s) * nt);
+                inj_weights(q.ax, q.px, g.nx, g.dx, false, e.wx);
+                inj_weights(q.az, q.pz, g.nz, g.dz, false, e.wz);
+                se.push_back(e);
+            } else {
+                // moment tensor [3][ns][nt] = (mxx, mzz, mxz): the force -d/dx (mxx, mxz) and
+                // -d/dz (mxz, mzz) of the delta (we_op.cpp:640-654; injector.cpp:191-217)
+                const long long txx = (long long)s * nt, tzz = (long long)(nshots + s) * nt,
+                                txz = (long long)(2 * nshots + s) * nt;
+                PointEntry a = footprint(q, s, txx, txz), b = footprint(q, s, txz, tzz);
+                inj_weights(q.ax, q.px, g.nx, g.dx, true, a.wx);
+                inj_weights(q.az, q.pz, g.nz, g.dz, false, a.wz);
+                inj_weights(q.ax, q.px, g.nx, g.dx, false, b.wx);
+                inj_weights(q.az, q.pz, g.nz, g.dz, true, b.wz);
+                for (int i = 0; i < 3; i++) {
+                    a.wx[i] = -a.wx[i];
+                    b.wz[i] = -b.wz[i];
+                }
+                se.push_back(a);
+                se.push_back(b);
             }
-            rcv_shot_ptr[s + 1] = r;
+            sp[s + 1] = (int)se.size();
+            for (int k = 0; k < nrec[s]; k++, r++) {
+                const double x = rxz[2 * r], z = rxz[2 * r + 1];
+                const long long tx = (long long)r * nt, tz = (long long)(nrt + r) * nt;
+                if (gl <= 0) {
+                    const Pivot p = pivot(x, z, ox, oz);
+                    PointEntry e = footprint(p, s, tx, tz);
+                    inj_weights(p.ax, p.px, g.nx, g.dx, false, e.wx);
+                    inj_weights(p.az, p.pz, g.nz, g.dz, false, e.wz);
+                    re.push_back(e);
+                    ext_weights(p.ax, e.wx);
+                    ext_weights(p.az, e.wz);
+                    xe.push_back(e);
+                } else {
+                    // DAS channel: the gauge ends along the dip; injection -first + second end,
+                    // extraction second - first end (injector.cpp:333-378)
+                    const double c = std::cos(rdip[r]), sn = std::sin(rdip[r]);
+                    const Pivot p1 = pivot(x - gl / 2 * c, z - gl / 2 * sn, ox, oz);
+                    const Pivot p2 = pivot(x + gl / 2 * c, z + gl / 2 * sn, ox, oz);
+                    PointEntry e1 = footprint(p1, s, tx, tz), e2 = footprint(p2, s, tx, tz);
+                    inj_weights(p1.ax, p1.px, g.nx, g.dx, false, e1.wx);
+                    inj_weights(p1.az, p1.pz, g.nz, g.dz, false, e1.wz);
+                    inj_weights(p2.ax, p2.px, g.nx, g.dx, false, e2.wx);
+                    inj_weights(p2.az, p2.pz, g.nz, g.dz, false, e2.wz);
+                    for (int i = 0; i < 3; i++) e1.wx[i] = -e1.wx[i];
+                    re.push_back(e1);
+                    re.push_back(e2);
+                    ext_weights(p2.ax, e2.wx);
+                    ext_weights(p2.az, e2.wz);
+                    ext_weights(p1.ax, e1.wx);
+                    ext_weights(p1.az, e1.wz);
+                    for (int i = 0; i < 3; i++) e1.wx[i] = -e1.wx[i];
+                    xe.push_back(e2);
+                    xe.push_back(e1);
+                }
+                xp.push_back((int)xe.size());
+            }
+            rp[s + 1] = (int)re.size();
         }
+        ns = nshots;
+        nr = nrt;
+        src_nc = mt ? 3 : 2;
+        gauge = gl > 0 ? gl : 0.0;
+        src_ents.swap(se);
+        rcv_ents.swap(re);
+        rcv_ext_ents.swap(xe);
+        rcv_ext_ptr.swap(xp);
+        src_shot_ptr.swap(sp);
+        rcv_shot_ptr.swap(rp);
         auto up = [&](DevMem& d, const void* h, size_t b) {
             d.reserve(std::max<size_t>(b, 16));
             if (b) HIPCHECK(hipMemcpyAsync(d.p, h, b, hipMemcpyHostToDevice, st));
@@ -550,6 +628,7 @@ struct Engine {
         up(d_src_ents, src_ents.data(), src_ents.size() * sizeof(PointEntry));
         up(d_rcv_ents, rcv_ents.data(), rcv_ents.size() * sizeof(PointEntry));
         up(d_rcv_ext_ents, rcv_ext_ents.data(), rcv_ext_ents.size() * sizeof(PointEntry));
+        up(d_rcv_ext_ptr, rcv_ext_ptr.data(), rcv_ext_ptr.size() * sizeof(int));
         up(d_src_shot_ptr, src_shot_ptr.data(), src_shot_ptr.size() * sizeof(int));
         up(d_rcv_shot_ptr, rcv_shot_ptr.data(), rcv_shot_ptr.size() * sizeof(int));
         inj_src.build(src_ents, st);
@@ -649,10 +728,10 @@ struct Engine {
         L.next = 2;
         if (adjoint)
             launch_init_prev(L.P(), 2 * nxz, nxz, ldz, ns, d_rcv_shot_ptr.as<int>(), d_rcv_ents.as<PointEntry>(), traces,
-                             (long long)nr * g.nt, st);
+                             st);
         else
             launch_init_prev(L.P(), 2 * nxz, nxz, ldz, ns, d_src_shot_ptr.as<int>(), d_src_ents.as<PointEntry>(), traces,
-                             (long long)ns * g.nt, st);
+                             st);
         launch_scale_prev(L.P(), d_model.as<float>() + 2 * nxz, (float)(0.5 * g.dt * g.dt), nxz, ns, st);
     }
 
@@ -691,8 +770,8 @@ struct Engine {
         }
         // extraction of the current level (forward)
         if (rcv)
-            launch_extract(L.C(), 2 * nxz, nxz, ldz, d_rcv_ext_ents.as<PointEntry>(), nr, rcv, (long long)nr * g.nt, it,
-                           st);
+            launch_extract(L.C(), 2 * nxz, nxz, ldz, d_rcv_ext_ents.as<PointEntry>(), d_rcv_ext_ptr.as<int>(), nr, rcv,
+                           it, st);
         // edge bands on their own stream, concurrent with the core boxes (disjoint cells),
         // or after them on the same stream
         StepArgs e = a;
@@ -732,8 +811,8 @@ struct Engine {
             im.rho_off = (long long)(a.stiff ? 4 : 2) * nxz;
         }
         launch_inject_cells(inj.d_cells.as<InjCell>(), (int)inj.cells.size(), inj.d_ptr.as<int>(),
-                            inj.d_ctrace.as<long long>(), inj.d_cw.as<float>(), traces,
-                            (long long)(adjoint ? nr : ns) * g.nt, it, L.N(), 2 * nxz, nxz, g.nx, g.nz, ldz,
+                            inj.d_ctx.as<long long>(), inj.d_ctz.as<long long>(), inj.d_cw.as<float>(), traces, it,
+                            L.N(), 2 * nxz, nxz, g.nx, g.nz, ldz,
                             d_deriv.as<float>(), d_tx.as<float>(), d_tz.as<float>(), d_div_tb.as<float>(),
                             d_div_lr.as<float>(), im, st);
         if (img_idx >= 0 && !fuse_img) {
@@ -766,7 +845,9 @@ struct Engine {
             for (int k = nsnap; k > K; k--) {
                 const int nseg_k = (nsnap + k - 1) / k;
                 if (2.0 * (nseg_k - 1) * ck_level + (k + 2.0) * level <= budget) {
-                    K = k;
+                    // the same number of segments of balanced length, so that the last one
+                    // (kept whole) is not a short remainder
+                    K = (nsnap + nseg_k - 1) / nseg_k;
                     break;
                 }
             }
@@ -869,7 +950,8 @@ struct Engine {
                 opt_bf16, opt_stiff, opt_edge_stream, opt_spb_img, opt_image_fused, opt_img_buffers, opt_img_stream, opt_img_sbp, P(d_ckpt16.p), P(d_ckpt_scale.p), P(d_amax.p),
                 P(d_fwd.p), P(d_adj.p), P(d_ckpt.p), P(d_seg.p), P(d_gshot.p), P(d_model.p), P(d_deriv.p),
                 P(d_boxes.p), P(d_list_core.p), P(d_list_bz.p), P(d_list_bz8.p), P(d_list_bx.p), P(inj_src.d_cells.p), P(inj_src.d_ptr.p),
-                P(inj_src.d_ctrace.p), P(inj_src.d_cw.p), P(inj_rcv.d_cells.p), P(inj_rcv.d_ptr.p), P(inj_rcv.d_ctrace.p),
+                P(inj_src.d_ctx.p), P(inj_src.d_ctz.p), P(inj_src.d_cw.p), P(inj_rcv.d_cells.p), P(inj_rcv.d_ptr.p),
+                P(inj_rcv.d_ctx.p), P(inj_rcv.d_ctz.p), P(d_rcv_ext_ptr.p), src_nc,
                 P(inj_rcv.d_cw.p), (long long)inj_src.cells.size(), (long long)inj_rcv.cells.size(), P(d_src_ents.p),
                 P(d_rcv_ents.p), P(d_rcv_ext_ents.p), P(d_src_shot_ptr.p), P(d_rcv_shot_ptr.p), P(d_tx.p), P(d_tz.p),
                 P(d_div_tb.p), P(d_div_lr.p), (long long)list_core.size(), (long long)list_bz.size(), (long long)list_bz8.size(),
@@ -987,7 +1069,7 @@ struct Engine {
         require_ready();
         if (keep && g.sub <= 0) throw std::runtime_error("checkpoints need sub > 0");
         ckpt_epoch = -1;  // any forward run replaces the recomputation sources
-        const long long ntr_s = (long long)2 * ns * g.nt, ntr_r = (long long)2 * nr * g.nt;
+        const long long ntr_s = (long long)src_nc * ns * g.nt, ntr_r = (long long)2 * nr * g.nt;
         const float* dsrc = src_dev ? src : upload_traces(d_src, src, ntr_s);
         if (opt_autotune && !tuned) autotune(dsrc);
         float* drcv = rcv;
@@ -1040,7 +1122,7 @@ struct Engine {
         }
         if (keep) capture(L, nt - 1, jlast);
         if (nr > 0)
-            launch_extract(L.C(), 2 * nxz, nxz, ldz, d_rcv_ext_ents.as<PointEntry>(), nr, drcv, (long long)nr * nt,
+            launch_extract(L.C(), 2 * nxz, nxz, ldz, d_rcv_ext_ents.as<PointEntry>(), d_rcv_ext_ptr.as<int>(), nr, drcv,
                            nt - 1, st);
         if (!rcv_dev && nr > 0)
             HIPCHECK(hipMemcpyAsync(rcv, drcv, (size_t)ntr_r * sizeof(float), hipMemcpyDeviceToHost, st));
@@ -1435,6 +1517,16 @@ int fwi2d_engine_set_shots(fwi2d_engine* e, int nshots, const double* sxz, const
         if (!e || !sxz || !nrec) throw std::runtime_error("null argument");
         HIPCHECK(hipSetDevice(e->impl->device));
         e->impl->set_shots(nshots, sxz, nrec, rxz, ox, oz);
+    });
+}
+
+int fwi2d_engine_set_acquisition(fwi2d_engine* e, int nshots, const double* sxz, const int* nrec, const double* rxz,
+                                 double ox, double oz, int moment_tensor, double gauge_length, const double* rdip)
+{
+    return guarded([&] {
+        if (!e || !sxz || !nrec) throw std::runtime_error("null argument");
+        HIPCHECK(hipSetDevice(e->impl->device));
+        e->impl->set_shots(nshots, sxz, nrec, rxz, ox, oz, moment_tensor != 0, gauge_length, rdip);
     });
 }
 

@@ -369,16 +369,24 @@ struct TraceFilter {
     bool empty() const { return f.empty(); }
 
     // y = C x (adjoint: C^T x); x and y must not overlap
-    void apply(bool adjoint, const double* x, double* y, Work& w) const
+    void apply(bool adjoint, const double* x, double* y, Work& w) const { apply2(adjoint, x, y, nullptr, nullptr, w); }
+
+    // y0 = C x0 and, when x1 is given, y1 = C x1, with one complex FFT pair for both on the
+    // FFT path: x0 + i x1 is transformed, and since the filter is real the real and
+    // imaginary parts of the filtered signal are the two filtered traces.  No output may
+    // overlap an input.
+    void apply2(bool adjoint, const double* x0, double* y0, const double* x1, double* y1, Work& w) const
     {
         if (!N) {
-            conv_trace(f, centered, adjoint, x, y, n);
+            conv_trace(f, centered, adjoint, x0, y0, n);
+            if (x1) conv_trace(f, centered, adjoint, x1, y1, n);
             return;
         }
         const int sh = centered ? ((int)f.size() + 1) / 2 - 1 : 0;
         w.re.assign(N, 0.0);
         w.im.assign(N, 0.0);
-        std::copy(x, x + n, w.re.begin());
+        std::copy(x0, x0 + n, w.re.begin());
+        if (x1) std::copy(x1, x1 + n, w.im.begin());
         fft_pow2(w.re, w.im, twr, twi, false);
         for (size_t k = 0; k < N; k++) {
             const double a = w.re[k], b = w.im[k], c = fr[k], d = adjoint ? -fi[k] : fi[k];
@@ -389,14 +397,15 @@ struct TraceFilter {
         const double s = 1.0 / (double)N;
         for (int t = 0; t < n; t++) {
             const long long m = adjoint ? ((long long)t - sh + (long long)N) % (long long)N : (long long)t + sh;
-            y[t] = w.re[(size_t)m] * s;
+            y0[t] = w.re[(size_t)m] * s;
+            if (x1) y1[t] = w.im[(size_t)m] * s;
         }
     }
 };
 
-// per-thread scratch of the objective's trace loop
+// per-thread scratch of the objective's trace loop (a pair of traces at a time)
 struct TraceWork {
-    std::vector<double> v, s, c;
+    std::vector<double> v[2], s[2], c[2];
     TraceFilter::Work fw;
 };
 
@@ -480,10 +489,10 @@ std::vector<double> prepare_filter(const float* f, int nf, int phase)
 
 struct fwi2d_op {
     fwi2d_op_desc d{};
-    std::vector<double> sxz, rxz;
+    std::vector<double> sxz, rxz, rdip;
     std::vector<int> nrec;
     int nr = 0;
-    std::vector<double> allsrc;  // [2][ns][nt_data]
+    std::vector<double> allsrc;  // [nc][ns][nt_data], nc = 2 (forces) or 3 (moment tensors)
     fwi2d_engine* eng = nullptr;
     fwi2d_grid grid{};
     TimeAxis axis{0, 0, 0};
@@ -494,6 +503,7 @@ struct fwi2d_op {
     bool shots_set = false;
 
     int np() const { return d.nparam; }
+    int nc() const { return d.mt ? 3 : 2; }
     long long nxz() const { return (long long)d.nx * d.nz; }
 
     // model (physical) -> analysed + converted float model; sets the time axis
@@ -538,11 +548,12 @@ struct fwi2d_op {
         g.gdamp_power = d.gdamp_power;
         grid = g;
         check(fwi2d_engine_create(&g, d.device, &eng));
-        check(fwi2d_engine_set_shots(eng, d.ns, sxz.data(), nrec.data(), rxz.data(), d.ox, d.oz));
+        check(fwi2d_engine_set_acquisition(eng, d.ns, sxz.data(), nrec.data(), rxz.data(), d.ox, d.oz, d.mt,
+                                           d.gl > 0 ? d.gl : 0.0, d.gl > 0 ? rdip.data() : nullptr));
         ckpt_model.clear();
         // sources at the propagation step (apply_forward, we_op.cpp:953-958)
-        std::vector<double> sp((size_t)2 * d.ns * t.nt);
-        resample(false, d.resampling == 1, d.sinc_half_length, 2 * d.ns, d.nt_data, d.dt_data, t.nt, t.dt,
+        std::vector<double> sp((size_t)nc() * d.ns * t.nt);
+        resample(false, d.resampling == 1, d.sinc_half_length, nc() * d.ns, d.nt_data, d.dt_data, t.nt, t.dt,
                  allsrc.data(), sp.data());
         src_prop.assign(sp.begin(), sp.end());
         build_trace_maps();
@@ -573,6 +584,29 @@ struct fwi2d_op {
         tm_adj = to_matrix(A, ntd);
     }
 
+    // dipole_to_strain (misc.cpp:18-43) on data [2][nr][nt_data]: forward
+    // p0 = (cos p0 + sin p1) / gl, p1 = 0; adjoint p1 = sin p0 / gl, p0 = cos p0 / gl.  It
+    // commutes with the per-trace time maps (the same linear map on every trace), so it
+    // runs on the data axis here where the reference applies it on the propagation axis.
+    void dipole_to_strain(bool adjoint, float* p) const
+    {
+        const int n = d.nt_data;
+        for (int r = 0; r < nr; r++) {
+            const double c = std::cos(rdip[r]), sn = std::sin(rdip[r]);
+            float* p0 = p + (size_t)r * n;
+            float* p1 = p + (size_t)(nr + r) * n;
+            for (int t = 0; t < n; t++) {
+                if (!adjoint) {
+                    p0[t] = (float)((c * p0[t] + sn * p1[t]) / d.gl);
+                    p1[t] = 0.f;
+                } else {
+                    p1[t] = (float)(sn * p0[t] / d.gl);
+                    p0[t] = (float)(c * p0[t] / d.gl);
+                }
+            }
+        }
+    }
+
     void run_forward(const std::vector<float>& m, std::vector<float>& rcv)
     {
         check(fwi2d_engine_set_model(eng, m.data()));
@@ -598,6 +632,7 @@ struct fwi2d_op {
         const bool keep = axis.sub > 0;
         fwi2d::engine_forward_traces(eng, src_prop.data(), keep, tm_fwd, out.data());
         if (keep) ckpt_model = m;
+        if (d.gl > 0) dipole_to_strain(false, out.data());
         for (size_t i = 0; i < n; i++) pdat[i] = add ? pdat[i] + out[i] : out[i];
     }
 
@@ -612,7 +647,13 @@ struct fwi2d_op {
                 std::vector<float> rcv;
                 run_forward(m, rcv);
             }
-            fwi2d::engine_adjoint_traces(eng, pdat, tm_adj, graw.data());
+            if (d.gl > 0) {  // strain -> gauge-end differences (we_op.cpp:1130-1135)
+                std::vector<float> p(pdat, pdat + (size_t)2 * nr * d.nt_data);
+                dipole_to_strain(true, p.data());
+                fwi2d::engine_adjoint_traces(eng, p.data(), tm_adj, graw.data());
+            } else {
+                fwi2d::engine_adjoint_traces(eng, pdat, tm_adj, graw.data());
+            }
         }
         // H quadrature (we_op.cpp:1192-1199) and chain rule (we_op.cpp:1208-1219)
         std::vector<double> gd(graw.begin(), graw.end());
@@ -695,19 +736,24 @@ struct fwi2d_nlls {
     void process(std::vector<double>& u) const
     {
         const int n = nt();
-        std::vector<double> c(filt.empty() ? 0 : n);
-        TraceFilter::Work fw;
-        for (size_t x = 0; x < ntr(); x++) {
-            double* p = u.data() + x * n;
-            if (integrate) integral_trace(p, false);
-            if (!w.empty())
-                for (int t = 0; t < n; t++) p[t] *= w[x * n + t];
-            if (!filt.empty()) {
-                filt.apply(false, p, c.data(), fw);
-                std::copy(c.begin(), c.end(), p);
+        const size_t T = ntr();
+        parallel_traces<TraceWork>((T + 1) / 2, [&](size_t pr, TraceWork& ws) {
+            const int m = 2 * pr + 1 < T ? 2 : 1;
+            double* p[2] = {u.data() + 2 * pr * n, m == 2 ? u.data() + (2 * pr + 1) * n : nullptr};
+            for (int q = 0; q < m; q++) {
+                if (integrate) integral_trace(p[q], false);
+                if (!w.empty())
+                    for (int t = 0; t < n; t++) p[q][t] *= w[(2 * pr + q) * n + t];
             }
-            if (trace_norm) normalize_trace(p);
-        }
+            if (!filt.empty()) {
+                ws.c[0].resize(n);
+                ws.c[1].resize(n);
+                filt.apply2(false, p[0], ws.c[0].data(), p[1], ws.c[1].data(), ws.fw);
+                for (int q = 0; q < m; q++) std::copy(ws.c[q].begin(), ws.c[q].end(), p[q]);
+            }
+            if (trace_norm)
+                for (int q = 0; q < m; q++) normalize_trace(p[q]);
+        });
     }
 
     void evaluate(const float* m, double* func, float* g)
@@ -726,48 +772,60 @@ struct fwi2d_nlls {
         // |r|^2 - 1/2 (r[0]^2 + r[n-1]^2); and the adjoint source of grad
         // (optimization.hpp:390-455, 780-905): I^T W C^T N'^T Ht r, with
         // N' = I/|s| - s s^T/|s|^3 the Jacobian of the trace normalisation at s
-        parallel_traces<TraceWork>(T, [&](size_t x, TraceWork& ws) {
-            std::vector<double>& v = ws.v;
-            std::vector<double>& s = ws.s;
-            v.assign(u.begin() + x * n, u.begin() + (x + 1) * n);
-            const float* wx = w.empty() ? nullptr : w.data() + x * n;
-            if (integrate) integral_trace(v.data(), false);
-            if (wx)
-                for (int t = 0; t < n; t++) v[t] *= wx[t];
-            if (!filt.empty()) {
-                ws.c.resize(n);
-                filt.apply(false, v.data(), ws.c.data(), ws.fw);
-                v.swap(ws.c);
+        // Traces go in pairs (2p, 2p + 1), whose filters share one complex FFT pair.
+        parallel_traces<TraceWork>((T + 1) / 2, [&](size_t pr, TraceWork& ws) {
+            const int m = 2 * pr + 1 < T ? 2 : 1;
+            double nrm[2] = {1.0, 1.0};
+            auto filter_pair = [&](bool adjoint) {
+                ws.c[0].resize(n);
+                ws.c[1].resize(n);
+                filt.apply2(adjoint, ws.v[0].data(), ws.c[0].data(), m == 2 ? ws.v[1].data() : nullptr,
+                            ws.c[1].data(), ws.fw);
+                for (int q = 0; q < m; q++) ws.v[q].swap(ws.c[q]);
+            };
+            for (int q = 0; q < m; q++) {
+                const size_t x = 2 * pr + q;
+                std::vector<double>& v = ws.v[q];
+                v.assign(u.begin() + x * n, u.begin() + (x + 1) * n);
+                if (integrate) integral_trace(v.data(), false);
+                if (!w.empty())
+                    for (int t = 0; t < n; t++) v[t] *= w[x * n + t];
             }
-            double nrm = 1.0;
-            if (trace_norm) {
-                s = v;
-                nrm = normalize_trace(v.data());
+            if (!filt.empty()) filter_pair(false);
+            for (int q = 0; q < m; q++) {
+                const size_t x = 2 * pr + q;
+                std::vector<double>& v = ws.v[q];
+                std::vector<double>& s = ws.s[q];
+                if (trace_norm) {
+                    s = v;
+                    nrm[q] = normalize_trace(v.data());
+                }
+                float* rx = r.data() + x * n;
+                const double* dx = d.data() + x * n;
+                double f = 0;
+                for (int t = 0; t < n; t++) {
+                    rx[t] = (float)(v[t] - dx[t]);
+                    f += (double)rx[t] * rx[t];
+                }
+                fpart[x] = f - 0.5 * ((double)rx[0] * rx[0] + (double)rx[n - 1] * rx[n - 1]);
+                if (!g) continue;
+                for (int t = 0; t < n; t++) v[t] = (double)rx[t] * ((t == 0 || t == n - 1) ? 0.5 * dt : dt);
+                if (trace_norm) {
+                    double vs = 0;
+                    for (int t = 0; t < n; t++) vs += v[t] * s[t];
+                    for (int t = 0; t < n; t++) v[t] = v[t] / nrm[q] - vs / (nrm[q] * nrm[q] * nrm[q]) * s[t];
+                }
             }
-            float* rx = r.data() + x * n;
-            const double* dx = d.data() + x * n;
-            double f = 0;
-            for (int t = 0; t < n; t++) {
-                rx[t] = (float)(v[t] - dx[t]);
-                f += (double)rx[t] * rx[t];
-            }
-            fpart[x] = f - 0.5 * ((double)rx[0] * rx[0] + (double)rx[n - 1] * rx[n - 1]);
             if (!g) return;
-            for (int t = 0; t < n; t++) v[t] = (double)rx[t] * ((t == 0 || t == n - 1) ? 0.5 * dt : dt);
-            if (trace_norm) {
-                double vs = 0;
-                for (int t = 0; t < n; t++) vs += v[t] * s[t];
-                for (int t = 0; t < n; t++) v[t] = v[t] / nrm - vs / (nrm * nrm * nrm) * s[t];
+            if (!filt.empty()) filter_pair(true);
+            for (int q = 0; q < m; q++) {
+                const size_t x = 2 * pr + q;
+                std::vector<double>& v = ws.v[q];
+                if (!w.empty())
+                    for (int t = 0; t < n; t++) v[t] *= w[x * n + t];
+                if (integrate) integral_trace(v.data(), true);
+                for (int t = 0; t < n; t++) af[x * n + t] = (float)v[t];
             }
-            if (!filt.empty()) {
-                ws.c.resize(n);
-                filt.apply(true, v.data(), ws.c.data(), ws.fw);
-                v.swap(ws.c);
-            }
-            if (wx)
-                for (int t = 0; t < n; t++) v[t] *= wx[t];
-            if (integrate) integral_trace(v.data(), true);
-            for (int t = 0; t < n; t++) af[x * n + t] = (float)v[t];
         });
         if (func) {
             double f = 0;
@@ -813,9 +871,17 @@ int fwi2d_op_create(const fwi2d_op_desc* desc, const float* allsrc, fwi2d_op** o
         op->nr = 0;
         for (int r : op->nrec) op->nr += r;
         op->rxz.assign(d.rxz, d.rxz + 2 * (size_t)op->nr);
+        if (d.gl > 0) {
+            if (!d.rdip) {
+                delete op;
+                throw std::runtime_error("DAS receivers (gl > 0) need their dips (rdip)");
+            }
+            op->rdip.assign(d.rdip, d.rdip + op->nr);
+        }
         op->d.sxz = nullptr;
         op->d.nrec = nullptr;
         op->d.rxz = nullptr;
+        op->d.rdip = nullptr;
         if (op->d.sinc_half_length <= 0) op->d.sinc_half_length = 11;
         // shot and receiver positions inside the model (analyzeGeometry, we_op.cpp:98-127)
         const double prec = std::min(d.dx, d.dz) * 1e-3;
@@ -828,12 +894,24 @@ int fwi2d_op_create(const fwi2d_op_desc* desc, const float* allsrc, fwi2d_op** o
                 delete op;
                 throw std::runtime_error("Shot falls outside subsurface model boundaries");
             }
-        for (int r = 0; r < op->nr; r++)
-            if (!inside(op->rxz[2 * r], op->rxz[2 * r + 1])) {
+        for (int r = 0; r < op->nr; r++) {
+            const double x = op->rxz[2 * r], z = op->rxz[2 * r + 1];
+            if (d.gl <= 0 && !inside(x, z)) {
                 delete op;
                 throw std::runtime_error("Receiver " + std::to_string(r) + " falls outside subsurface model boundaries");
             }
-        op->allsrc.assign(allsrc, allsrc + (size_t)2 * d.ns * d.nt_data);
+            if (d.gl > 0) {  // both gauge ends inside the model (we_op.cpp:134-148)
+                const double hx = d.gl / 2 * std::cos(op->rdip[r]), hz = d.gl / 2 * std::sin(op->rdip[r]);
+                const bool out = x - hx < d.ox || x - hx > xmax || x + hx < d.ox || x + hx > xmax || z - hz < d.oz ||
+                                 z - hz > zmax || z + hz < d.oz || z + hz > zmax;
+                if (out) {
+                    delete op;
+                    throw std::runtime_error("DAS channel " + std::to_string(r) + " must be away from the subsurface "
+                                             "model boundaries by half of the gauge length along the fiber");
+                }
+            }
+        }
+        op->allsrc.assign(allsrc, allsrc + (size_t)(d.mt ? 3 : 2) * d.ns * d.nt_data);
         *out = op;
     });
 }
@@ -939,8 +1017,13 @@ int fwi2d_filter_apply(const double* filter, int nf, int centered, int adjoint, 
         if (nf < 1 || nt < 1 || ntr < 0) throw std::runtime_error("invalid sizes");
         TraceFilter f;
         f.init(std::vector<double>(filter, filter + nf), centered != 0, nt);
-        TraceFilter::Work w;
-        for (long long x = 0; x < ntr; x++) f.apply(adjoint != 0, in + x * nt, out + x * nt, w);
+        if (in < out + ntr * nt && out < in + ntr * nt) throw std::runtime_error("in and out overlap");
+        parallel_traces<TraceFilter::Work>((size_t)(ntr + 1) / 2, [&](size_t p, TraceFilter::Work& w) {
+            const long long x = 2 * (long long)p;
+            const bool two = x + 1 < ntr;
+            f.apply2(adjoint != 0, in + x * nt, out + x * nt, two ? in + (x + 1) * nt : nullptr,
+                     two ? out + (x + 1) * nt : nullptr, w);
+        });
     });
 }
 

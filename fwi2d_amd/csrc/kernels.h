@@ -13,18 +13,23 @@ namespace fwi2d {
 constexpr int kTileZ = 64;     // z extent of a tile: 16 quads
 constexpr int kThreads = 256;  // 4 wavefronts per workgroup
 
-// One point injector (delta_m3): 3x3 footprint starting at (xi0, zi0) with separable
-// weights, and the offset of its x-component trace (z component at + comp_stride).
+// One 3x3 footprint of a point injector or extractor, starting at (xi0, zi0), with
+// separable weights wx[i] wz[j]; it feeds (or is read into) trace sample offset `trace`
+// for the x field component and `tz` for the z component.  A delta_m3 point is one
+// footprint (trace and tz the two components of one trace), a ddelta_m3 moment tensor
+// two (the x- and z-derivative parts, reading the mxx/mxz and mxz/mzz traces), a
+// dipole_m3 DAS channel two (the gauge ends, with opposite signs).
 struct PointEntry {
     int xi0, zi0;
     float wx[3], wz[3];
-    long long trace;
+    long long trace, tz;
     int shot;
     int pad;
 };
 
 // A cell touched by point injectors: shot and grid position; its contributions are
-// listed in a CSR (trace offset of the x component, weight) in injector order.
+// listed in a CSR (trace offsets of the x and z field components, weight) in injector
+// order.
 struct InjCell {
     int shot, gx, gz, pad;
 };
@@ -103,7 +108,7 @@ inline int core_z_end(int nz) { return ((nz - 8) / 4) * 4; }
 void launch_step(const StepArgs& a, bool vti, BoxClass c, const int4* boxes, const int* list, int n, bool image,
                  hipStream_t s);
 void launch_init_prev(float* prev, long long shot_stride, long long nxz, int ldz, int nshots, const int* shot_ptr,
-                      const PointEntry* ents, const float* trace, long long comp_stride, hipStream_t s);
+                      const PointEntry* ents, const float* trace, hipStream_t s);
 void launch_scale_prev(float* prev, const float* rho, float half_dt2, long long nxz, int nshots, hipStream_t s);
 // Density term of a fused imaging step for the injected increment of the new level
 // (on = 0: no imaging); rho_off: offset of the density component in a gradient slab.
@@ -115,13 +120,14 @@ struct InjImage {
     float* grad;
     long long gslab_stride, rho_off;
 };
-void launch_inject_cells(const InjCell* cells, int ncell, const int* ptr, const long long* ctrace, const float* cw,
-                         const float* trace, long long comp_stride, int it, float* next, long long shot_stride,
+void launch_inject_cells(const InjCell* cells, int ncell, const int* ptr, const long long* ctx, const long long* ctz,
+                         const float* cw, const float* trace, int it, float* next, long long shot_stride,
                          long long nxz, int nx, int nz, int ldz, const float* irho, const float* taper_x,
                          const float* taper_z, const float* div_tb, const float* div_lr, const InjImage& im,
                          hipStream_t s);
+// receiver r sums the footprints ents[rptr[r]] .. ents[rptr[r+1]-1] into its traces
 void launch_extract(const float* state, long long shot_stride, long long nxz, int ldz, const PointEntry* ents,
-                    int n, float* trace, long long comp_stride, int it, hipStream_t s);
+                    const int* rptr, int nrec, float* trace, int it, hipStream_t s);
 void launch_image(const ImageArgs& a, bool vti, int tile_x, hipStream_t s);
 // imaging condition after an adjoint step over the tile_x x 64 tile grid (a.ntx x a.ntz
 // tiles, a.ngroups groups of a.spb shots); a.curr lagged adjoint level, a.next the new

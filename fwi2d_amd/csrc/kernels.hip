@@ -1376,7 +1376,7 @@ __global__ void __launch_bounds__(kThreads) k_img(const StepArgs a)
 // Initial condition: prev = 1/2 dt^2/rho * inject(trace[0])  (we_op.cpp:655-663)
 // ---------------------------------------------------------------------------
 __global__ void k_init_prev(float* prev, long long shot_stride, long long nxz, int ldz, const int* shot_ptr,
-                            const PointEntry* ents, const float* trace, long long comp_stride)
+                            const PointEntry* ents, const float* trace)
 {
     // one single-thread block per shot: deterministic accumulation order over its injectors
     const int s = blockIdx.x;
@@ -1385,7 +1385,7 @@ __global__ void k_init_prev(float* prev, long long shot_stride, long long nxz, i
     float* pz = px + nxz;
     for (int k = b; k < e; k++) {
         const PointEntry en = ents[k];
-        const float vx = trace[en.trace], vz = trace[en.trace + comp_stride];
+        const float vx = trace[en.trace], vz = trace[en.tz];
         for (int ix = 0; ix < 3; ix++)
             for (int iz = 0; iz < 3; iz++) {
                 const long long g = (long long)(en.xi0 + ix) * ldz + (en.zi0 + iz);
@@ -1407,17 +1407,21 @@ __global__ void k_scale_prev(float* prev, const float* rho, float half_dt2, long
 
 // extraction of one level from global memory (used for the last level)
 __global__ void k_extract(const float* state, long long shot_stride, long long nxz, int ldz, const PointEntry* ents,
-                          int n, float* trace, long long comp_stride, int it)
+                          const int* rptr, int nrec, float* trace, int it)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= 2 * n) return;
-    const PointEntry en = ents[k >> 1];
-    const int comp = k & 1;
-    const float* u = state + en.shot * shot_stride + comp * nxz;
+    if (k >= 2 * nrec) return;
+    const int r = k >> 1, comp = k & 1;
     float v = 0.f;
-    for (int ix = 0; ix < 3; ix++)
-        for (int iz = 0; iz < 3; iz++) v += en.wx[ix] * en.wz[iz] * u[(long long)(en.xi0 + ix) * ldz + en.zi0 + iz];
-    trace[en.trace + comp * comp_stride + it] = v;
+    long long out = 0;
+    for (int f = rptr[r]; f < rptr[r + 1]; f++) {
+        const PointEntry en = ents[f];
+        const float* u = state + en.shot * shot_stride + comp * nxz;
+        for (int ix = 0; ix < 3; ix++)
+            for (int iz = 0; iz < 3; iz++) v += en.wx[ix] * en.wz[iz] * u[(long long)(en.xi0 + ix) * ldz + en.zi0 + iz];
+        out = comp ? en.tz : en.trace;
+    }
+    if (rptr[r + 1] > rptr[r]) trace[out + it] = v;
 }
 
 // ---------------------------------------------------------------------------
@@ -1427,8 +1431,8 @@ __global__ void k_extract(const float* state, long long shot_stride, long long n
 // source term s, so the sources add dt^2/rho * s / d * taper (d: the absorbing divisor on
 // the perimeter, 1 elsewhere).
 // ---------------------------------------------------------------------------
-__global__ void k_inject_cells(const InjCell* cells, int ncell, const int* ptr, const long long* ctrace,
-                               const float* cw, const float* trace, long long comp_stride, int it, float* next,
+__global__ void k_inject_cells(const InjCell* cells, int ncell, const int* ptr, const long long* ctx,
+                               const long long* ctz, const float* cw, const float* trace, int it, float* next,
                                long long shot_stride, long long nxz, int nx, int nz, int ldz, const float* irho,
                                const float* taper_x, const float* taper_z, const float* div_tb, const float* div_lr,
                                const InjImage im)
@@ -1438,9 +1442,8 @@ __global__ void k_inject_cells(const InjCell* cells, int ncell, const int* ptr, 
     const InjCell cell = cells[c];
     float fx = 0.f, fz = 0.f;
     for (int k = ptr[c]; k < ptr[c + 1]; k++) {
-        const long long tr = ctrace[k] + it;
-        fx += cw[k] * trace[tr];
-        fz += cw[k] * trace[tr + comp_stride];
+        fx += cw[k] * trace[ctx[k] + it];
+        fz += cw[k] * trace[ctz[k] + it];
     }
     const int gx = cell.gx, gz = cell.gz;
     const long long g = (long long)gx * ldz + gz;
@@ -1768,10 +1771,9 @@ void launch_image(const ImageArgs& a, bool vti, int tile_x, hipStream_t s)
 }
 
 void launch_init_prev(float* prev, long long shot_stride, long long nxz, int ldz, int nshots, const int* shot_ptr,
-                      const PointEntry* ents, const float* trace, long long comp_stride, hipStream_t s)
+                      const PointEntry* ents, const float* trace, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_init_prev, dim3(nshots), dim3(1), 0, s, prev, shot_stride, nxz, ldz, shot_ptr, ents, trace,
-                       comp_stride);
+    hipLaunchKernelGGL(k_init_prev, dim3(nshots), dim3(1), 0, s, prev, shot_stride, nxz, ldz, shot_ptr, ents, trace);
 }
 
 void launch_scale_prev(float* prev, const float* rho, float half_dt2, long long nxz, int nshots, hipStream_t s)
@@ -1780,24 +1782,24 @@ void launch_scale_prev(float* prev, const float* rho, float half_dt2, long long 
                        nshots);
 }
 
-void launch_inject_cells(const InjCell* cells, int ncell, const int* ptr, const long long* ctrace, const float* cw,
-                         const float* trace, long long comp_stride, int it, float* next, long long shot_stride,
+void launch_inject_cells(const InjCell* cells, int ncell, const int* ptr, const long long* ctx, const long long* ctz,
+                         const float* cw, const float* trace, int it, float* next, long long shot_stride,
                          long long nxz, int nx, int nz, int ldz, const float* irho, const float* taper_x,
                          const float* taper_z, const float* div_tb, const float* div_lr, const InjImage& im,
                          hipStream_t s)
 {
     if (ncell <= 0) return;
     hipLaunchKernelGGL(k_inject_cells, dim3((ncell + kThreads - 1) / kThreads), dim3(kThreads), 0, s, cells, ncell,
-                       ptr, ctrace, cw, trace, comp_stride, it, next, shot_stride, nxz, nx, nz, ldz, irho, taper_x,
+                       ptr, ctx, ctz, cw, trace, it, next, shot_stride, nxz, nx, nz, ldz, irho, taper_x,
                        taper_z, div_tb, div_lr, im);
 }
 
-void launch_extract(const float* state, long long shot_stride, long long nxz, int ldz, const PointEntry* ents, int n,
-                    float* trace, long long comp_stride, int it, hipStream_t s)
+void launch_extract(const float* state, long long shot_stride, long long nxz, int ldz, const PointEntry* ents,
+                    const int* rptr, int nrec, float* trace, int it, hipStream_t s)
 {
-    if (n <= 0) return;
-    hipLaunchKernelGGL(k_extract, dim3((2 * n + kThreads - 1) / kThreads), dim3(kThreads), 0, s, state, shot_stride,
-                       nxz, ldz, ents, n, trace, comp_stride, it);
+    if (nrec <= 0) return;
+    hipLaunchKernelGGL(k_extract, dim3((2 * nrec + kThreads - 1) / kThreads), dim3(kThreads), 0, s, state,
+                       shot_stride, nxz, ldz, ents, rptr, nrec, trace, it);
 }
 
 void launch_sum_shots(const float* per_shot, long long shot_stride, int nshots, long long n, float* out, int acc,

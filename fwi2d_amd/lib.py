@@ -37,7 +37,8 @@ class Fwi2dOpDesc(C.Structure):
                 ("rhomin", C.c_double), ("rhomax", C.c_double), ("deltamin", C.c_double),
                 ("deltamax", C.c_double), ("epsilonmin", C.c_double), ("epsilonmax", C.c_double),
                 ("soft_clip", C.c_int), ("gdamp_tmin", C.c_double), ("gdamp_tmax", C.c_double),
-                ("gdamp_power", C.c_double), ("device", C.c_int)]
+                ("gdamp_power", C.c_double), ("device", C.c_int), ("mt", C.c_int), ("gl", C.c_double),
+                ("rdip", C.POINTER(C.c_double))]
 
 
 EXPORTS = [
@@ -48,7 +49,7 @@ EXPORTS = [
     "fwi2d_nlls_create", "fwi2d_nlls_evaluate", "fwi2d_nlls_residual", "fwi2d_nlls_dnorm", "fwi2d_nlls_destroy",
     "fwi2d_nlls_objective", "fwi2d_lbfgs", "fwi2d_nlls_create_filtered", "fwi2d_filter_prepare",
     "fwi2d_filter_apply", "fwi2d_dot_f64_noalloc", "fwi2d_read_header", "fwi2d_read_data", "fwi2d_write",
-    "fwi2d_read_coord", "fwi2d_regular_geometry",
+    "fwi2d_read_coord", "fwi2d_regular_geometry", "fwi2d_engine_set_acquisition",
 ]
 
 FORMAT_SEP, FORMAT_BINARY = 0, 1
@@ -91,6 +92,7 @@ def load():
     lib.fwi2d_engine_create.argtypes = [C.POINTER(Fwi2dGrid), C.c_int, C.POINTER(P)]
     lib.fwi2d_engine_set_model.argtypes = [P, P]
     lib.fwi2d_engine_set_shots.argtypes = [P, C.c_int, P, P, P, C.c_double, C.c_double]
+    lib.fwi2d_engine_set_acquisition.argtypes = [P, C.c_int, P, P, P, C.c_double, C.c_double, C.c_int, C.c_double, P]
     lib.fwi2d_engine_forward.argtypes = [P, P, P, C.c_int, C.c_int]
     lib.fwi2d_engine_adjoint.argtypes = [P, P, P, C.c_int]
     lib.fwi2d_engine_dot_f64.argtypes = [P, P, P, P, C.c_longlong, C.POINTER(C.c_double)]
@@ -163,7 +165,7 @@ class Engine:
         self.np = 5 if grid.vti else 3
 
     def close(self):
-        if self._h:
+        if getattr(self, "_h", None):
             load().fwi2d_engine_destroy(self._h)
             self._h = C.c_void_p()
 
@@ -174,18 +176,25 @@ class Engine:
         assert m.shape == (self.np, self.grid.nx, self.grid.nz), m.shape
         _check(load().fwi2d_engine_set_model(self._h, _ptr(m)))
 
-    def set_shots(self, sxz, rxz, ox=0.0, oz=0.0):
+    def set_shots(self, sxz, rxz, ox=0.0, oz=0.0, moment_tensor=False, gauge_length=0.0, rdip=None):
+        """Point forces (or moment tensors) at sxz, point receivers (or DAS channels of
+        gauge_length along the dips rdip, one array per shot) at rxz (one array per shot)."""
         sxz = np.ascontiguousarray(sxz, dtype=np.float64).reshape(-1, 2)
         nrec = np.ascontiguousarray([len(r) for r in rxz], dtype=np.int32)
         allr = np.ascontiguousarray(np.concatenate([np.asarray(r, np.float64).reshape(-1, 2) for r in rxz], 0))
         self.ns, self.nr = len(sxz), int(nrec.sum())
-        self._geom = (sxz, nrec, allr)
-        _check(load().fwi2d_engine_set_shots(self._h, self.ns, _ptr(sxz), _ptr(nrec), _ptr(allr), ox, oz))
+        self.src_nc = 3 if moment_tensor else 2
+        dips = None
+        if gauge_length > 0:
+            dips = np.ascontiguousarray(np.concatenate([np.asarray(r, np.float64) for r in rdip]))
+        self._geom = (sxz, nrec, allr, dips)
+        _check(load().fwi2d_engine_set_acquisition(self._h, self.ns, _ptr(sxz), _ptr(nrec), _ptr(allr), ox, oz,
+                                                   int(moment_tensor), float(gauge_length), _ptr(dips)))
 
     def forward(self, src, keep_checkpoints=False):
         """src [2][ns][nt] (host numpy) -> receivers [2][nr][nt]."""
         s = _f32(src)
-        assert s.shape == (2, self.ns, self.grid.nt), s.shape
+        assert s.shape == (getattr(self, "src_nc", 2), self.ns, self.grid.nt), s.shape
         rcv = np.zeros((2, self.nr, self.grid.nt), np.float32)
         _check(load().fwi2d_engine_forward(self._h, _ptr(s), _ptr(rcv), int(keep_checkpoints), 0))
         return rcv
@@ -258,14 +267,20 @@ class WeOp:
         d.nrec = nrec.ctypes.data_as(C.POINTER(C.c_int))
         d.rxz = rxz.ctypes.data_as(C.POINTER(C.c_double))
         self.nr = int(nrec.sum())
+        d.mt = int(prob.get("mt", 0))
+        d.gl = float(prob.get("gl", 0.0))
+        if d.gl > 0:
+            self._rdip = np.ascontiguousarray(np.concatenate([np.asarray(r, np.float64) for r in prob["rdip"]]))
+            assert self._rdip.size == self.nr
+            d.rdip = self._rdip.ctypes.data_as(C.POINTER(C.c_double))
         src = _f32(allsrc)
-        assert src.shape == (2, d.ns, d.nt_data), src.shape
+        assert src.shape == (3 if d.mt else 2, d.ns, d.nt_data), src.shape
         self._h = C.c_void_p()
         _check(lib.fwi2d_op_create(C.byref(d), _ptr(src), C.byref(self._h)))
         self.nt_data = d.nt_data
 
     def close(self):
-        if self._h:
+        if getattr(self, "_h", None):
             load().fwi2d_op_destroy(self._h)
             self._h = C.c_void_p()
 
@@ -318,7 +333,7 @@ class Nlls:
                                                  C.byref(self._h)))
 
     def close(self):
-        if self._h:
+        if getattr(self, "_h", None):
             load().fwi2d_nlls_destroy(self._h)
             self._h = C.c_void_p()
 

@@ -68,6 +68,17 @@ int fwi2d_engine_set_model(fwi2d_engine* e, const float* model);
 int fwi2d_engine_set_shots(fwi2d_engine* e, int nshots, const double* sxz, const int* nrec,
                            const double* rxz, double ox, double oz);
 
+/* fwi2d_engine_set_shots with the reference's other injectors (par.mt, par.gl):
+   moment_tensor != 0: the sources are moment tensors injected with ddelta_m3
+   (src/cpp/injector.cpp:114-239; we_op.cpp:640-654) and the source traces of
+   fwi2d_engine_forward are [3][nshots][nt] = (mxx, mzz, mxz);
+   gauge_length > 0: the receivers are DAS channels extracted and injected with dipole_m3
+   (injector.cpp:241-378) at x -/+ gl/2 cos(dip), z -/+ gl/2 sin(dip), rdip [nrec_total]
+   their dips (rad); each receiver trace pair [2][r][nt] then holds the differences of
+   the two displacement components between the gauge ends (before dipole_to_strain). */
+int fwi2d_engine_set_acquisition(fwi2d_engine* e, int nshots, const double* sxz, const int* nrec, const double* rxz,
+                                 double ox, double oz, int moment_tensor, double gauge_length, const double* rdip);
+
 /* Forward propagation of all shots (propagate(adj=false), src/cpp/we_op.cpp:564-902).
    src: [2][nshots][nt] source traces at the propagation step (x then z force);
    rcv: [2][nrec_total][nt] displacement traces (output).  keep_checkpoints != 0 keeps
@@ -160,7 +171,7 @@ const char* fwi2d_last_error(void);
 
 /* Operator description: the reference's param fields used on this path
    (src/include/param.hpp:10-82) plus the model domain and source time axis.
-   Point forces (mt=0) and point receivers (gl=0) are supported. */
+   Point forces or moment tensors (mt), point receivers or DAS channels (gl). */
 typedef struct {
     int nx, nz; double dx, dz, ox, oz;            /* model domain axes 2, 1 */
     int nt_data; double dt_data;                  /* source/data time axis (origin 0) */
@@ -178,6 +189,14 @@ typedef struct {
     int soft_clip;
     double gdamp_tmin, gdamp_tmax, gdamp_power;
     int device;
+    int mt;                                       /* par.mt: moment-tensor sources, allsrc [3][ns][nt_data]
+                                                     = (mxx, mzz, mxz) (analyzeWavelet, we_op.cpp:216-238) */
+    double gl;                                    /* par.gl > 0: DAS receivers of this gauge length (km);
+                                                     data [2][nrec][nt_data] with the strain (rate) in
+                                                     component 0 and zeros in 1 (dipole_to_strain,
+                                                     misc.cpp:18-43) */
+    const double* rdip;                           /* par.rdip: receiver dips (rad), nrec_total values,
+                                                     read when gl > 0 */
 } fwi2d_op_desc;
 
 typedef struct fwi2d_op fwi2d_op;
@@ -254,7 +273,8 @@ int fwi2d_filter_prepare(const float* filter, int nf, int filter_phase, double* 
 
 /* conv1dnd apply_forward / apply_adjoint (operator.cpp:2092-2137) alone, host only, in
    double precision: out = C in (adjoint: C^T in) for ntr traces of nt samples, with the
-   filter's centred (shift floor((nf + 1)/2) - 1) or causal convolution. */
+   filter's centred (shift floor((nf + 1)/2) - 1) or causal convolution, on up to 16
+   host threads; in and out must not overlap. */
 int fwi2d_filter_apply(const double* filter, int nf, int centered, int adjoint, long long ntr, int nt,
                        const double* in, double* out);
 

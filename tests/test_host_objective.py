@@ -73,3 +73,21 @@ def test_argument_checks(op):
     h = lib.C.c_void_p()
     assert L.fwi2d_nlls_create(op._h, d.ctypes.data_as(lib.C.c_void_p), None, None, 8, lib.C.byref(h)) != 0
     assert b"unknown objective flags" in L.fwi2d_last_error()
+
+
+def test_das_and_moment_tensor_arguments_are_checked():
+    """DAS channels must keep both gauge ends inside the model (we_op.cpp:134-148), need
+    their dips, and moment-tensor sources come as three traces per shot."""
+    from fwi2d_amd import lib
+    nx, nz, h = 40, 30, 0.01
+    rx = np.stack([np.arange(2, nx - 2, 4) * h, np.full(len(range(2, nx - 2, 4)), 0.05)], 1)
+    prob = W.base_problem(nx, nz, h, h, 101, 0.002, [[0.1, 0.1]], [rx], taper=5, dt=0.001, sub=-1)
+    w = W.ricker(101, 0.002, 12.0)
+    prob_das = dict(prob, gl=0.05, rdip=[np.zeros(len(rx))])
+    with pytest.raises(lib.Fwi2dError, match="half of the gauge length"):
+        lib.WeOp(prob_das, W.vertical_force(w, 1))  # the first channel's gauge starts at x = -0.005 km
+    ok = dict(prob_das, rxz=[rx[1:]], rdip=[np.zeros(len(rx) - 1)])
+    lib.WeOp(ok, W.vertical_force(w, 1)).close()
+    with pytest.raises(AssertionError):
+        lib.WeOp(dict(prob, mt=1), W.vertical_force(w, 1))  # [2][ns][nt] given, [3][ns][nt] expected
+    lib.WeOp(dict(prob, mt=1), np.stack([w[None], w[None], 0 * w[None]])).close()
