@@ -48,6 +48,13 @@ def parse():
     ap.add_argument("--checkpoint-budget", type=int, default=0, metavar="MIB",
                     help="device memory (MiB) for checkpoints and snapshot slots of the batched form; longer "
                          "checkpoint segments leave fewer steps to recompute (0: the sqrt scheme's minimum)")
+    ap.add_argument("--image-fused", type=int, choices=[-1, 0, 1], default=-1,
+                    help="1: the imaging condition runs inside the adjoint step kernels (by parts in time); "
+                         "0: in a separate imaging kernel after each adjoint step; -1: the engine's default "
+                         "(1 on isotropic grids, 0 on VTI grids)")
+    ap.add_argument("--engine-option", action="append", default=[], metavar="KEY=VALUE",
+                    help="an engine option (fwi2d_engine_set_option) set after the autotuner ran, e.g. "
+                         "shots_per_block_img=8; repeatable")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-check", action="store_true", help="skip the small-case gradient error check")
     ap.add_argument("--cpu-steps", type=int, default=1600, help="time steps of the CPU baseline sample")
@@ -75,6 +82,8 @@ def parse():
                     help="with --fwi: regular_wolfe's a1, a first step of a1 |m| / |g| if that is larger than "
                          "a0 / |g| (the reference's default 0)")
     ap.add_argument("--fwi-trials", type=int, default=10, help="with --fwi: trial steps per line search")
+    ap.add_argument("--fwi-mask-tapers", action="store_true",
+                    help="with --fwi: gradient mask zero in the absorbing taper zones (sides and bottom)")
     ap.add_argument("--step-only", type=int, metavar="EDGE_STREAM", default=None,
                     help="only autotune and time the forward time step of the workload with this edge_stream "
                          "setting (0: every launch of a step on one stream, for per-launch kernel traces), "
@@ -194,10 +203,11 @@ class Batch:
     """One engine holding a shot batch, with device-resident traces (torch tensors)."""
 
     def __init__(self, fwi2d_amd, torch, prob, model_true, model, sxz, rxz, src, nt, dt, tile_x, tuned=True,
-                 checkpoint_budget=0):
+                 checkpoint_budget=0, image_fused=0):
         self.torch = torch
         self.eng = fwi2d_amd.Engine(fwi2d_amd.grid_from(prob, nt, dt, 1))
         self.eng.set_option("checkpoint_budget", checkpoint_budget)
+        self.eng.set_option("image_fused", image_fused)
         if tuned:
             # batched form: step shapes picked by the autotuner at the first forward run,
             # time loops captured in hipGraphs and replayed
@@ -240,7 +250,7 @@ class Batch:
         self.grad_norm = self.eng.dot_f64(self.grad.data_ptr(), 0, 0, self.grad.numel()) ** 0.5
 
 
-def grad_error_small(vti=False, fmt=0):
+def grad_error_small(vti=False, fmt=0, image_fused=0):
     """Gradient rel-L2 error vs the FP64 oracle on a reduced single-shot case (largest over
     the lambda / mu / rho components); fmt: checkpoint format (0 FP32, 1 BF16, 2 FP16)."""
     import fwi2d_amd
@@ -271,6 +281,7 @@ def grad_error_small(vti=False, fmt=0):
     eng = fwi2d_amd.Engine(fwi2d_amd.grid_from(prob, nt, dt, 1))
     eng.set_option("checkpoint_format", fmt)
     eng.set_option("segment", 25)
+    eng.set_option("image_fused", image_fused)
     eng.set_model(m1)
     eng.set_shots(sxz, rxz)
     eng.forward(src, keep_checkpoints=True)
@@ -407,6 +418,12 @@ def fwi_run(args):
         gmask[:] = (np.sin(0.5 * np.pi * ramp) ** 2)[None, None, :]
     if args.fwi_params == "vp":
         gmask[1:] = 0.0
+    nx, nz = prob["nx"], prob["nz"]
+    inner = np.zeros((nx, nz), bool)  # outside the absorbing taper zones
+    inner[prob["taper_left"]:nx - prob["taper_right"], prob["taper_top"]:nz - prob["taper_bottom"]] = True
+    if args.fwi_mask_tapers:
+        gmask[:, ~inner] = 0.0
+    vp_err_inner = lambda x: float(np.linalg.norm((x[0] - true[0])[inner]) / np.linalg.norm(true[0][inner]))
     op = fwi2d_amd.WeOp(prob, src)
     dobs = op.apply_forward(true)
     log("observed data modelled: %d shots, %d traces" % (ns, 2 * op.nr))
@@ -430,14 +447,21 @@ def fwi_run(args):
         m1, hist, k, info = fwi2d_amd.lbfgs(evaluate, m, niter=args.fwi, max_trial=args.fwi_trials, a1=args.fwi_a1,
                                             report=True)
         el = time.perf_counter() - t0
+        dv = m1[0] - m[0]
+        ix, iz = np.unravel_index(int(np.argmax(np.abs(dv))), dv.shape)
         out.append({"band_hz": fc, "filter_taps": int(filt.size), "filter_pass_seconds": round(t_filter, 3),
                     "iterations": k, "evaluations": count[0],
                     "seconds": round(el, 1), "functional_history": [float(v) for v in hist],
                     "model_rel_err": [err(m), err(m1)], "vp_rel_err": [vp_err(m), vp_err(m1)],
                     "vp_rel_err_below_200m": [vp_err_deep(m), vp_err_deep(m1)],
+                    "vp_rel_err_outside_tapers": [vp_err_inner(m), vp_err_inner(m1)],
+                    "vp_range": [float(m1[0].min()), float(m1[0].max())],
+                    "max_vp_change": [float(dv[ix, iz]), round(ix * h, 3), round(iz * h, 3)],
                     "trials": [[int(r[0]), int(r[1]), float(r[2]), float(r[3])] for r in info["trials"]]})
-        log("band %g Hz: %d iterations, %d evaluations, %.1f s, functional %.4e -> %.4e, vp error %.4f -> %.4f"
-            % (fc, k, count[0], el, hist[0], hist[k], vp_err(m), vp_err(m1)))
+        log("band %g Hz: %d iterations, %d evaluations, %.1f s, functional %.4e -> %.4e, vp error %.4f -> %.4f "
+            "(outside tapers %.4f -> %.4f), vp in [%.3f, %.3f], largest change %.3f at x %.2f z %.2f km"
+            % (fc, k, count[0], el, hist[0], hist[k], vp_err(m), vp_err(m1), vp_err_inner(m), vp_err_inner(m1),
+               m1[0].min(), m1[0].max(), dv[ix, iz], ix * h, iz * h))
         m = m1
         obj.close()
     op.close()
@@ -447,10 +471,10 @@ def fwi_run(args):
                       "vp_rel_err_below_200m": [vp_err_deep(start), vp_err_deep(m)],
                       "config": {"workload": "%d shots, 600x2000 h=10 m, 4000 samples of 1 ms, %g Hz Ricker, "
                                              "low-pass bands %s Hz, %d l-BFGS iterations per band, inverting %s, "
-                                             "mute %g km, a1 %g"
+                                             "mute %g km, a1 %g, tapers %s"
                                              % (ns, args.fwi_source_hz, args.bands, args.fwi,
                                                 "vp" if args.fwi_params == "vp" else "vp, vs, rho", args.fwi_mute,
-                                                args.fwi_a1)}}))
+                                                args.fwi_a1, "masked" if args.fwi_mask_tapers else "free")}}))
 
 
 def main():
@@ -490,8 +514,11 @@ def main():
     # ---- batched form -------------------------------------------------------
     log("workload ready: %d shots, %dx%d, nt=%d" % (ns, nz, nx, nt))
     B = Batch(fwi2d_amd, torch, prob, m_true, m0, sxz, rxz, src, nt, dt, args.tile_x,
-              checkpoint_budget=args.checkpoint_budget)
+              checkpoint_budget=args.checkpoint_budget, image_fused=args.image_fused)
     log("observed data modelled")
+    for kv in args.engine_option:
+        k, v = kv.split("=", 1)
+        B.eng.set_option(k, int(v))
     for _ in range(args.warmup):
         B.gradient()
     torch.cuda.synchronize()
@@ -511,6 +538,7 @@ def main():
 
     # ---- dominant kernel: fused step, timed with HIP events on its stream ----
     # (eager launches: one step = the core-box launch plus the edge-band launches beside it)
+    image_fused = B.eng.stat("image_fused")
     tune = {"tile_x": B.eng.stat("tile_x"), "shots_per_block": B.eng.stat("shots_per_block"),
             "shots_per_block_edge": B.eng.stat("shots_per_block_edge"), "edge_stream": B.eng.stat("edge_stream"),
             "report": B.eng.tune_report()}
@@ -547,10 +575,11 @@ def main():
         plain_shots_per_s = P / t_plain
         log("plain form done")
 
-    grad_err = None if args.no_check else grad_error_small(vti)
+    grad_err = None if args.no_check else grad_error_small(vti, 0, args.image_fused)
     grad_err16 = None
     if vti and not args.no_check:
-        grad_err16 = {"bf16": grad_error_small(True, 1), "fp16": grad_error_small(True, 2)}
+        grad_err16 = {"bf16": grad_error_small(True, 1, args.image_fused),
+                      "fp16": grad_error_small(True, 2, args.image_fused)}
     log("gradient check done")
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_steps if not vti else max(args.cpu_steps // 4, 50),
                                                   args.workload)
@@ -578,6 +607,8 @@ def main():
         "recompute_steps_per_gradient": int(recompute // max(args.steps + args.warmup, 1)),
         "checkpoint_segment": segment,
         "checkpoint_budget_mib": args.checkpoint_budget,
+        "image_fused": image_fused,
+        "engine_options": args.engine_option,
         "grad_rel_err_vs_fp64": grad_err,
         "grad_err_sample": "1 shot, 160x100 grid%s, nt=600, raw lambda/mu/rho gradients vs FP64 oracle"
                            % (" (VTI)" if vti else ""),

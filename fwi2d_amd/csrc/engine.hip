@@ -146,7 +146,11 @@ struct Engine {
     int opt_spb = 8;       // shots advanced one after the other by each core-box workgroup
     int opt_spb_edge = 1;  // the same for edge tiles (their workgroups are latency-bound)
     int opt_spb_img = 0;   // shots per imaging workgroup (0: 4, or opt_spb when fused)
-    int opt_image_fused = 0;  // 1: imaging condition fused into the adjoint step kernels
+    // 1: imaging condition fused into the adjoint step kernels (by parts in time, sub = 1);
+    // 0: the imaging kernel after each adjoint step; -1: fused on isotropic grids, where it
+    // is faster (the VTI fused kernels exceed the register file)
+    int opt_image_fused = -1;
+    bool image_fused() const { return opt_image_fused < 0 ? !vti : opt_image_fused != 0; }
     int opt_img_buffers = 1;  // LDS window buffers of the imaging kernel (1 or 2)
     int opt_img_stream = 1;   // 1: imaging kernel on its own stream beside the next step
     int opt_img_sbp = 1;      // 1: density term by parts in time (sub = 1, separate imaging)
@@ -739,17 +743,26 @@ struct Engine {
     // be loaded): the step's current level is the lagged adjoint level and its new level
     // the adjoint level of the reference's compute_gradients call
     // sbp: img_idx is snapshot j of the by-parts imaging (imaged after this step with the
-    // step's previous, current and new levels, see sbp_image_args)
+    // step's previous, current and new levels, see sbp_image_args), or, with the imaging
+    // fused into the step (image_fused), the density term of snapshot j and the stiffness
+    // terms of snapshot j-1, whose derivatives meet those of the step's current level
     void step(Levels& L, int it, bool adjoint, const float* traces, float* rcv, int img_idx = -1, bool sbp = false)
     {
         StepArgs a = base_args();
         // imaging fused into the step kernels, or (default) a separate k_img pass after the
         // step and its point sources
-        const bool fuse_img = img_idx >= 0 && opt_image_fused && !sbp;
+        const bool fuse_img = img_idx >= 1 && image_fused() && sbp;
         StepArgs ia{};
         if (img_idx >= 0) {
             if (sbp) sbp_image_args(a, img_idx);
             else set_image_args(a, img_idx);
+            if (fuse_img) {
+                int mode;
+                float wrho;
+                image_weights(img_idx - 1, mode, a.wlm, wrho);
+                a.fm = seg_slot(img_idx / seg_K, img_idx - 1);
+                a.img_deriv = 1;
+            }
             ia = a;
             // shots per imaging workgroup: the separate kernel is fastest with small
             // groups (more workgroups in flight; measured 4 < 16 < 64 at 64 shots)
@@ -800,11 +813,8 @@ struct Engine {
         InjImage im{};
         if (img) {
             im.on = 1;
-            im.mode = a.img_mode;
-            im.wrho = a.wrho;
+            im.wn = a.wn;
             im.f0 = a.f0;
-            im.fm = a.fm;
-            im.fp = a.fp;
             im.fshot_stride = a.fshot_stride;
             im.grad = a.grad;
             im.gslab_stride = a.gslab_stride;
@@ -1358,7 +1368,7 @@ struct Engine {
         const int nt = g.nt, sub = g.sub;
         // the density term by parts needs each step's previous level while the next step
         // runs: four adjoint levels
-        const bool sbp = fused && opt_img_sbp && sub == 1 && !opt_image_fused;
+        const bool sbp = fused && opt_img_sbp && sub == 1;
         Levels A = levels_in(d_adj, sbp ? 4 : 3);
         init_levels(A, true, dadj);
         if (sbp) {
@@ -1381,6 +1391,8 @@ struct Engine {
                 ia.aprev = Ac.P();
                 ia.curr = Ac.C();
                 ia.next = Ac.C();  // weight 0
+                // fused: the stiffness terms of snapshot 0 came with the last step
+                if (image_fused()) ia.img_deriv = 0;
                 stat_image_calls++;
                 run_img(ia, nt - 1, true);
                 img_join();
@@ -1626,7 +1638,8 @@ int fwi2d_engine_set_option(fwi2d_engine* e, const char* key, long long value)
             if (value < 1 || value > 64) throw std::runtime_error("shots_per_block_edge must be in [1, 64]");
             x->opt_spb_edge = (int)value;
         } else if (k == "image_fused") {
-            x->opt_image_fused = value ? 1 : 0;
+            if (value < -1 || value > 1) throw std::runtime_error("image_fused must be -1, 0 or 1");
+            x->opt_image_fused = (int)value;
         } else if (k == "image_sbp") {
             x->opt_img_sbp = value ? 1 : 0;
         } else if (k == "image_stream") {
@@ -1655,7 +1668,7 @@ long long fwi2d_engine_get_stat(fwi2d_engine* e, const char* key)
     if (k == "shots_per_block_edge") return x->opt_spb_edge;
     if (k == "shots_per_block_img") return x->opt_spb_img;
     if (k == "edge_stream") return x->opt_edge_stream;
-    if (k == "image_fused") return x->opt_image_fused;
+    if (k == "image_fused") return x->image_fused() ? 1 : 0;
     if (k == "image_buffers") return x->opt_img_buffers;
     if (k == "image_stream") return x->opt_img_stream;
     if (k == "image_sbp") return x->opt_img_sbp;

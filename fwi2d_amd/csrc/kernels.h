@@ -110,12 +110,13 @@ void launch_step(const StepArgs& a, bool vti, BoxClass c, const int4* boxes, con
 void launch_init_prev(float* prev, long long shot_stride, long long nxz, int ldz, int nshots, const int* shot_ptr,
                       const PointEntry* ents, const float* trace, hipStream_t s);
 void launch_scale_prev(float* prev, const float* rho, float half_dt2, long long nxz, int nshots, hipStream_t s);
-// Density term of a fused imaging step for the injected increment of the new level
-// (on = 0: no imaging); rho_off: offset of the density component in a gradient slab.
+// Density-term share of the point sources of a fused imaging step (by parts in time,
+// on = 0: no imaging): wn f0 . (injected x, z) into the shot's gradient slab at rho_off,
+// the offset of the density component in a slab.
 struct InjImage {
-    int on, mode;
-    float wrho;
-    const float *f0, *fm, *fp;
+    int on;
+    float wn;
+    const float* f0;
     long long fshot_stride;
     float* grad;
     long long gslab_stride, rho_off;

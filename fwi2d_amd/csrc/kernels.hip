@@ -811,10 +811,15 @@ __device__ __forceinline__ void quad_to_lds(const float* src, float* lds_wave)
 // core (EDGE = false) hold only cells whose stencils are all interior, so no closure or
 // SAT code is compiled in; boxes of the edge bands (EDGE = true) take the closure and
 // SAT paths where a quad needs them.
-// IMG (the image_fused option; by default the imaging runs in k_img): adjoint step with
-// the imaging condition of forward snapshot idx fused in; the gradient contributions of
-// the workgroup's shots are summed in registers and added once per launch to the
-// gradient slab of the group's first shot.
+// IMG (the image_fused option; by default the imaging runs in k_img): adjoint step it with
+// the imaging by parts in time fused in (see k_img with SBP): the density term of
+// snapshot j = f0 against wn next + wc curr + wp prev at the cells (prev as stored, before
+// the step's taper; next before the step's point sources, whose share k_inject_cells
+// adds), and the stiffness terms of snapshot j-1 = fm, whose derivatives meet those of the
+// step's current level (the reference's lag).  fm's window goes through the LDS of the
+// products once the new level is done.  The gradient contributions of the workgroup's
+// shots are summed in registers and added once per launch to the gradient slab of the
+// group's first shot.
 template <bool VTI, int TX, int TZ, bool EDGE, bool IMG>
 __global__ void __launch_bounds__(kThreads)
 k_step(const StepArgs a, const int4* boxes, const int* list)
@@ -828,9 +833,10 @@ k_step(const StepArgs a, const int4* boxes, const int* list)
     __shared__ __attribute__((aligned(16))) float sPz[VTI ? S::IX * S::PZ : 4];
     __shared__ __attribute__((aligned(16))) float sQx[S::IX * S::PZ];
     __shared__ __attribute__((aligned(16))) float sQz[S::IX * S::PZ];
-    __shared__ __attribute__((aligned(16))) float sFx[IMG ? S::IX * S::PZ : 4];  // forward snapshot idx,
-    __shared__ __attribute__((aligned(16))) float sFz[IMG ? S::IX * S::PZ : 4];  // product-range layout
     StepSmem sm{sUx, sUz, sMu, sMuX, sPx, sPz, sQx, sQz};
+    // IMG: snapshot fm's window in the product layout, over the products (dead by then)
+    float* const sFx = sPx;
+    float* const sFz = sQx;
 
     const int lin = xcd_remap(blockIdx.x, gridDim.x);
     TileInfo t;
@@ -905,7 +911,8 @@ k_step(const StepArgs a, const int4* boxes, const int* list)
     }
 
     // ---- D2 weights of the new-level quads (core boxes, isotropic): once per workgroup ----
-    constexpr bool CW = !EDGE && !VTI && S::NQ3 == 1;
+    // (not with IMG: the imaging's accumulators and second phase need the registers)
+    constexpr bool CW = !EDGE && !VTI && S::NQ3 == 1 && !IMG;
     QuadW qw[CW ? S::NQ3 : 1];
     if constexpr (CW) {
         __syncthreads();  // the coefficient window is in LDS
@@ -948,7 +955,7 @@ k_step(const StepArgs a, const int4* boxes, const int* list)
     // core boxes: the state window goes straight from global memory into LDS at the start
     // of each shot (no register staging); other boxes prefetch the next shot's window
     // into registers during this shot
-    constexpr bool DMA = !EDGE && !IMG && S::WZ == 4 * S::WQ;
+    constexpr bool DMA = !EDGE && S::WZ == 4 * S::WQ;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     if (!DMA && s0 < s1) load_window(s0, tid, t);
 
@@ -1012,22 +1019,6 @@ k_step(const StepArgs a, const int4* boxes, const int* list)
                     st4(&sUx[o], wux[m]);
                     st4(&sUz[o], wuz[m]);
                 }
-            }
-        }
-        if constexpr (IMG) {
-            const float* F0 = a.f0 + shot * a.fshot_stride;
-#pragma unroll
-            for (int m = 0; m < S::NQ2; m++) {
-                const int i = tid + m * kThreads;
-                int lc, lq;
-                if (!prod_slot<S>(i, lc, lq)) continue;
-                const int gx = t.plx + lc, z0 = t.pz0 + 4 * lq;
-                const bool ok = gx < t.phx && z0 + 3 >= t.plz && z0 < t.phz;
-                const int g = ok ? gx * ldz + z0 : 0;
-                const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
-                const int o = lc * S::PZ + 4 * lq;
-                st4(&sFx[o], ok ? ld4(F0 + g) : z4);
-                st4(&sFz[o], ok ? ld4(F0 + nxz + g) : z4);
             }
         }
         __syncthreads();
@@ -1115,35 +1106,73 @@ k_step(const StepArgs a, const int4* boxes, const int* list)
             st4(nux + g, make_float4(ox[0], ox[1], ox[2], ox[3]));
             st4(nuz + g, make_float4(oz[0], oz[1], oz[2], oz[3]));
             if constexpr (IMG) {
-                // the step's temporaries are dead here: keep the scheduler from mixing the
-                // imaging reads into the step's arithmetic (register pressure)
+                // density term of snapshot j: f0 against the weighted adjoint levels
                 __builtin_amdgcn_sched_barrier(0);
-                const int q = (gx - t.plx) * S::PZ + (z0 - t.pz0);
-                {
-                    // density term: the new level against the second time difference of
-                    // snapshots idx-1, idx, idx+1
-                    float f0x[4], f0z[4], fmx[4], fmz[4], fpx[4], fpz[4];
-                    unpack(ld4(&sFx[q]), f0x);
-                    unpack(ld4(&sFz[q]), f0z);
-                    const long long fo = shot * a.fshot_stride + g;
-                    if (a.img_mode != 1) {
-                        unpack(ld4(a.fm + fo), fmx);
-                        unpack(ld4(a.fm + fo + nxz), fmz);
+                const long long fo = shot * a.fshot_stride + g;
+                float f0x[4], f0z[4], pvx[4], pvz[4];
+                unpack(ld4(a.f0 + fo), f0x);
+                unpack(ld4(a.f0 + fo + nxz), f0z);
+                unpack(qprx[m], pvx);
+                unpack(qprz[m], pvz);
+                const bool st = VTI && a.stiff;
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const float ax = a.wn * ox[k] + a.wc * cx[k] + a.wp * pvx[k];
+                    const float az = a.wn * oz[k] + a.wc * cz[k] + a.wp * pvz[k];
+                    const float r = z0 + k < nz ? f0x[k] * ax + f0z[k] * az : 0.f;
+                    if (VTI) {
+                        gr[m][k] += st ? 0.f : r;
+                        g4[VTI ? m : 0][k] += st ? r : 0.f;
+                    } else {
+                        gr[m][k] += r;
                     }
-                    if (a.img_mode != 2) {
-                        unpack(ld4(a.fp + fo), fpx);
-                        unpack(ld4(a.fp + fo + nxz), fpz);
-                    }
-                    image_rho<VTI>(a, z0, f0x, f0z, fmx, fmz, fpx, fpz, ox, oz, gr[m], g4[VTI ? m : 0]);
                 }
-                __builtin_amdgcn_sched_barrier(0);
-                // adjoint derivatives of the step's current level (the reference's lagged
-                // u_x, u_z), forward derivatives of snapshot idx (we_op.cpp:519-541)
+            }
+        }
+        if constexpr (IMG) {
+            // stiffness terms of snapshot j-1: its window replaces the products
+            __syncthreads();  // every wavefront is done with the products
+            const float* Fm = a.fm + shot * a.fshot_stride;
+#pragma unroll
+            for (int m = 0; m < S::NQ2; m++) {
+                const int i = tid + m * kThreads;
+                int lc, lq;
+                const bool slot = prod_slot<S>(i, lc, lq);
+                const int gx = t.plx + lc, z0 = t.pz0 + 4 * lq;
+                const bool ok = slot && gx < nx && z0 >= 0 && z0 < ldz;
+                const int g = ok ? gx * ldz + z0 : 0;
+                if constexpr (DMA && !S::ROWMAJ) {
+                    // quad i at float 4 i (column-major product slots), as the window DMA
+                    if (slot) {
+                        const int o = 4 * (m * kThreads + wave * 64);
+                        quad_to_lds(ok ? Fm + g : kZeroQuad, &sFx[o]);
+                        quad_to_lds(ok ? Fm + nxz + g : kZeroQuad, &sFz[o]);
+                    }
+                } else {
+                    if (!slot) continue;
+                    const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+                    const int o = lc * S::PZ + 4 * lq;
+                    st4(&sFx[o], ok ? ld4(Fm + g) : z4);
+                    st4(&sFz[o], ok ? ld4(Fm + nxz + g) : z4);
+                }
+            }
+            if constexpr (DMA && !S::ROWMAJ) __builtin_amdgcn_s_waitcnt(0);
+            __syncthreads();
+#pragma unroll
+            for (int m = 0; m < S::NQ3; m++) {
+                const int i = tid + m * kThreads;
+                int gx, z0;
+                out_quad<S>(t, i, gx, z0);
+                if (gx >= t.ex || z0 >= t.ez) continue;
+                const int wq = (gx - t.wsx) * S::WZ + (z0 - t.wsz);
+                const int q = (gx - t.plx) * S::PZ + (z0 - t.pz0);
+                // adjoint derivatives of the step's current level, forward derivatives of
+                // snapshot j-1 (we_op.cpp:519-541)
                 float axx[4], azx[4], axz[4], azz[4], t0x[4], t1z[4], t2x[4], t3z[4];
                 quad_derivs<EDGE, S::WZ>(a, sUx, sUz, wq, gx, z0, axx, azx, axz, azz);
                 quad_derivs<EDGE, S::PZ>(a, sFx, sFz, q, gx, z0, t0x, t3z, t2x, t1z);
-                image_stiff<VTI>(a, g, z0, axx, azx, axz, azz, t0x, t1z, t2x, t3z, gl[m], gm[m], gr[m],
-                                 g3[VTI ? m : 0]);
+                image_stiff<VTI>(a, (long long)gx * ldz + z0, z0, axx, azx, axz, azz, t0x, t1z, t2x, t3z, gl[m],
+                                 gm[m], gr[m], g3[VTI ? m : 0]);
             }
         }
     }
@@ -1458,22 +1487,10 @@ __global__ void k_inject_cells(const InjCell* cells, int ncell, const int* ptr, 
     u[0] += ix;
     u[nxz] += iz;
     if (im.on) {
-        // the fused imaging step took the new level before this increment: add its
-        // density-term contribution (we_op.cpp:533-541) to the shot's own gradient slab
+        // the fused imaging step took the new level before this increment: add its share
+        // of the density term by parts, wn f0 . (ix, iz), to the shot's own gradient slab
         const long long fo = cell.shot * im.fshot_stride + g;
-        const float f0x = im.f0[fo], f0z = im.f0[fo + nxz];
-        float d2x, d2z;
-        if (im.mode == 0) {
-            d2x = im.fp[fo] - 2.f * f0x + im.fm[fo];
-            d2z = im.fp[fo + nxz] - 2.f * f0z + im.fm[fo + nxz];
-        } else if (im.mode == 1) {
-            d2x = im.fp[fo] - f0x;
-            d2z = im.fp[fo + nxz] - f0z;
-        } else {
-            d2x = -f0x + im.fm[fo];
-            d2z = -f0z + im.fm[fo + nxz];
-        }
-        im.grad[cell.shot * im.gslab_stride + im.rho_off + g] += im.wrho * (ix * d2x + iz * d2z);
+        im.grad[cell.shot * im.gslab_stride + im.rho_off + g] += im.wn * (ix * im.f0[fo] + iz * im.f0[fo + nxz]);
     }
 }
 

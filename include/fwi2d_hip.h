@@ -123,8 +123,10 @@ int fwi2d_dot_f64_noalloc(int device, const float* a, const float* b, const floa
    enqueued before it; 2: the same, enqueued after it; 0: after it on the same stream),
    "shots_per_block_img" (shots per imaging workgroup; 0: 4, or "shots_per_block" when
    fused),
-   "image_fused" (1: the imaging condition runs inside the adjoint step kernels; 0, the
-   default: in its own kernel after each adjoint step and its point sources),
+   "image_fused" (1: the imaging condition runs inside the adjoint step kernels, by parts
+   in time (with sub = 1 and the imaging lag; otherwise as 0); 0: in its own kernel after
+   each adjoint step and its point sources; -1, the default: 1 on isotropic grids, 0 on
+   VTI grids; the stat reads the choice in effect),
    "image_buffers" (1, the default, or 2 LDS window buffers in the imaging kernel; with
    2 the next shot's windows load during this shot's imaging),
    "image_stream" (1, the default: the imaging kernel of adjoint step it runs on a second

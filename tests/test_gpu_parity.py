@@ -266,11 +266,13 @@ def test_raw_gradient_matches_oracle(lib, orc64, vti):
     eng.forward(src, keep_checkpoints=True)
     assert np.array_equal(eng.adjoint(adj), g1)
     eng.set_option("image_stream", 1)
-    # imaging fused into the step kernels (the same gradient up to the order of the sums,
-    # which the cancellations of the VTI lambda term amplify) and 32-wide tiles (step and
-    # imaging; the forward wavefield itself differs in rounding): within the FP32 gap
+    # the imaging fused into the step kernels or in its own kernel, whichever the default
+    # is not (the same gradient up to the order of the sums, which the cancellations of
+    # the VTI lambda term amplify), and 32-wide tiles (step and imaging; the forward
+    # wavefield itself differs in rounding): within the FP32 gap
+    assert eng.stat("image_fused") == (0 if vti else 1)
     eng.set_option("shots_per_block", 16)
-    for opt, val, tol in (("image_fused", 1, 1e-5), ("tile_x", 32, 1e-5)):
+    for opt, val, tol in (("image_fused", 1 if vti else 0, 1e-5), ("tile_x", 32, 1e-5)):
         eng.set_option(opt, val)
         eng.forward(src, keep_checkpoints=True)
         g2 = eng.adjoint(adj)
@@ -278,7 +280,7 @@ def test_raw_gradient_matches_oracle(lib, orc64, vti):
             err = rel(g2[c], gk[c].astype(np.float64))
             print("vti=%s %s=%d component %d rel diff %.2e" % (vti, opt, val, c, err))
             assert err < tol
-        eng.set_option(opt, 0 if opt == "image_fused" else 16)
+        eng.set_option(opt, -1 if opt == "image_fused" else 16)
 
 
 def oracle_gradient(orc64, prob, model, sxz, rxz, src, nt, dt, adj, vti, sub=1):
@@ -310,6 +312,17 @@ def test_raw_gradient_small_and_ragged_grids(lib, orc64, nx, nz, vti):
     for c in range(3):
         err = rel(gk[c], gref[c])
         print("nx=%d nz=%d vti=%s component %d raw gradient rel err %.3e" % (nx, nz, vti, c, err))
+        assert err <= 1e-4
+    # the imaging in the adjoint step kernels of every box class (and the density share of
+    # the adjoint point sources in the injection kernel), or in its own kernel: whichever
+    # the default is not
+    eng.set_option("image_fused", 1 if vti else 0)
+    eng.forward(src, keep_checkpoints=True)
+    gf = eng.adjoint(adj)
+    for c in range(3):
+        err = rel(gf[c], gref[c])
+        print("nx=%d nz=%d vti=%s component %d image_fused=%d raw gradient rel err %.3e"
+              % (nx, nz, vti, c, 1 if vti else 0, err))
         assert err <= 1e-4
 
 
