@@ -52,6 +52,8 @@ def parse():
                     help="1: the imaging condition runs inside the adjoint step kernels (by parts in time); "
                          "0: in a separate imaging kernel after each adjoint step; -1: the engine's default "
                          "(1 on isotropic grids, 0 on VTI grids)")
+    ap.add_argument("--engine-preset", action="append", default=[], metavar="KEY=VALUE",
+                    help="an engine option set before the autotuner runs, e.g. band_rows=16; repeatable")
     ap.add_argument("--engine-option", action="append", default=[], metavar="KEY=VALUE",
                     help="an engine option (fwi2d_engine_set_option) set after the autotuner ran, e.g. "
                          "shots_per_block_img=8; repeatable")
@@ -203,9 +205,12 @@ class Batch:
     """One engine holding a shot batch, with device-resident traces (torch tensors)."""
 
     def __init__(self, fwi2d_amd, torch, prob, model_true, model, sxz, rxz, src, nt, dt, tile_x, tuned=True,
-                 checkpoint_budget=0, image_fused=0):
+                 checkpoint_budget=0, image_fused=-1, presets=()):
         self.torch = torch
         self.eng = fwi2d_amd.Engine(fwi2d_amd.grid_from(prob, nt, dt, 1))
+        for kv in presets:
+            k, v = kv.split("=", 1)
+            self.eng.set_option(k, int(v))
         self.eng.set_option("checkpoint_budget", checkpoint_budget)
         self.eng.set_option("image_fused", image_fused)
         if tuned:
@@ -329,7 +334,7 @@ def step_only(args):
     wl = WORKLOADS[args.workload]
     ns, nt = args.shots or wl[4], 41
     prob, m_true, m0, sxz, rxz, src, dt = workload(args.workload, ns, nt)
-    B = Batch(fwi2d_amd, torch, prob, m_true, m0, sxz, rxz, src, nt, dt, args.tile_x)
+    B = Batch(fwi2d_amd, torch, prob, m_true, m0, sxz, rxz, src, nt, dt, args.tile_x, presets=args.engine_preset)
     B.eng.set_option("graph", 0)
     B.eng.set_option("edge_stream", args.step_only)
     B.eng.set_option("timing", 1)
@@ -337,6 +342,7 @@ def step_only(args):
         B.eng.forward_device(B.src.data_ptr(), B.rcv.data_ptr(), keep_checkpoints=False)
     us = B.eng.stat("fwd_loop_ns") / 1e3 / B.eng.stat("fwd_loop_steps")
     print(json.dumps({"workload": args.workload, "edge_stream": args.step_only, "step_us": round(us, 2),
+                      "presets": args.engine_preset,
                       "tile_x": B.eng.stat("tile_x"), "shots_per_block": B.eng.stat("shots_per_block"),
                       "shots_per_block_edge": B.eng.stat("shots_per_block_edge")}))
 
@@ -514,7 +520,7 @@ def main():
     # ---- batched form -------------------------------------------------------
     log("workload ready: %d shots, %dx%d, nt=%d" % (ns, nz, nx, nt))
     B = Batch(fwi2d_amd, torch, prob, m_true, m0, sxz, rxz, src, nt, dt, args.tile_x,
-              checkpoint_budget=args.checkpoint_budget, image_fused=args.image_fused)
+              checkpoint_budget=args.checkpoint_budget, image_fused=args.image_fused, presets=args.engine_preset)
     log("observed data modelled")
     for kv in args.engine_option:
         k, v = kv.split("=", 1)
@@ -608,7 +614,7 @@ def main():
         "checkpoint_segment": segment,
         "checkpoint_budget_mib": args.checkpoint_budget,
         "image_fused": image_fused,
-        "engine_options": args.engine_option,
+        "engine_options": args.engine_preset + args.engine_option,
         "grad_rel_err_vs_fp64": grad_err,
         "grad_err_sample": "1 shot, 160x100 grid%s, nt=600, raw lambda/mu/rho gradients vs FP64 oracle"
                            % (" (VTI)" if vti else ""),

@@ -145,6 +145,8 @@ struct Engine {
     std::string tune_report;
     int opt_spb = 8;       // shots advanced one after the other by each core-box workgroup
     int opt_spb_edge = 1;  // the same for edge tiles (their workgroups are latency-bound)
+    int opt_spb_edge_img = 0;  // edge tiles of adjoint steps with the imaging fused in (0: opt_spb_edge)
+    int opt_band_rows = 8;     // rows of the top band (the bottom band: 8-11 more than band_rows - 8)
     int opt_spb_img = 0;   // shots per imaging workgroup (0: 4, or opt_spb when fused)
     // 1: imaging condition fused into the adjoint step kernels (by parts in time, sub = 1);
     // 0: the imaging kernel after each adjoint step; -1: fused on isotropic grids, where it
@@ -174,6 +176,7 @@ struct Engine {
     int ns = 0, nr = 0;
     int src_nc = 2;        // source trace components: 2 (force x, z) or 3 (mxx, mzz, mxz)
     double gauge = 0.0;    // > 0: DAS receivers of this gauge length
+    DevMem d_das_cs;       // DAS: cos, sin of each channel's dip (the operator level's strain)
     std::vector<PointEntry> src_ents, rcv_ents;  // flat, ordered by shot
     std::vector<int> src_shot_ptr, rcv_shot_ptr;
     DevMem d_src_ents, d_rcv_ents, d_src_shot_ptr, d_rcv_shot_ptr;
@@ -253,7 +256,9 @@ struct Engine {
     // Step decomposition: the core [6, nx-6) x [8, core_z_end) where every stencil is
     // interior, cut into tile_x x 64 boxes (the tile grid clipped to the core), and the
     // edge bands around it: 120 x 8 / 64 x 16 boxes along the top and bottom (corners included),
-    // 8 x 128 boxes along the left and right sides.  Boxes partition the grid.
+    // 8 x 128 boxes along the left and right sides.  Boxes partition the grid.  The
+    // band_rows option moves the core's top and bottom by band_rows - 8 rows into the
+    // grid (thicker bands in 64 x 16 boxes).
     void build_boxes()
     {
         boxes.clear();
@@ -263,7 +268,8 @@ struct Engine {
         list_bx.clear();
         const int nx = g.nx, nz = g.nz;
         const int cx0 = kCoreX0, cx1 = nx - kCoreX0;
-        const int cz0 = kCoreZ0, cz1 = std::max(core_z_end(nz), kCoreZ0);
+        const int cz0 = std::min(kCoreZ0 + (opt_band_rows - 8), nz);
+        const int cz1 = std::min(std::max(core_z_end(nz) - (opt_band_rows - 8), cz0), nz);
         auto add = [&](std::vector<int>& list, int sx, int ex, int sz, int ez) {
             if (ex <= sx || ez <= sz) return;
             list.push_back((int)boxes.size());
@@ -279,15 +285,21 @@ struct Engine {
             }
         }
         // top and bottom bands: 8-row bands in 120 x 8 boxes, taller ones (the bottom band
-        // holds 8 to 11 rows) in 64 x 16 boxes
-        auto zband = [&](int sz, int ez) {
-            const bool thin = ez - sz <= 8;
-            const int w = thin ? kBandZ8Width : 64;
-            const int nb = (nx + w - 1) / w;
-            for (int k = 0; k < nb; k++) {
-                int sx, ex;
-                tile_range_x(k, nb, w, nx, sx, ex);
-                add(thin ? list_bz8 : list_bz, sx, ex, sz, ez);
+        // holds 8 to 11 rows with the default band_rows) in 64 x 16 boxes; a band of more
+        // than 16 rows in rows of boxes split evenly (8 to 16 rows each).  Only a box at the
+        // grid's top or bottom takes the thin class, whose compact window serves no other.
+        auto zband = [&](int sz0, int ez) {
+            const int n = ez - sz0, nchunk = (n + 15) / 16;
+            for (int c = 0; c < nchunk; c++) {
+                const int sz = sz0 + n * c / nchunk, e = sz0 + n * (c + 1) / nchunk;
+                const bool thin = e - sz <= 8 && (sz == 0 || e == nz);
+                const int w = thin ? kBandZ8Width : 64;
+                const int nb = (nx + w - 1) / w;
+                for (int k = 0; k < nb; k++) {
+                    int sx, ex;
+                    tile_range_x(k, nb, w, nx, sx, ex);
+                    add(thin ? list_bz8 : list_bz, sx, ex, sz, e);
+                }
             }
         };
         zband(0, cz0);
@@ -619,6 +631,16 @@ struct Engine {
         nr = nrt;
         src_nc = mt ? 3 : 2;
         gauge = gl > 0 ? gl : 0.0;
+        if (gl > 0) {
+            std::vector<double> cs(2 * (size_t)nrt);
+            for (int k = 0; k < nrt; k++) {
+                cs[2 * k] = std::cos(rdip[k]);
+                cs[2 * k + 1] = std::sin(rdip[k]);
+            }
+            d_das_cs.reserve(std::max<size_t>(cs.size() * sizeof(double), 16));
+            if (nrt) HIPCHECK(hipMemcpyAsync(d_das_cs.p, cs.data(), cs.size() * sizeof(double), hipMemcpyHostToDevice, st));
+            HIPCHECK(hipStreamSynchronize(st));
+        }
         src_ents.swap(se);
         rcv_ents.swap(re);
         rcv_ext_ents.swap(xe);
@@ -788,7 +810,7 @@ struct Engine {
         // edge bands on their own stream, concurrent with the core boxes (disjoint cells),
         // or after them on the same stream
         StepArgs e = a;
-        e.spb = std::max(1, std::min(opt_spb_edge, ns));
+        e.spb = std::max(1, std::min(fuse_img && opt_spb_edge_img > 0 ? opt_spb_edge_img : opt_spb_edge, ns));
         e.ngroups = (ns + e.spb - 1) / e.spb;
         const bool img = fuse_img;
         const int4* bx = d_boxes.as<int4>();
@@ -921,6 +943,8 @@ struct Engine {
         d_tdata.reserve((size_t)std::max<long long>(ntr * m.nrow, 1) * sizeof(float));
         launch_trace_matvec(d_tm_ptr.as<int>(), d_tm_col.as<int>(), d_tm_w.as<double>(), m.nrow, m.ncol,
                             d_rcv.as<float>(), d_tdata.as<float>(), ntr, st);
+        // DAS: gauge-end differences -> strain; it commutes with the per-trace time map
+        if (gauge > 0) launch_das_strain(d_tdata.as<float>(), d_das_cs.as<double>(), nr, m.nrow, gauge, false, st);
         if (ntr * m.nrow > 0)
             HIPCHECK(hipMemcpyAsync(out, d_tdata.p, (size_t)(ntr * m.nrow) * sizeof(float), hipMemcpyDeviceToHost, st));
         HIPCHECK(hipStreamSynchronize(st));
@@ -932,14 +956,16 @@ struct Engine {
         require_ready();
         const long long ntr = 2LL * nr;
         upload_matrix(m, -1, g.nt);
-        const float* ddata = upload_traces(d_tdata, data, std::max<long long>(ntr * m.ncol, 1));
+        float* ddata = upload_traces(d_tdata, data, std::max<long long>(ntr * m.ncol, 1));
+        // DAS: strain -> gauge-end differences (we_op.cpp:1130-1135)
+        if (gauge > 0) launch_das_strain(ddata, d_das_cs.as<double>(), nr, m.ncol, gauge, true, st);
         d_adjsrc.reserve((size_t)std::max<long long>(ntr * g.nt, 1) * sizeof(float));
         launch_trace_matvec(d_tm_ptr.as<int>(), d_tm_col.as<int>(), d_tm_w.as<double>(), m.nrow, m.ncol, ddata,
                             d_adjsrc.as<float>(), ntr, st);
         adjoint(d_adjsrc.as<float>(), grad, true, false);
     }
 
-    const float* upload_traces(DevMem& d, const float* h, long long n)
+    float* upload_traces(DevMem& d, const float* h, long long n)
     {
         d.reserve((size_t)n * sizeof(float));
         HIPCHECK(hipMemcpyAsync(d.p, h, (size_t)n * sizeof(float), hipMemcpyHostToDevice, st));
@@ -956,7 +982,7 @@ struct Engine {
     std::vector<long long> loop_key(int kind, const void* p0, const void* p1, int flag) const
     {
         auto P = [](const void* p) { return (long long)(uintptr_t)p; };
-        return {kind, P(p0), P(p1), flag, g.nt, ns, nr, tile_x, opt_spb, opt_spb_edge, opt_lag, seg_K, nseg,
+        return {kind, P(p0), P(p1), flag, g.nt, ns, nr, tile_x, opt_spb, opt_spb_edge, opt_spb_edge_img, opt_lag, seg_K, nseg,
                 opt_bf16, opt_stiff, opt_edge_stream, opt_spb_img, opt_image_fused, opt_img_buffers, opt_img_stream, opt_img_sbp, P(d_ckpt16.p), P(d_ckpt_scale.p), P(d_amax.p),
                 P(d_fwd.p), P(d_adj.p), P(d_ckpt.p), P(d_seg.p), P(d_gshot.p), P(d_model.p), P(d_deriv.p),
                 P(d_boxes.p), P(d_list_core.p), P(d_list_bz.p), P(d_list_bz8.p), P(d_list_bx.p), P(inj_src.d_cells.p), P(inj_src.d_ptr.p),
@@ -1637,6 +1663,13 @@ int fwi2d_engine_set_option(fwi2d_engine* e, const char* key, long long value)
         } else if (k == "shots_per_block_edge") {
             if (value < 1 || value > 64) throw std::runtime_error("shots_per_block_edge must be in [1, 64]");
             x->opt_spb_edge = (int)value;
+        } else if (k == "band_rows") {
+            if (value != 8 && value != 12 && value != 16) throw std::runtime_error("band_rows must be 8, 12 or 16");
+            x->opt_band_rows = (int)value;
+            x->build_boxes();
+        } else if (k == "shots_per_block_edge_img") {
+            if (value < 0 || value > 64) throw std::runtime_error("shots_per_block_edge_img must be in [0, 64]");
+            x->opt_spb_edge_img = (int)value;
         } else if (k == "image_fused") {
             if (value < -1 || value > 1) throw std::runtime_error("image_fused must be -1, 0 or 1");
             x->opt_image_fused = (int)value;
@@ -1666,6 +1699,8 @@ long long fwi2d_engine_get_stat(fwi2d_engine* e, const char* key)
     if (k == "tile_x") return x->tile_x;
     if (k == "shots_per_block") return x->opt_spb;
     if (k == "shots_per_block_edge") return x->opt_spb_edge;
+    if (k == "shots_per_block_edge_img") return x->opt_spb_edge_img;
+    if (k == "band_rows") return x->opt_band_rows;
     if (k == "shots_per_block_img") return x->opt_spb_img;
     if (k == "edge_stream") return x->opt_edge_stream;
     if (k == "image_fused") return x->image_fused() ? 1 : 0;

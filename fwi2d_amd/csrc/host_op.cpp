@@ -584,29 +584,6 @@ struct fwi2d_op {
         tm_adj = to_matrix(A, ntd);
     }
 
-    // dipole_to_strain (misc.cpp:18-43) on data [2][nr][nt_data]: forward
-    // p0 = (cos p0 + sin p1) / gl, p1 = 0; adjoint p1 = sin p0 / gl, p0 = cos p0 / gl.  It
-    // commutes with the per-trace time maps (the same linear map on every trace), so it
-    // runs on the data axis here where the reference applies it on the propagation axis.
-    void dipole_to_strain(bool adjoint, float* p) const
-    {
-        const int n = d.nt_data;
-        for (int r = 0; r < nr; r++) {
-            const double c = std::cos(rdip[r]), sn = std::sin(rdip[r]);
-            float* p0 = p + (size_t)r * n;
-            float* p1 = p + (size_t)(nr + r) * n;
-            for (int t = 0; t < n; t++) {
-                if (!adjoint) {
-                    p0[t] = (float)((c * p0[t] + sn * p1[t]) / d.gl);
-                    p1[t] = 0.f;
-                } else {
-                    p1[t] = (float)(sn * p0[t] / d.gl);
-                    p0[t] = (float)(c * p0[t] / d.gl);
-                }
-            }
-        }
-    }
-
     void run_forward(const std::vector<float>& m, std::vector<float>& rcv)
     {
         check(fwi2d_engine_set_model(eng, m.data()));
@@ -632,7 +609,6 @@ struct fwi2d_op {
         const bool keep = axis.sub > 0;
         fwi2d::engine_forward_traces(eng, src_prop.data(), keep, tm_fwd, out.data());
         if (keep) ckpt_model = m;
-        if (d.gl > 0) dipole_to_strain(false, out.data());
         for (size_t i = 0; i < n; i++) pdat[i] = add ? pdat[i] + out[i] : out[i];
     }
 
@@ -647,13 +623,9 @@ struct fwi2d_op {
                 std::vector<float> rcv;
                 run_forward(m, rcv);
             }
-            if (d.gl > 0) {  // strain -> gauge-end differences (we_op.cpp:1130-1135)
-                std::vector<float> p(pdat, pdat + (size_t)2 * nr * d.nt_data);
-                dipole_to_strain(true, p.data());
-                fwi2d::engine_adjoint_traces(eng, p.data(), tm_adj, graw.data());
-            } else {
-                fwi2d::engine_adjoint_traces(eng, pdat, tm_adj, graw.data());
-            }
+            // (DAS channels: the device maps strain to gauge-end differences first,
+            // dipole_to_strain's adjoint; it commutes with the per-trace time map)
+            fwi2d::engine_adjoint_traces(eng, pdat, tm_adj, graw.data());
         }
         // H quadrature (we_op.cpp:1192-1199) and chain rule (we_op.cpp:1208-1219)
         std::vector<double> gd(graw.begin(), graw.end());

@@ -142,6 +142,9 @@ void launch_fill(float* p, long long n, float v, hipStream_t s);
 void launch_trace_matvec(const int* ptr, const int* col, const double* w, int nrow, int ncol, const float* in,
                          float* out, long long ntr, hipStream_t s);
 void launch_copy(float* dst, const float* src, long long n, hipStream_t s);
+// dipole_to_strain of nr DAS channels in place on traces [2][nr][nt] (cs: cos, sin of
+// each channel's dip; forward or adjoint)
+void launch_das_strain(float* tr, const double* cs, int nr, int nt, double gl, bool adjoint, hipStream_t s);
 // per-shot max-abs scaled 16-bit packing of a time level, FP16 (f16) or BF16 (amax:
 // nshots words of scratch)
 void launch_pack16(bool f16, const float* u, unsigned short* q, float* scale, unsigned* amax, long long per_shot,

@@ -1600,6 +1600,27 @@ __global__ void k_fill(float* p, long long n, float v)
     for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) p[i] = v;
 }
 
+// dipole_to_strain (misc.cpp:18-43) of DAS channels, in place on traces [2][nr][nt]:
+// forward x = (c x + s z) / gl, z = 0; adjoint z = s x / gl, x = c x / gl, with cs = (cos,
+// sin) of each channel's dip; in double precision, one sample per thread.
+__global__ void k_das_strain(float* tr, const double* cs, int nr, int nt, double gl, int adjoint)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)nr * nt) return;
+    const int r = (int)(i / nt);
+    const double c = cs[2 * r], sn = cs[2 * r + 1];
+    float* px = tr + i;
+    float* pz = tr + (long long)nr * nt + i;
+    const double x = *px;
+    if (!adjoint) {
+        *px = (float)((c * x + sn * (double)*pz) / gl);
+        *pz = 0.f;
+    } else {
+        *pz = (float)(sn * x / gl);
+        *px = (float)(c * x / gl);
+    }
+}
+
 // The per-trace time map of the operator level (resampling, time derivative, quadrature
 // weights, reversal, composed on the host): out[x][i] = sum_k w[k] in[x][col[k]] over the
 // row-compressed entries k of row i, accumulated in double precision; one output sample
@@ -1836,6 +1857,13 @@ void launch_trace_matvec(const int* ptr, const int* col, const double* w, int nr
     const long long n = ntr * nrow;
     if (n <= 0) return;
     hipLaunchKernelGGL(k_trace_matvec, dim3(grid_for(n)), dim3(kThreads), 0, s, ptr, col, w, nrow, ncol, in, out, n);
+}
+
+void launch_das_strain(float* tr, const double* cs, int nr, int nt, double gl, bool adjoint, hipStream_t s)
+{
+    const long long n = (long long)nr * nt;
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_das_strain, dim3(grid_for(n)), dim3(kThreads), 0, s, tr, cs, nr, nt, gl, adjoint ? 1 : 0);
 }
 
 void launch_pack16(bool f16, const float* u, unsigned short* q, float* scale, unsigned* amax, long long per_shot,

@@ -21,11 +21,14 @@ struct TraceMatrix {
 };
 
 // Forward run of every shot (sources at the propagation step, host memory) whose receiver
-// traces [2 nr][nt] are mapped by m (ncol = nt) on the device into out [2 nr][m.nrow] (host).
+// traces [2 nr][nt] are mapped by m (ncol = nt) on the device into out [2 nr][m.nrow] (host);
+// DAS channels (a gauge length set with the acquisition) then go to strain on the device
+// (dipole_to_strain, misc.cpp:18-43).
 void engine_forward_traces(fwi2d_engine* e, const float* src, bool keep, const TraceMatrix& m, float* out);
 
 // Adjoint run whose adjoint sources are m applied on the device to data [2 nr][m.ncol]
-// (host, m.nrow = nt); the raw gradient [np][nx][nz] goes to grad (host).
+// (host, m.nrow = nt; DAS channels: dipole_to_strain's adjoint first); the raw gradient
+// [np][nx][nz] goes to grad (host).
 void engine_adjoint_traces(fwi2d_engine* e, const float* data, const TraceMatrix& m, float* grad);
 
 }  // namespace fwi2d

@@ -119,6 +119,8 @@ int fwi2d_dot_f64_noalloc(int device, const float* a, const float* b, const floa
 /* Tuning and test options (no reference counterpart): "tile_x" (16 | 32 cells per tile
    along x), "shots_per_block" (1..64 shots advanced in turn by one step workgroup of an
    interior tile), "shots_per_block_edge" (the same for tiles at the grid edge),
+   "shots_per_block_edge_img" (the same for edge tiles of adjoint steps with the imaging
+   fused in; 0, the default: "shots_per_block_edge"),
    "edge_stream" (1: edge-band launches run on a second stream beside the core launch,
    enqueued before it; 2: the same, enqueued after it; 0: after it on the same stream),
    "shots_per_block_img" (shots per imaging workgroup; 0: 4, or "shots_per_block" when
@@ -152,7 +154,7 @@ int fwi2d_dot_f64_noalloc(int device, const float* a, const float* b, const floa
    reference's one-step imaging lag, 0 none), "timing" (1 time the forward step loop with
    HIP events), "graph" (1: capture each forward / adjoint time loop in a hipGraph the
    first time it runs in a configuration and replay it afterwards).  Statistics: "steps", "recompute_steps",
-   "image_calls", "segment", "nseg", "tile_x", "shots_per_block", "shots_per_block_edge", "edge_stream",
+   "image_calls", "segment", "nseg", "tile_x", "shots_per_block", "shots_per_block_edge", "shots_per_block_edge_img", "edge_stream",
    "image_fused", "image_buffers", "image_stream", "image_sbp", "ntiles", "interior_tiles", "graph_captures", "graph_replays", "nrec",
    "nshots", "fwd_loop_ns", "fwd_loop_steps"; -1 for an unknown key. */
 int fwi2d_engine_set_option(fwi2d_engine* e, const char* key, long long value);

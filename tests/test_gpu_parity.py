@@ -326,6 +326,34 @@ def test_raw_gradient_small_and_ragged_grids(lib, orc64, nx, nz, vti):
         assert err <= 1e-4
 
 
+@pytest.mark.parametrize("band_rows,vti", [(12, False), (16, False), (16, True)])
+def test_thick_edge_bands(lib, orc64, orc32, band_rows, vti):
+    """The core moved band_rows - 8 rows into the grid at the top and bottom, so that the
+    bands (the top one from row 0) run in 64 x 16 boxes: forward traces and raw gradients
+    (imaging fused into the adjoint step on the isotropic grid) against the FP64 oracle,
+    also on a grid whose core is empty."""
+    for nx, nz in ((69, 133), (21, 30)):
+        prob, model, sxz, rxz, src, nt, dt = engine_case(nx=nx, nz=nz, nt=200, taper=4, ns=2, vti=vti)
+        case = (prob, model, sxz, rxz, src, nt, dt)
+        ref = oracle_forward(orc64, *case)
+        gap = rel(oracle_forward(orc32, *case), ref)
+        eng, out = run_engine(lib, *case, sub=1, keep=True, options=(("band_rows", band_rows),))
+        assert eng.stat("band_rows") == band_rows
+        err = rel(out, ref)
+        tol = max(1e-5, 2 * gap)
+        print("band_rows=%d vti=%s %dx%d forward rel err %.3e (tol %.1e)" % (band_rows, vti, nx, nz, err, tol))
+        assert err <= tol
+        rng = np.random.default_rng(nx + nz)
+        adj = rng.standard_normal((2, sum(len(r) for r in rxz), nt)) * np.hanning(nt)
+        gref = oracle_gradient(orc64, prob, model, sxz, rxz, src, nt, dt, adj, vti)
+        gk = eng.adjoint(adj)
+        for c in range(3):
+            err = rel(gk[c], gref[c])
+            print("band_rows=%d vti=%s %dx%d component %d raw gradient rel err %.3e" % (band_rows, vti, nx, nz, c, err))
+            assert err <= 1e-4
+        eng.close()
+
+
 def test_operator_level_forward_and_gradient(lib, orc64):
     """apply_forward / apply_jacobianT with resampling (data 2 ms, propagation 1 ms),
     particle-velocity data and the H-quadrature + chain rule post-processing."""
