@@ -549,16 +549,23 @@ def main():
             "shots_per_block_edge": B.eng.stat("shots_per_block_edge"), "edge_stream": B.eng.stat("edge_stream"),
             "report": B.eng.tune_report()}
     graph_replays = B.eng.stat("graph_replays")
+    recompute = B.eng.stat("recompute_steps")
     B.eng.set_option("graph", 0)
     B.eng.set_option("timing", 1)
     B.eng.forward_device(B.src.data_ptr(), B.rcv.data_ptr(), keep_checkpoints=False)
     loop_ns = B.eng.stat("fwd_loop_ns")
     nsteps = B.eng.stat("fwd_loop_steps")
-    B.eng.set_option("timing", 0)
     step_s = loop_ns * 1e-9 / nsteps
+    # one more gradient, eager, for the time split of its adjoint run: the recomputation
+    # steps at the forward step's time, the rest the adjoint steps with their imaging
+    rec0 = B.eng.stat("recompute_steps")
+    B.gradient()
+    adj_s = B.eng.stat("adj_ns") * 1e-9
+    rec_steps = B.eng.stat("recompute_steps") - rec0
+    B.eng.set_option("timing", 0)
+    adjoint_step_us = 1e6 * (adj_s - rec_steps * step_s) / (nt - 1)
     cells_per_launch = ns * nx * nz
     achieved = bytes_per_cell * cells_per_launch / step_s / 1e9
-    recompute = B.eng.stat("recompute_steps")
     segment = B.eng.stat("segment")
     del B
     torch.cuda.empty_cache()
@@ -611,6 +618,9 @@ def main():
         "plain_one_shot_per_launch_shots_per_s": plain_shots_per_s and round(plain_shots_per_s, 4),
         "batched_vs_plain": plain_shots_per_s and round(shots_per_s / plain_shots_per_s, 3),
         "recompute_steps_per_gradient": int(recompute // max(args.steps + args.warmup, 1)),
+        "time_split_eager_s": {"forward": round(step_s * (nt - 1), 3), "adjoint_run": round(adj_s, 3),
+                               "recompute_steps": int(rec_steps),
+                               "adjoint_step_us_with_imaging": round(adjoint_step_us, 1)},
         "checkpoint_segment": segment,
         "checkpoint_budget_mib": args.checkpoint_budget,
         "image_fused": image_fused,

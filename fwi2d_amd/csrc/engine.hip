@@ -165,6 +165,7 @@ struct Engine {
     hipEvent_t ev_img_in = nullptr, ev_img_done[2] = {nullptr, nullptr};
     bool img_out[2] = {false, false};
     double last_fwd_loop_ms = 0.0;  // HIP-event time of the last forward step loop (timing option)
+    double last_adj_ms = 0.0;       // HIP-event time of the last adjoint run, recomputation included
     int last_fwd_loop_steps = 0;
     bool vti = false;
 
@@ -1389,6 +1390,12 @@ struct Engine {
         if (opt_stiff && (!vti || !fused))
             throw std::runtime_error("stiffness gradients need a VTI grid and the imaging lag");
         const int nslab = ns;
+        hipEvent_t ev0 = nullptr, ev1 = nullptr;
+        if (opt_timing) {
+            HIPCHECK(hipEventCreate(&ev0));
+            HIPCHECK(hipEventCreate(&ev1));
+            HIPCHECK(hipEventRecord(ev0, st));
+        }
         d_gshot.reserve((size_t)nslab * np * nxz * sizeof(float));
         launch_fill(d_gshot.as<float>(), (long long)nslab * np * nxz, 0.f, st);
         const int nt = g.nt, sub = g.sub;
@@ -1452,6 +1459,15 @@ struct Engine {
         d_gout.reserve((size_t)np * nxz * sizeof(float));
         float* out = d_gout.as<float>();
         launch_sum_shots(d_gshot.as<float>(), (long long)np * nxz, nslab, (long long)np * nxz, out, 0, st);
+        if (opt_timing) {
+            HIPCHECK(hipEventRecord(ev1, st));
+            HIPCHECK(hipEventSynchronize(ev1));
+            float ms = 0.f;
+            HIPCHECK(hipEventElapsedTime(&ms, ev0, ev1));
+            last_adj_ms = ms;
+            (void)hipEventDestroy(ev0);
+            (void)hipEventDestroy(ev1);
+        }
         // [np][nx][ldz] -> [np][nx][nz]
         HIPCHECK(hipMemcpy2DAsync(grad, (size_t)g.nz * sizeof(float), out, (size_t)ldz * sizeof(float),
                                   (size_t)g.nz * sizeof(float), (size_t)np * g.nx,
@@ -1715,6 +1731,7 @@ long long fwi2d_engine_get_stat(fwi2d_engine* e, const char* key)
     if (k == "nshots") return x->ns;
     if (k == "fwd_loop_ns") return (long long)(x->last_fwd_loop_ms * 1e6);
     if (k == "fwd_loop_steps") return x->last_fwd_loop_steps;
+    if (k == "adj_ns") return (long long)(x->last_adj_ms * 1e6);
     return -1;
 }
 

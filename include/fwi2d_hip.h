@@ -156,7 +156,8 @@ int fwi2d_dot_f64_noalloc(int device, const float* a, const float* b, const floa
    first time it runs in a configuration and replay it afterwards).  Statistics: "steps", "recompute_steps",
    "image_calls", "segment", "nseg", "tile_x", "shots_per_block", "shots_per_block_edge", "shots_per_block_edge_img", "edge_stream",
    "image_fused", "image_buffers", "image_stream", "image_sbp", "ntiles", "interior_tiles", "graph_captures", "graph_replays", "nrec",
-   "nshots", "fwd_loop_ns", "fwd_loop_steps"; -1 for an unknown key. */
+   "nshots", "fwd_loop_ns", "fwd_loop_steps", "adj_ns" (with the "timing" option: the last
+   adjoint run's device time, recomputation included); -1 for an unknown key. */
 int fwi2d_engine_set_option(fwi2d_engine* e, const char* key, long long value);
 long long fwi2d_engine_get_stat(fwi2d_engine* e, const char* key);
 
