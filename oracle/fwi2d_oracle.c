@@ -1045,6 +1045,185 @@ int orc_propagate(const orc_grid* g, const real* model,
 }
 
 /* ------------------------------------------------------------------------- */
+/* Acoustic operator nl_we_op_a (we_op.cpp:1863-2135): one pressure-like field, */
+/* model (K, 1/rho) = (rho vp^2, 1/rho)                                         */
+/* ------------------------------------------------------------------------- */
+/* cell k rows (or columns) into side `side` (0 top, 1 bottom, 2 left, 3 right) on line l */
+static inline long aside_cell(int side, int l, int k, int nx, int nz)
+{
+    if (side == 0) return (long)l * nz + k;
+    if (side == 1) return (long)l * nz + nz - 1 - k;
+    if (side == 2) return (long)k * nz + l;
+    return (long)(nx - 1 - k) * nz + l;
+}
+
+/* asat_dirichlet_*, asat_absorbing_*, asat_neumann_* (spatial_operators.cpp:551-777),
+   added into out over the whole side: kind 1 Dirichlet (the free surface; a its
+   stiffness parameter), 2 absorbing (in1 the previous level), 3 Neumann. */
+void orc_asat(int side, int kind, const real* in0, const real* in1, real* out, const real* K, const real* ri,
+              int nx, int nz, double dx, double dz, double dt, double a)
+{
+    const int nline = side < 2 ? nx : nz;
+    const double d = side < 2 ? dz : dx;
+    for (int l = 0; l < nline; l++) {
+        const long b = aside_cell(side, l, 0, nx, nz);
+        if (kind == 1) {
+            /* + H^-1 S^T (ri in0)_0, - H^-1 (ri in0 / (h a))_0 */
+            for (int k = 0; k < 4; k++)
+                out[aside_cell(side, l, k, nx, nz)] += (real)(1.0 / (HCOEF[k] * d * d)) * (real)SCOEF[k] * ri[b] * in0[b];
+            out[b] -= ri[b] * in0[b] / (real)(a * HCOEF[0] * d * d);
+        } else {
+            /* - H^-1 (ri S in0 [- in1 / (2 dt rho v)])_0 */
+            real sum = 0;
+            for (int k = 0; k < 4; k++) sum += (real)SCOEF[k] * in0[aside_cell(side, l, k, nx, nz)];
+            real v = ri[b] * sum / (real)d;
+            if (kind == 2) v -= in1[b] / ((real)(2 * dt) * (real)sqrt(K[b] / ri[b]));
+            out[b] -= (real)(1.0 / (HCOEF[0] * d)) * v;
+        }
+    }
+}
+
+/* asat_scale_boundaries (spatial_operators.cpp:779-860): a cell on an absorbing side is
+   divided by 1 + the sum, over the absorbing sides it lies on, of v dt / (2 h H0) with
+   v = sqrt(K / rho) of the cell (the corners take both of their sides, as the reference's
+   sc1/sc2 bookkeeping does). */
+void orc_asat_scale(real* u, const real* K, const real* ri, int nx, int nz, double dx, double dz, double dt,
+                    int top, int bottom, int left, int right)
+{
+    for (int ix = 0; ix < nx; ix++)
+        for (int iz = 0; iz < nz; iz++) {
+            const int on_t = top && iz == 0, on_b = bottom && iz == nz - 1;
+            const int on_l = left && ix == 0, on_r = right && ix == nx - 1;
+            if (!(on_t || on_b || on_l || on_r)) continue;
+            const long i = (long)ix * nz + iz;
+            const real v = (real)sqrt(K[i] * ri[i]) * (real)dt;
+            real s = 0;
+            if (on_t) s += v / (real)(2 * dz * HCOEF[0]);
+            if (on_l) s += v / (real)(2 * dx * HCOEF[0]);
+            if (on_b) s += v / (real)(2 * dz * HCOEF[0]);
+            if (on_r) s += v / (real)(2 * dx * HCOEF[0]);
+            u[i] = u[i] / (1 + s);
+        }
+}
+
+/* nl_we_op_a::compute_gradients (we_op.cpp:1883-1935): K gradient (with the -1/K^2 of
+   the K^-1 parameterisation) from the second time difference of the snapshots against
+   the adjoint level, 1/rho gradient from first derivatives; grad [2][nx][nz] accumulated */
+static void acoustic_gradients(const orc_grid* g, const real* model, const real* full, const real* adj, real* tmp,
+                               real* grad, int idx, double dtg)
+{
+    const int nx = g->nx, nz = g->nz;
+    const long nxz = (long)nx * nz;
+    const real* K = model;
+    const real* F0 = full + (long)idx * nxz;
+    real *t0 = tmp, *t1 = tmp + nxz, *t2 = tmp + 2 * nxz, *t3 = tmp + 3 * nxz;
+    orc_Dx(0, F0, t0, NULL, nx, nz, g->dx, 1.0);
+    orc_Dz(0, F0, t1, NULL, nx, nz, g->dz, 1.0);
+    orc_Dx(0, adj, t2, NULL, nx, nz, g->dx, 1.0);
+    orc_Dz(0, adj, t3, NULL, nx, nz, g->dz, 1.0);
+    const double t = idx * dtg;
+    double w = 1;
+    if (g->gdamp_tmax - t > 0 && t - g->gdamp_tmin >= 0)
+        w = cos(g->gdamp_power * M_PI * (t - g->gdamp_tmin) / (g->gdamp_tmax - g->gdamp_tmin));
+    w *= w;
+    const int last = g->nt / g->sub;
+    for (long i = 0; i < nxz; i++) {
+        real d2;
+        real wr = (real)(w * dtg);
+        if (idx > 0 && idx < last) {
+            d2 = full[(long)(idx + 1) * nxz + i] - 2 * F0[i] + full[(long)(idx - 1) * nxz + i];
+        } else if (idx == 0) {
+            d2 = full[(long)(idx + 1) * nxz + i] - F0[i];
+            wr = (real)(w * 0.5 * dtg);
+        } else {
+            d2 = -F0[i] + full[(long)(idx - 1) * nxz + i];
+        }
+        grad[i] -= (real)w / ((real)dtg * K[i] * K[i]) * adj[i] * d2;
+        grad[nxz + i] += wr * (t2[i] * t0[i] + t3[i] * t1[i]);
+    }
+}
+
+/* nl_we_op_a::propagate (we_op.cpp:1937-2135) with point injectors and extractors of one
+   component; the boundary kinds come from bc_* (1 Dirichlet free surface, 2 absorbing,
+   3 Neumann), alpha is the Dirichlet SAT parameter 0.2508560249 / free_surface_stiffness */
+static int propagate_acoustic_pts(const orc_grid* g, const real* model, double alpha, const pts* inj, const real* ib,
+                                  const pts* ext, real* ob, real* full, real* grad)
+{
+    d2b_init();
+    const int nx = g->nx, nz = g->nz, nt = g->nt, sub = g->sub;
+    const long nxz = (long)nx * nz;
+    const double dx = g->dx, dz = g->dz, dt = g->dt;
+    if (nx < 12 || nz < 12 || nt < 2) return 1;
+    if (grad && (sub <= 0 || !full)) return 2;
+    const int bc[4] = {g->bc_top, g->bc_bottom, g->bc_left, g->bc_right};
+    for (int k = 0; k < 4; k++)
+        if (bc[k] < 1 || bc[k] > 3) return 4;
+    real* work = (real*)calloc((size_t)nxz * 7, sizeof(real));
+    if (!work) return 3;
+    real* prev = work;
+    real* curr = work + nxz;
+    real* next = work + 2 * nxz;
+    real* tmp = work + 3 * nxz;  /* 4 fields for the gradients */
+    const real* K = model;
+    const real* ri = model + nxz;
+
+    /* prev = K/2 dt^2 inject(src[0])  (we_op.cpp:1977-1979) */
+    pts_inject(inj, ib, NULL, 0, prev, NULL);
+    for (long i = 0; i < nxz; i++) prev[i] *= (real)0.5 * K[i] * (real)dt * (real)dt;
+
+    for (int it = 0; it < nt - 1; it++) {
+        if (!grad && sub > 0 && it % sub == 0 && full) memcpy(full + (long)(it / sub) * nxz, curr, nxz * sizeof(real));
+        if (!grad && ext) pts_extract(ext, ob, NULL, it, curr, NULL);
+        if (grad && (nt - 1 - it) % sub == 0 && it != 0)
+            acoustic_gradients(g, model, full, curr, tmp, grad, (nt - 1 - it) / sub, sub * dt);
+        orc_Dxx_var(0, curr, next, ri, nx, nz, dx, 1.0);
+        orc_Dzz_var(1, curr, next, ri, nx, nz, dz, 1.0);
+        pts_inject(inj, ib, NULL, it, next, NULL);
+        for (int side = 0; side < 4; side++)
+            orc_asat(side, bc[side], curr, prev, next, K, ri, nx, nz, dx, dz, dt, alpha);
+        for (long i = 0; i < nxz; i++) next[i] = (real)dt * (real)dt * next[i] * K[i] + 2 * curr[i] - prev[i];
+        orc_asat_scale(next, K, ri, nx, nz, dx, dz, dt, bc[0] == 2, bc[1] == 2, bc[2] == 2, bc[3] == 2);
+        orc_taper(curr, nx, nz, g->taper_top, g->taper_bottom, g->taper_left, g->taper_right, g->taper_strength);
+        orc_taper(next, nx, nz, g->taper_top, g->taper_bottom, g->taper_left, g->taper_right, g->taper_strength);
+        real* b = prev; prev = curr; curr = next; next = b;
+    }
+    if (!grad && sub > 0 && (nt - 1) % sub == 0 && full) memcpy(full + (long)((nt - 1) / sub) * nxz, curr, nxz * sizeof(real));
+    if (!grad && ext) pts_extract(ext, ob, NULL, nt - 1, curr, NULL);
+    if (grad) acoustic_gradients(g, model, full, curr, tmp, grad, 0, sub * dt);
+    free(work);
+    return 0;
+}
+
+/* delta_m3 footprints of one component (the acoustic pressure) */
+static void pts_delta1(pts* p, int ntr, const int* xi, const int* zi, const real* xw, const real* zw, int row, int nt,
+                       int nz)
+{
+    pts_alloc(p, 9 * ntr);
+    for (int t = 0; t < ntr; t++)
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++)
+                pts_add(p, 0, (long)xi[3 * t + a] * nz + zi[3 * t + b], (long)t * nt,
+                        xw[6 * t + 3 * row + a] * zw[6 * t + 3 * row + b]);
+}
+
+/* acoustic propagate with delta_m3 injectors and extractors: in [ninj][nt] source traces,
+   out [next][nt] receiver traces (accumulated), full [nsnap][nx][nz] snapshots (forward)
+   or read (adjoint with grad [2][nx][nz]); model (K, 1/rho) */
+int orc_propagate_acoustic(const orc_grid* g, const real* model, double alpha, int ninj, const int* ixi,
+                           const int* izi, const real* ixw, const real* izw, const real* in, int next_,
+                           const int* exi, const int* ezi, const real* exw, const real* ezw, real* out, real* full,
+                           real* grad)
+{
+    pts inj, ext;
+    pts_delta1(&inj, ninj, ixi, izi, ixw, izw, 0, g->nt, g->nz);
+    if (next_ > 0) pts_delta1(&ext, next_, exi, ezi, exw, ezw, 1, g->nt, g->nz);
+    const int rc = propagate_acoustic_pts(g, model, alpha, &inj, in, next_ > 0 ? &ext : NULL, out, full, grad);
+    pts_free(&inj);
+    if (next_ > 0) pts_free(&ext);
+    return rc;
+}
+
+/* ------------------------------------------------------------------------- */
 /* Model preparation: analyzeModel (we_op.cpp:267-474) and convert_model         */
 /* (we_op.cpp:476-494, 1441-1462)                                               */
 /* ------------------------------------------------------------------------- */

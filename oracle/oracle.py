@@ -188,6 +188,45 @@ class Oracle:
         assert rc == 0, "orc_propagate rc=%d" % rc
         return out
 
+    # ---- acoustic operator (nl_we_op_a) ----------------------------------
+    def asat(self, side, kind, in0, in1, out, K, ri, dx, dz, dt, a):
+        """out + the acoustic SAT of one side (kind 1 Dirichlet, 2 absorbing, 3 Neumann)."""
+        out = self.arr(out).copy()
+        in0, K, ri = self.arr(in0), self.arr(K), self.arr(ri)
+        in1 = self.arr(in1) if in1 is not None else in0
+        self.lib.orc_asat(side, kind, self.p(in0), self.p(in1), self.p(out), self.p(K), self.p(ri), out.shape[0],
+                          out.shape[1], C.c_double(dx), C.c_double(dz), C.c_double(dt), C.c_double(a))
+        return out
+
+    def asat_scale(self, u, K, ri, dx, dz, dt, top, bottom, left, right):
+        u = self.arr(u).copy()
+        self.lib.orc_asat_scale(self.p(u), self.p(self.arr(K)), self.p(self.arr(ri)), u.shape[0], u.shape[1],
+                                C.c_double(dx), C.c_double(dz), C.c_double(dt), top, bottom, left, right)
+        return u
+
+    def propagate_acoustic(self, g, model, inj, inj_traces, ext=None, full=None, grad=None, alpha=0.2508560249):
+        """Acoustic propagation, model (K, 1/rho) [2][nx][nz]; inj = (xi, zi, xw, zw) tables,
+        inj_traces [ninj][nt].  Returns receiver traces [next][nt] in forward mode.  full:
+        [nsnap][nx][nz] snapshots (written forward, read with grad [2][nx][nz])."""
+        model = self.arr(model)
+        xi, zi, xw, zw = inj
+        tr = self.arr(inj_traces)
+        nt = g.nt
+        assert tr.shape[-1] == nt
+        if ext is not None:
+            exi, ezi, exw, ezw = ext
+            nr = exi.shape[0]
+            out = np.zeros((nr, nt), self.dtype)
+            ep = (_ptr(exi, C.c_int), _ptr(ezi, C.c_int), self.p(self.arr(exw)), self.p(self.arr(ezw)))
+        else:
+            nr, out, ep = 0, None, (None, None, None, None)
+        rc = self.lib.orc_propagate_acoustic(C.byref(g), self.p(model), C.c_double(alpha), xi.shape[0],
+                                             _ptr(xi, C.c_int), _ptr(zi, C.c_int), self.p(self.arr(xw)),
+                                             self.p(self.arr(zw)), self.p(tr), nr, ep[0], ep[1], ep[2], ep[3],
+                                             self.p(out), self.p(full), self.p(grad))
+        assert rc == 0, "orc_propagate_acoustic rc=%d" % rc
+        return out
+
     # ---- model and traces ------------------------------------------------
     def analyze_model(self, mp, model, dx, dz):
         model = self.arr(model).copy()
@@ -509,6 +548,20 @@ class RefSBP:
         self.lib.ref_Dxx_var(0, self.p(u), self.p(out), self.p(c), u.shape[0], u.shape[1], C.c_double(d),
                              C.c_double(1.0))
         return out
+
+    def asat(self, side, kind, in0, in1, out, K, ri, dx, dz, dt, a):
+        out = self.arr(out).copy()
+        in0, K, ri = self.arr(in0), self.arr(K), self.arr(ri)
+        in1 = self.arr(in1) if in1 is not None else in0
+        self.lib.ref_asat(side, kind, self.p(in0), self.p(in1), self.p(out), self.p(K), self.p(ri), out.shape[0],
+                          out.shape[1], C.c_double(dx), C.c_double(dz), C.c_double(dt), C.c_double(a))
+        return out
+
+    def asat_scale(self, u, K, ri, dx, dz, dt, top, bottom, left, right):
+        u = self.arr(u).copy()
+        self.lib.ref_asat_scale(self.p(u), self.p(self.arr(K)), self.p(self.arr(ri)), u.shape[0], u.shape[1],
+                                C.c_double(dx), C.c_double(dz), C.c_double(dt), top, bottom, left, right)
+        return u
 
     def Dzz(self, u, d):
         u = self.arr(u)

@@ -91,3 +91,45 @@ EXPORT void ref_applyH(int axis_x, int inv, const data_t* in, data_t* out, int n
     if (axis_x) applyHx(inv != 0, false, in, out, nx, nz, d, 0, nx, 0, nz);
     else applyHz(inv != 0, false, in, out, nx, nz, d, 0, nx, 0, nz);
 }
+
+// Acoustic SAT terms (asat_dirichlet_*, asat_absorbing_*, asat_neumann_*) on one side over
+// its whole length, out += SAT; kind 1 Dirichlet (stiffness parameter a), 2 absorbing
+// (in1 the previous level), 3 Neumann; model (K, 1/rho).
+EXPORT void ref_asat(int side, int kind, const data_t* in0, const data_t* in1, data_t* out, const data_t* K,
+                     const data_t* ri, int nx, int nz, data_t dx, data_t dz, data_t dt, data_t a)
+{
+    const data_t* par[2] = {K, ri};
+    const data_t* in[2] = {in0, in1};
+    const int n = side < 2 ? nx : nz;
+    if (kind == 1) {
+        switch (side) {
+        case 0: asat_dirichlet_top(true, in, out, nx, nz, dx, dz, 0, n, par, a); break;
+        case 1: asat_dirichlet_bottom(true, in, out, nx, nz, dx, dz, 0, n, par, a); break;
+        case 2: asat_dirichlet_left(true, in, out, nx, nz, dx, dz, 0, n, par, a); break;
+        default: asat_dirichlet_right(true, in, out, nx, nz, dx, dz, 0, n, par, a); break;
+        }
+    } else if (kind == 2) {
+        switch (side) {
+        case 0: asat_absorbing_top(true, in, out, nx, nz, dx, dz, dt, 0, n, par, a); break;
+        case 1: asat_absorbing_bottom(true, in, out, nx, nz, dx, dz, dt, 0, n, par, a); break;
+        case 2: asat_absorbing_left(true, in, out, nx, nz, dx, dz, dt, 0, n, par, a); break;
+        default: asat_absorbing_right(true, in, out, nx, nz, dx, dz, dt, 0, n, par, a); break;
+        }
+    } else {
+        switch (side) {
+        case 0: asat_neumann_top(true, in, out, nx, nz, dx, dz, 0, n, par, a); break;
+        case 1: asat_neumann_bottom(true, in, out, nx, nz, dx, dz, 0, n, par, a); break;
+        case 2: asat_neumann_left(true, in, out, nx, nz, dx, dz, 0, n, par, a); break;
+        default: asat_neumann_right(true, in, out, nx, nz, dx, dz, 0, n, par, a); break;
+        }
+    }
+}
+
+// asat_scale_boundaries over the whole grid; model (K, 1/rho)
+EXPORT void ref_asat_scale(data_t* u, const data_t* K, const data_t* ri, int nx, int nz, data_t dx, data_t dz,
+                           data_t dt, int top, int bottom, int left, int right)
+{
+    const data_t* par[2] = {K, ri};
+    data_t* in[1] = {u};
+    asat_scale_boundaries(in, nx, nz, dx, dz, 0, nx, 0, nz, par, dt, top, bottom, left, right);
+}

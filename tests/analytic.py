@@ -182,3 +182,22 @@ def plane_wave_speed(traces, pol, dt, v_expected, f0=10.0, d=(0.2, 0.5)):
     y0, y1, y2 = cc[k - 1], cc[k], cc[k + 1]
     lag = (k - (nt - 1) + 0.5 * (y0 - y2) / (y0 - 2 * y1 + y2)) * dt
     return (d[1] - d[0]) / lag
+
+
+def acoustic_greens_traces(src, dt, v, rho, offsets, nt=None, pad=16):
+    """Pressure traces [len(offsets)][nt] at receiver offsets (dx, dz) from a point source
+    s(t) of the acoustic operator, u_tt = K (div(grad u / rho) + s(t) delta(x - xs)), in a
+    homogeneous full space: u = K g_v * s / v^2 = rho g_v * s with the 2-D Helmholtz
+    Green's function g_v of the module docstring."""
+    src = np.asarray(src, np.float64)
+    nt = len(src) if nt is None else nt
+    N = pad * len(src)
+    F = np.fft.rfft(src, N) * dt
+    w = 2 * np.pi * np.fft.rfftfreq(N, dt)[1:]
+    out = np.zeros((len(offsets), nt))
+    for n, (ox, oz) in enumerate(offsets):
+        r = float(np.hypot(ox, oz))
+        G = np.zeros(len(w) + 1, complex)
+        G[1:] = rho * (-0.25j) * hankel2(0, w / v * r)
+        out[n] = np.fft.irfft(G * F, N)[:nt] / dt
+    return out
