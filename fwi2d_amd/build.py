@@ -16,7 +16,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libfwi2d_hip.so")
 
-HIP_SOURCES = ["kernels.hip", "engine.hip"]
+HIP_SOURCES = ["kernels.hip", "engine.hip", "acoustic.hip"]
 CXX_SOURCES = ["host_op.cpp", "solver.cpp", "io.cpp"]
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-I" + os.path.join(ROOT, "include"), "-I" + CSRC]

@@ -155,4 +155,45 @@ void launch_dot_f64(const float* a, const float* b, const float* w, long long n,
                     int nblocks, hipStream_t s);
 void launch_reduce_f64(const double* partial, int n, double* out, hipStream_t s);
 
+// ---------------------------------------------------------------------------
+// Acoustic operator (nl_we_op_a, we_op.cpp:1863-2135): one field per shot, [nshots][nx][ldz]
+// ---------------------------------------------------------------------------
+constexpr int kAcTX = 16, kAcTZ = 64;  // step and imaging tiles (x, z)
+
+struct AcousticArgs {
+    int nx, nz, ldz, nshots, ntx, ntz, taper_prev;
+    int bc[4];                   // top, bottom, left, right: 1 Dirichlet, 2 absorbing, 3 Neumann
+    float rdx2, rdz2;            // 1 / dx^2, 1 / dz^2
+    float dir_x[4], dir_z[4];    // Dirichlet SAT S^T terms SC[k] / (H[k] h^2)
+    float pen_x, pen_z;          // Dirichlet penalty 1 / (a H0 h^2)
+    float neu_x, neu_z;          // Neumann / absorbing 1 / (H0 h^2)
+    float abs_x, abs_z;          // absorbing 1 / (2 dt H0 h)
+    const float *K, *ri, *kdt2, *div, *taper_x, *taper_z;  // [nx][ldz] (tapers per axis)
+    const float* prev;
+    const float* curr;
+    float* next;
+    long long shot_stride;       // nx * ldz
+    // imaging (k_agrad): adjoint level curr against snapshots f0 (idx), fm, fp
+    const float *f0, *fm, *fp;
+    long long fshot_stride;
+    float* grad;                 // per-shot slabs [nshots][2][nx][ldz]
+    long long gslab_stride;
+    int mode;                    // 0 interior snapshot, 1 the first, 2 the last
+    float wk;                    // K term weight w / dtg
+    float wr;                    // 1/rho term weight w dtg (halved for the first snapshot)
+    float rdx, rdz;              // 1 / dx, 1 / dz
+};
+// one time step of every shot: next from prev and curr (no point sources)
+void launch_astep(const AcousticArgs& a, hipStream_t s);
+// imaging condition of one snapshot for every shot, into the per-shot slabs
+void launch_agrad(const AcousticArgs& a, hipStream_t s);
+// point sources: next[cell] += scale[cell] * sum_k w[k] trace[off[k] + it] / div * taper
+// (scale = dt^2 K; with init: prev = 1/2 dt^2 K sum_k ..., no divisor or taper)
+void launch_ainject(const InjCell* cells, int ncell, const int* ptr, const long long* off, const float* w,
+                    const float* trace, int it, float* u, long long shot_stride, int ldz, const float* kdt2,
+                    const float* div, const float* taper_x, const float* taper_z, bool init, hipStream_t s);
+// receivers: trace[r][it] = sum over the 9 footprint cells of w * u (shot of the receiver)
+void launch_aextract(const float* u, long long shot_stride, int ldz, const PointEntry* ents, int nrec, float* trace,
+                     int it, hipStream_t s);
+
 }  // namespace fwi2d

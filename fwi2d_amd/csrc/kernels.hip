@@ -1900,4 +1900,225 @@ void launch_reduce_f64(const double* partial, int n, double* out, hipStream_t s)
     hipLaunchKernelGGL(k_reduce_f64, dim3(1), dim3(kThreads), 0, s, partial, n, out);
 }
 
+// ---------------------------------------------------------------------------
+// Acoustic operator (nl_we_op_a::propagate, we_op.cpp:1937-2135; the SAT terms and their
+// scaling, spatial_operators.cpp:551-860).  A workgroup owns a 16 (x) x 64 (z) tile of
+// one shot; the level and 1/rho windows (halo 8, which holds every closure row a tile
+// can need) go through LDS, and each thread forms the new level of one quad (column-
+// major, so a wavefront covers 4 whole columns): D2x + D2z of the 1/rho-weighted
+// Laplacian (interior weights or closure rows per axis), the SAT terms of the boundary
+// cells, the leapfrog with dt^2 K, the absorbing boundary divisor and the cos^2 taper.
+// The reference tapers the current level in place after each step; here the previous
+// level is tapered on load instead (taper_prev), which gives the same values.
+// ---------------------------------------------------------------------------
+constexpr int kAcH = 8;
+constexpr int kAcWX = kAcTX + 2 * kAcH, kAcWZ = kAcTZ + 2 * kAcH, kAcWQ = kAcWZ / 4;
+static_assert(kAcTX * (kAcTZ / 4) == kThreads, "one quad per thread");
+
+// the level and 1/rho windows of tile (sx, sz) of one shot into LDS
+__device__ __forceinline__ void ac_windows(const float* u, const float* c, float* sU, float* sC, int wsx, int wsz,
+                                           int nx, int ldz)
+{
+    for (int i = threadIdx.x; i < kAcWX * kAcWQ; i += kThreads) {
+        const int lx = i / kAcWQ, lq = i - lx * kAcWQ;
+        const int gx = wsx + lx, gz = wsz + 4 * lq;
+        const bool ok = gx >= 0 && gx < nx && gz >= 0 && gz < ldz;
+        const long long g = ok ? (long long)gx * ldz + gz : 0;
+        const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+        st4(&sU[lx * kAcWZ + 4 * lq], ok ? ld4(u + g) : z4);
+        if (c) st4(&sC[lx * kAcWZ + 4 * lq], ok ? ld4(c + g) : z4);
+    }
+}
+
+// SAT terms of one side at a cell k cells in from it (0 <= k < 4), added to rhs: pu / pc
+// point at the side's boundary cell of the cell's line in the level / 1/rho windows, st
+// the window stride pointing inwards, p the cell's previous level (absorbing), K the
+// boundary cell's bulk modulus (absorbing).
+__device__ __forceinline__ float ac_sat(int kind, int k, const float* pu, const float* pc, int st, float p, float K,
+                                        const float* dir, float pen, float neu, float absc)
+{
+    if (kind == 1) {
+        // Dirichlet (free surface): + H^-1 S^T (ri u)_0 on the first 4 cells, - penalty at 0
+        float v = dir[k] * pc[0] * pu[0];
+        if (k == 0) v -= pc[0] * pu[0] * pen;
+        return v;
+    }
+    if (k != 0) return 0.f;
+    // Neumann: - H^-1 (ri S u)_0; absorbing: + the impedance term of the previous level
+    float v = -neu * pc[0] * bnd_deriv(pu, st);
+    if (kind == 2) v += absc * p / sqrtf(K / pc[0]);
+    return v;
+}
+
+__global__ void __launch_bounds__(kThreads) k_astep(const AcousticArgs a)
+{
+    __shared__ __attribute__((aligned(16))) float sU[kAcWX * kAcWZ];
+    __shared__ __attribute__((aligned(16))) float sR[kAcWX * kAcWZ];
+    const int tile = blockIdx.x, shot = blockIdx.y;
+    const int tix = tile / a.ntz, tiz = tile - tix * a.ntz;
+    const int sx = tix * kAcTX, sz = tiz * kAcTZ;
+    const int ex = min(sx + kAcTX, a.nx), ez = min(sz + kAcTZ, a.ldz);
+    const int wsx = sx - kAcH, wsz = sz - kAcH;
+    const int nx = a.nx, nz = a.nz, ldz = a.ldz;
+    const long long so = (long long)shot * a.shot_stride;
+    ac_windows(a.curr + so, a.ri, sU, sR, wsx, wsz, nx, ldz);
+    __syncthreads();
+    const int q = threadIdx.x;
+    const int gx = sx + q / (kAcTZ / 4), z0 = sz + 4 * (q % (kAcTZ / 4));
+    if (gx >= ex || z0 >= ez) return;
+    const long long g = (long long)gx * ldz + z0;
+    float pv[4], kd[4], dv[4], tz[4], out[4];
+    unpack(ld4(a.prev + so + g), pv);
+    unpack(ld4(a.kdt2 + g), kd);
+    unpack(ld4(a.div + g), dv);
+    unpack(ld4(a.taper_z + z0), tz);
+    const float tx = a.taper_x[gx];
+    const int col = (gx - wsx) * kAcWZ;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int gz = z0 + k;
+        if (gz >= nz) {
+            out[k] = 0.f;  // rows past nz stay zero
+            continue;
+        }
+        const int w = col + (gz - wsz);
+        const float* pu = &sU[w];
+        const float* pc = &sR[w];
+        const float lx = (gx >= 6 && gx < nx - 6) ? d2_apply(d2_weights(pc, kAcWZ), pu, kAcWZ)
+                                                 : d2_closure(pu, pc, kAcWZ, gx, nx);
+        const float lz = (gz >= 6 && gz < nz - 6) ? d2_apply(d2_weights(pc, 1), pu, 1) : d2_closure(pu, pc, 1, gz, nz);
+        float rhs = lx * a.rdx2 + lz * a.rdz2;
+        float p = pv[k];
+        if (a.taper_prev) p = (p * tz[k]) * tx;
+        if (gz < 4) {  // top side, boundary cell (gx, 0)
+            const int b = col - wsz;
+            rhs += ac_sat(a.bc[0], gz, &sU[b], &sR[b], 1, p, gz == 0 ? a.K[g - z0] : 0.f, a.dir_z, a.pen_z, a.neu_z,
+                          a.abs_z);
+        }
+        if (gz >= nz - 4) {  // bottom side, boundary cell (gx, nz - 1)
+            const int b = col + (nz - 1 - wsz);
+            rhs += ac_sat(a.bc[1], nz - 1 - gz, &sU[b], &sR[b], -1, p,
+                          gz == nz - 1 ? a.K[(long long)gx * ldz + nz - 1] : 0.f, a.dir_z, a.pen_z, a.neu_z, a.abs_z);
+        }
+        if (gx < 4) {  // left side, boundary cell (0, gz)
+            const int b = (0 - wsx) * kAcWZ + (gz - wsz);
+            rhs += ac_sat(a.bc[2], gx, &sU[b], &sR[b], kAcWZ, p, gx == 0 ? a.K[gz] : 0.f, a.dir_x, a.pen_x, a.neu_x,
+                          a.abs_x);
+        }
+        if (gx >= nx - 4) {  // right side, boundary cell (nx - 1, gz)
+            const int b = (nx - 1 - wsx) * kAcWZ + (gz - wsz);
+            rhs += ac_sat(a.bc[3], nx - 1 - gx, &sU[b], &sR[b], -kAcWZ, p,
+                          gx == nx - 1 ? a.K[(long long)(nx - 1) * ldz + gz] : 0.f, a.dir_x, a.pen_x, a.neu_x, a.abs_x);
+        }
+        // leapfrog (we_op.cpp:2093-2098), boundary divisor (asat_scale_boundaries), taper
+        const float v = (kd[k] * rhs + (2.f * pu[0] - p)) / dv[k];
+        out[k] = (v * tz[k]) * tx;
+    }
+    st4(a.next + so + g, make_float4(out[0], out[1], out[2], out[3]));
+}
+
+// nl_we_op_a::compute_gradients (we_op.cpp:1883-1935) for every shot: the K term from the
+// second time difference of the snapshots against the adjoint level, the 1/rho term
+// from first derivatives of both; added to the shot's gradient slab
+__global__ void __launch_bounds__(kThreads) k_agrad(const AcousticArgs a)
+{
+    __shared__ __attribute__((aligned(16))) float sA[kAcWX * kAcWZ];
+    __shared__ __attribute__((aligned(16))) float sF[kAcWX * kAcWZ];
+    const int tile = blockIdx.x, shot = blockIdx.y;
+    const int tix = tile / a.ntz, tiz = tile - tix * a.ntz;
+    const int sx = tix * kAcTX, sz = tiz * kAcTZ;
+    const int ex = min(sx + kAcTX, a.nx), ez = min(sz + kAcTZ, a.ldz);
+    const int wsx = sx - kAcH, wsz = sz - kAcH;
+    const int nx = a.nx, nz = a.nz, ldz = a.ldz;
+    const long long so = (long long)shot * a.shot_stride, fo = (long long)shot * a.fshot_stride;
+    ac_windows(a.curr + so, a.f0 + fo, sA, sF, wsx, wsz, nx, ldz);
+    __syncthreads();
+    const int q = threadIdx.x;
+    const int gx = sx + q / (kAcTZ / 4), z0 = sz + 4 * (q % (kAcTZ / 4));
+    if (gx >= ex || z0 >= ez) return;
+    const long long g = (long long)gx * ldz + z0;
+    float K[4], fm[4] = {0.f, 0.f, 0.f, 0.f}, fp[4] = {0.f, 0.f, 0.f, 0.f}, gk[4], gr[4];
+    unpack(ld4(a.K + g), K);
+    if (a.mode != 1) unpack(ld4(a.fm + fo + g), fm);
+    if (a.mode != 2) unpack(ld4(a.fp + fo + g), fp);
+    float* G = a.grad + (long long)shot * a.gslab_stride + g;
+    unpack(ld4(G), gk);
+    unpack(ld4(G + a.shot_stride), gr);
+    const int col = (gx - wsx) * kAcWZ;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int gz = z0 + k;
+        if (gz >= nz) continue;
+        const int w = col + (gz - wsz);
+        const float* pa = &sA[w];
+        const float* pf = &sF[w];
+        const float ax = d1<false>(pa, kAcWZ, gx, nx) * a.rdx, az = d1<false>(pa, 1, gz, nz) * a.rdz;
+        const float fx = d1<false>(pf, kAcWZ, gx, nx) * a.rdx, fz = d1<false>(pf, 1, gz, nz) * a.rdz;
+        const float d2 = a.mode == 0 ? fp[k] - 2.f * pf[0] + fm[k] : (a.mode == 1 ? fp[k] - pf[0] : -pf[0] + fm[k]);
+        gk[k] -= a.wk / (K[k] * K[k]) * pa[0] * d2;
+        gr[k] += a.wr * (ax * fx + az * fz);
+    }
+    st4(G, make_float4(gk[0], gk[1], gk[2], gk[3]));
+    st4(G + a.shot_stride, make_float4(gr[0], gr[1], gr[2], gr[3]));
+}
+
+// point sources of the acoustic operator: per touched cell, the sum of its contributions
+// in injector order, times dt^2 K (the leapfrog's factor), over the boundary divisor and
+// times the taper, as the reference's injection into the right-hand side becomes after
+// the step; init: the initial condition prev = 1/2 dt^2 K inject(trace sample 0)
+__global__ void k_ainject(const InjCell* cells, int ncell, const int* ptr, const long long* off, const float* w,
+                          const float* trace, int it, float* u, long long shot_stride, int ldz, const float* kdt2,
+                          const float* div, const float* taper_x, const float* taper_z, int init)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    const InjCell cell = cells[c];
+    float f = 0.f;
+    for (int k = ptr[c]; k < ptr[c + 1]; k++) f += w[k] * trace[off[k] + it];
+    const long long g = (long long)cell.gx * ldz + cell.gz;
+    float* p = u + cell.shot * shot_stride + g;
+    if (init) *p += 0.5f * kdt2[g] * f;
+    else *p += ((kdt2[g] * f / div[g]) * taper_z[cell.gz]) * taper_x[cell.gx];
+}
+
+__global__ void k_aextract(const float* u, long long shot_stride, int ldz, const PointEntry* ents, int nrec,
+                           float* trace, int it)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrec) return;
+    const PointEntry e = ents[r];
+    const float* p = u + e.shot * shot_stride;
+    float v = 0.f;
+    for (int ix = 0; ix < 3; ix++)
+        for (int iz = 0; iz < 3; iz++) v += e.wx[ix] * e.wz[iz] * p[(long long)(e.xi0 + ix) * ldz + e.zi0 + iz];
+    trace[e.trace + it] = v;
+}
+
+void launch_astep(const AcousticArgs& a, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_astep, dim3(a.ntx * a.ntz, a.nshots), dim3(kThreads), 0, s, a);
+}
+
+void launch_agrad(const AcousticArgs& a, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_agrad, dim3(a.ntx * a.ntz, a.nshots), dim3(kThreads), 0, s, a);
+}
+
+void launch_ainject(const InjCell* cells, int ncell, const int* ptr, const long long* off, const float* w,
+                    const float* trace, int it, float* u, long long shot_stride, int ldz, const float* kdt2,
+                    const float* div, const float* taper_x, const float* taper_z, bool init, hipStream_t s)
+{
+    if (ncell <= 0) return;
+    hipLaunchKernelGGL(k_ainject, dim3((ncell + 127) / 128), dim3(128), 0, s, cells, ncell, ptr, off, w, trace, it, u,
+                       shot_stride, ldz, kdt2, div, taper_x, taper_z, init ? 1 : 0);
+}
+
+void launch_aextract(const float* u, long long shot_stride, int ldz, const PointEntry* ents, int nrec, float* trace,
+                     int it, hipStream_t s)
+{
+    if (nrec <= 0) return;
+    hipLaunchKernelGGL(k_aextract, dim3((nrec + 127) / 128), dim3(128), 0, s, u, shot_stride, ldz, ents, nrec, trace,
+                       it);
+}
+
 }  // namespace fwi2d

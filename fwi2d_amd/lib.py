@@ -49,7 +49,9 @@ EXPORTS = [
     "fwi2d_nlls_create", "fwi2d_nlls_evaluate", "fwi2d_nlls_residual", "fwi2d_nlls_dnorm", "fwi2d_nlls_destroy",
     "fwi2d_nlls_objective", "fwi2d_lbfgs", "fwi2d_nlls_create_filtered", "fwi2d_filter_prepare",
     "fwi2d_filter_apply", "fwi2d_dot_f64_noalloc", "fwi2d_read_header", "fwi2d_read_data", "fwi2d_write",
-    "fwi2d_read_coord", "fwi2d_regular_geometry", "fwi2d_engine_set_acquisition",
+    "fwi2d_read_coord", "fwi2d_regular_geometry", "fwi2d_engine_set_acquisition", "fwi2d_acoustic_create",
+    "fwi2d_acoustic_set_model", "fwi2d_acoustic_set_shots", "fwi2d_acoustic_forward", "fwi2d_acoustic_adjoint",
+    "fwi2d_acoustic_destroy",
 ]
 
 FORMAT_SEP, FORMAT_BINARY = 0, 1
@@ -103,6 +105,12 @@ def load():
     lib.fwi2d_engine_tune_report.argtypes = [P]
     lib.fwi2d_engine_tune_report.restype = C.c_char_p
     lib.fwi2d_engine_destroy.argtypes = [P]
+    lib.fwi2d_acoustic_create.argtypes = [C.POINTER(Fwi2dGrid), C.c_double, C.c_int, C.POINTER(P)]
+    lib.fwi2d_acoustic_set_model.argtypes = [P, P]
+    lib.fwi2d_acoustic_set_shots.argtypes = [P, C.c_int, P, P, P, C.c_double, C.c_double]
+    lib.fwi2d_acoustic_forward.argtypes = [P, P, P, C.c_int, C.c_int]
+    lib.fwi2d_acoustic_adjoint.argtypes = [P, P, P, C.c_int]
+    lib.fwi2d_acoustic_destroy.argtypes = [P]
     lib.fwi2d_op_create.argtypes = [C.POINTER(Fwi2dOpDesc), P, C.POINTER(P)]
     lib.fwi2d_op_apply_forward.argtypes = [P, C.c_int, P, P]
     lib.fwi2d_op_apply_jacobianT.argtypes = [P, C.c_int, P, P, P]
@@ -152,6 +160,54 @@ def grid_from(prob, nt, dt, sub):
             setattr(g, k, prob[k])
     g.nt, g.dt, g.sub = int(nt), float(dt), int(sub)
     return g
+
+
+class AcousticEngine:
+    """Batched device propagation of the acoustic operator (nl_we_op_a), engine level of
+    the C ABI: model (K, 1/rho) [2][nx][nz], one trace component."""
+
+    def __init__(self, grid, free_surface_stiffness=1.0, device=0):
+        lib = load()
+        self._h = C.c_void_p()
+        self.grid = grid
+        _check(lib.fwi2d_acoustic_create(C.byref(grid), float(free_surface_stiffness), int(device),
+                                         C.byref(self._h)))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().fwi2d_acoustic_destroy(self._h)
+            self._h = C.c_void_p()
+
+    __del__ = close
+
+    def set_model(self, model):
+        m = _f32(model)
+        assert m.shape == (2, self.grid.nx, self.grid.nz), m.shape
+        _check(load().fwi2d_acoustic_set_model(self._h, _ptr(m)))
+
+    def set_shots(self, sxz, rxz, ox=0.0, oz=0.0):
+        sxz = np.ascontiguousarray(sxz, dtype=np.float64).reshape(-1, 2)
+        nrec = np.ascontiguousarray([len(r) for r in rxz], dtype=np.int32)
+        allr = np.ascontiguousarray(np.concatenate([np.asarray(r, np.float64).reshape(-1, 2) for r in rxz], 0))
+        self.ns, self.nr = len(sxz), int(nrec.sum())
+        self._geom = (sxz, nrec, allr)
+        _check(load().fwi2d_acoustic_set_shots(self._h, self.ns, _ptr(sxz), _ptr(nrec), _ptr(allr), ox, oz))
+
+    def forward(self, src, keep_snapshots=False):
+        """src [ns][nt] (host numpy) -> receivers [nr][nt]."""
+        s = _f32(src)
+        assert s.shape == (self.ns, self.grid.nt), s.shape
+        out = np.zeros((self.nr, self.grid.nt), np.float32)
+        _check(load().fwi2d_acoustic_forward(self._h, _ptr(s), _ptr(out), int(keep_snapshots), 0))
+        return out
+
+    def adjoint(self, adj):
+        """adjoint sources [nr][nt] -> raw (K, 1/rho) gradient [2][nx][nz]."""
+        a = _f32(adj)
+        assert a.shape == (self.nr, self.grid.nt), a.shape
+        g = np.zeros((2, self.grid.nx, self.grid.nz), np.float32)
+        _check(load().fwi2d_acoustic_adjoint(self._h, _ptr(a), _ptr(g), 0))
+        return g
 
 
 class Engine:

@@ -171,6 +171,42 @@ void fwi2d_engine_destroy(fwi2d_engine* e);
 const char* fwi2d_last_error(void);
 
 /* ------------------------------------------------------------------------- */
+/* Acoustic engine: nl_we_op_a::propagate (src/cpp/we_op.cpp:1937-2135) for all */
+/* shots at once (the reference runs this operator on the CPU only)            */
+/* ------------------------------------------------------------------------- */
+typedef struct fwi2d_acoustic fwi2d_acoustic;
+
+/* The grid as for the elastic engine (vti ignored); the boundaries are 1 (free surface,
+   the Dirichlet SAT asat_dirichlet_*), 2 (absorbing, asat_absorbing_*) or 3 (rigid,
+   asat_neumann_*); free_surface_stiffness is par.free_surface_stiffness (raised to 1 as
+   the reference does for this operator, we_op.cpp:16; the SAT parameter is
+   0.2508560249 / stiffness, we_op.cpp:1962). */
+int fwi2d_acoustic_create(const fwi2d_grid* grid, double free_surface_stiffness, int device, fwi2d_acoustic** out);
+
+/* The converted model [2][nx][nz]: K = rho vp^2 and 1/rho (nl_we_op_a::convert_model,
+   we_op.cpp:1863-1881).  Invalidates snapshots of a previous forward run. */
+int fwi2d_acoustic_set_model(fwi2d_acoustic* e, const float* model);
+
+/* Point sources and receivers with delta_m3 footprints (src/cpp/injector.cpp:3-64), as
+   fwi2d_engine_set_shots. */
+int fwi2d_acoustic_set_shots(fwi2d_acoustic* e, int nshots, const double* sxz, const int* nrec, const double* rxz,
+                             double ox, double oz);
+
+/* propagate(adj = false) of every shot: source traces src [nshots][nt], receiver traces
+   rcv [nrec total][nt]; keep_snapshots: keep every sub-th level (1 + nt/sub levels per
+   shot) on the device for fwi2d_acoustic_adjoint.  on_device: src and rcv are device
+   pointers. */
+int fwi2d_acoustic_forward(fwi2d_acoustic* e, const float* src, float* rcv, int keep_snapshots, int on_device);
+
+/* propagate(adj = true) with compute_gradients (we_op.cpp:1883-1935): adjoint sources adj
+   [nrec total][nt] injected at the receivers; grad [2][nx][nz] receives the raw K and
+   1/rho gradients summed over the shots (before the H quadrature and the chain rule of
+   apply_jacobianT, we_op.cpp:1192-1230). */
+int fwi2d_acoustic_adjoint(fwi2d_acoustic* e, const float* adj, float* grad, int on_device);
+
+void fwi2d_acoustic_destroy(fwi2d_acoustic* e);
+
+/* ------------------------------------------------------------------------- */
 /* Operator level (drop-in for nl_we_op_e / nl_we_op_vti)                      */
 /* ------------------------------------------------------------------------- */
 

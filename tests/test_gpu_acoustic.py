@@ -1,0 +1,120 @@
+"""The acoustic engine (fwi2d_acoustic_*, nl_we_op_a::propagate, src/cpp/we_op.cpp:1937-2135)
+against the FP64 oracle: receiver traces within the reference algorithm's own FP32 gap
+(at least 1e-5) and raw (K, 1/rho) gradients within 1e-4, for every boundary kind (free
+surface, absorbing, rigid), with and without tapers, ragged grids and sub > 1; the batch
+of shots equals each shot alone bit for bit."""
+import numpy as np
+import pytest
+
+from fwi2d_amd import workloads as W
+from oracle.oracle import grid_struct
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(b))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    import fwi2d_amd
+    fwi2d_amd.load()
+    return fwi2d_amd
+
+
+def case(nx=90, nz=70, nt=400, bc=(1, 2, 2, 2), taper=10, ns=2, sub=1, seed=0):
+    h, dt = 0.01, 0.001
+    rng = np.random.default_rng(seed)
+    x = np.arange(nx)[:, None] * h
+    z = np.arange(nz)[None, :] * h
+    vp = 1.8 + 1.2 * z + 0.2 * np.exp(-((x - 0.4) ** 2 + (z - 0.3) ** 2) / 0.01) + 0 * x
+    rho = 2.0 + 0.2 * np.sin(2 * np.pi * x / 0.6) * np.cos(2 * np.pi * z / 0.5)
+    model = np.stack([rho * vp ** 2, 1 / rho])
+    sxz = np.stack([rng.uniform(0.1, (nx - 10) * h, ns), rng.uniform(0.02, 0.15, ns)], 1)
+    rxz = [np.stack([np.linspace(0.03, (nx - 4) * h, 12 + 3 * s), np.full(12 + 3 * s, 0.02 + 0.01 * s)], 1)
+           for s in range(ns)]
+    grid = dict(nx=nx, nz=nz, dx=h, dz=h, nt=nt, dt=dt, sub=sub, bc_top=bc[0], bc_bottom=bc[1], bc_left=bc[2],
+                bc_right=bc[3], taper_top=0 if bc[0] != 2 else taper, taper_bottom=taper, taper_left=taper,
+                taper_right=taper, taper_strength=0.05, vti=0, gdamp_tmin=0.0, gdamp_tmax=0.0, gdamp_power=1.0)
+    src = np.stack([W.ricker(nt, dt, 15.0 + 3 * s) for s in range(ns)])
+    return grid, model, sxz, rxz, src
+
+
+def oracle_run(orc, grid, model, sxz, rxz, src, adj=None, stiffness=1.0):
+    g = grid_struct(grid)
+    nx, nz, h = grid["nx"], grid["nz"], grid["dx"]
+    alpha = 0.2508560249 / max(stiffness, 1.0)
+    nt, sub = grid["nt"], grid["sub"]
+    outs, grads = [], np.zeros((2, nx, nz))
+    r0 = 0
+    for s in range(len(sxz)):
+        inj = orc.delta_m3(nx, nz, h, h, 0, 0, [sxz[s, 0]], [sxz[s, 1]])
+        ext = orc.delta_m3(nx, nz, h, h, 0, 0, rxz[s][:, 0], rxz[s][:, 1])
+        full = np.zeros((1 + nt // sub, nx, nz)) if sub > 0 else None
+        outs.append(orc.propagate_acoustic(g, model, inj, src[s:s + 1], ext=ext, full=full, alpha=alpha))
+        if adj is not None:
+            n = len(rxz[s])
+            orc.propagate_acoustic(g, model, ext, adj[r0:r0 + n], full=full, grad=grads, alpha=alpha)
+            r0 += n
+    return np.concatenate(outs, 0), grads
+
+
+def engine(lib, grid, model, sxz, rxz, stiffness=1.0):
+    from fwi2d_amd.lib import Fwi2dGrid
+    g = Fwi2dGrid()
+    for k, v in grid.items():
+        setattr(g, k, v)
+    e = lib.AcousticEngine(g, stiffness)
+    e.set_model(model)
+    e.set_shots(sxz, rxz)
+    return e
+
+
+@pytest.mark.parametrize("bc,taper,nx,nz,sub", [((1, 2, 2, 2), 10, 90, 70, 1), ((2, 2, 2, 2), 0, 61, 47, 1),
+                                                  ((3, 3, 3, 3), 0, 37, 29, 1), ((1, 3, 2, 1), 8, 130, 83, 2)])
+def test_acoustic_traces_and_gradient_match_oracle(lib, orc64, orc32, bc, taper, nx, nz, sub):
+    grid, model, sxz, rxz, src = case(nx=nx, nz=nz, bc=bc, taper=taper, sub=sub)
+    e = engine(lib, grid, model, sxz, rxz)
+    d = e.forward(src, keep_snapshots=True)
+    nr = sum(len(r) for r in rxz)
+    adj = np.random.default_rng(nx).standard_normal((nr, grid["nt"])) * np.hanning(grid["nt"])
+    ref, gref = oracle_run(orc64, grid, model, sxz, rxz, src, adj=adj)
+    gap = rel(oracle_run(orc32, grid, model, sxz, rxz, src)[0], ref)
+    err = rel(d, ref)
+    tol = max(1e-5, 2 * gap)
+    print("bc=%s taper=%d %dx%d sub=%d: traces rel err %.3e (tol %.1e)" % (bc, taper, nx, nz, sub, err, tol))
+    assert err <= tol
+    gk = e.adjoint(adj)
+    for c, name in enumerate(("K", "1/rho")):
+        errg = rel(gk[c], gref[c])
+        print("  %s gradient rel err %.3e" % (name, errg))
+        assert errg <= 1e-4
+    e.close()
+
+
+def test_acoustic_batch_equals_single_shots(lib):
+    grid, model, sxz, rxz, src = case(ns=3)
+    e = engine(lib, grid, model, sxz, rxz)
+    d = e.forward(src)
+    r0 = 0
+    for s in range(3):
+        one = engine(lib, grid, model, sxz[s:s + 1], rxz[s:s + 1])
+        n = len(rxz[s])
+        assert np.array_equal(one.forward(src[s:s + 1]), d[r0:r0 + n])
+        r0 += n
+        one.close()
+    e.close()
+
+
+def test_acoustic_stiffness_and_errors(lib, orc64):
+    grid, model, sxz, rxz, src = case(nx=50, nz=40, nt=200)
+    e = engine(lib, grid, model, sxz, rxz, stiffness=2.5)
+    ref, _ = oracle_run(orc64, grid, model, sxz, rxz, src, stiffness=2.5)
+    assert rel(e.forward(src), ref) < 1e-5
+    with pytest.raises(lib.Fwi2dError, match="snapshots"):
+        e.adjoint(np.zeros((sum(len(r) for r in rxz), grid["nt"])))
+    e.close()
+    bad = dict(grid, bc_top=4)
+    with pytest.raises(lib.Fwi2dError, match="boundaries"):
+        engine(lib, bad, model, sxz, rxz)
