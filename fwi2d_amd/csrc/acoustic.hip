@@ -17,6 +17,7 @@
 #include "engine_common.h"
 #include "fwi2d_hip.h"
 #include "kernels.h"
+#include "trace_matrix.h"
 
 namespace fwi2d {
 
@@ -278,6 +279,43 @@ struct Acoustic {
         launch_agrad(a, st);
     }
 
+    // the operator level's per-trace time maps on the device (see trace_matrix.h)
+    DevMem d_tm_ptr, d_tm_col, d_tm_w, d_tdata;
+    void upload_matrix(const TraceMatrix& m, int ncol_expected, int nrow_expected)
+    {
+        check_trace_matrix(m, ncol_expected, nrow_expected);
+        up(d_tm_ptr, m.ptr.data(), m.ptr.size() * sizeof(int));
+        up(d_tm_col, m.col.data(), m.col.size() * sizeof(int));
+        up(d_tm_w, m.w.data(), m.w.size() * sizeof(double));
+    }
+
+    void forward_traces(const float* src, bool keep, const TraceMatrix& m, float* out)
+    {
+        require_ready();
+        upload_matrix(m, g.nt, -1);
+        d_rcv.reserve(std::max<size_t>((size_t)nr * g.nt * sizeof(float), 16));
+        up(d_src, src, (size_t)ns * g.nt * sizeof(float));
+        forward(d_src.as<float>(), d_rcv.as<float>(), keep, true);
+        d_tdata.reserve(std::max<size_t>((size_t)nr * m.nrow * sizeof(float), 16));
+        launch_trace_matvec(d_tm_ptr.as<int>(), d_tm_col.as<int>(), d_tm_w.as<double>(), m.nrow, m.ncol,
+                            d_rcv.as<float>(), d_tdata.as<float>(), nr, st);
+        if ((long long)nr * m.nrow > 0)
+            HIPCHECK(hipMemcpyAsync(out, d_tdata.p, (size_t)nr * m.nrow * sizeof(float), hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+        HIPCHECK(hipGetLastError());
+    }
+
+    void adjoint_traces(const float* data, const TraceMatrix& m, float* grad)
+    {
+        require_ready();
+        upload_matrix(m, -1, g.nt);
+        up(d_tdata, data, std::max<size_t>((size_t)nr * m.ncol * sizeof(float), 4));
+        d_adj.reserve(std::max<size_t>((size_t)nr * g.nt * sizeof(float), 16));
+        launch_trace_matvec(d_tm_ptr.as<int>(), d_tm_col.as<int>(), d_tm_w.as<double>(), m.nrow, m.ncol,
+                            d_tdata.as<float>(), d_adj.as<float>(), nr, st);
+        adjoint(d_adj.as<float>(), grad, true, false);
+    }
+
     void forward(const float* src, float* rcv, bool keep, bool on_device)
     {
         require_ready();
@@ -305,7 +343,7 @@ struct Acoustic {
         snap_epoch = keep ? model_epoch : -1;
     }
 
-    void adjoint(const float* adj, float* grad, bool on_device)
+    void adjoint(const float* adj, float* grad, bool adj_dev, bool grad_dev)
     {
         require_ready();
         if (g.sub <= 0) throw std::runtime_error("the adjoint needs sub > 0");
@@ -314,7 +352,7 @@ struct Acoustic {
                                      "with keep_snapshots first");
         const int nt = g.nt;
         const float* dadj = adj;  // not read without receivers
-        if (!on_device && nr > 0) {
+        if (!adj_dev && nr > 0) {
             up(d_adj, adj, (size_t)nr * nt * sizeof(float));
             dadj = d_adj.as<float>();
         }
@@ -326,7 +364,7 @@ struct Acoustic {
         // [2][nx][ldz] -> [2][nx][nz]
         HIPCHECK(hipMemcpy2DAsync(grad, (size_t)g.nz * sizeof(float), d_gout.p, (size_t)ldz * sizeof(float),
                                   (size_t)g.nz * sizeof(float), (size_t)2 * g.nx,
-                                  on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, st));
+                                  grad_dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, st));
         HIPCHECK(hipStreamSynchronize(st));
         HIPCHECK(hipGetLastError());
     }
@@ -351,6 +389,22 @@ int guarded_ac(F&& f)
 struct fwi2d_acoustic {
     fwi2d::Acoustic* impl;
 };
+
+namespace fwi2d {
+
+void acoustic_forward_traces(fwi2d_acoustic* e, const float* src, bool keep, const TraceMatrix& m, float* out)
+{
+    HIPCHECK(hipSetDevice(e->impl->device));
+    e->impl->forward_traces(src, keep, m, out);
+}
+
+void acoustic_adjoint_traces(fwi2d_acoustic* e, const float* data, const TraceMatrix& m, float* grad)
+{
+    HIPCHECK(hipSetDevice(e->impl->device));
+    e->impl->adjoint_traces(data, m, grad);
+}
+
+}  // namespace fwi2d
 
 extern "C" {
 
@@ -400,7 +454,7 @@ int fwi2d_acoustic_adjoint(fwi2d_acoustic* e, const float* adj, float* grad, int
     return fwi2d::guarded_ac([&] {
         if (!e || !grad || (!adj && e->impl->nr > 0)) throw std::runtime_error("null argument");
         HIPCHECK(hipSetDevice(e->impl->device));
-        e->impl->adjoint(adj, grad, on_device != 0);
+        e->impl->adjoint(adj, grad, on_device != 0, on_device != 0);
     });
 }
 

@@ -785,14 +785,7 @@ struct Engine {
     // the trace matrix m on the device, its shape and indices checked against the traces
     void upload_matrix(const TraceMatrix& m, int ncol_expected, int nrow_expected)
     {
-        if ((ncol_expected >= 0 && m.ncol != ncol_expected) || (nrow_expected >= 0 && m.nrow != nrow_expected) ||
-            m.nrow < 0 || m.ncol < 1 || (int)m.ptr.size() != m.nrow + 1 || m.ptr[0] != 0 ||
-            (size_t)m.ptr[m.nrow] != m.col.size() || m.col.size() != m.w.size())
-            throw std::runtime_error("trace matrix does not match the traces");
-        for (int i = 0; i < m.nrow; i++)
-            if (m.ptr[i + 1] < m.ptr[i]) throw std::runtime_error("trace matrix rows out of order");
-        for (int c : m.col)
-            if (c < 0 || c >= m.ncol) throw std::runtime_error("trace matrix column out of range");
+        check_trace_matrix(m, ncol_expected, nrow_expected);
         auto up = [&](DevMem& d, const void* h, size_t b) {
             d.reserve(std::max<size_t>(b, 16));
             if (b) HIPCHECK(hipMemcpyAsync(d.p, h, b, hipMemcpyHostToDevice, st));

@@ -50,6 +50,14 @@ TimeAxis analyze_model(const fwi2d_op_desc& d, std::vector<double>& m)
     double* vp = m.data();
     double* vs = vp + n;
     double* rho = vp + 2 * n;
+    auto clip = [&](double* a, double lo, double hi) {
+        for (long long i = 0; i < n; i++) a[i] = std::max(lo, std::min(hi, a[i]));
+    };
+    if (d.nparam == 2) {  // acoustic (vp, rho): always clipped (we_op.cpp:399-421)
+        rho = vp + n;
+        clip(vp, d.vpmin, d.vpmax);
+        clip(rho, d.rhomin, d.rhomax);
+    }
     if (d.nparam == 5) {
         double* del = vp + 3 * n;
         for (long long i = 0; i < n; i++) {
@@ -59,10 +67,7 @@ TimeAxis analyze_model(const fwi2d_op_desc& d, std::vector<double>& m)
             else if (del[i] > v2) del[i] = 0.9999 * v2;
         }
     }
-    auto clip = [&](double* a, double lo, double hi) {
-        for (long long i = 0; i < n; i++) a[i] = std::max(lo, std::min(hi, a[i]));
-    };
-    if (!d.soft_clip) {
+    if (d.nparam != 2 && !d.soft_clip) {
         clip(vp, d.vpmin, d.vpmax);
         clip(vs, d.vsmin, d.vsmax);
         clip(rho, d.rhomin, d.rhomax);
@@ -93,9 +98,17 @@ TimeAxis analyze_model(const fwi2d_op_desc& d, std::vector<double>& m)
     return t;
 }
 
-// convert_model (we_op.cpp:476-494, 1441-1462): (vp, vs, rho[, delta, eps]) -> (lambda, mu, rho[, c13, eps])
+// convert_model (we_op.cpp:476-494, 1441-1462): (vp, vs, rho[, delta, eps]) -> (lambda, mu, rho[, c13, eps]);
+// the acoustic nl_we_op_a::convert_model (we_op.cpp:1863-1881): (vp, rho) -> (K = rho vp^2, 1/rho)
 void convert_model(std::vector<double>& m, int np, long long n)
 {
+    if (np == 2) {
+        for (long long i = 0; i < n; i++) {
+            m[i] = m[n + i] * m[i] * m[i];
+            m[n + i] = 1.0 / m[n + i];
+        }
+        return;
+    }
     for (long long i = 0; i < n; i++) {
         const double vp = m[i], vs = m[n + i], rho = m[2 * n + i];
         const double la = rho * (vp * vp - 2 * vs * vs), mu = rho * vs * vs;
@@ -492,8 +505,9 @@ struct fwi2d_op {
     std::vector<double> sxz, rxz, rdip;
     std::vector<int> nrec;
     int nr = 0;
-    std::vector<double> allsrc;  // [nc][ns][nt_data], nc = 2 (forces) or 3 (moment tensors)
+    std::vector<double> allsrc;  // [nc][ns][nt_data], nc = 2 (forces), 3 (moment tensors), 1 (acoustic)
     fwi2d_engine* eng = nullptr;
+    fwi2d_acoustic* aeng = nullptr;  // the acoustic operator's engine (nparam = 2)
     fwi2d_grid grid{};
     TimeAxis axis{0, 0, 0};
     std::vector<float> ckpt_model;  // converted model of the last checkpointed forward
@@ -503,7 +517,9 @@ struct fwi2d_op {
     bool shots_set = false;
 
     int np() const { return d.nparam; }
-    int nc() const { return d.mt ? 3 : 2; }
+    bool acoustic() const { return d.nparam == 2; }
+    int nc() const { return acoustic() ? 1 : (d.mt ? 3 : 2); }  // source components
+    int ncd() const { return acoustic() ? 1 : 2; }               // data components
     long long nxz() const { return (long long)d.nx * d.nz; }
 
     // model (physical) -> analysed + converted float model; sets the time axis
@@ -521,9 +537,11 @@ struct fwi2d_op {
 
     void ensure_engine(const TimeAxis& t)
     {
-        if (eng && t.nt == axis.nt && t.dt == axis.dt && t.sub == axis.sub) return;
+        if ((eng || aeng) && t.nt == axis.nt && t.dt == axis.dt && t.sub == axis.sub) return;
         if (eng) fwi2d_engine_destroy(eng);
+        if (aeng) fwi2d_acoustic_destroy(aeng);
         eng = nullptr;
+        aeng = nullptr;
         axis = t;
         fwi2d_grid g{};
         g.nx = d.nx;
@@ -547,9 +565,14 @@ struct fwi2d_op {
         g.gdamp_tmax = d.gdamp_tmax;
         g.gdamp_power = d.gdamp_power;
         grid = g;
-        check(fwi2d_engine_create(&g, d.device, &eng));
-        check(fwi2d_engine_set_acquisition(eng, d.ns, sxz.data(), nrec.data(), rxz.data(), d.ox, d.oz, d.mt,
-                                           d.gl > 0 ? d.gl : 0.0, d.gl > 0 ? rdip.data() : nullptr));
+        if (acoustic()) {
+            check(fwi2d_acoustic_create(&g, d.free_surface_stiffness, d.device, &aeng));
+            check(fwi2d_acoustic_set_shots(aeng, d.ns, sxz.data(), nrec.data(), rxz.data(), d.ox, d.oz));
+        } else {
+            check(fwi2d_engine_create(&g, d.device, &eng));
+            check(fwi2d_engine_set_acquisition(eng, d.ns, sxz.data(), nrec.data(), rxz.data(), d.ox, d.oz, d.mt,
+                                               d.gl > 0 ? d.gl : 0.0, d.gl > 0 ? rdip.data() : nullptr));
+        }
         ckpt_model.clear();
         // sources at the propagation step (apply_forward, we_op.cpp:953-958)
         std::vector<double> sp((size_t)nc() * d.ns * t.nt);
@@ -584,12 +607,19 @@ struct fwi2d_op {
         tm_adj = to_matrix(A, ntd);
     }
 
+    void set_model(const std::vector<float>& m)
+    {
+        if (acoustic()) check(fwi2d_acoustic_set_model(aeng, m.data()));
+        else check(fwi2d_engine_set_model(eng, m.data()));
+    }
+
     void run_forward(const std::vector<float>& m, std::vector<float>& rcv)
     {
-        check(fwi2d_engine_set_model(eng, m.data()));
-        rcv.assign((size_t)2 * nr * axis.nt, 0.f);
+        set_model(m);
+        rcv.assign((size_t)ncd() * nr * axis.nt, 0.f);
         const int keep = axis.sub > 0;
-        check(fwi2d_engine_forward(eng, src_prop.data(), rcv.data(), keep, 0));
+        if (acoustic()) check(fwi2d_acoustic_forward(aeng, src_prop.data(), rcv.data(), keep, 0));
+        else check(fwi2d_engine_forward(eng, src_prop.data(), rcv.data(), keep, 0));
         if (keep) ckpt_model = m;
         else ckpt_model.clear();
     }
@@ -602,12 +632,13 @@ struct fwi2d_op {
             run_forward(m, rcv);
             return;
         }
-        check(fwi2d_engine_set_model(eng, m.data()));
+        set_model(m);
         ckpt_model.clear();
-        const size_t n = (size_t)2 * nr * d.nt_data;
+        const size_t n = (size_t)ncd() * nr * d.nt_data;
         std::vector<float> out(n);
         const bool keep = axis.sub > 0;
-        fwi2d::engine_forward_traces(eng, src_prop.data(), keep, tm_fwd, out.data());
+        if (acoustic()) fwi2d::acoustic_forward_traces(aeng, src_prop.data(), keep, tm_fwd, out.data());
+        else fwi2d::engine_forward_traces(eng, src_prop.data(), keep, tm_fwd, out.data());
         if (keep) ckpt_model = m;
         for (size_t i = 0; i < n; i++) pdat[i] = add ? pdat[i] + out[i] : out[i];
     }
@@ -617,7 +648,8 @@ struct fwi2d_op {
         std::vector<float> m = prepare(pmod0);
         if (axis.sub <= 0) throw std::runtime_error("apply_jacobianT needs sub != 0");
         const long long n = nxz();
-        std::vector<float> graw((size_t)np() * n, 0.f);
+        const int ng = acoustic() ? 2 : 3;  // raw gradient fields
+        std::vector<float> graw((size_t)(acoustic() ? 2 : np()) * n, 0.f);
         if (nr > 0) {  // no traces: no adjoint sources, a zero gradient
             if (ckpt_model != m) {  // no checkpoints of this model: run the forward first
                 std::vector<float> rcv;
@@ -625,28 +657,40 @@ struct fwi2d_op {
             }
             // (DAS channels: the device maps strain to gauge-end differences first,
             // dipole_to_strain's adjoint; it commutes with the per-trace time map)
-            fwi2d::engine_adjoint_traces(eng, pdat, tm_adj, graw.data());
+            if (acoustic()) fwi2d::acoustic_adjoint_traces(aeng, pdat, tm_adj, graw.data());
+            else fwi2d::engine_adjoint_traces(eng, pdat, tm_adj, graw.data());
         }
-        // H quadrature (we_op.cpp:1192-1199) and chain rule (we_op.cpp:1208-1219)
+        // H quadrature (we_op.cpp:1192-1199) and chain rule (we_op.cpp:1208-1230)
         std::vector<double> gd(graw.begin(), graw.end());
-        for (int c = 0; c < 3; c++)
+        for (int c = 0; c < ng; c++)
             for (int ix = 0; ix < d.nx; ix++)
                 for (int iz = 0; iz < d.nz; iz++)
                     gd[c * n + (long long)ix * d.nz + iz] *= h_weight(d.nz, iz) * d.dz * h_weight(d.nx, ix) * d.dx;
-        for (long long i = 0; i < n; i++) {
-            const double la = m[i], mu = m[n + i], rho = m[2 * n + i];
-            const double gl = gd[i], gm = gd[n + i], gr = gd[2 * n + i];
-            gd[2 * n + i] = (la / rho) * gl + (mu / rho) * gm + gr;
-            gd[n + i] = 2 * std::sqrt(mu * rho) * (gm - 2 * gl);
-            gd[i] = 2 * std::sqrt(rho * (la + 2 * mu)) * gl;
+        if (acoustic()) {
+            // (K, 1/rho) -> (vp, rho): g_rho = (1/rho) K g_K - (1/rho)^2 g_1/rho, g_vp = 2 sqrt(K rho) g_K
+            for (long long i = 0; i < n; i++) {
+                const double K = m[i], ri = m[n + i];
+                const double gk = gd[i], gri = gd[n + i];
+                gd[n + i] = ri * K * gk - ri * ri * gri;
+                gd[i] = 2 * std::sqrt(K / ri) * gk;
+            }
+        } else {
+            for (long long i = 0; i < n; i++) {
+                const double la = m[i], mu = m[n + i], rho = m[2 * n + i];
+                const double gl = gd[i], gm = gd[n + i], gr = gd[2 * n + i];
+                gd[2 * n + i] = (la / rho) * gl + (mu / rho) * gm + gr;
+                gd[n + i] = 2 * std::sqrt(mu * rho) * (gm - 2 * gl);
+                gd[i] = 2 * std::sqrt(rho * (la + 2 * mu)) * gl;
+            }
+            for (long long i = 3 * n; i < np() * n; i++) gd[i] = 0.0;
         }
-        for (long long i = 3 * n; i < np() * n; i++) gd[i] = 0.0;
         for (long long i = 0; i < np() * n; i++) pmod[i] = (float)((add ? (double)pmod[i] : 0.0) + gd[i]);
     }
 
     ~fwi2d_op()
     {
         if (eng) fwi2d_engine_destroy(eng);
+        if (aeng) fwi2d_acoustic_destroy(aeng);
     }
 };
 
@@ -832,18 +876,23 @@ int fwi2d_op_create(const fwi2d_op_desc* desc, const float* allsrc, fwi2d_op** o
     return op_guard([&] {
         if (!desc || !allsrc || !out) throw std::runtime_error("null argument");
         const fwi2d_op_desc& d = *desc;
-        if (d.nparam != 3 && d.nparam != 5)
-            throw std::runtime_error("the model must contain 3 (vp, vs, rho) or 5 (vp, vs, rho, delta, eps) parameters");
+        if (d.nparam != 2 && d.nparam != 3 && d.nparam != 5)
+            throw std::runtime_error("The model must contain two (Vp, rho), three (Vp, Vs, rho) or five (Vp, Vs, rho, "
+                                     "delta, eps) parameters");
         if (d.ns < 1) throw std::runtime_error("need at least one shot");
         if (d.nt_data < 2 || !(d.dt_data > 0)) throw std::runtime_error("invalid data time axis");
         fwi2d_op* op = new fwi2d_op;
         op->d = d;
+        if (d.nparam == 2) {  // the acoustic operator: point sources and receivers (we_op.cpp:13-20)
+            op->d.mt = 0;
+            op->d.gl = 0;
+        }
         op->sxz.assign(d.sxz, d.sxz + 2 * d.ns);
         op->nrec.assign(d.nrec, d.nrec + d.ns);
         op->nr = 0;
         for (int r : op->nrec) op->nr += r;
         op->rxz.assign(d.rxz, d.rxz + 2 * (size_t)op->nr);
-        if (d.gl > 0) {
+        if (op->d.gl > 0) {
             if (!d.rdip) {
                 delete op;
                 throw std::runtime_error("DAS receivers (gl > 0) need their dips (rdip)");
@@ -868,11 +917,11 @@ int fwi2d_op_create(const fwi2d_op_desc* desc, const float* allsrc, fwi2d_op** o
             }
         for (int r = 0; r < op->nr; r++) {
             const double x = op->rxz[2 * r], z = op->rxz[2 * r + 1];
-            if (d.gl <= 0 && !inside(x, z)) {
+            if (op->d.gl <= 0 && !inside(x, z)) {
                 delete op;
                 throw std::runtime_error("Receiver " + std::to_string(r) + " falls outside subsurface model boundaries");
             }
-            if (d.gl > 0) {  // both gauge ends inside the model (we_op.cpp:134-148)
+            if (op->d.gl > 0) {  // both gauge ends inside the model (we_op.cpp:134-148)
                 const double hx = d.gl / 2 * std::cos(op->rdip[r]), hz = d.gl / 2 * std::sin(op->rdip[r]);
                 const bool out = x - hx < d.ox || x - hx > xmax || x + hx < d.ox || x + hx > xmax || z - hz < d.oz ||
                                  z - hz > zmax || z + hz < d.oz || z + hz > zmax;
@@ -883,7 +932,7 @@ int fwi2d_op_create(const fwi2d_op_desc* desc, const float* allsrc, fwi2d_op** o
                 }
             }
         }
-        op->allsrc.assign(allsrc, allsrc + (size_t)(d.mt ? 3 : 2) * d.ns * d.nt_data);
+        op->allsrc.assign(allsrc, allsrc + (size_t)op->nc() * d.ns * d.nt_data);
         *out = op;
     });
 }
@@ -934,7 +983,7 @@ int fwi2d_nlls_create_filtered(fwi2d_op* op, const float* d, const float* w, con
         if (filter && nf < 1) throw std::runtime_error("the filter needs at least one sample");
         if (filter_phase < FWI2D_FILTER_DEFAULT || filter_phase > FWI2D_FILTER_MINIMUM)
             throw std::runtime_error("unknown filter phase");
-        const size_t ndata = (size_t)2 * op->nr * op->d.nt_data, nmodel = (size_t)op->np() * op->nxz();
+        const size_t ndata = (size_t)op->ncd() * op->nr * op->d.nt_data, nmodel = (size_t)op->np() * op->nxz();
         auto nonneg = [](const float* p, size_t n) {
             for (size_t i = 0; i < n; i++)
                 if (!(p[i] >= 0)) return false;

@@ -38,7 +38,7 @@ class Fwi2dOpDesc(C.Structure):
                 ("deltamax", C.c_double), ("epsilonmin", C.c_double), ("epsilonmax", C.c_double),
                 ("soft_clip", C.c_int), ("gdamp_tmin", C.c_double), ("gdamp_tmax", C.c_double),
                 ("gdamp_power", C.c_double), ("device", C.c_int), ("mt", C.c_int), ("gl", C.c_double),
-                ("rdip", C.POINTER(C.c_double))]
+                ("rdip", C.POINTER(C.c_double)), ("free_surface_stiffness", C.c_double)]
 
 
 EXPORTS = [
@@ -295,7 +295,8 @@ def op_desc_from(prob, device=0):
         if k in prob:
             setattr(d, k, prob[k])
     d.sinc_half_length = int(prob.get("sinc_hl", 11))
-    d.nparam = 5 if prob.get("vti") else 3
+    d.nparam = 2 if prob.get("acoustic") else (5 if prob.get("vti") else 3)
+    d.free_surface_stiffness = float(prob.get("free_surface_stiffness", 1.0))
     d.nt_data, d.dt_data = int(mp["nt_data"]), float(mp["dt_data"])
     d.courant, d.dt, d.sub = float(mp["courant"]), float(mp["dt"]), int(mp["sub"])
     d.vpmin, d.vpmax, d.vsmin, d.vsmax = mp["vpmin"], mp["vpmax"], mp["vsmin"], mp["vsmax"]
@@ -307,13 +308,16 @@ def op_desc_from(prob, device=0):
 
 
 class WeOp:
-    """Drop-in for the reference's nl_we_op_e / nl_we_op_vti: apply_forward and
-    apply_jacobianT on host numpy arrays, computed by the HIP engine."""
+    """Drop-in for the reference's nl_we_op_e / nl_we_op_vti / nl_we_op_a: apply_forward
+    and apply_jacobianT on host numpy arrays, computed by the HIP engine.  prob["acoustic"]
+    selects nl_we_op_a: model (vp, rho), sources [1][ns][nt_data], data [1][nr][nt_data]."""
 
     def __init__(self, prob, allsrc, device=0):
         lib = load()
         self.prob = prob
-        self.np = 5 if prob.get("vti") else 3
+        self.acoustic = bool(prob.get("acoustic", 0))
+        self.np = 2 if self.acoustic else (5 if prob.get("vti") else 3)
+        self.ncd = 1 if self.acoustic else 2
         d = op_desc_from(prob, device)
         sxz = np.ascontiguousarray(prob["sxz"], dtype=np.float64)
         nrec = np.ascontiguousarray([len(r) for r in prob["rxz"]], dtype=np.int32)
@@ -330,7 +334,8 @@ class WeOp:
             assert self._rdip.size == self.nr
             d.rdip = self._rdip.ctypes.data_as(C.POINTER(C.c_double))
         src = _f32(allsrc)
-        assert src.shape == (3 if d.mt else 2, d.ns, d.nt_data), src.shape
+        nc = 1 if self.acoustic else (3 if d.mt else 2)
+        assert src.shape == (nc, d.ns, d.nt_data), src.shape
         self._h = C.c_void_p()
         _check(lib.fwi2d_op_create(C.byref(d), _ptr(src), C.byref(self._h)))
         self.nt_data = d.nt_data
@@ -344,7 +349,7 @@ class WeOp:
 
     def apply_forward(self, model, add_to=None):
         m = _f32(model)
-        out = np.zeros((2, self.nr, self.nt_data), np.float32) if add_to is None else _f32(add_to).copy()
+        out = np.zeros((self.ncd, self.nr, self.nt_data), np.float32) if add_to is None else _f32(add_to).copy()
         _check(load().fwi2d_op_apply_forward(self._h, int(add_to is not None), _ptr(m), _ptr(out)))
         return out
 
@@ -377,7 +382,7 @@ class Nlls:
                  filter=None, filter_phase="default"):
         self.op = op  # the objective uses the operator handle
         self._d = _f32(d)
-        assert self._d.shape == (2, op.nr, op.nt_data), self._d.shape
+        assert self._d.shape == (op.ncd, op.nr, op.nt_data), self._d.shape
         w = None if w is None else _f32(w)
         gmask = None if gmask is None else _f32(gmask)
         filt = None if filter is None else _f32(filter).ravel()

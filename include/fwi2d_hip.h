@@ -207,16 +207,20 @@ int fwi2d_acoustic_adjoint(fwi2d_acoustic* e, const float* adj, float* grad, int
 void fwi2d_acoustic_destroy(fwi2d_acoustic* e);
 
 /* ------------------------------------------------------------------------- */
-/* Operator level (drop-in for nl_we_op_e / nl_we_op_vti)                      */
+/* Operator level (drop-in for nl_we_op_e / nl_we_op_vti / nl_we_op_a)         */
 /* ------------------------------------------------------------------------- */
 
 /* Operator description: the reference's param fields used on this path
    (src/include/param.hpp:10-82) plus the model domain and source time axis.
-   Point forces or moment tensors (mt), point receivers or DAS channels (gl). */
+   Point forces or moment tensors (mt), point receivers or DAS channels (gl).  With
+   nparam = 2 the operator is the acoustic nl_we_op_a (model (vp, rho); one source and
+   one data component, allsrc [ns][nt_data] and data [nrec][nt_data]; mt and gl ignored,
+   as the reference's analyzeGeometry does, we_op.cpp:13-20). */
 typedef struct {
     int nx, nz; double dx, dz, ox, oz;            /* model domain axes 2, 1 */
     int nt_data; double dt_data;                  /* source/data time axis (origin 0) */
-    int nparam;                                   /* 3 iso (vp,vs,rho) or 5 vti (+delta,eps) */
+    int nparam;                                   /* 2 acoustic (vp,rho), 3 iso (vp,vs,rho) or
+                                                     5 vti (+delta,eps) */
     int ns; const double* sxz;                    /* shots (x, z) */
     const int* nrec; const double* rxz;           /* receivers per shot, (x, z) concatenated */
     double courant, dt;                           /* par.courant, par.dt (<0 auto) */
@@ -238,6 +242,9 @@ typedef struct {
                                                      misc.cpp:18-43) */
     const double* rdip;                           /* par.rdip: receiver dips (rad), nrec_total values,
                                                      read when gl > 0 */
+    double free_surface_stiffness;                /* par.free_surface_stiffness of the acoustic
+                                                     free surface (raised to 1 as the reference
+                                                     does; 0 reads as 1) */
 } fwi2d_op_desc;
 
 typedef struct fwi2d_op fwi2d_op;

@@ -1245,6 +1245,11 @@ void orc_analyze_model(orc_model_par* p, real* model, int nparam, int nx, int nz
     real* vp = model;
     real* vs = model + nxz;
     real* rho = model + 2 * nxz;
+#define CLIP(a, lo, hi) do { for (long i = 0; i < nxz; i++) { real v_ = (a)[i]; if (v_ > (real)(hi)) v_ = (real)(hi); if (v_ < (real)(lo)) v_ = (real)(lo); (a)[i] = v_; } } while (0)
+    if (nparam == 2) {  /* acoustic (vp, rho): always hard-clipped, vmin/vmax from vp (we_op.cpp:401-428) */
+        CLIP(vp, p->vpmin, p->vpmax);
+        CLIP(model + nxz, p->rhomin, p->rhomax);
+    }
     if (nparam == 5) {
         real* del = model + 3 * nxz;
         for (long i = 0; i < nxz; i++) {
@@ -1254,8 +1259,7 @@ void orc_analyze_model(orc_model_par* p, real* model, int nparam, int nx, int nz
             else if (del[i] > v2) del[i] = (real)(0.9999 * v2);
         }
     }
-#define CLIP(a, lo, hi) do { for (long i = 0; i < nxz; i++) { real v_ = (a)[i]; if (v_ > (real)(hi)) v_ = (real)(hi); if (v_ < (real)(lo)) v_ = (real)(lo); (a)[i] = v_; } } while (0)
-    if (!p->soft_clip) {
+    if (nparam != 2 && !p->soft_clip) {
         CLIP(vp, p->vpmin, p->vpmax);
         CLIP(vs, p->vsmin, p->vsmax);
         CLIP(rho, p->rhomin, p->rhomax);
@@ -1268,7 +1272,11 @@ void orc_analyze_model(orc_model_par* p, real* model, int nparam, int nx, int nz
     }
 #undef CLIP
     double vpmx = -1e300, vsmn = 1e300;
-    for (long i = 0; i < nxz; i++) { if (vp[i] > vpmx) vpmx = vp[i]; if (vs[i] < vsmn) vsmn = vs[i]; }
+    for (long i = 0; i < nxz; i++) {
+        if (vp[i] > vpmx) vpmx = vp[i];
+        const real vmin_i = nparam == 2 ? vp[i] : vs[i];
+        if (vmin_i < vsmn) vsmn = vmin_i;
+    }
     p->vmax = vpmx;
     p->vmin = vsmn;
     if (nparam == 5) {
@@ -1298,6 +1306,13 @@ void orc_analyze_model(orc_model_par* p, real* model, int nparam, int nx, int nz
 /* forward: (vp, vs, rho[, delta, eps]) -> (lambda, mu, rho[, c13, eps]) */
 void orc_convert_model(real* m, int nparam, long n)
 {
+    if (nparam == 2) {  /* nl_we_op_a::convert_model (we_op.cpp:1863-1881): (vp, rho) -> (K, 1/rho) */
+        for (long i = 0; i < n; i++) {
+            m[i] = m[n + i] * m[i] * m[i];
+            m[n + i] = (real)1 / m[n + i];
+        }
+        return;
+    }
     for (long i = 0; i < n; i++) {
         real vp = m[i], vs = m[n + i], rho = m[2 * n + i];
         m[i] = rho * (vp * vp - 2 * vs * vs);
@@ -1403,11 +1418,13 @@ typedef struct {
     int mt;                /* 1: moment-tensor sources, allsrc [3][ns][nt] (mxx, mzz, mxz) */
     double gl;             /* > 0: DAS receivers of this gauge length (dipole_m3) */
     const double* rdip;    /* [nr_total] receiver dips (rad), used when gl > 0 */
+    int acoustic;          /* 1: nl_we_op_a, model (vp, rho), allsrc [1][ns][nt], data [1][nr][nt] */
+    double stiffness;      /* acoustic free-surface SAT stiffness (>= 1 keeps the exact limit) */
 } orc_op;
 
 static void op_prepare(orc_op* op, const real* pmod, real** model_out)
 {
-    const int np = op->g.vti ? 5 : 3;
+    const int np = op->acoustic ? 2 : (op->g.vti ? 5 : 3);
     const long nxz = (long)op->g.nx * op->g.nz;
     real* m = (real*)malloc((size_t)np * nxz * sizeof(real));
     memcpy(m, pmod, (size_t)np * nxz * sizeof(real));
@@ -1456,6 +1473,121 @@ static void receiver_pts(const orc_op* op, int r0, int n, int row, int nt, pts* 
     free(rx); free(rz); free(xi); free(zi); free(xw); free(zw);
 }
 
+/* The acoustic operator (nl_we_op_a through nl_we_op_e::apply_forward / apply_jacobianT,
+   we_op.cpp:904-1239 with nmodels = 2): one source and one data component, delta_m3
+   points, propagate_acoustic_pts per shot, snapshots [ns][1+nt/sub][nx][nz]. */
+static double acoustic_alpha(const orc_op* op)
+{
+    return 0.2508560249 / (op->stiffness > 1.0 ? op->stiffness : 1.0);
+}
+
+static void acoustic_pts(const orc_op* op, int n, const double* xz, int row, int nt, pts* p)
+{
+    const int nx = op->g.nx, nz = op->g.nz, m = n > 0 ? n : 1;
+    double* x = (double*)malloc(sizeof(double) * m);
+    double* z = (double*)malloc(sizeof(double) * m);
+    for (int r = 0; r < n; r++) { x[r] = xz[2 * r]; z[r] = xz[2 * r + 1]; }
+    int* xi = (int*)malloc(sizeof(int) * 3 * m);
+    int* zi = (int*)malloc(sizeof(int) * 3 * m);
+    real* xw = (real*)malloc(sizeof(real) * 6 * m);
+    real* zw = (real*)malloc(sizeof(real) * 6 * m);
+    orc_delta_m3(nx, nz, op->g.dx, op->g.dz, op->g.ox, op->g.oz, n, x, z, xi, zi, xw, zw);
+    pts_delta1(p, n, xi, zi, xw, zw, row, nt, nz);
+    free(x); free(z); free(xi); free(zi); free(xw); free(zw);
+}
+
+static int apply_forward_a(orc_op* op, int add, const real* m, const real* allsrc, real* data, real* full)
+{
+    const int nz = op->g.nz, nt = op->g.nt, ns = op->ns, ntd = op->mp.nt_data;
+    const double dtd = op->mp.dt_data, dt = op->g.dt;
+    const long nxz = (long)op->g.nx * nz;
+    const int nr = nr_total(op);
+    real* src = (real*)calloc((size_t)ns * nt, sizeof(real));
+    real* rcv = (real*)calloc((size_t)(nr > 0 ? nr : 1) * nt, sizeof(real));
+    resample(op, 0, 0, ns, ntd, dtd, nt, dt, (real*)allsrc, src);
+    const int nsnap = op->g.sub > 0 ? 1 + nt / op->g.sub : 0;
+    int r0 = 0, rc = 0;
+    for (int s = 0; s < ns && !rc; s++) {
+        pts inj, ext;
+        acoustic_pts(op, 1, op->sxz + 2 * s, 0, nt, &inj);
+        const int n = op->nrec[s];
+        if (n > 0) acoustic_pts(op, n, op->rxz + 2 * (long)r0, 1, nt, &ext);
+        real* fs = full && nsnap ? full + (long)s * nsnap * nxz : NULL;
+        rc = propagate_acoustic_pts(&op->g, m, acoustic_alpha(op), &inj, src + (long)s * nt, n > 0 ? &ext : NULL,
+                                    rcv + (long)r0 * nt, fs, NULL);
+        pts_free(&inj);
+        if (n > 0) pts_free(&ext);
+        r0 += n;
+    }
+    if (!rc) {
+        if (op->seismotype == 1) {
+            real* t = (real*)calloc((size_t)(nr > 0 ? nr : 1) * nt, sizeof(real));
+            orc_Dt(0, nr, nt, dt, rcv, t);
+            free(rcv);
+            rcv = t;
+        }
+        real alpha = (real)(dt / dtd);
+        for (long i = 0; i < (long)nr * nt; i++) rcv[i] *= alpha;
+        resample(op, 1, add, nr, ntd, dtd, nt, dt, data, rcv);
+    }
+    free(src); free(rcv);
+    return rc;
+}
+
+static int apply_jacobianT_a(orc_op* op, int add, real* grad, const real* m, const real* data, const real* full)
+{
+    const int nx = op->g.nx, nz = op->g.nz, nt = op->g.nt, ns = op->ns, ntd = op->mp.nt_data;
+    const double dtd = op->mp.dt_data, dt = op->g.dt;
+    const long nxz = (long)nx * nz;
+    const int nr = nr_total(op);
+    if (op->g.sub <= 0) return 2;
+    const int nsnap = 1 + nt / op->g.sub;
+    real* g = (real*)calloc((size_t)2 * nxz, sizeof(real));
+    real* rcv = (real*)calloc((size_t)(nr > 0 ? nr : 1) * nt, sizeof(real));
+    resample(op, 0, 0, nr, ntd, dtd, nt, dt, (real*)data, rcv);
+    orc_Ht(1, nr, nt, dt, rcv);
+    real alpha = (real)(dt / dtd);
+    for (int x = 0; x < nr; x++) {
+        real* a = rcv + (long)x * nt;
+        for (int t = 0; t < nt / 2; t++) { real v = a[t]; a[t] = a[nt - 1 - t]; a[nt - 1 - t] = v; }
+        for (int t = 0; t < nt; t++) a[t] *= -alpha;
+    }
+    if (op->seismotype == 1) {
+        real* t = (real*)calloc((size_t)(nr > 0 ? nr : 1) * nt, sizeof(real));
+        orc_Dt(1, nr, nt, dt, rcv, t);
+        free(rcv);
+        rcv = t;
+    }
+    int r0 = 0, rc = 0;
+    for (int s = 0; s < ns && !rc; s++) {
+        const int n = op->nrec[s];
+        if (n > 0) {
+            pts inj;
+            acoustic_pts(op, n, op->rxz + 2 * (long)r0, 0, nt, &inj);
+            rc = propagate_acoustic_pts(&op->g, m, acoustic_alpha(op), &inj, rcv + (long)r0 * nt, NULL, NULL,
+                                        (real*)full + (long)s * nsnap * nxz, g);
+            pts_free(&inj);
+        }
+        r0 += n;
+    }
+    if (!rc) {
+        for (int c = 0; c < 2; c++) {
+            orc_applyH(0, 0, g + c * nxz, nx, nz, op->g.dz);
+            orc_applyH(1, 0, g + c * nxz, nx, nz, op->g.dx);
+        }
+        /* chain rule (we_op.cpp:1220-1229): g_vp = 2 sqrt(K/(1/rho)) g_K,
+           g_rho = (1/rho) K g_K - (1/rho)^2 g_(1/rho) */
+        for (long i = 0; i < nxz; i++) {
+            const real K = m[i], ri = m[nxz + i], gk = g[i], gr = g[nxz + i];
+            g[nxz + i] = ri * K * gk - ri * ri * gr;
+            g[i] = (real)(2 * sqrt(K / ri)) * gk;
+        }
+        for (long i = 0; i < 2 * nxz; i++) grad[i] = (add ? grad[i] : 0) + g[i];
+    }
+    free(g); free(rcv);
+    return rc;
+}
+
 /* allsrc: [2][ns][nt_data] point forces, or [3][ns][nt_data] moment tensors (mxx, mzz,
    mxz) when op->mt; data: [2][nr_total][nt_data]; full (optional, sub>0):
    [ns][1+nt/sub][2][nx][nz].  we_op.cpp:904-1055. */
@@ -1463,6 +1595,11 @@ int orc_apply_forward(orc_op* op, int add, const real* pmod, const real* allsrc,
 {
     real* m;
     op_prepare(op, pmod, &m);
+    if (op->acoustic) {
+        const int rc = apply_forward_a(op, add, m, allsrc, data, full);
+        free(m);
+        return rc;
+    }
     const int nx = op->g.nx, nz = op->g.nz, nt = op->g.nt, ns = op->ns, ntd = op->mp.nt_data;
     const double dtd = op->mp.dt_data, dt = op->g.dt;
     const long nxz = (long)nx * nz;
@@ -1522,6 +1659,11 @@ int orc_apply_jacobianT(orc_op* op, int add, real* grad, const real* pmod0, cons
 {
     real* m;
     op_prepare(op, pmod0, &m);
+    if (op->acoustic) {
+        const int rc = apply_jacobianT_a(op, add, grad, m, data, full);
+        free(m);
+        return rc;
+    }
     const int nx = op->g.nx, nz = op->g.nz, nt = op->g.nt, ns = op->ns, ntd = op->mp.nt_data;
     const double dtd = op->mp.dt_data, dt = op->g.dt;
     const long nxz = (long)nx * nz;

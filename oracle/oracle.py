@@ -39,7 +39,7 @@ class OrcOp(C.Structure):
     _fields_ = [("g", OrcGrid), ("mp", OrcModelPar), ("ns", C.c_int), ("sxz", C.POINTER(C.c_double)),
                 ("nrec", C.POINTER(C.c_int)), ("rxz", C.POINTER(C.c_double)), ("seismotype", C.c_int),
                 ("resampling", C.c_int), ("sinc_hl", C.c_int), ("mt", C.c_int), ("gl", C.c_double),
-                ("rdip", C.POINTER(C.c_double))]
+                ("rdip", C.POINTER(C.c_double)), ("acoustic", C.c_int), ("stiffness", C.c_double)]
 
 
 def grid_struct(p):
@@ -283,8 +283,10 @@ class Oracle:
         op.seismotype = prob.get("seismotype", 0)
         op.resampling = prob.get("resampling", 0)
         op.sinc_hl = prob.get("sinc_hl", 11)
-        op.mt = int(prob.get("mt", 0))
-        op.gl = float(prob.get("gl", 0.0))
+        op.acoustic = int(prob.get("acoustic", 0))
+        op.stiffness = float(prob.get("free_surface_stiffness", 1.0))
+        op.mt = 0 if op.acoustic else int(prob.get("mt", 0))
+        op.gl = 0.0 if op.acoustic else float(prob.get("gl", 0.0))
         rdip = np.ascontiguousarray(np.concatenate(prob["rdip"]) if "rdip" in prob else np.zeros(len(rxz)),
                                     dtype=np.float64)
         op.rdip = _ptr(rdip, C.c_double)
@@ -328,7 +330,8 @@ class Oracle:
         model = self.arr(model)
         allsrc = self.arr(allsrc)
         nr = int(op._keep[1].sum())
-        data = np.zeros((2, nr, op.mp.nt_data), self.dtype)
+        ncd, nfld = (1, 1) if op.acoustic else (2, 2)
+        data = np.zeros((ncd, nr, op.mp.nt_data), self.dtype)
         full = None
         if keep_full:
             # size from a dry analysis of the model
@@ -336,7 +339,7 @@ class Oracle:
             self.analyze_model(tmp, model, op.g.dx, op.g.dz)
             sub = tmp.sub_out
             nsnap = 1 + tmp.nt_prop // sub
-            full = np.zeros((op.ns, nsnap, 2, op.g.nx, op.g.nz), self.dtype)
+            full = np.zeros((op.ns, nsnap, nfld, op.g.nx, op.g.nz), self.dtype)
         rc = self.lib.orc_apply_forward(C.byref(op), 0, self.p(model), self.p(allsrc), self.p(data), self.p(full))
         assert rc == 0, rc
         return data, full
@@ -344,7 +347,7 @@ class Oracle:
     def apply_jacobianT(self, op, model0, data, full):
         model0 = self.arr(model0)
         data = self.arr(data)
-        np_ = 5 if op.g.vti else 3
+        np_ = 2 if op.acoustic else (5 if op.g.vti else 3)
         grad = np.zeros((np_, op.g.nx, op.g.nz), self.dtype)
         rc = self.lib.orc_apply_jacobianT(C.byref(op), 0, self.p(grad), self.p(model0), self.p(data), self.p(full))
         assert rc == 0, rc

@@ -118,3 +118,69 @@ def test_acoustic_stiffness_and_errors(lib, orc64):
     bad = dict(grid, bc_top=4)
     with pytest.raises(lib.Fwi2dError, match="boundaries"):
         engine(lib, bad, model, sxz, rxz)
+
+
+def op_case(seismotype=0, bc=(1, 2, 2, 2), dt_data=0.002, nt_data=201, nx=120, nz=80, stiffness=1.0):
+    """nl_we_op_a at the operator level: model (vp, rho), sources [1][ns][nt_data],
+    data [1][nr][nt_data]."""
+    h = 0.01
+    x = np.arange(nx)[:, None] * h
+    z = np.arange(nz)[None, :] * h
+    vp = 1.8 + 1.2 * z + 0.25 * np.exp(-((x - 0.6) ** 2 + (z - 0.45) ** 2) / 0.01)
+    rho = 2.0 + 0.15 * np.sin(2 * np.pi * x / 0.55) * np.cos(2 * np.pi * z / 0.4)
+    true = np.stack([vp, rho])
+    start = np.stack([1.8 + 1.2 * z + 0 * x, np.full((nx, nz), 2.05)])
+    rx = np.stack([np.arange(4, nx - 4, 3) * h, np.full(len(range(4, nx - 4, 3)), 0.04)], 1)
+    rx2 = np.stack([np.full(15, 0.9), np.linspace(0.1, 0.7, 15)], 1)
+    prob = W.base_problem(nx, nz, h, h, nt_data, dt_data, [[0.3, 0.1], [0.75, 0.06], [1.0, 0.3]], [rx, rx2, rx[::2]],
+                          taper=12, bc=bc, seismotype=seismotype, dt=0.001, sub=-1)
+    prob["acoustic"] = 1
+    prob["free_surface_stiffness"] = stiffness
+    src = np.stack([W.ricker(nt_data, dt_data, f) for f in (12.0, 9.0, 14.0)])[None]
+    return prob, true.astype(np.float32), start.astype(np.float32), src.astype(np.float32)
+
+
+@pytest.mark.parametrize("seismotype,bc,dt_data,stiffness", [(0, (1, 2, 2, 2), 0.002, 1.0),
+                                                             (1, (2, 3, 2, 1), 0.001, 1.0),
+                                                             (0, (1, 2, 2, 2), 0.003, 2.0)])
+def test_acoustic_operator_matches_oracle(lib, orc64, seismotype, bc, dt_data, stiffness):
+    """WeOp with prob["acoustic"] (nl_we_op_a): apply_forward (resampling, time derivative,
+    dt/dt_data scaling) and apply_jacobianT (H quadrature, chain rule to vp and rho) against
+    the oracle's operator within 1e-4."""
+    prob, true, start, src = op_case(seismotype, bc, dt_data, stiffness=stiffness)
+    dobs, _ = orc64.apply_forward(orc64.make_op(prob), true, src)
+    d0, full = orc64.apply_forward(orc64.make_op(prob), start, src, keep_full=True)
+    op = lib.WeOp(prob, src)
+    assert op.np == 2 and op.ncd == 1
+    d0k = op.apply_forward(start)
+    assert d0k.shape == d0.shape == (1, op.nr, prob["mp"]["nt_data"])
+    err_d = rel(d0k, d0)
+    print("seismotype=%d bc=%s dt_data=%g: forward rel err %.3e" % (seismotype, bc, dt_data, err_d))
+    assert err_d <= 1e-4
+    rw = (d0 - dobs) * dt_data
+    rw[..., 0] *= 0.5
+    rw[..., -1] *= 0.5
+    gref = orc64.apply_jacobianT(orc64.make_op(prob), start, rw, full)
+    gk = op.apply_jacobianT(start, rw)
+    assert gk.shape == (2,) + start.shape[1:]
+    for c, name in enumerate(("vp", "rho")):
+        err = rel(gk[c], gref[c])
+        print("  %s gradient rel err %.3e" % (name, err))
+        assert err <= 1e-4
+    # the least-squares objective on the acoustic operator: 1/2 dt |r|^2 (trapezoid) of its
+    # own residual, and the gradient J^T of that residual with the trapezoid weights (the
+    # residual is a few percent of the data, so its FP32 error is compared on itself)
+    nl = lib.Nlls(op, dobs)
+    f, g = nl.evaluate(start)
+    rk = np.asarray(d0k, np.float64) - dobs
+    assert f == pytest.approx(orc64.nlls_functional(rk, dt_data), rel=1e-5)
+    rwk = rk * dt_data
+    rwk[..., 0] *= 0.5
+    rwk[..., -1] *= 0.5
+    grefk = orc64.apply_jacobianT(orc64.make_op(prob), start, rwk, full)
+    for c in range(2):
+        err = rel(g[c], grefk[c])
+        print("  objective gradient %d rel err %.3e" % (c, err))
+        assert err <= 1e-4
+    nl.close()
+    op.close()

@@ -220,3 +220,98 @@ def test_acoustic_gradient_finite_difference(orc64):
     # measured 8.6e-3 at 10 m and 1.8e-4 at 5 m on this smooth perturbation
     assert abs(e1["1/rho"]) < 0.03
     assert abs(e2["1/rho"]) < abs(e1["1/rho"]) / 4
+
+
+# ---------------------------------------------------------------------------
+# Operator level: nl_we_op_a through apply_forward / apply_jacobianT
+# (we_op.cpp:904-1239 with nmodels = 2)
+# ---------------------------------------------------------------------------
+def acoustic_op_problem(nt_data=151, dt_data=0.002, dt=0.001, sub=1, bc=(1, 2, 2, 2), seismotype=0, nx=50, nz=40):
+    h = 0.01
+    x = np.arange(nx)[:, None] * h
+    z = np.arange(nz)[None, :] * h
+    vp = 1.8 + 1.0 * z + 0 * x
+    rho = 2.0 + 0.15 * np.sin(2 * np.pi * x / 0.45) * np.cos(2 * np.pi * z / 0.35)
+    rx = np.stack([np.arange(5, nx - 5) * h, np.full(nx - 10, 0.05)], 1)
+    prob = W.base_problem(nx, nz, h, h, nt_data, dt_data, [[0.23, 0.12], [0.31, 0.09]], [rx, rx[::2]], taper=0,
+                          bc=bc, seismotype=seismotype, dt=dt, sub=sub)
+    prob["acoustic"] = 1
+    src = np.stack([W.ricker(nt_data, dt_data, 12.0), W.ricker(nt_data, dt_data, 9.0)])[None]
+    return prob, np.stack([vp, rho]), src
+
+
+@pytest.mark.parametrize("seismotype", [0, 1])
+def test_acoustic_op_forward_is_the_propagation(orc64, seismotype):
+    """At dt_data = dt the sinc resampler is the identity: the op's data are the
+    propagation's receiver traces on the (K, 1/rho) model with the free-surface penalty,
+    times dt/dt_data = 1, and their time derivative Dt when seismotype = 1; vp above vpmax
+    is clipped first."""
+    prob, m, src = acoustic_op_problem(nt_data=301, dt_data=0.001, seismotype=seismotype)
+    m_hi = m.copy()
+    m_hi[0, :5, :5] = 50.0  # clipped to vpmax = 8 by analyzeModel
+    d, _ = orc64.apply_forward(orc64.make_op(prob), m_hi, src)
+    m_clip = m_hi.copy()
+    m_clip[0] = np.minimum(m_clip[0], prob["mp"]["vpmax"])
+    model = np.stack([m_clip[1] * m_clip[0] ** 2, 1 / m_clip[1]])
+    g = grid_struct(dict(prob, nt=301, dt=0.001, sub=1))
+    nx, nz, h = prob["nx"], prob["nz"], prob["dx"]
+    ref = []
+    for s, rxz in enumerate(prob["rxz"]):
+        inj = orc64.delta_m3(nx, nz, h, h, 0, 0, [prob["sxz"][s, 0]], [prob["sxz"][s, 1]])
+        ext = orc64.delta_m3(nx, nz, h, h, 0, 0, rxz[:, 0], rxz[:, 1])
+        ref.append(orc64.propagate_acoustic(g, model, inj, src[0, s:s + 1], ext=ext))
+    ref = np.concatenate(ref)
+    if seismotype == 1:
+        ref = orc64.Dt(False, ref, 0.001)
+    assert d.shape == (1, ref.shape[0], 301)
+    assert np.abs(d[0] - ref).max() <= 1e-9 * np.abs(ref).max()
+
+
+def _acoustic_op_misfit(orc, prob, true, start, src):
+    dt = prob["mp"]["dt_data"]
+    op = orc.make_op(prob)
+    dobs, _ = orc.apply_forward(op, true, src)
+    d0, full = orc.apply_forward(op, start, src, keep_full=True)
+
+    def J(m):
+        r = orc.apply_forward(orc.make_op(prob), m, src)[0] - dobs
+        return 0.5 * dt * (np.sum(r ** 2) - 0.5 * np.sum(r[..., 0] ** 2 + r[..., -1] ** 2))
+
+    rw = (d0 - dobs) * dt
+    rw[..., 0] *= 0.5
+    rw[..., -1] *= 0.5
+    return J, orc.apply_jacobianT(op, start, rw, full)
+
+
+@pytest.mark.parametrize("seismotype,bc", [(0, (1, 3, 3, 3)), (1, (1, 3, 3, 3)), (0, (1, 2, 2, 2))])
+def test_acoustic_op_gradient_finite_difference(orc64, seismotype, bc):
+    """apply_jacobianT is the gradient of the trapezoid-rule misfit through the resampler
+    (dt_data = 2 dt), the time derivative (seismotype 1), the H quadrature and the chain
+    rule to (vp, rho): along vp (rho fixed) only the exact K gradient enters; along rho at
+    fixed K (vp = sqrt(K/rho)) the chain rule's K and 1/rho terms both enter, with the
+    1/rho term's narrow-versus-wide stencil difference (0.1-0.3 % here).  Absorbing sides
+    take their impedance from the boundary cells' velocity, a dependence the reference's
+    gradient leaves out (0.4 % along vp on this problem): with absorbing sides the
+    perturbation is kept off the absorbing boundary cells, and the gradient is exact again."""
+    from scipy.ndimage import gaussian_filter
+    prob, true, src = acoustic_op_problem(seismotype=seismotype, bc=bc)
+    prob["free_surface_stiffness"] = 2.0
+    start = true.copy()
+    start[0] = gaussian_filter(true[0], 4) * 0.97
+    start[1] = gaussian_filter(true[1], 4) * 1.03
+    J, g = _acoustic_op_misfit(orc64, prob, true, start, src)
+    assert g.shape == (2,) + true.shape[1:]
+    s = gaussian_filter(np.random.default_rng(3).standard_normal(true[0].shape), 3) * 0.01
+    if bc[1] == 2:
+        s[:1], s[-1:], s[:, -1:] = 0, 0, 0  # off the absorbing boundary cells
+    K0 = start[1] * start[0] ** 2
+    paths = {"vp": lambda e: np.stack([start[0] * (1 + e * s), start[1]]),
+             "rho@K": lambda e: np.stack([np.sqrt(K0 / (start[1] * (1 + e * s))), start[1] * (1 + e * s)])}
+    errs = {}
+    for name, path in paths.items():
+        fd = (J(path(0.1)) - J(path(-0.1))) / 0.2
+        dm = (path(1e-6) - path(-1e-6)) / 2e-6
+        errs[name] = (np.sum(g * dm) - fd) / fd
+        print("seismotype %d bc %s %s: rel err %.2e" % (seismotype, bc, name, errs[name]))
+    assert abs(errs["vp"]) < 1e-5
+    assert abs(errs["rho@K"]) < 0.01
