@@ -166,7 +166,7 @@ class AcousticEngine:
     """Batched device propagation of the acoustic operator (nl_we_op_a), engine level of
     the C ABI: model (K, 1/rho) [2][nx][nz], one trace component."""
 
-    def __init__(self, grid, free_surface_stiffness=1.0, device=0):
+    def __init__(self, grid, free_surface_stiffness=1.05, device=0):
         lib = load()
         self._h = C.c_void_p()
         self.grid = grid
@@ -296,7 +296,7 @@ def op_desc_from(prob, device=0):
             setattr(d, k, prob[k])
     d.sinc_half_length = int(prob.get("sinc_hl", 11))
     d.nparam = 2 if prob.get("acoustic") else (5 if prob.get("vti") else 3)
-    d.free_surface_stiffness = float(prob.get("free_surface_stiffness", 1.0))
+    d.free_surface_stiffness = float(prob.get("free_surface_stiffness", 1.05))  # param.hpp:35
     d.nt_data, d.dt_data = int(mp["nt_data"]), float(mp["dt_data"])
     d.courant, d.dt, d.sub = float(mp["courant"]), float(mp["dt"]), int(mp["sub"])
     d.vpmin, d.vpmax, d.vsmin, d.vsmax = mp["vpmin"], mp["vpmax"], mp["vsmin"], mp["vsmax"]

@@ -237,6 +237,49 @@ def notebook_elastic_fwi():
     return prob, true.astype(f32), m0.astype(f32), allsrc.astype(f32)
 
 
+def notebook_acoustic_fwi():
+    """The published acoustic FWI example of the reference (examples/acoustic_fwi.ipynb
+    cells 3, 5, 8 and 11; examples/par/acoustic_fwi.p), as the reference programs see it.
+
+    - Wavelet (cell 3): the elastic example's 10 Hz Ricker, 1001 samples of 4 ms, shifted
+      50 samples, float32; duplicated once per shot as the acoustic source (analyzeWavelet
+      with nmodels = 2, src/cpp/we_op.cpp:178-194).
+    - Models (cell 5): 801 x 201 cells of 10 m; (vp, rho) = (1.5, 1.0) above z index 100
+      and (2.5, 2.0) below; the true model has (2.0, 1.75) in x indices 375-425, z indices
+      70-79.
+    - Geometry (cell 8, acoustic_fwi.p): ns=10 sx0=2 sz0=0.1 sxinc=0.4 nr=401 rx0=2 rz0=0.1
+      rxinc=0.01 in float32 as readParameters builds it (src/cpp/param.cpp:226-244).
+    - Operator: bc_top=1, other sides 2 with 50-cell tapers of strength 0.08, dt=-1
+      (courant 0.6), seismotype=0 (pressure), sub=-1, free_surface_stiffness 1.05 (the
+      default, src/include/param.hpp:35); the inversion's bounds vp in [1, 3] and rho in
+      [1, 2.5] (acoustic_fwi.p).
+
+    Returns (prob, true_model, start_model, allsrc [1][10][1001] float32)."""
+    f32 = np.float32
+    nt, dt, f0, shift = 1001, 0.004, 10.0, 50
+    t = np.linspace(0, (nt - 1) * dt, nt)
+    sig = np.sqrt(2) / (2 * f0 * np.pi)
+    w = ((1.0 - ((t - shift * dt) / sig) ** 2) * np.exp(-0.5 * ((t - shift * dt) / sig) ** 2)).astype(f32)
+    nz, nx, h = 201, 801, 0.01
+    m0 = np.zeros((2, nx, nz))
+    m0[0, :, :100], m0[0, :, 100:] = 1.5, 2.5
+    m0[1, :, :100], m0[1, :, 100:] = 1.0, 2.0
+    true = m0.copy()
+    true[0, 375:426, 70:80], true[1, 375:426, 70:80] = 2.0, 1.75
+    ns, nr = 10, 401
+    sx = f32(2) + np.arange(ns, dtype=f32) * f32(0.4)
+    sxz = np.stack([sx, np.full(ns, f32(0.1), f32)], 1).astype(np.float64)
+    rx = f32(2) + np.arange(nr, dtype=f32) * f32(0.01)
+    rxz = np.stack([rx, np.full(nr, f32(0.1), f32)], 1).astype(np.float64)
+    prob = base_problem(nx, nz, float(f32(h)), float(f32(h)), nt, float(f32(dt)), sxz, [rxz] * ns,
+                        taper=50, taper_strength=float(f32(0.08)), seismotype=0, dt=-1.0, sub=-1)
+    prob["acoustic"] = 1
+    prob["free_surface_stiffness"] = 1.05
+    prob["mp"].update(vpmin=1.0, vpmax=3.0, rhomin=1.0, rhomax=2.5)
+    allsrc = np.tile(w, (ns, 1))[None]
+    return prob, true.astype(f32), m0.astype(f32), allsrc.astype(f32)
+
+
 def config4(nshots=48, f0=5.0, seed=0):
     """End-to-end FWI: 48 shots, 600 x 2000, h = 10 m, nt = 4000 (dt = 1 ms); true = depth
     gradient + seeded lens anomalies, start = the true model smoothed over 500 m; Ricker

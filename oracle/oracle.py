@@ -284,7 +284,7 @@ class Oracle:
         op.resampling = prob.get("resampling", 0)
         op.sinc_hl = prob.get("sinc_hl", 11)
         op.acoustic = int(prob.get("acoustic", 0))
-        op.stiffness = float(prob.get("free_surface_stiffness", 1.0))
+        op.stiffness = float(prob.get("free_surface_stiffness", 1.05))
         op.mt = 0 if op.acoustic else int(prob.get("mt", 0))
         op.gl = 0.0 if op.acoustic else float(prob.get("gl", 0.0))
         rdip = np.ascontiguousarray(np.concatenate(prob["rdip"]) if "rdip" in prob else np.zeros(len(rxz)),
