@@ -32,6 +32,13 @@ BYTES_PER_CELL_UPDATE_FWD = 36.0   # iso forward step: read curr 2 + prev 2 + mo
 # step with imaging at sub = 1 (68 B iso, 76 B VTI: + 8 B of forward state, + 24 B read-modify-
 # write of the 3 gradient fields); recomputation of the forward wavefield counts as overhead
 BYTES_PER_GRADIENT_CELL_STEP = {False: 36.0 + 68.0, True: 44.0 + 76.0}
+# acoustic operator (nl_we_op_a), per shot-cell: a step reads prev and curr and writes next
+# (12 B; its per-cell factors dt^2 K, 1/rho and the boundary divisor are read once per cell
+# for all shots); imaging every sub steps reads the adjoint level and snapshots j-1, j, j+1
+# and read-modify-writes the K and 1/rho slabs (32 B); the forward run writes snapshot j
+ACOUSTIC_BYTES_FWD = 12.0
+ACOUSTIC_BYTES_MODEL = 12.0
+ACOUSTIC_BYTES_IMAGING = 32.0
 
 
 def parse():
@@ -86,6 +93,9 @@ def parse():
     ap.add_argument("--fwi-trials", type=int, default=10, help="with --fwi: trial steps per line search")
     ap.add_argument("--fwi-mask-tapers", action="store_true",
                     help="with --fwi: gradient mask zero in the absorbing taper zones (sides and bottom)")
+    ap.add_argument("--acoustic", type=int, metavar="SUB", default=None,
+                    help="the acoustic operator (nl_we_op_a) on the C2 geometry: batched gradients with snapshots "
+                         "every SUB steps, kept whole")
     ap.add_argument("--step-only", type=int, metavar="EDGE_STREAM", default=None,
                     help="only autotune and time the forward time step of the workload with this edge_stream "
                          "setting (0: every launch of a step on one stream, for per-launch kernel traces), "
@@ -386,6 +396,158 @@ def operator_level(args):
                                                                              ", VTI" if vti else "")}}))
 
 
+def acoustic_grad_error_small():
+    """Acoustic gradient rel-L2 error vs the FP64 oracle on a reduced single-shot case
+    (largest over the raw K and 1/rho gradients)."""
+    import fwi2d_amd
+    from fwi2d_amd import workloads as W
+    from oracle.oracle import Oracle, grid_struct
+    nx, nz, h, nt, dt, sub = 160, 100, 0.01, 600, 0.001, 2
+    phys = W.add_ellipses(W.gradient_model(nx, nz, h, h, seed=0), h, h)
+    start = W.smooth_model(phys, h, h, 0.1)
+    m0 = np.stack([phys[2] * phys[0] ** 2, 1 / phys[2]]).astype(np.float32).astype(np.float64)
+    m1 = np.stack([start[2] * start[0] ** 2, 1 / start[2]]).astype(np.float32).astype(np.float64)
+    prob = dict(nx=nx, nz=nz, dx=h, dz=h, bc_top=1, bc_bottom=2, bc_left=2, bc_right=2, taper_top=0,
+                taper_bottom=20, taper_left=20, taper_right=20, taper_strength=0.05, vti=0)
+    sxz = np.array([[0.8, 0.02]])
+    rxz = [np.stack([np.arange(2, nx - 2, 4) * h, np.full(len(range(2, nx - 2, 4)), 0.02)], 1)]
+    src = W.ricker(nt, dt, 12.0)[None]
+    orc = Oracle(64)
+    g = grid_struct(dict(prob, nt=nt, dt=dt, sub=sub))
+    inj = orc.delta_m3(nx, nz, h, h, 0, 0, sxz[:, 0], sxz[:, 1])
+    ext = orc.delta_m3(nx, nz, h, h, 0, 0, rxz[0][:, 0], rxz[0][:, 1])
+    alpha = 0.2508560249 / 1.05
+    dobs = orc.propagate_acoustic(g, m0, inj, src, ext=ext, alpha=alpha)
+    full = np.zeros((1 + nt // sub, nx, nz))
+    d = orc.propagate_acoustic(g, m1, inj, src, ext=ext, full=full, alpha=alpha)
+    adj = -(d - dobs)[:, ::-1].copy()
+    gref = np.zeros((2, nx, nz))
+    orc.propagate_acoustic(g, m1, ext, adj, full=full, grad=gref, alpha=alpha)
+    eng = fwi2d_amd.AcousticEngine(fwi2d_amd.grid_from(prob, nt, dt, sub), 1.05)
+    eng.set_model(m1)
+    eng.set_shots(sxz, rxz)
+    eng.forward(src, keep_snapshots=True)
+    gk = eng.adjoint(adj)
+    eng.close()
+    return max(float(np.linalg.norm(gk[c] - gref[c]) / np.linalg.norm(gref[c])) for c in range(2))
+
+
+def acoustic_cpu_baseline(nt_sample, sub):
+    """The oracle's FP32 acoustic propagation (OpenMP over the host cores) on one shot of
+    the C2 grid: nt_sample forward steps with snapshots every sub steps + the adjoint run
+    with imaging."""
+    from fwi2d_amd import workloads as W
+    from oracle.oracle import Oracle, grid_struct
+    orc = Oracle(32)
+    nz, nx, h, dt, _, _, _, gseed, _ = WORKLOADS["c2"]
+    phys = W.gradient_model(nx, nz, h, h, seed=0)
+    m = np.stack([phys[2] * phys[0] ** 2, 1 / phys[2]]).astype(np.float32)
+    prob = dict(nx=nx, nz=nz, dx=h, dz=h, bc_top=1, bc_bottom=2, bc_left=2, bc_right=2, taper_top=0,
+                taper_bottom=30, taper_left=30, taper_right=30, taper_strength=0.05, vti=0)
+    sxz, rxz, freqs = W.config2_geometry(1, nx, h, seed=gseed)
+    g = grid_struct(dict(prob, nt=nt_sample, dt=dt, sub=sub))
+    src = W.ricker(nt_sample, dt, float(freqs[0]))[None].astype(np.float32)
+    inj = orc.delta_m3(nx, nz, h, h, 0, 0, sxz[:, 0], sxz[:, 1])
+    ext = orc.delta_m3(nx, nz, h, h, 0, 0, rxz[0][:, 0], rxz[0][:, 1])
+    full = np.zeros((1 + nt_sample // sub, nx, nz), np.float32)
+    grad = np.zeros((2, nx, nz), np.float32)
+    t0 = time.perf_counter()
+    d = orc.propagate_acoustic(g, m, inj, src, ext=ext, full=full)
+    orc.propagate_acoustic(g, m, ext, d[:, ::-1].copy(), full=full, grad=grad)
+    el = time.perf_counter() - t0
+    return dict(cell_updates_per_s=2.0 * nt_sample * nx * nz / el, seconds=el,
+                threads=int(orc.lib.orc_num_threads()),
+                sample="1 shot, %dx%d grid, %d forward + %d adjoint steps (sub=%d)" % (nz, nx, nt_sample, nt_sample,
+                                                                                      sub))
+
+
+def acoustic_run(args):
+    """The acoustic operator (nl_we_op_a) on the C2 geometry: `shots` shots (64) on
+    500 x 1700 cells of 10 m, nt = 4000 steps of 1 ms, snapshots every `--acoustic` steps
+    kept whole on the device.  `steps` timed batched gradients (forward with snapshots,
+    FP64 misfit, adjoint with imaging) after `warmup` untimed ones, all on device
+    buffers; then one eager forward run without snapshots timed step by step."""
+    import torch
+    import fwi2d_amd
+    from fwi2d_amd import workloads as W
+    fwi2d_amd.load()
+    sub = args.acoustic
+    nz, nx, h, dt, _, _, _, gseed, _ = WORKLOADS["c2"]
+    ns = args.shots or 64
+    nt = args.nt or 4000
+    phys = W.gradient_model(nx, nz, h, h, seed=0)
+    start = W.smooth_model(phys, h, h, 0.3)
+    mtrue = np.stack([phys[2] * phys[0] ** 2, 1 / phys[2]]).astype(np.float32)
+    m0 = np.stack([start[2] * start[0] ** 2, 1 / start[2]]).astype(np.float32)
+    sxz, rxz, freqs = W.config2_geometry(ns, nx, h, seed=gseed)
+    src_np = np.stack([W.ricker(nt, dt, f) for f in freqs]).astype(np.float32)
+    prob = W.base_problem(nx, nz, h, h, nt, dt, sxz, rxz, dt=dt, sub=sub)
+    eng = fwi2d_amd.AcousticEngine(fwi2d_amd.grid_from(prob, nt, dt, sub))
+    eng.set_shots(sxz, rxz)
+    dev = "cuda"
+    src = torch.from_numpy(src_np).to(dev)
+    rcv = torch.zeros((eng.nr, nt), dtype=torch.float32, device=dev)
+    grad = torch.zeros((2, nx, nz), dtype=torch.float32, device=dev)
+    eng.set_model(mtrue)
+    eng.forward_device(src.data_ptr(), rcv.data_ptr())
+    dobs = rcv.clone()
+    eng.set_model(m0)
+    log("acoustic workload ready: %d shots, %dx%d, nt=%d, sub=%d" % (ns, nz, nx, nt, sub))
+    res = {}
+
+    def gradient():
+        eng.forward_device(src.data_ptr(), rcv.data_ptr(), keep_snapshots=True)
+        r = dobs - rcv
+        res["misfit"] = 0.5 * float(torch.sum(r.double() ** 2))
+        adj = torch.flip(r, dims=[1]).contiguous()
+        torch.cuda.synchronize()
+        eng.adjoint_device(adj.data_ptr(), grad.data_ptr())
+
+    for _ in range(args.warmup):
+        gradient()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        gradient()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    log("timed acoustic gradients done: %.3f s" % el)
+    t1 = time.perf_counter()
+    eng.forward_device(src.data_ptr(), rcv.data_ptr(), keep_snapshots=False)
+    torch.cuda.synchronize()
+    fwd_s = time.perf_counter() - t1
+    eng.close()
+    shots_per_s = ns * args.steps / el
+    step_s = fwd_s / (nt - 1)
+    cells = ns * nx * nz
+    fwd_bytes = ACOUSTIC_BYTES_FWD * cells + ACOUSTIC_BYTES_MODEL * nx * nz
+    grad_bytes = (2 * fwd_bytes * (nt - 1) + (4.0 + ACOUSTIC_BYTES_IMAGING) * cells * (nt // sub + 1))
+    err = None if args.no_check else acoustic_grad_error_small()
+    cpu = None if args.no_cpu else acoustic_cpu_baseline(800, sub)
+    line = {"metric": "acoustic FWI gradient shots/s (nl_we_op_a, batched)", "value": round(shots_per_s, 4),
+            "unit": "shots/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": round(1000 * el / args.steps, 2), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "acoustic gradients: %d shots, 500x1700 grid h=10 m, nt=%d dt=1 ms, snapshots "
+                                   "every %d steps" % (ns, nt, sub), "shots_per_step": ns, "sub": sub},
+            "gcell_updates_per_s": round(shots_per_s * 2.0 * nt * nx * nz / 1e9, 2),
+            "misfit_f64": res.get("misfit"),
+            "grad_rel_err_vs_fp64": err,
+            "grad_err_sample": "1 shot, 160x100 grid, nt=600, sub=2, raw K and 1/rho gradients vs FP64 oracle",
+            "roofline": {"bound": "hbm", "kernel": "k_astep + k_ainject per step (eager forward, no snapshots)",
+                         "achieved": round(fwd_bytes / step_s / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(fwd_bytes / step_s / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
+                         "avg_step_us": round(step_s * 1e6, 2),
+                         "bytes_per_step": fwd_bytes,
+                         "gradient_achieved": round(grad_bytes * args.steps / el / 1e9, 1),
+                         "gradient_frac": round(grad_bytes * args.steps / el / 1e9 / HBM_PEAK_GBS, 4)}}
+    if cpu:
+        line["cpu_baseline"] = {"value": round(cpu["cell_updates_per_s"] / (2.0 * nt * nx * nz), 6),
+                                "unit": "shots/s", "cores": cpu["threads"], "kind": "port", "sample": cpu["sample"],
+                                "gcell_updates_per_s": round(cpu["cell_updates_per_s"] / 1e9, 4)}
+    print(json.dumps(line))
+
+
 def lowpass_filter(fc, dt):
     """A zero-phase low-pass FIR: a Hann-windowed sinc of cut-off fc (Hz), 1.5 periods of
     the cut-off on each side, unit gain at zero frequency."""
@@ -499,6 +661,9 @@ def main():
         return
     if args.fwi is not None:
         fwi_run(args)
+        return
+    if args.acoustic is not None:
+        acoustic_run(args)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU only: the engine batches the shots of one MI355X; across GPUs "

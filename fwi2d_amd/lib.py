@@ -209,6 +209,15 @@ class AcousticEngine:
         _check(load().fwi2d_acoustic_adjoint(self._h, _ptr(a), _ptr(g), 0))
         return g
 
+    def forward_device(self, src_ptr, rcv_ptr, keep_snapshots=False):
+        """Device pointers: src [ns][nt] -> rcv [nr][nt]."""
+        _check(load().fwi2d_acoustic_forward(self._h, C.c_void_p(src_ptr), C.c_void_p(rcv_ptr),
+                                             int(keep_snapshots), 1))
+
+    def adjoint_device(self, adj_ptr, grad_ptr):
+        """Device pointers: adjoint sources [nr][nt] -> raw gradient [2][nx][nz]."""
+        _check(load().fwi2d_acoustic_adjoint(self._h, C.c_void_p(adj_ptr), C.c_void_p(grad_ptr), 1))
+
 
 class Engine:
     """Batched device propagation (engine level of the C ABI)."""
