@@ -59,7 +59,7 @@ struct Acoustic {
         HIPCHECK(hipSetDevice(device));
         HIPCHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
         upload_constants(st);
-        ldz = (g.nz + 3) / 4 * 4;
+        ldz = (g.nz + 31) / 32 * 32;  // columns start on 128-byte lines
         nxz = (long long)g.nx * ldz;
         ntx = (g.nx + kAcTX - 1) / kAcTX;
         ntz = (ldz + kAcTZ - 1) / kAcTZ;
@@ -177,6 +177,18 @@ struct Acoustic {
         a.nshots = ns;
         a.ntx = ntx;
         a.ntz = ntz;
+        // shots per step workgroup: up to 8 while the launch keeps 2048 or more workgroups
+        a.spb = (int)std::max(1LL, std::min(8LL, (long long)ntx * ntz * ns / 2048));
+        // the rectangle of tiles whose cells are all interior along both axes (no closure
+        // rows, no SAT terms): columns 6..nx-7, rows 6..nz-7
+        auto interior = [](int n, int t, int ntiles, int& i0, int& i1) {
+            i0 = (6 + t - 1) / t;
+            i1 = std::min(ntiles, (n - 6) / t);  // tile i ends at (i + 1) t <= n - 6
+            if (i1 <= i0) i0 = i1 = 0;
+        };
+        interior(g.nx, kAcTX, ntx, a.itx0, a.itx1);
+        interior(g.nz, kAcTZ, ntz, a.itz0, a.itz1);
+        if (a.itx1 == a.itx0 || a.itz1 == a.itz0) a.itx0 = a.itx1 = a.itz0 = a.itz1 = 0;
         a.bc[0] = g.bc_top;
         a.bc[1] = g.bc_bottom;
         a.bc[2] = g.bc_left;

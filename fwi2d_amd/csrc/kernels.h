@@ -162,6 +162,8 @@ constexpr int kAcTX = 16, kAcTZ = 64;  // step and imaging tiles (x, z)
 
 struct AcousticArgs {
     int nx, nz, ldz, nshots, ntx, ntz, taper_prev;
+    int spb;                     // k_astep: shots per workgroup (consecutive shots of one tile)
+    int itx0, itx1, itz0, itz1;  // k_astep: the tiles [itx0, itx1) x [itz0, itz1) hold interior cells only
     int bc[4];                   // top, bottom, left, right: 1 Dirichlet, 2 absorbing, 3 Neumann
     float rdx2, rdz2;            // 1 / dx^2, 1 / dz^2
     float dir_x[4], dir_z[4];    // Dirichlet SAT S^T terms SC[k] / (H[k] h^2)

@@ -1950,71 +1950,201 @@ __device__ __forceinline__ float ac_sat(int kind, int k, const float* pu, const 
     return v;
 }
 
-__global__ void __launch_bounds__(kThreads) k_astep(const AcousticArgs a)
+// The step's level window is narrower in x than the imaging windows: 2 columns before the
+// tile and 6 after it, the window moved left to start at column nx - 8 at the far side.
+// That holds every point a tile's cells read: interior D2 rows +-2, the closure rows of
+// columns 0-5 read columns 0-7 (first tile) and those of columns nx-6..nx-1 columns
+// nx-8..nx-1 (a tile holding such a column ends at most 5 columns before its window does),
+// and the SAT terms the boundary column and its 3 neighbours.
+constexpr int kAcSHL = 2, kAcSWX = kAcTX + kAcSHL + 6;
+
+__device__ __forceinline__ float sel4(const float (&v)[4], int k)
 {
-    __shared__ __attribute__((aligned(16))) float sU[kAcWX * kAcWZ];
-    __shared__ __attribute__((aligned(16))) float sR[kAcWX * kAcWZ];
-    const int tile = blockIdx.x, shot = blockIdx.y;
-    const int tix = tile / a.ntz, tiz = tile - tix * a.ntz;
+    return k == 0 ? v[0] : (k == 1 ? v[1] : (k == 2 ? v[2] : v[3]));
+}
+
+__device__ __forceinline__ void ac_window_x(const float* u, float* sU, int wsx, int wsz, int nx, int ldz)
+{
+    for (int i = threadIdx.x; i < kAcSWX * kAcWQ; i += kThreads) {
+        const int lx = i / kAcWQ, lq = i - lx * kAcWQ;
+        const int gx = wsx + lx, gz = wsz + 4 * lq;
+        const bool ok = gx >= 0 && gx < nx && gz >= 0 && gz < ldz;
+        st4(&sU[lx * kAcWZ + 4 * lq], ok ? ld4(u + (long long)gx * ldz + gz) : make_float4(0.f, 0.f, 0.f, 0.f));
+    }
+}
+
+// A workgroup steps one 16 x 64 tile of a.spb consecutive shots: the 1/rho window, dt^2 K,
+// the boundary divisor and the tapers are loaded once (and, for tiles of interior cells
+// only, the D2 weights of the cells formed once); the shots' level windows alternate
+// between two LDS buffers, the next shot's loaded while the current one is computed (one
+// barrier per shot).  EDGE = false: a tile of the rectangle of interior tiles; EDGE = true:
+// a tile of the frame around it, whose quads of interior cells take the interior rows
+// (weights formed per shot) and whose other quads the closure rows and SAT terms, one
+// cell at a time.
+template <bool EDGE>
+__device__ __forceinline__ void astep_tile(const AcousticArgs& a, int tix, int tiz, int grp, float* sR,
+                                           float (*sU)[kAcSWX * kAcWZ])
+{
     const int sx = tix * kAcTX, sz = tiz * kAcTZ;
-    const int ex = min(sx + kAcTX, a.nx), ez = min(sz + kAcTZ, a.ldz);
-    const int wsx = sx - kAcH, wsz = sz - kAcH;
     const int nx = a.nx, nz = a.nz, ldz = a.ldz;
-    const long long so = (long long)shot * a.shot_stride;
-    ac_windows(a.curr + so, a.ri, sU, sR, wsx, wsz, nx, ldz);
-    __syncthreads();
-    const int q = threadIdx.x;
-    const int gx = sx + q / (kAcTZ / 4), z0 = sz + 4 * (q % (kAcTZ / 4));
-    if (gx >= ex || z0 >= ez) return;
-    const long long g = (long long)gx * ldz + z0;
-    float pv[4], kd[4], dv[4], tz[4], out[4];
-    unpack(ld4(a.prev + so + g), pv);
+    const int ex = min(sx + kAcTX, nx), ez = min(sz + kAcTZ, ldz);
+    const int wsx = min(sx - kAcSHL, nx - 8), wsz = sz - kAcH;
+    const int s0 = grp * a.spb, s1 = min(s0 + a.spb, a.nshots);
+    for (int i = threadIdx.x; i < kAcSWX * kAcWQ; i += kThreads) {
+        const int lx = i / kAcWQ, lq = i - lx * kAcWQ;
+        const int gx = wsx + lx, gz = wsz + 4 * lq;
+        const bool ok = gx >= 0 && gx < nx && gz >= 0 && gz < ldz;
+        st4(&sR[lx * kAcWZ + 4 * lq], ok ? ld4(a.ri + (long long)gx * ldz + gz) : make_float4(0.f, 0.f, 0.f, 0.f));
+    }
+    ac_window_x(a.curr + (long long)s0 * a.shot_stride, sU[0], wsx, wsz, nx, ldz);
+    const int q = threadIdx.x;  // quads column-major: a wavefront covers 4 whole columns
+    const int col_q = q / (kAcTZ / 4), row_q = q % (kAcTZ / 4);
+    const int gx = sx + col_q, z0 = sz + 4 * row_q;
+    const bool active = gx < ex && z0 < ez;
+    const long long g = active ? (long long)gx * ldz + z0 : 0;
+    float kd[4], dv[4], tz[4];
     unpack(ld4(a.kdt2 + g), kd);
     unpack(ld4(a.div + g), dv);
-    unpack(ld4(a.taper_z + z0), tz);
-    const float tx = a.taper_x[gx];
-    const int col = (gx - wsx) * kAcWZ;
+    unpack(ld4(a.taper_z + (active ? z0 : 0)), tz);
+    const float tx = a.taper_x[active ? gx : 0];
+    const int col = (active ? gx - wsx : 2) * kAcWZ;  // an in-window column either way
+    const int zw = (active ? z0 : sz) - wsz;
+    __syncthreads();
+    W5 wx[4], wz[4];
+    if (!EDGE) {
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-        const int gz = z0 + k;
-        if (gz >= nz) {
-            out[k] = 0.f;  // rows past nz stay zero
-            continue;
+        for (int k = 0; k < 4; k++) {
+            wx[k] = d2_weights(&sR[col + zw + k], kAcWZ);
+            wz[k] = d2_weights(&sR[col + zw + k], 1);
         }
-        const int w = col + (gz - wsz);
-        const float* pu = &sU[w];
-        const float* pc = &sR[w];
-        const float lx = (gx >= 6 && gx < nx - 6) ? d2_apply(d2_weights(pc, kAcWZ), pu, kAcWZ)
-                                                 : d2_closure(pu, pc, kAcWZ, gx, nx);
-        const float lz = (gz >= 6 && gz < nz - 6) ? d2_apply(d2_weights(pc, 1), pu, 1) : d2_closure(pu, pc, 1, gz, nz);
-        float rhs = lx * a.rdx2 + lz * a.rdz2;
-        float p = pv[k];
-        if (a.taper_prev) p = (p * tz[k]) * tx;
-        if (gz < 4) {  // top side, boundary cell (gx, 0)
-            const int b = col - wsz;
-            rhs += ac_sat(a.bc[0], gz, &sU[b], &sR[b], 1, p, gz == 0 ? a.K[g - z0] : 0.f, a.dir_z, a.pen_z, a.neu_z,
-                          a.abs_z);
-        }
-        if (gz >= nz - 4) {  // bottom side, boundary cell (gx, nz - 1)
-            const int b = col + (nz - 1 - wsz);
-            rhs += ac_sat(a.bc[1], nz - 1 - gz, &sU[b], &sR[b], -1, p,
-                          gz == nz - 1 ? a.K[(long long)gx * ldz + nz - 1] : 0.f, a.dir_z, a.pen_z, a.neu_z, a.abs_z);
-        }
-        if (gx < 4) {  // left side, boundary cell (0, gz)
-            const int b = (0 - wsx) * kAcWZ + (gz - wsz);
-            rhs += ac_sat(a.bc[2], gx, &sU[b], &sR[b], kAcWZ, p, gx == 0 ? a.K[gz] : 0.f, a.dir_x, a.pen_x, a.neu_x,
-                          a.abs_x);
-        }
-        if (gx >= nx - 4) {  // right side, boundary cell (nx - 1, gz)
-            const int b = (nx - 1 - wsx) * kAcWZ + (gz - wsz);
-            rhs += ac_sat(a.bc[3], nx - 1 - gx, &sU[b], &sR[b], -kAcWZ, p,
-                          gx == nx - 1 ? a.K[(long long)(nx - 1) * ldz + gz] : 0.f, a.dir_x, a.pen_x, a.neu_x, a.abs_x);
-        }
-        // leapfrog (we_op.cpp:2093-2098), boundary divisor (asat_scale_boundaries), taper
-        const float v = (kd[k] * rhs + (2.f * pu[0] - p)) / dv[k];
-        out[k] = (v * tz[k]) * tx;
     }
-    st4(a.next + so + g, make_float4(out[0], out[1], out[2], out[3]));
+    const bool inx = gx >= 6 && gx < nx - 6;
+    for (int sh = s0; sh < s1; sh++) {
+        const int b = (sh - s0) & 1;
+        const long long so = (long long)sh * a.shot_stride;
+        if (sh + 1 < s1) ac_window_x(a.curr + so + a.shot_stride, sU[b ^ 1], wsx, wsz, nx, ldz);
+        if (active) {
+            float pv[4], out[4];
+            unpack(ld4(a.prev + so + g), pv);
+            const float* U = sU[b];
+            if (!EDGE) {
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const float* pu = &U[col + zw + k];
+                    float p = pv[k];
+                    if (a.taper_prev) p = (p * tz[k]) * tx;
+                    const float rhs = d2_apply(wx[k], pu, kAcWZ) * a.rdx2 + d2_apply(wz[k], pu, 1) * a.rdz2;
+                    // leapfrog (we_op.cpp:2093-2098), boundary divisor, taper
+                    const float v = (kd[k] * rhs + (2.f * pu[0] - p)) / dv[k];
+                    out[k] = (v * tz[k]) * tx;
+                }
+            } else if (inx && z0 >= 6 && z0 + 3 < nz - 6) {
+                // a quad of interior cells in a frame tile: the interior rows, their weights
+                // formed per shot
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const float* pu = &U[col + zw + k];
+                    const float* pc = &sR[col + zw + k];
+                    float p = pv[k];
+                    if (a.taper_prev) p = (p * tz[k]) * tx;
+                    const float rhs = d2_apply(d2_weights(pc, kAcWZ), pu, kAcWZ) * a.rdx2 +
+                                      d2_apply(d2_weights(pc, 1), pu, 1) * a.rdz2;
+                    const float v = (kd[k] * rhs + (2.f * pu[0] - p)) / dv[k];
+                    out[k] = (v * tz[k]) * tx;
+                }
+            } else {
+                // one cell at a time: unrolled, the closure rows and SAT terms of the four
+                // cells would keep hundreds of window values live
+#pragma unroll 1
+                for (int k = 0; k < 4; k++) {
+                    const int gz = z0 + k;
+                    const int w = col + zw + k;
+                    const float* pu = &U[w];
+                    const float* pc = &sR[w];
+                    const float tzk = sel4(tz, k);
+                    float p = sel4(pv, k);
+                    if (a.taper_prev) p = (p * tzk) * tx;
+                    float v = 0.f;  // rows past nz stay zero
+                    if (gz < nz) {
+                        const float lx = inx ? d2_apply(d2_weights(pc, kAcWZ), pu, kAcWZ)
+                                             : d2_closure(pu, pc, kAcWZ, gx, nx);
+                        const float lz = (gz >= 6 && gz < nz - 6) ? d2_apply(d2_weights(pc, 1), pu, 1)
+                                                                   : d2_closure(pu, pc, 1, gz, nz);
+                        float rhs = lx * a.rdx2 + lz * a.rdz2;
+                        if (gz < 4) {  // top side, boundary cell (gx, 0)
+                            const int bb = col - wsz;
+                            rhs += ac_sat(a.bc[0], gz, &U[bb], &sR[bb], 1, p, gz == 0 ? a.K[g - z0] : 0.f, a.dir_z,
+                                          a.pen_z, a.neu_z, a.abs_z);
+                        }
+                        if (gz >= nz - 4) {  // bottom side, boundary cell (gx, nz - 1)
+                            const int bb = col + (nz - 1 - wsz);
+                            rhs += ac_sat(a.bc[1], nz - 1 - gz, &U[bb], &sR[bb], -1, p,
+                                          gz == nz - 1 ? a.K[(long long)gx * ldz + nz - 1] : 0.f, a.dir_z, a.pen_z,
+                                          a.neu_z, a.abs_z);
+                        }
+                        if (gx < 4) {  // left side, boundary cell (0, gz)
+                            const int bb = (0 - wsx) * kAcWZ + (gz - wsz);
+                            rhs += ac_sat(a.bc[2], gx, &U[bb], &sR[bb], kAcWZ, p, gx == 0 ? a.K[gz] : 0.f, a.dir_x,
+                                          a.pen_x, a.neu_x, a.abs_x);
+                        }
+                        if (gx >= nx - 4) {  // right side, boundary cell (nx - 1, gz)
+                            const int bb = (nx - 1 - wsx) * kAcWZ + (gz - wsz);
+                            rhs += ac_sat(a.bc[3], nx - 1 - gx, &U[bb], &sR[bb], -kAcWZ, p,
+                                          gx == nx - 1 ? a.K[(long long)(nx - 1) * ldz + gz] : 0.f, a.dir_x, a.pen_x,
+                                          a.neu_x, a.abs_x);
+                        }
+                        // leapfrog (we_op.cpp:2093-2098), boundary divisor (asat_scale_boundaries), taper
+                        v = (sel4(kd, k) * rhs + (2.f * pu[0] - p)) / sel4(dv, k);
+                        v = (v * tzk) * tx;
+                    }
+                    out[0] = k == 0 ? v : out[0];
+                    out[1] = k == 1 ? v : out[1];
+                    out[2] = k == 2 ? v : out[2];
+                    out[3] = k == 3 ? v : out[3];
+                }
+            }
+            st4(a.next + so + g, make_float4(out[0], out[1], out[2], out[3]));
+        }
+        __syncthreads();  // the next shot's window is in; this shot's window is read
+    }
+}
+
+// One launch for all tiles: the frame tiles' workgroups first (their closure rows make
+// them the slowest), then the interior rectangle's, each class spread over all XCDs (the
+// frame's block count rounded up to a multiple of 8, the extra blocks idle) and remapped
+// within its class so that an XCD runs neighbouring tiles.
+__global__ void __launch_bounds__(kThreads) k_astep(const AcousticArgs a)
+{
+    __shared__ __attribute__((aligned(16))) float sR[kAcSWX * kAcWZ];
+    __shared__ __attribute__((aligned(16))) float sU[2][kAcSWX * kAcWZ];
+    const int nin_x = a.itx1 - a.itx0, nin_z = a.itz1 - a.itz0;
+    const int nin = nin_x * nin_z, nedge = a.ntx * a.ntz - nin;
+    const int groups = (a.nshots + a.spb - 1) / a.spb;
+    const int ne = nedge * groups, ne8 = (ne + 7) & ~7;
+    const int b = blockIdx.x;
+    if (b < ne8) {
+        if (b >= ne) return;  // padding: the whole workgroup leaves before any barrier
+        // frame: the rows of tiles above and below the rectangle, then its side columns
+        const int lin = xcd_remap(b, ne);
+        const int grp = lin / nedge, tile = lin - grp * nedge;
+        const int top = a.itz0 * a.ntx, bot = (a.ntz - a.itz1) * a.ntx;
+        int tix, tiz;
+        if (tile < top + bot) {
+            const int r = tile < top ? tile : tile - top;
+            tix = r % a.ntx;
+            tiz = (tile < top ? 0 : a.itz1) + r / a.ntx;
+        } else {
+            const int r = tile - top - bot, w = a.ntx - nin_x;
+            const int c = r % w;
+            tiz = a.itz0 + r / w;
+            tix = c < a.itx0 ? c : a.itx1 + (c - a.itx0);
+        }
+        astep_tile<true>(a, tix, tiz, grp, sR, sU);
+    } else {
+        const int lin = xcd_remap(b - ne8, gridDim.x - ne8);
+        const int grp = lin / nin, tile = lin - grp * nin;
+        astep_tile<false>(a, a.itx0 + tile / nin_z, a.itz0 + tile % nin_z, grp, sR, sU);
+    }
 }
 
 // nl_we_op_a::compute_gradients (we_op.cpp:1883-1935) for every shot: the K term from the
@@ -2096,7 +2226,9 @@ __global__ void k_aextract(const float* u, long long shot_stride, int ldz, const
 
 void launch_astep(const AcousticArgs& a, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_astep, dim3(a.ntx * a.ntz, a.nshots), dim3(kThreads), 0, s, a);
+    const int groups = (a.nshots + a.spb - 1) / a.spb;
+    const int nin = (a.itx1 - a.itx0) * (a.itz1 - a.itz0), ne = (a.ntx * a.ntz - nin) * groups;
+    hipLaunchKernelGGL(k_astep, dim3(((ne + 7) & ~7) + nin * groups), dim3(kThreads), 0, s, a);
 }
 
 void launch_agrad(const AcousticArgs& a, hipStream_t s)
