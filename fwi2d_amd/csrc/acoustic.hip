@@ -5,9 +5,11 @@
 // model (K, 1/rho) = (rho vp^2, 1/rho) as convert_model (we_op.cpp:1863-1881) produces
 // it.  A time step is one launch over 16 x 64 tiles of every shot (k_astep), then the
 // point sources (k_ainject).  The forward run keeps every sub-th level on the device (the
-// reference's full wavefield), and the adjoint run images them (k_agrad) the way the
-// reference's compute_gradients does, into one gradient slab per shot that are summed
-// at the end.  The reference never ran this operator on a GPU.
+// reference's full wavefield) when that history fits the snapshot budget, and otherwise
+// checkpoints of segments of about sqrt(2 nsnap) snapshots from which the adjoint run
+// recomputes them, one segment at a time; the adjoint run images them (k_agrad) the way
+// the reference's compute_gradients does, into one gradient slab per shot that are
+// summed at the end.  The reference never ran this operator on a GPU.
 #include <algorithm>
 #include <cmath>
 #include <stdexcept>
@@ -41,6 +43,13 @@ struct Acoustic {
     long long model_epoch = 0, snap_epoch = -1;
     DevMem d_K, d_ri, d_kdt2, d_div, d_tx, d_tz;
     DevMem d_lev, d_full, d_gslab, d_gout, d_src, d_rcv, d_adj;
+    // snapshot segments: checkpoints, the segment buffer, recompute levels, the forward
+    // run's source traces
+    DevMem d_ckpt, d_seg, d_rlev, d_fsrc;
+    long long snapshot_budget_mib = 32768;  // whole snapshot history up to this size
+    bool whole = true;
+    int seg_K = 0, nseg = 0, seg_loaded = -1;
+    long long stat_recompute_steps = 0;
     InjList inj_src, inj_rcv;
     std::vector<PointEntry> ext;
     DevMem d_ext;
@@ -223,45 +232,132 @@ struct Acoustic {
     float* level(int k) { return d_lev.as<float>() + (long long)k * ns * nxz; }
     float* snap(int idx) { return d_full.as<float>() + (long long)idx * ns * nxz; }
 
+    // Segments of K snapshots (K about sqrt(2 nsnap)); segment j owns snapshots jK ..
+    // (j+1)K - 1 and its buffer holds jK - 1 .. (j+1)K (K + 2 slots), every snapshot its
+    // imaging reads.  Segment j >= 1 restarts from the levels (curr, prev) at step
+    // (jK - 1) sub, stored by the forward run, which also keeps the last segment's
+    // snapshots; the adjoint run images the segments last to first, recomputing each.
+    void plan_segments()
+    {
+        const int n = nsnap();
+        const double history = (double)n * ns * nxz * sizeof(float);
+        whole = history <= (double)snapshot_budget_mib * 1048576.0;
+        seg_K = std::max(1, std::min(n, (int)std::ceil(std::sqrt(2.0 * n))));
+        nseg = (n + seg_K - 1) / seg_K;
+    }
+    int seg_base(int j) const { return std::max(j * seg_K - 1, 0); }
+    int seg_last(int j) const { return std::min((j + 1) * seg_K, nsnap() - 1); }
+    float* slot(int j, int idx) { return d_seg.as<float>() + (long long)(idx - seg_base(j)) * ns * nxz; }
+    float* ckpt(int k) { return d_ckpt.as<float>() + (long long)k * ns * nxz; }
+    // snapshot idx as the imaging of a snapshot of segment j reads it (segment j loaded:
+    // its buffer holds the neighbours jK - 1 and (j+1)K too)
+    float* snapshot(int j, int idx) { return whole ? snap(idx) : slot(j, idx); }
+
+    // one forward step it on levels (p, c, n): the step kernel, then the point sources
+    void step(float* p, float* c, float* n, int it, const InjList& inj, const float* tr)
+    {
+        AcousticArgs a = base_args();
+        a.taper_prev = it > 0;
+        a.prev = p;
+        a.curr = c;
+        a.next = n;
+        launch_astep(a, st);
+        launch_ainject(inj.d_cells.as<InjCell>(), (int)inj.cells.size(), inj.d_ptr.as<int>(),
+                       inj.d_ctx.as<long long>(), inj.d_cw.as<float>(), tr, it, n, nxz, ldz, a.kdt2, a.div, a.taper_x,
+                       a.taper_z, false, st);
+    }
+
+    // the initial condition prev = 1/2 K dt^2 inject(trace sample 0) (we_op.cpp:1977-1979)
+    // on zeroed levels
+    void init_levels(float* lev, const InjList& inj, const float* tr)
+    {
+        const long long ss = (long long)ns * nxz;
+        launch_fill(lev, 3 * ss, 0.f, st);
+        AcousticArgs a = base_args();
+        launch_ainject(inj.d_cells.as<InjCell>(), (int)inj.cells.size(), inj.d_ptr.as<int>(),
+                       inj.d_ctx.as<long long>(), inj.d_cw.as<float>(), tr, 0, lev, nxz, ldz, a.kdt2, a.div, a.taper_x,
+                       a.taper_z, true, st);
+    }
+
+    // recompute segment j's snapshots from its checkpoint into the segment buffer
+    void load_segment(int j)
+    {
+        if (seg_loaded == j) return;
+        const long long ss = (long long)ns * nxz;
+        const int sub = g.sub, lastw = (g.nt - 1) / sub;  // last written snapshot
+        const int b = seg_base(j), e = seg_last(j);
+        launch_fill(slot(j, b), (long long)(e - b + 1) * ss, 0.f, st);  // slots past lastw stay zero
+        d_rlev.reserve((size_t)3 * ss * sizeof(float));
+        float* R = d_rlev.as<float>();
+        int ip = 0, ic = 1, in = 2, it = 0;
+        if (j == 0) {
+            init_levels(R, inj_src, d_fsrc.as<float>());
+        } else {
+            HIPCHECK(hipMemcpyAsync(R + ic * ss, ckpt(2 * (j - 1)), ss * sizeof(float), hipMemcpyDeviceToDevice, st));
+            HIPCHECK(hipMemcpyAsync(R + ip * ss, ckpt(2 * (j - 1) + 1), ss * sizeof(float), hipMemcpyDeviceToDevice, st));
+            it = b * sub;
+        }
+        const int end = std::min(e, lastw) * sub;
+        for (;; it++) {
+            if (it % sub == 0 && it / sub >= b)
+                HIPCHECK(hipMemcpyAsync(slot(j, it / sub), R + ic * ss, ss * sizeof(float), hipMemcpyDeviceToDevice, st));
+            if (it >= end) break;
+            step(R + ip * ss, R + ic * ss, R + in * ss, it, inj_src, d_fsrc.as<float>());
+            stat_recompute_steps++;
+            const int t = ip;
+            ip = ic;
+            ic = in;
+            in = t;
+        }
+        seg_loaded = j;
+    }
+
     // one propagation (we_op.cpp:1937-2135): sources inj with traces tr; the forward run
-    // extracts into rcv and keeps snapshots, the adjoint run images them into the slabs
+    // extracts into rcv and keeps snapshots (or checkpoints), the adjoint run images them
+    // into the slabs
     void propagate(bool adjoint, const InjList& inj, const float* tr, float* rcv, bool keep)
     {
         const int nt = g.nt, sub = g.sub;
         const long long ss = (long long)ns * nxz;
         d_lev.reserve((size_t)3 * ss * sizeof(float));
-        launch_fill(d_lev.as<float>(), 3 * ss, 0.f, st);
+        init_levels(level(0), inj, tr);
         int ip = 0, ic = 1, in = 2;
-        AcousticArgs a = base_args();
-        // prev = 1/2 K dt^2 inject(trace sample 0)  (we_op.cpp:1977-1979)
-        launch_ainject(inj.d_cells.as<InjCell>(), (int)inj.cells.size(), inj.d_ptr.as<int>(),
-                       inj.d_ctx.as<long long>(), inj.d_cw.as<float>(), tr, 0, level(ip), nxz, ldz, a.kdt2, a.div,
-                       a.taper_x, a.taper_z, true, st);
+        const int jlast = nseg - 1;
+        auto capture = [&](int it) {  // forward run: snapshots and checkpoints of level it
+            if (!keep) return;
+            if (whole) {
+                if (it % sub == 0)
+                    HIPCHECK(hipMemcpyAsync(snap(it / sub), level(ic), ss * sizeof(float), hipMemcpyDeviceToDevice, st));
+                return;
+            }
+            if (it % sub == 0 && it / sub >= seg_base(jlast))
+                HIPCHECK(hipMemcpyAsync(slot(jlast, it / sub), level(ic), ss * sizeof(float), hipMemcpyDeviceToDevice,
+                                        st));
+            if (it % sub == 0 && (it / sub + 1) % seg_K == 0 && it / sub + 1 < nseg * seg_K && it < nt - 1) {
+                const int j = (it / sub + 1) / seg_K;  // level (jK - 1) sub starts segment j
+                if (j >= 1 && j < nseg) {
+                    HIPCHECK(hipMemcpyAsync(ckpt(2 * (j - 1)), level(ic), ss * sizeof(float), hipMemcpyDeviceToDevice,
+                                            st));
+                    HIPCHECK(hipMemcpyAsync(ckpt(2 * (j - 1) + 1), level(ip), ss * sizeof(float),
+                                            hipMemcpyDeviceToDevice, st));
+                }
+            }
+        };
         for (int it = 0; it < nt - 1; it++) {
             if (!adjoint) {
-                if (keep && it % sub == 0)
-                    HIPCHECK(hipMemcpyAsync(snap(it / sub), level(ic), ss * sizeof(float), hipMemcpyDeviceToDevice, st));
+                capture(it);
                 if (rcv) launch_aextract(level(ic), nxz, ldz, d_ext.as<PointEntry>(), nr, rcv, it, st);
             } else if ((nt - 1 - it) % sub == 0 && it != 0) {
                 image(level(ic), (nt - 1 - it) / sub);
             }
-            a.taper_prev = it > 0;
-            a.prev = level(ip);
-            a.curr = level(ic);
-            a.next = level(in);
-            launch_astep(a, st);
-            launch_ainject(inj.d_cells.as<InjCell>(), (int)inj.cells.size(), inj.d_ptr.as<int>(),
-                           inj.d_ctx.as<long long>(), inj.d_cw.as<float>(), tr, it, level(in), nxz, ldz, a.kdt2,
-                           a.div, a.taper_x, a.taper_z, false, st);
+            step(level(ip), level(ic), level(in), it, inj, tr);
             const int t = ip;
             ip = ic;
             ic = in;
             in = t;
         }
         if (!adjoint) {
-            if (keep && (nt - 1) % sub == 0)
-                HIPCHECK(hipMemcpyAsync(snap((nt - 1) / sub), level(ic), ss * sizeof(float), hipMemcpyDeviceToDevice,
-                                        st));
+            capture(nt - 1);
             if (rcv) launch_aextract(level(ic), nxz, ldz, d_ext.as<PointEntry>(), nr, rcv, nt - 1, st);
         } else {
             image(level(ic), 0);
@@ -281,10 +377,12 @@ struct Acoustic {
         a.mode = (idx > 0 && idx < last) ? 0 : (idx == 0 ? 1 : 2);
         a.wk = (float)(w / dtg);
         a.wr = (float)(idx == 0 ? w * 0.5 * dtg : w * dtg);
+        const int j = whole ? 0 : idx / seg_K;
+        if (!whole) load_segment(j);
         a.curr = adj;
-        a.f0 = snap(idx);
-        a.fm = a.mode != 1 ? snap(idx - 1) : nullptr;
-        a.fp = a.mode != 2 ? snap(idx + 1) : nullptr;
+        a.f0 = snapshot(j, idx);
+        a.fm = a.mode != 1 ? snapshot(j, idx - 1) : nullptr;
+        a.fp = a.mode != 2 ? snapshot(j, idx + 1) : nullptr;
         a.fshot_stride = nxz;
         a.grad = d_gslab.as<float>();
         a.gslab_stride = 2 * nxz;
@@ -344,8 +442,21 @@ struct Acoustic {
             drcv = d_rcv.as<float>();
         }
         if (keep) {
-            d_full.reserve((size_t)nsnap() * ns * nxz * sizeof(float));
-            for (int idx = (nt - 1) / g.sub + 1; idx < nsnap(); idx++) launch_fill(snap(idx), (long long)ns * nxz, 0.f, st);
+            plan_segments();
+            const long long ss = (long long)ns * nxz;
+            if (whole) {
+                d_full.reserve((size_t)nsnap() * ss * sizeof(float));
+                for (int idx = (nt - 1) / g.sub + 1; idx < nsnap(); idx++) launch_fill(snap(idx), ss, 0.f, st);
+            } else {
+                d_ckpt.reserve((size_t)std::max(2 * (nseg - 1), 1) * ss * sizeof(float));
+                d_seg.reserve((size_t)(seg_K + 2) * ss * sizeof(float));
+                const int jl = nseg - 1;
+                launch_fill(slot(jl, seg_base(jl)), (long long)(seg_last(jl) - seg_base(jl) + 1) * ss, 0.f, st);
+                // the recomputation re-runs the sources: keep their traces
+                d_fsrc.reserve((size_t)ns * nt * sizeof(float));
+                HIPCHECK(hipMemcpyAsync(d_fsrc.p, dsrc, (size_t)ns * nt * sizeof(float), hipMemcpyDeviceToDevice, st));
+                seg_loaded = jl;
+            }
         }
         propagate(false, inj_src, dsrc, nr > 0 ? drcv : nullptr, keep);
         if (!on_device && nr > 0)
@@ -468,6 +579,35 @@ int fwi2d_acoustic_adjoint(fwi2d_acoustic* e, const float* adj, float* grad, int
         HIPCHECK(hipSetDevice(e->impl->device));
         e->impl->adjoint(adj, grad, on_device != 0, on_device != 0);
     });
+}
+
+int fwi2d_acoustic_set_option(fwi2d_acoustic* e, const char* key, long long value)
+{
+    return fwi2d::guarded_ac([&] {
+        if (!e || !key) throw std::runtime_error("null argument");
+        const std::string k = key;
+        if (k == "snapshot_budget") {
+            if (value < 0) throw std::runtime_error("snapshot_budget must be >= 0 (MiB)");
+            e->impl->snapshot_budget_mib = value;
+            e->impl->snap_epoch = -1;  // the next forward run plans the snapshots again
+        } else {
+            throw std::runtime_error("unknown acoustic engine option: " + k);
+        }
+    });
+}
+
+long long fwi2d_acoustic_get_stat(fwi2d_acoustic* e, const char* key)
+{
+    if (!e || !key) return -1;
+    const std::string k = key;
+    const fwi2d::Acoustic& a = *e->impl;
+    if (k == "recompute_steps") return a.stat_recompute_steps;
+    if (k == "whole_snapshots") return a.whole ? 1 : 0;
+    if (k == "segment") return a.seg_K;
+    if (k == "nseg") return a.nseg;
+    if (k == "snapshot_budget") return a.snapshot_budget_mib;
+    if (k == "shots_per_block") return a.base_args().spb;
+    return -1;
 }
 
 void fwi2d_acoustic_destroy(fwi2d_acoustic* e)

@@ -51,7 +51,7 @@ EXPORTS = [
     "fwi2d_filter_apply", "fwi2d_dot_f64_noalloc", "fwi2d_read_header", "fwi2d_read_data", "fwi2d_write",
     "fwi2d_read_coord", "fwi2d_regular_geometry", "fwi2d_engine_set_acquisition", "fwi2d_acoustic_create",
     "fwi2d_acoustic_set_model", "fwi2d_acoustic_set_shots", "fwi2d_acoustic_forward", "fwi2d_acoustic_adjoint",
-    "fwi2d_acoustic_destroy",
+    "fwi2d_acoustic_set_option", "fwi2d_acoustic_get_stat", "fwi2d_acoustic_destroy",
 ]
 
 FORMAT_SEP, FORMAT_BINARY = 0, 1
@@ -110,6 +110,9 @@ def load():
     lib.fwi2d_acoustic_set_shots.argtypes = [P, C.c_int, P, P, P, C.c_double, C.c_double]
     lib.fwi2d_acoustic_forward.argtypes = [P, P, P, C.c_int, C.c_int]
     lib.fwi2d_acoustic_adjoint.argtypes = [P, P, P, C.c_int]
+    lib.fwi2d_acoustic_set_option.argtypes = [P, C.c_char_p, C.c_longlong]
+    lib.fwi2d_acoustic_get_stat.argtypes = [P, C.c_char_p]
+    lib.fwi2d_acoustic_get_stat.restype = C.c_longlong
     lib.fwi2d_acoustic_destroy.argtypes = [P]
     lib.fwi2d_op_create.argtypes = [C.POINTER(Fwi2dOpDesc), P, C.POINTER(P)]
     lib.fwi2d_op_apply_forward.argtypes = [P, C.c_int, P, P]
@@ -208,6 +211,12 @@ class AcousticEngine:
         g = np.zeros((2, self.grid.nx, self.grid.nz), np.float32)
         _check(load().fwi2d_acoustic_adjoint(self._h, _ptr(a), _ptr(g), 0))
         return g
+
+    def set_option(self, key, value):
+        _check(load().fwi2d_acoustic_set_option(self._h, key.encode(), int(value)))
+
+    def stat(self, key):
+        return load().fwi2d_acoustic_get_stat(self._h, key.encode())
 
     def forward_device(self, src_ptr, rcv_ptr, keep_snapshots=False):
         """Device pointers: src [ns][nt] -> rcv [nr][nt]."""

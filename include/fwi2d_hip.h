@@ -193,9 +193,11 @@ int fwi2d_acoustic_set_shots(fwi2d_acoustic* e, int nshots, const double* sxz, c
                              double ox, double oz);
 
 /* propagate(adj = false) of every shot: source traces src [nshots][nt], receiver traces
-   rcv [nrec total][nt]; keep_snapshots: keep every sub-th level (1 + nt/sub levels per
-   shot) on the device for fwi2d_acoustic_adjoint.  on_device: src and rcv are device
-   pointers. */
+   rcv [nrec total][nt]; keep_snapshots: keep what fwi2d_acoustic_adjoint needs of the
+   1 + nt/sub snapshots per shot (we_op.cpp:923) on the device: all of them when they fit
+   the "snapshot_budget" option, otherwise checkpoints of segments of about sqrt(2 nsnap)
+   snapshots, from which the adjoint run recomputes them (one more forward run's steps;
+   the source traces are copied for it).  on_device: src and rcv are device pointers. */
 int fwi2d_acoustic_forward(fwi2d_acoustic* e, const float* src, float* rcv, int keep_snapshots, int on_device);
 
 /* propagate(adj = true) with compute_gradients (we_op.cpp:1883-1935): adjoint sources adj
@@ -203,6 +205,14 @@ int fwi2d_acoustic_forward(fwi2d_acoustic* e, const float* src, float* rcv, int 
    1/rho gradients summed over the shots (before the H quadrature and the chain rule of
    apply_jacobianT, we_op.cpp:1192-1230). */
 int fwi2d_acoustic_adjoint(fwi2d_acoustic* e, const float* adj, float* grad, int on_device);
+
+/* Options (no counterpart in the reference, which keeps every snapshot on the host):
+   "snapshot_budget" (MiB, default 32768): the largest snapshot history kept whole; 0
+   always checkpoints.  Statistics: "recompute_steps" (total), "whole_snapshots" (1 or 0,
+   the last forward run's choice), "segment", "nseg", "snapshot_budget",
+   "shots_per_block" (of the step kernel); -1 for an unknown key. */
+int fwi2d_acoustic_set_option(fwi2d_acoustic* e, const char* key, long long value);
+long long fwi2d_acoustic_get_stat(fwi2d_acoustic* e, const char* key);
 
 void fwi2d_acoustic_destroy(fwi2d_acoustic* e);
 

@@ -203,3 +203,32 @@ def test_acoustic_shot_groups_and_interior_tiles(lib, orc64):
     print("shot 64 of 65: traces rel err %.3e" % err)
     assert err <= 2e-5
     e.close()
+
+
+@pytest.mark.parametrize("nt,sub", [(400, 1), (400, 2), (400, 3), (401, 4), (72, 2)])
+def test_acoustic_checkpointed_snapshots_equal_the_whole_history(lib, nt, sub):
+    """With the snapshot budget at 0 the forward run keeps checkpoints of segments of about
+    sqrt(2 nsnap) snapshots and the adjoint run recomputes them; the gradient equals the
+    one imaged from the whole history bit for bit, including nt a multiple of sub (the
+    reference's zero last slot, at 72 steps and sub 2 alone in the last segment)."""
+    grid, model, sxz, rxz, src = case(nx=70, nz=60, nt=nt, sub=sub, ns=3, seed=sub)
+    nr = sum(len(r) for r in rxz)
+    adj = np.random.default_rng(nt + sub).standard_normal((nr, nt)).astype(np.float32)
+    grads, traces = [], []
+    for budget in (1 << 20, 0):
+        e = engine(lib, grid, model, sxz, rxz)
+        e.set_option("snapshot_budget", budget)
+        traces.append(e.forward(src, keep_snapshots=True))
+        grads.append(e.adjoint(adj))
+        assert e.stat("whole_snapshots") == (1 if budget else 0)
+        n = 1 + nt // sub
+        K = e.stat("segment")
+        if budget == 0:
+            # every segment but the last is recomputed once, from one snapshot before its
+            # own to one after: about one forward run
+            assert 0 < e.stat("recompute_steps") <= nt + 2 * sub * e.stat("nseg")
+            assert e.stat("nseg") == (n + K - 1) // K and K * K >= 2 * n
+        e.close()
+    assert np.array_equal(traces[0], traces[1])
+    assert np.array_equal(grads[0], grads[1])
+    assert np.abs(grads[0]).max() > 0
