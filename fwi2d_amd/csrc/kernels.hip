@@ -2019,6 +2019,15 @@ __device__ __forceinline__ void astep_tile(const AcousticArgs& a, int tix, int t
         }
     }
     const bool inx = gx >= 6 && gx < nx - 6;
+    // bulk moduli of the boundary cells the absorbing SAT terms of this quad read
+    // (the side column's cells for a quad in column 0 or nx - 1)
+    float kbt = 0.f, kbb = 0.f, kbs[4] = {0.f, 0.f, 0.f, 0.f};
+    if (EDGE && active) {
+        const long long c0 = (long long)gx * ldz;
+        if (z0 < 4) kbt = a.K[c0];
+        if (z0 + 4 > nz - 4 && z0 < nz) kbb = a.K[c0 + nz - 1];
+        if (gx == 0 || gx == nx - 1) unpack(ld4(a.K + c0 + z0), kbs);
+    }
     for (int sh = s0; sh < s1; sh++) {
         const int b = (sh - s0) & 1;
         const long long so = (long long)sh * a.shot_stride;
@@ -2073,25 +2082,23 @@ __device__ __forceinline__ void astep_tile(const AcousticArgs& a, int tix, int t
                         float rhs = lx * a.rdx2 + lz * a.rdz2;
                         if (gz < 4) {  // top side, boundary cell (gx, 0)
                             const int bb = col - wsz;
-                            rhs += ac_sat(a.bc[0], gz, &U[bb], &sR[bb], 1, p, gz == 0 ? a.K[g - z0] : 0.f, a.dir_z,
-                                          a.pen_z, a.neu_z, a.abs_z);
+                            rhs += ac_sat(a.bc[0], gz, &U[bb], &sR[bb], 1, p, kbt, a.dir_z, a.pen_z, a.neu_z,
+                                          a.abs_z);
                         }
                         if (gz >= nz - 4) {  // bottom side, boundary cell (gx, nz - 1)
                             const int bb = col + (nz - 1 - wsz);
-                            rhs += ac_sat(a.bc[1], nz - 1 - gz, &U[bb], &sR[bb], -1, p,
-                                          gz == nz - 1 ? a.K[(long long)gx * ldz + nz - 1] : 0.f, a.dir_z, a.pen_z,
+                            rhs += ac_sat(a.bc[1], nz - 1 - gz, &U[bb], &sR[bb], -1, p, kbb, a.dir_z, a.pen_z,
                                           a.neu_z, a.abs_z);
                         }
                         if (gx < 4) {  // left side, boundary cell (0, gz)
                             const int bb = (0 - wsx) * kAcWZ + (gz - wsz);
-                            rhs += ac_sat(a.bc[2], gx, &U[bb], &sR[bb], kAcWZ, p, gx == 0 ? a.K[gz] : 0.f, a.dir_x,
+                            rhs += ac_sat(a.bc[2], gx, &U[bb], &sR[bb], kAcWZ, p, sel4(kbs, k), a.dir_x,
                                           a.pen_x, a.neu_x, a.abs_x);
                         }
                         if (gx >= nx - 4) {  // right side, boundary cell (nx - 1, gz)
                             const int bb = (nx - 1 - wsx) * kAcWZ + (gz - wsz);
                             rhs += ac_sat(a.bc[3], nx - 1 - gx, &U[bb], &sR[bb], -kAcWZ, p,
-                                          gx == nx - 1 ? a.K[(long long)(nx - 1) * ldz + gz] : 0.f, a.dir_x, a.pen_x,
-                                          a.neu_x, a.abs_x);
+                                          sel4(kbs, k), a.dir_x, a.pen_x, a.neu_x, a.abs_x);
                         }
                         // leapfrog (we_op.cpp:2093-2098), boundary divisor (asat_scale_boundaries), taper
                         v = (sel4(kd, k) * rhs + (2.f * pu[0] - p)) / sel4(dv, k);
