@@ -345,8 +345,9 @@ class WeOp:
         d.nrec = nrec.ctypes.data_as(C.POINTER(C.c_int))
         d.rxz = rxz.ctypes.data_as(C.POINTER(C.c_double))
         self.nr = int(nrec.sum())
-        d.mt = int(prob.get("mt", 0))
-        d.gl = float(prob.get("gl", 0.0))
+        # the acoustic operator has point sources and receivers only (we_op.cpp:13-20)
+        d.mt = 0 if self.acoustic else int(prob.get("mt", 0))
+        d.gl = 0.0 if self.acoustic else float(prob.get("gl", 0.0))
         if d.gl > 0:
             self._rdip = np.ascontiguousarray(np.concatenate([np.asarray(r, np.float64) for r in prob["rdip"]]))
             assert self._rdip.size == self.nr

@@ -91,3 +91,44 @@ def test_das_and_moment_tensor_arguments_are_checked():
     with pytest.raises(AssertionError):
         lib.WeOp(dict(prob, mt=1), W.vertical_force(w, 1))  # [2][ns][nt] given, [3][ns][nt] expected
     lib.WeOp(dict(prob, mt=1), np.stack([w[None], w[None], 0 * w[None]])).close()
+
+
+def test_acoustic_operator_arguments():
+    """The acoustic operator (nparam 2, nl_we_op_a): one source trace per shot
+    (analyzeWavelet's acoustic copy, we_op.cpp:178-194) and one data component; the
+    objective's data and gradient mask take those shapes; moment tensors and DAS are
+    switched off as we_op.cpp:13-20 does; parameter counts other than 2, 3 and 5 fail."""
+    from fwi2d_amd import lib
+    nx, nz, h = 40, 30, 0.01
+    rx = np.stack([np.arange(2, nx - 2, 4) * h, np.full(len(range(2, nx - 2, 4)), 0.05)], 1)
+    prob = W.base_problem(nx, nz, h, h, 101, 0.002, [[0.1, 0.1], [0.2, 0.1]], [rx, rx], taper=5, dt=0.001, sub=-1)
+    prob["acoustic"] = 1
+    w = W.ricker(101, 0.002, 12.0)
+    src = np.stack([w, w])[None].astype(np.float32)
+    with pytest.raises(AssertionError):
+        lib.WeOp(prob, W.vertical_force(w, 2))  # [2][ns][nt]: a force, not an acoustic source
+    op = lib.WeOp(dict(prob, mt=1, gl=0.05), src)  # mt and gl ignored, as the reference does
+    assert op.np == 2 and op.ncd == 1
+    d = np.random.default_rng(1).standard_normal((1, op.nr, op.nt_data)).astype(np.float32)
+    assert lib.Nlls(op, d, normalize=True).dnorm() == pytest.approx(0.002 * np.sum(d.astype(np.float64) ** 2),
+                                                                    rel=1e-12)
+    with pytest.raises(AssertionError):
+        lib.Nlls(op, np.zeros((2, op.nr, op.nt_data), np.float32))
+    mask = np.ones((2, nx, nz), np.float32)
+    mask[1, 3, 3] = -1.0
+    with pytest.raises(lib.Fwi2dError, match="Gradient mask must be non-negative"):
+        lib.Nlls(op, d, gmask=mask)
+    op.close()
+    d4 = lib.op_desc_from(prob)
+    d4.nparam = 4
+    sxz = np.ascontiguousarray(prob["sxz"])
+    nrec = np.array([len(rx), len(rx)], np.int32)
+    rxz = np.ascontiguousarray(np.concatenate([rx, rx]))
+    d4.ns = 2
+    d4.sxz = sxz.ctypes.data_as(lib.C.POINTER(lib.C.c_double))
+    d4.nrec = nrec.ctypes.data_as(lib.C.POINTER(lib.C.c_int))
+    d4.rxz = rxz.ctypes.data_as(lib.C.POINTER(lib.C.c_double))
+    h4 = lib.C.c_void_p()
+    L = lib.load()
+    assert L.fwi2d_op_create(lib.C.byref(d4), src.ctypes.data_as(lib.C.c_void_p), lib.C.byref(h4)) != 0
+    assert b"two (Vp, rho), three (Vp, Vs, rho) or five" in L.fwi2d_last_error()
