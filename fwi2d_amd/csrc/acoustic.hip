@@ -479,11 +479,12 @@ struct Acoustic {
             up(d_adj, adj, (size_t)nr * nt * sizeof(float));
             dadj = d_adj.as<float>();
         }
-        d_gslab.reserve((size_t)ns * 2 * nxz * sizeof(float));
-        launch_fill(d_gslab.as<float>(), (long long)ns * 2 * nxz, 0.f, st);
+        const int groups = (ns + base_args().spb - 1) / base_args().spb;  // one slab per shot group
+        d_gslab.reserve((size_t)groups * 2 * nxz * sizeof(float));
+        launch_fill(d_gslab.as<float>(), (long long)groups * 2 * nxz, 0.f, st);
         propagate(true, inj_rcv, dadj, nullptr, false);
         d_gout.reserve((size_t)2 * nxz * sizeof(float));
-        launch_sum_shots(d_gslab.as<float>(), 2 * nxz, ns, 2 * nxz, d_gout.as<float>(), 0, st);
+        launch_sum_shots(d_gslab.as<float>(), 2 * nxz, groups, 2 * nxz, d_gout.as<float>(), 0, st);
         // [2][nx][ldz] -> [2][nx][nz]
         HIPCHECK(hipMemcpy2DAsync(grad, (size_t)g.nz * sizeof(float), d_gout.p, (size_t)ldz * sizeof(float),
                                   (size_t)g.nz * sizeof(float), (size_t)2 * g.nx,

@@ -178,7 +178,7 @@ struct AcousticArgs {
     // imaging (k_agrad): adjoint level curr against snapshots f0 (idx), fm, fp
     const float *f0, *fm, *fp;
     long long fshot_stride;
-    float* grad;                 // per-shot slabs [nshots][2][nx][ldz]
+    float* grad;                 // per shot-group slabs [groups][2][nx][ldz] (groups of spb shots)
     long long gslab_stride;
     int mode;                    // 0 interior snapshot, 1 the first, 2 the last
     float wk;                    // K term weight w / dtg

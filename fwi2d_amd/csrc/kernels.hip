@@ -1903,32 +1903,18 @@ void launch_reduce_f64(const double* partial, int n, double* out, hipStream_t s)
 // ---------------------------------------------------------------------------
 // Acoustic operator (nl_we_op_a::propagate, we_op.cpp:1937-2135; the SAT terms and their
 // scaling, spatial_operators.cpp:551-860).  A workgroup owns a 16 (x) x 64 (z) tile of
-// one shot; the level and 1/rho windows (halo 8, which holds every closure row a tile
-// can need) go through LDS, and each thread forms the new level of one quad (column-
-// major, so a wavefront covers 4 whole columns): D2x + D2z of the 1/rho-weighted
-// Laplacian (interior weights or closure rows per axis), the SAT terms of the boundary
-// cells, the leapfrog with dt^2 K, the absorbing boundary divisor and the cos^2 taper.
-// The reference tapers the current level in place after each step; here the previous
-// level is tapered on load instead (taper_prev), which gives the same values.
+// a group of shots; the level and 1/rho windows go through LDS (8 rows above and below
+// the tile, which holds every closure row a tile can need; the columns as below), and
+// each thread forms the new level of one quad (column-major, so a wavefront covers 4
+// whole columns): D2x + D2z of the 1/rho-weighted Laplacian (interior weights or closure
+// rows per axis), the SAT terms of the boundary cells, the leapfrog with dt^2 K, the
+// absorbing boundary divisor and the cos^2 taper.  The reference tapers the current
+// level in place after each step; here the previous level is tapered on load instead
+// (taper_prev), which gives the same values.
 // ---------------------------------------------------------------------------
 constexpr int kAcH = 8;
-constexpr int kAcWX = kAcTX + 2 * kAcH, kAcWZ = kAcTZ + 2 * kAcH, kAcWQ = kAcWZ / 4;
+constexpr int kAcWZ = kAcTZ + 2 * kAcH, kAcWQ = kAcWZ / 4;
 static_assert(kAcTX * (kAcTZ / 4) == kThreads, "one quad per thread");
-
-// the level and 1/rho windows of tile (sx, sz) of one shot into LDS
-__device__ __forceinline__ void ac_windows(const float* u, const float* c, float* sU, float* sC, int wsx, int wsz,
-                                           int nx, int ldz)
-{
-    for (int i = threadIdx.x; i < kAcWX * kAcWQ; i += kThreads) {
-        const int lx = i / kAcWQ, lq = i - lx * kAcWQ;
-        const int gx = wsx + lx, gz = wsz + 4 * lq;
-        const bool ok = gx >= 0 && gx < nx && gz >= 0 && gz < ldz;
-        const long long g = ok ? (long long)gx * ldz + gz : 0;
-        const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
-        st4(&sU[lx * kAcWZ + 4 * lq], ok ? ld4(u + g) : z4);
-        if (c) st4(&sC[lx * kAcWZ + 4 * lq], ok ? ld4(c + g) : z4);
-    }
-}
 
 // SAT terms of one side at a cell k cells in from it (0 <= k < 4), added to rhs: pu / pc
 // point at the side's boundary cell of the cell's line in the level / 1/rho windows, st
@@ -2156,47 +2142,73 @@ __global__ void __launch_bounds__(kThreads) k_astep(const AcousticArgs a)
 
 // nl_we_op_a::compute_gradients (we_op.cpp:1883-1935) for every shot: the K term from the
 // second time difference of the snapshots against the adjoint level, the 1/rho term
-// from first derivatives of both; added to the shot's gradient slab
+// from first derivatives of both.  A workgroup images one tile of a.spb consecutive
+// shots (the step kernel's groups and windows, adjoint level and snapshot windows double
+// buffered), sums their terms in shot order in registers and adds the sum to the group's
+// gradient slab once.
 __global__ void __launch_bounds__(kThreads) k_agrad(const AcousticArgs a)
 {
-    __shared__ __attribute__((aligned(16))) float sA[kAcWX * kAcWZ];
-    __shared__ __attribute__((aligned(16))) float sF[kAcWX * kAcWZ];
-    const int tile = blockIdx.x, shot = blockIdx.y;
+    __shared__ __attribute__((aligned(16))) float sA[2][kAcSWX * kAcWZ];
+    __shared__ __attribute__((aligned(16))) float sF[2][kAcSWX * kAcWZ];
+    const int ntile = a.ntx * a.ntz;
+    const int lin = xcd_remap(blockIdx.x, gridDim.x);
+    const int grp = lin / ntile, tile = lin - grp * ntile;
     const int tix = tile / a.ntz, tiz = tile - tix * a.ntz;
     const int sx = tix * kAcTX, sz = tiz * kAcTZ;
-    const int ex = min(sx + kAcTX, a.nx), ez = min(sz + kAcTZ, a.ldz);
-    const int wsx = sx - kAcH, wsz = sz - kAcH;
     const int nx = a.nx, nz = a.nz, ldz = a.ldz;
-    const long long so = (long long)shot * a.shot_stride, fo = (long long)shot * a.fshot_stride;
-    ac_windows(a.curr + so, a.f0 + fo, sA, sF, wsx, wsz, nx, ldz);
-    __syncthreads();
+    const int ex = min(sx + kAcTX, nx), ez = min(sz + kAcTZ, ldz);
+    // the step's window: D1 rows read +-2 points inside, the closure rows the 6 points
+    // next to a side
+    const int wsx = min(sx - kAcSHL, nx - 8), wsz = sz - kAcH;
+    const int s0 = grp * a.spb, s1 = min(s0 + a.spb, a.nshots);
+    ac_window_x(a.curr + (long long)s0 * a.shot_stride, sA[0], wsx, wsz, nx, ldz);
+    ac_window_x(a.f0 + (long long)s0 * a.fshot_stride, sF[0], wsx, wsz, nx, ldz);
     const int q = threadIdx.x;
     const int gx = sx + q / (kAcTZ / 4), z0 = sz + 4 * (q % (kAcTZ / 4));
-    if (gx >= ex || z0 >= ez) return;
-    const long long g = (long long)gx * ldz + z0;
-    float K[4], fm[4] = {0.f, 0.f, 0.f, 0.f}, fp[4] = {0.f, 0.f, 0.f, 0.f}, gk[4], gr[4];
+    const bool active = gx < ex && z0 < ez;
+    const long long g = active ? (long long)gx * ldz + z0 : 0;
+    float K[4], gk[4], gr[4];
     unpack(ld4(a.K + g), K);
-    if (a.mode != 1) unpack(ld4(a.fm + fo + g), fm);
-    if (a.mode != 2) unpack(ld4(a.fp + fo + g), fp);
-    float* G = a.grad + (long long)shot * a.gslab_stride + g;
+    float* G = a.grad + (long long)grp * a.gslab_stride + g;
     unpack(ld4(G), gk);
     unpack(ld4(G + a.shot_stride), gr);
-    const int col = (gx - wsx) * kAcWZ;
+    float kw[4];  // the K term's weight w / (dtg K^2)
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-        const int gz = z0 + k;
-        if (gz >= nz) continue;
-        const int w = col + (gz - wsz);
-        const float* pa = &sA[w];
-        const float* pf = &sF[w];
-        const float ax = d1<false>(pa, kAcWZ, gx, nx) * a.rdx, az = d1<false>(pa, 1, gz, nz) * a.rdz;
-        const float fx = d1<false>(pf, kAcWZ, gx, nx) * a.rdx, fz = d1<false>(pf, 1, gz, nz) * a.rdz;
-        const float d2 = a.mode == 0 ? fp[k] - 2.f * pf[0] + fm[k] : (a.mode == 1 ? fp[k] - pf[0] : -pf[0] + fm[k]);
-        gk[k] -= a.wk / (K[k] * K[k]) * pa[0] * d2;
-        gr[k] += a.wr * (ax * fx + az * fz);
+    for (int k = 0; k < 4; k++) kw[k] = active ? a.wk / (K[k] * K[k]) : 0.f;
+    const int col = (active ? gx - wsx : 2) * kAcWZ;
+    const int zw = (active ? z0 : sz) - wsz;
+    __syncthreads();
+    for (int sh = s0; sh < s1; sh++) {
+        const int b = (sh - s0) & 1;
+        const long long fo = (long long)sh * a.fshot_stride;
+        if (sh + 1 < s1) {
+            ac_window_x(a.curr + (long long)(sh + 1) * a.shot_stride, sA[b ^ 1], wsx, wsz, nx, ldz);
+            ac_window_x(a.f0 + fo + a.fshot_stride, sF[b ^ 1], wsx, wsz, nx, ldz);
+        }
+        if (active) {
+            float fm[4] = {0.f, 0.f, 0.f, 0.f}, fp[4] = {0.f, 0.f, 0.f, 0.f};
+            if (a.mode != 1) unpack(ld4(a.fm + fo + g), fm);
+            if (a.mode != 2) unpack(ld4(a.fp + fo + g), fp);
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const int gz = z0 + k;
+                if (gz >= nz) continue;
+                const float* pa = &sA[b][col + zw + k];
+                const float* pf = &sF[b][col + zw + k];
+                const float ax = d1<false>(pa, kAcWZ, gx, nx) * a.rdx, az = d1<false>(pa, 1, gz, nz) * a.rdz;
+                const float fx = d1<false>(pf, kAcWZ, gx, nx) * a.rdx, fz = d1<false>(pf, 1, gz, nz) * a.rdz;
+                const float d2 =
+                    a.mode == 0 ? fp[k] - 2.f * pf[0] + fm[k] : (a.mode == 1 ? fp[k] - pf[0] : -pf[0] + fm[k]);
+                gk[k] -= kw[k] * pa[0] * d2;
+                gr[k] += a.wr * (ax * fx + az * fz);
+            }
+        }
+        __syncthreads();  // the next shot's windows are in; this shot's are read
     }
-    st4(G, make_float4(gk[0], gk[1], gk[2], gk[3]));
-    st4(G + a.shot_stride, make_float4(gr[0], gr[1], gr[2], gr[3]));
+    if (active) {
+        st4(G, make_float4(gk[0], gk[1], gk[2], gk[3]));
+        st4(G + a.shot_stride, make_float4(gr[0], gr[1], gr[2], gr[3]));
+    }
 }
 
 // point sources of the acoustic operator: per touched cell, the sum of its contributions
@@ -2240,7 +2252,8 @@ void launch_astep(const AcousticArgs& a, hipStream_t s)
 
 void launch_agrad(const AcousticArgs& a, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_agrad, dim3(a.ntx * a.ntz, a.nshots), dim3(kThreads), 0, s, a);
+    const int groups = (a.nshots + a.spb - 1) / a.spb;
+    hipLaunchKernelGGL(k_agrad, dim3(a.ntx * a.ntz * groups), dim3(kThreads), 0, s, a);
 }
 
 void launch_ainject(const InjCell* cells, int ncell, const int* ptr, const long long* off, const float* w,
