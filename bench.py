@@ -104,14 +104,15 @@ def parse():
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_step_iso_64shots_500x1700_traffic.md")
+ACOUSTIC_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_astep_acoustic_64shots_500x1700.md")
 
 
-def measured_traffic(spb=None):
+def measured_traffic(spb=None, path=TRAFFIC_FILE):
     """HBM bytes per forward step from the committed counter summary (FETCH_SIZE and
     WRITE_SIZE per dispatch, calibrated against a copy kernel of known size) for the
     core launch's shots per workgroup spb (the autotuner's choice), or None."""
     try:
-        lines = open(TRAFFIC_FILE).read().splitlines()
+        lines = open(path).read().splitlines()
     except OSError:
         return None
     for key in (["traffic_bytes_per_launch_spb%d:" % spb] if spb else []) + ["traffic_bytes_per_launch:"]:
@@ -536,7 +537,8 @@ def acoustic_run(args):
             "grad_err_sample": "1 shot, 160x100 grid, nt=600, sub=2, raw K and 1/rho gradients vs FP64 oracle",
             "roofline": {"bound": "hbm", "kernel": "k_astep + k_ainject per step (eager forward, no snapshots)",
                          "achieved": round(fwd_bytes / step_s / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": round(fwd_bytes / step_s / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
+                         "frac": round(fwd_bytes / step_s / 1e9 / HBM_PEAK_GBS, 4),
+                         "traffic": measured_traffic(path=ACOUSTIC_TRAFFIC_FILE) if ns == 64 else None,
                          "avg_step_us": round(step_s * 1e6, 2),
                          "bytes_per_step": fwd_bytes,
                          "gradient_achieved": round(grad_bytes * args.steps / el / 1e9, 1),
