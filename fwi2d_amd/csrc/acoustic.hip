@@ -3,7 +3,7 @@
 //
 // One pressure-like field per shot, [shot][nx][ldz] with ldz = nz rounded up to a quad;
 // model (K, 1/rho) = (rho vp^2, 1/rho) as convert_model (we_op.cpp:1863-1881) produces
-// it.  A time step is one launch over 16 x 64 tiles of every shot (k_astep), then the
+// it.  A time step is one launch over 32 x 32 tiles of every shot (k_astep), then the
 // point sources (k_ainject).  The forward run keeps every sub-th level on the device (the
 // reference's full wavefield) when that history fits the snapshot budget, and otherwise
 // checkpoints of segments of about sqrt(2 nsnap) snapshots from which the adjoint run
@@ -46,7 +46,7 @@ struct Acoustic {
     // snapshot segments: checkpoints, the segment buffer, recompute levels, the forward
     // run's source traces
     DevMem d_ckpt, d_seg, d_rlev, d_fsrc;
-    long long snapshot_budget_mib = 32768;  // whole snapshot history up to this size
+    long long snapshot_budget_mib = 131072;  // whole snapshot history up to this size (128 GiB)
     bool whole = true;
     int seg_K = 0, nseg = 0, seg_loaded = -1;
     long long stat_recompute_steps = 0;

@@ -158,7 +158,7 @@ void launch_reduce_f64(const double* partial, int n, double* out, hipStream_t s)
 // ---------------------------------------------------------------------------
 // Acoustic operator (nl_we_op_a, we_op.cpp:1863-2135): one field per shot, [nshots][nx][ldz]
 // ---------------------------------------------------------------------------
-constexpr int kAcTX = 16, kAcTZ = 64;  // step and imaging tiles (x, z)
+constexpr int kAcTX = 32, kAcTZ = 32;  // step and imaging tiles (x, z)
 
 struct AcousticArgs {
     int nx, nz, ldz, nshots, ntx, ntz, taper_prev;

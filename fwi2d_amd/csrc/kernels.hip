@@ -1902,10 +1902,10 @@ void launch_reduce_f64(const double* partial, int n, double* out, hipStream_t s)
 
 // ---------------------------------------------------------------------------
 // Acoustic operator (nl_we_op_a::propagate, we_op.cpp:1937-2135; the SAT terms and their
-// scaling, spatial_operators.cpp:551-860).  A workgroup owns a 16 (x) x 64 (z) tile of
+// scaling, spatial_operators.cpp:551-860).  A workgroup owns a 32 (x) x 32 (z) tile of
 // a group of shots; the level and 1/rho windows go through LDS (8 rows above and below
 // the tile, which holds every closure row a tile can need; the columns as below), and
-// each thread forms the new level of one quad (column-major, so a wavefront covers 4
+// each thread forms the new level of one quad (column-major, so a wavefront covers 8
 // whole columns): D2x + D2z of the 1/rho-weighted Laplacian (interior weights or closure
 // rows per axis), the SAT terms of the boundary cells, the leapfrog with dt^2 K, the
 // absorbing boundary divisor and the cos^2 taper.  The reference tapers the current
@@ -1959,7 +1959,7 @@ __device__ __forceinline__ void ac_window_x(const float* u, float* sU, int wsx, 
     }
 }
 
-// A workgroup steps one 16 x 64 tile of a.spb consecutive shots: the 1/rho window, dt^2 K,
+// A workgroup steps one 32 x 32 tile of a.spb consecutive shots: the 1/rho window, dt^2 K,
 // the boundary divisor and the tapers are loaded once (and, for tiles of interior cells
 // only, the D2 weights of the cells formed once); the shots' level windows alternate
 // between two LDS buffers, the next shot's loaded while the current one is computed (one
@@ -1983,7 +1983,7 @@ __device__ __forceinline__ void astep_tile(const AcousticArgs& a, int tix, int t
         st4(&sR[lx * kAcWZ + 4 * lq], ok ? ld4(a.ri + (long long)gx * ldz + gz) : make_float4(0.f, 0.f, 0.f, 0.f));
     }
     ac_window_x(a.curr + (long long)s0 * a.shot_stride, sU[0], wsx, wsz, nx, ldz);
-    const int q = threadIdx.x;  // quads column-major: a wavefront covers 4 whole columns
+    const int q = threadIdx.x;  // quads column-major: a wavefront covers 8 whole columns
     const int col_q = q / (kAcTZ / 4), row_q = q % (kAcTZ / 4);
     const int gx = sx + col_q, z0 = sz + 4 * row_q;
     const bool active = gx < ex && z0 < ez;

@@ -207,7 +207,7 @@ int fwi2d_acoustic_forward(fwi2d_acoustic* e, const float* src, float* rcv, int 
 int fwi2d_acoustic_adjoint(fwi2d_acoustic* e, const float* adj, float* grad, int on_device);
 
 /* Options (no counterpart in the reference, which keeps every snapshot on the host):
-   "snapshot_budget" (MiB, default 32768): the largest snapshot history kept whole; 0
+   "snapshot_budget" (MiB, default 131072): the largest snapshot history kept whole; 0
    always checkpoints.  Statistics: "recompute_steps" (total), "whole_snapshots" (1 or 0,
    the last forward run's choice), "segment", "nseg", "snapshot_budget",
    "shots_per_block" (of the step kernel); -1 for an unknown key. */
