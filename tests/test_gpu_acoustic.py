@@ -232,3 +232,49 @@ def test_acoustic_checkpointed_snapshots_equal_the_whole_history(lib, nt, sub):
     assert np.array_equal(traces[0], traces[1])
     assert np.array_equal(grads[0], grads[1])
     assert np.abs(grads[0]).max() > 0
+
+
+def test_acoustic_full_size_properties(lib):
+    """Size-independent properties of the acoustic engine on the 64-shot 500 x 1700 grid
+    of the batched benchmark (a short time axis): scaling in the source (powers of two
+    scale FP32 exactly away from the subnormal range: 1e-7), superposition of two source
+    sets, one shot alone equal to the batch bit for bit, scaling of the gradient in the
+    adjoint source, and the checkpointed gradient equal to the whole-history one."""
+    nz, nx, h, dt, nt, ns = 500, 1700, 0.01, 0.001, 240, 64
+    m = W.gradient_model(nx, nz, h, h, seed=0)
+    model = np.stack([m[2] * m[0] ** 2, 1 / m[2]]).astype(np.float32)
+    sxz, rxz, freqs = W.config2_geometry(ns, nx, h)
+    prob = W.base_problem(nx, nz, h, h, nt, dt, sxz, rxz, dt=dt, sub=1)
+    wav = np.stack([W.ricker(nt, dt, f) for f in freqs]).astype(np.float32)
+    s2 = (-0.5 * wav[:, ::-1]).copy()
+    e = lib.AcousticEngine(lib.grid_from(prob, nt, dt, 1))
+    e.set_model(model)
+    e.set_shots(sxz, rxz)
+    assert e.stat("shots_per_block") == 8  # the benchmark's shot groups
+    d1 = e.forward(wav)
+    err = rel(e.forward(2 * wav), 2 * d1.astype(np.float64))
+    print("scaling rel err %.2e" % err)
+    assert err < 1e-7
+    d2 = e.forward(s2)
+    err = rel(e.forward(wav + s2), d1.astype(np.float64) + d2)
+    print("superposition rel err %.2e" % err)
+    assert err < 1e-5
+    k = 37
+    one = lib.AcousticEngine(lib.grid_from(prob, nt, dt, 1))
+    one.set_model(model)
+    one.set_shots(sxz[k:k + 1], rxz[k:k + 1])
+    r0 = sum(len(r) for r in rxz[:k])
+    assert np.array_equal(one.forward(wav[k:k + 1]), d1[r0:r0 + len(rxz[k])])
+    one.close()
+    adj = (np.random.default_rng(4).standard_normal(d1.shape) * np.hanning(nt)).astype(np.float32)
+    e.forward(wav, keep_snapshots=True)
+    g1 = e.adjoint(adj)
+    err = rel(e.adjoint(2 * adj), 2 * g1.astype(np.float64))
+    print("gradient scaling rel err %.2e" % err)
+    assert err < 1e-7
+    assert np.all(np.isfinite(g1)) and np.linalg.norm(g1) > 0
+    e.set_option("snapshot_budget", 0)
+    e.forward(wav, keep_snapshots=True)
+    assert e.stat("whole_snapshots") == 0
+    assert np.array_equal(e.adjoint(adj), g1)
+    e.close()
