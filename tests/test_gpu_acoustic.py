@@ -278,3 +278,21 @@ def test_acoustic_full_size_properties(lib):
     assert e.stat("whole_snapshots") == 0
     assert np.array_equal(e.adjoint(adj), g1)
     e.close()
+
+
+def test_acoustic_shot_without_receivers_and_small_grids(lib, orc64):
+    """A shot with no receivers (a ragged acquisition) leaves the other shots' traces and
+    the gradient as the acquisition without it gives them; grids below 12 cells fail."""
+    grid, model, sxz, rxz, src = case(nx=80, nz=60, nt=300, ns=3, seed=7)
+    rxz0 = [rxz[0], np.zeros((0, 2)), rxz[2]]
+    e = engine(lib, grid, model, sxz, rxz0)
+    d = e.forward(src, keep_snapshots=True)
+    assert d.shape == (len(rxz[0]) + len(rxz[2]), 300)
+    ref = engine(lib, grid, model, sxz[[0, 2]], [rxz[0], rxz[2]])
+    assert np.array_equal(d, ref.forward(src[[0, 2]], keep_snapshots=True))
+    adj = np.random.default_rng(2).standard_normal(d.shape).astype(np.float32)
+    assert np.array_equal(e.adjoint(adj), ref.adjoint(adj))
+    e.close()
+    ref.close()
+    with pytest.raises(lib.Fwi2dError, match="nx, nz >= 12"):
+        engine(lib, dict(grid, nz=11), model[:, :, :11], sxz, rxz)
