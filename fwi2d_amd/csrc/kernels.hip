@@ -1959,17 +1959,60 @@ __device__ __forceinline__ void ac_window_x(const float* u, float* sU, int wsx, 
     }
 }
 
+// The new level of one frame cell with closure rows or SAT terms (the reference's
+// Dxx_var/Dzz_var closures and asat_* terms, then the leapfrog, the boundary divisor and
+// the taper), from the level window U and the 1/rho window R (w: the cell's window index)
+__device__ __forceinline__ float ac_frame_cell(const AcousticArgs& a, const float* U, const float* R, int w, int wsx,
+                                               int wsz, int gx, int gz, long long so)
+{
+    const int nx = a.nx, nz = a.nz, ldz = a.ldz;
+    const long long gi = (long long)gx * ldz + gz;
+    const float* pu = &U[w];
+    const float* pc = &R[w];
+    const float tzk = a.taper_z[gz], tx = a.taper_x[gx];
+    float p = a.prev[so + gi];
+    if (a.taper_prev) p = (p * tzk) * tx;
+    const float lx = (gx >= 6 && gx < nx - 6) ? d2_apply(d2_weights(pc, kAcWZ), pu, kAcWZ)
+                                              : d2_closure(pu, pc, kAcWZ, gx, nx);
+    const float lz = (gz >= 6 && gz < nz - 6) ? d2_apply(d2_weights(pc, 1), pu, 1) : d2_closure(pu, pc, 1, gz, nz);
+    float rhs = lx * a.rdx2 + lz * a.rdz2;
+    const int col = (gx - wsx) * kAcWZ;
+    if (gz < 4) {  // top side, boundary cell (gx, 0)
+        const int bb = col - wsz;
+        rhs += ac_sat(a.bc[0], gz, &U[bb], &R[bb], 1, p, gz == 0 ? a.K[gi - gz] : 0.f, a.dir_z, a.pen_z, a.neu_z,
+                      a.abs_z);
+    }
+    if (gz >= nz - 4) {  // bottom side, boundary cell (gx, nz - 1)
+        const int bb = col + (nz - 1 - wsz);
+        rhs += ac_sat(a.bc[1], nz - 1 - gz, &U[bb], &R[bb], -1, p, gz == nz - 1 ? a.K[gi] : 0.f, a.dir_z, a.pen_z,
+                      a.neu_z, a.abs_z);
+    }
+    if (gx < 4) {  // left side, boundary cell (0, gz)
+        const int bb = (0 - wsx) * kAcWZ + (gz - wsz);
+        rhs += ac_sat(a.bc[2], gx, &U[bb], &R[bb], kAcWZ, p, gx == 0 ? a.K[gi] : 0.f, a.dir_x, a.pen_x, a.neu_x,
+                      a.abs_x);
+    }
+    if (gx >= nx - 4) {  // right side, boundary cell (nx - 1, gz)
+        const int bb = (nx - 1 - wsx) * kAcWZ + (gz - wsz);
+        rhs += ac_sat(a.bc[3], nx - 1 - gx, &U[bb], &R[bb], -kAcWZ, p, gx == nx - 1 ? a.K[gi] : 0.f, a.dir_x,
+                      a.pen_x, a.neu_x, a.abs_x);
+    }
+    // leapfrog (we_op.cpp:2093-2098), boundary divisor (asat_scale_boundaries), taper
+    const float v = (a.kdt2[gi] * rhs + (2.f * pu[0] - p)) / a.div[gi];
+    return (v * tzk) * tx;
+}
+
 // A workgroup steps one 32 x 32 tile of a.spb consecutive shots: the 1/rho window, dt^2 K,
 // the boundary divisor and the tapers are loaded once (and, for tiles of interior cells
 // only, the D2 weights of the cells formed once); the shots' level windows alternate
 // between two LDS buffers, the next shot's loaded while the current one is computed (one
 // barrier per shot).  EDGE = false: a tile of the rectangle of interior tiles; EDGE = true:
-// a tile of the frame around it, whose quads of interior cells take the interior rows
-// (weights formed per shot) and whose other quads the closure rows and SAT terms, one
-// cell at a time.
+// a tile of the frame around it, whose interior cells take the interior rows (weights
+// formed per shot) and whose other cells the closure rows and SAT terms, dealt one cell
+// per thread.
 template <bool EDGE>
 __device__ __forceinline__ void astep_tile(const AcousticArgs& a, int tix, int tiz, int grp, float* sR,
-                                           float (*sU)[kAcSWX * kAcWZ])
+                                           float (*sU)[kAcSWX * kAcWZ], float* sO)
 {
     const int sx = tix * kAcTX, sz = tiz * kAcTZ;
     const int nx = a.nx, nz = a.nz, ldz = a.ldz;
@@ -1996,6 +2039,8 @@ __device__ __forceinline__ void astep_tile(const AcousticArgs& a, int tix, int t
     const int col = (active ? gx - wsx : 2) * kAcWZ;  // an in-window column either way
     const int zw = (active ? z0 : sz) - wsz;
     __syncthreads();
+    // interior D2 weights of the quad's cells, once for all shots (interior tiles; a frame
+    // tile forms them per shot: held across the shot loop they cost a wavefront per SIMD)
     W5 wx[4], wz[4];
     if (!EDGE) {
 #pragma unroll
@@ -2005,15 +2050,14 @@ __device__ __forceinline__ void astep_tile(const AcousticArgs& a, int tix, int t
         }
     }
     const bool inx = gx >= 6 && gx < nx - 6;
-    // bulk moduli of the boundary cells the absorbing SAT terms of this quad read
-    // (the side column's cells for a quad in column 0 or nx - 1)
-    float kbt = 0.f, kbb = 0.f, kbs[4] = {0.f, 0.f, 0.f, 0.f};
-    if (EDGE && active) {
-        const long long c0 = (long long)gx * ldz;
-        if (z0 < 4) kbt = a.K[c0];
-        if (z0 + 4 > nz - 4 && z0 < nz) kbb = a.K[c0 + nz - 1];
-        if (gx == 0 || gx == nx - 1) unpack(ld4(a.K + c0 + z0), kbs);
-    }
+    // frame tiles: the cells with closure rows or SAT terms (the tile's columns within 6 of
+    // a side, then its rows within 6 of a side in the other columns), dealt to all the
+    // workgroup's threads one cell each, consecutive threads along a row or down a column
+    const int zr = min(ez, nz), nrr = zr - sz, ncx = ex - sx;
+    const int cl = sx < 6 ? min(6, ex) - sx : 0, crs = max(nx - 6, sx + cl), cr = max(0, ex - crs);
+    const int rt = sz < 6 ? max(0, min(6, zr) - sz) : 0, rbs = max(nz - 6, sz + rt), rb = max(0, zr - rbs);
+    const int nsc = cl + cr, nsr = rt + rb, nnc = ncx - nsc;
+    const int nslow = EDGE ? nsc * nrr + nnc * nsr : 0;
     for (int sh = s0; sh < s1; sh++) {
         const int b = (sh - s0) & 1;
         const long long so = (long long)sh * a.shot_stride;
@@ -2033,70 +2077,46 @@ __device__ __forceinline__ void astep_tile(const AcousticArgs& a, int tix, int t
                     const float v = (kd[k] * rhs + (2.f * pu[0] - p)) / dv[k];
                     out[k] = (v * tz[k]) * tx;
                 }
-            } else if (inx && z0 >= 6 && z0 + 3 < nz - 6) {
-                // a quad of interior cells in a frame tile: the interior rows, their weights
-                // formed per shot
+            } else {
+                // the interior cells of the quad (weights formed per shot) and the rows
+                // past nz (zero) into the output tile; the other cells follow below
 #pragma unroll
                 for (int k = 0; k < 4; k++) {
+                    const int gz = z0 + k;
                     const float* pu = &U[col + zw + k];
                     const float* pc = &sR[col + zw + k];
-                    float p = pv[k];
-                    if (a.taper_prev) p = (p * tz[k]) * tx;
-                    const float rhs = d2_apply(d2_weights(pc, kAcWZ), pu, kAcWZ) * a.rdx2 +
-                                      d2_apply(d2_weights(pc, 1), pu, 1) * a.rdz2;
-                    const float v = (kd[k] * rhs + (2.f * pu[0] - p)) / dv[k];
-                    out[k] = (v * tz[k]) * tx;
-                }
-            } else {
-                // one cell at a time: unrolled, the closure rows and SAT terms of the four
-                // cells would keep hundreds of window values live
-#pragma unroll 1
-                for (int k = 0; k < 4; k++) {
-                    const int gz = z0 + k;
-                    const int w = col + zw + k;
-                    const float* pu = &U[w];
-                    const float* pc = &sR[w];
-                    const float tzk = sel4(tz, k);
-                    float p = sel4(pv, k);
-                    if (a.taper_prev) p = (p * tzk) * tx;
-                    float v = 0.f;  // rows past nz stay zero
-                    if (gz < nz) {
-                        const float lx = inx ? d2_apply(d2_weights(pc, kAcWZ), pu, kAcWZ)
-                                             : d2_closure(pu, pc, kAcWZ, gx, nx);
-                        const float lz = (gz >= 6 && gz < nz - 6) ? d2_apply(d2_weights(pc, 1), pu, 1)
-                                                                   : d2_closure(pu, pc, 1, gz, nz);
-                        float rhs = lx * a.rdx2 + lz * a.rdz2;
-                        if (gz < 4) {  // top side, boundary cell (gx, 0)
-                            const int bb = col - wsz;
-                            rhs += ac_sat(a.bc[0], gz, &U[bb], &sR[bb], 1, p, kbt, a.dir_z, a.pen_z, a.neu_z,
-                                          a.abs_z);
-                        }
-                        if (gz >= nz - 4) {  // bottom side, boundary cell (gx, nz - 1)
-                            const int bb = col + (nz - 1 - wsz);
-                            rhs += ac_sat(a.bc[1], nz - 1 - gz, &U[bb], &sR[bb], -1, p, kbb, a.dir_z, a.pen_z,
-                                          a.neu_z, a.abs_z);
-                        }
-                        if (gx < 4) {  // left side, boundary cell (0, gz)
-                            const int bb = (0 - wsx) * kAcWZ + (gz - wsz);
-                            rhs += ac_sat(a.bc[2], gx, &U[bb], &sR[bb], kAcWZ, p, sel4(kbs, k), a.dir_x,
-                                          a.pen_x, a.neu_x, a.abs_x);
-                        }
-                        if (gx >= nx - 4) {  // right side, boundary cell (nx - 1, gz)
-                            const int bb = (nx - 1 - wsx) * kAcWZ + (gz - wsz);
-                            rhs += ac_sat(a.bc[3], nx - 1 - gx, &U[bb], &sR[bb], -kAcWZ, p,
-                                          sel4(kbs, k), a.dir_x, a.pen_x, a.neu_x, a.abs_x);
-                        }
-                        // leapfrog (we_op.cpp:2093-2098), boundary divisor (asat_scale_boundaries), taper
-                        v = (sel4(kd, k) * rhs + (2.f * pu[0] - p)) / sel4(dv, k);
-                        v = (v * tzk) * tx;
+                    if (gz >= nz) {
+                        sO[(gx - sx) * kAcTZ + (gz - sz)] = 0.f;  // rows past nz stay zero
+                    } else if (inx && gz >= 6 && gz < nz - 6) {
+                        float p = pv[k];
+                        if (a.taper_prev) p = (p * tz[k]) * tx;
+                        const float rhs = d2_apply(d2_weights(pc, kAcWZ), pu, kAcWZ) * a.rdx2 +
+                                          d2_apply(d2_weights(pc, 1), pu, 1) * a.rdz2;
+                        const float v = (kd[k] * rhs + (2.f * pu[0] - p)) / dv[k];
+                        sO[(gx - sx) * kAcTZ + (gz - sz)] = (v * tz[k]) * tx;
                     }
-                    out[0] = k == 0 ? v : out[0];
-                    out[1] = k == 1 ? v : out[1];
-                    out[2] = k == 2 ? v : out[2];
-                    out[3] = k == 3 ? v : out[3];
                 }
             }
-            st4(a.next + so + g, make_float4(out[0], out[1], out[2], out[3]));
+            if (!EDGE) st4(a.next + so + g, make_float4(out[0], out[1], out[2], out[3]));
+        }
+        if (EDGE) {
+#pragma unroll 1
+            for (int c = threadIdx.x; c < nslow; c += kThreads) {
+                int cx, cz;
+                if (c < nsc * nrr) {
+                    const int j = c / nrr;
+                    cz = sz + (c - j * nrr);
+                    cx = j < cl ? sx + j : crs + (j - cl);
+                } else {
+                    const int c2 = c - nsc * nrr, r = c2 / nnc;
+                    cx = sx + cl + (c2 - r * nnc);
+                    cz = r < rt ? sz + r : rbs + (r - rt);
+                }
+                sO[(cx - sx) * kAcTZ + (cz - sz)] =
+                    ac_frame_cell(a, sU[b], sR, (cx - wsx) * kAcWZ + (cz - wsz), wsx, wsz, cx, cz, so);
+            }
+            __syncthreads();  // the output tile is complete
+            if (active) st4(a.next + so + g, ld4(&sO[(gx - sx) * kAcTZ + (z0 - sz)]));
         }
         __syncthreads();  // the next shot's window is in; this shot's window is read
     }
@@ -2110,6 +2130,7 @@ __global__ void __launch_bounds__(kThreads) k_astep(const AcousticArgs a)
 {
     __shared__ __attribute__((aligned(16))) float sR[kAcSWX * kAcWZ];
     __shared__ __attribute__((aligned(16))) float sU[2][kAcSWX * kAcWZ];
+    __shared__ __attribute__((aligned(16))) float sO[kAcTX * kAcTZ];  // a frame tile's new level
     const int nin_x = a.itx1 - a.itx0, nin_z = a.itz1 - a.itz0;
     const int nin = nin_x * nin_z, nedge = a.ntx * a.ntz - nin;
     const int groups = (a.nshots + a.spb - 1) / a.spb;
@@ -2132,11 +2153,11 @@ __global__ void __launch_bounds__(kThreads) k_astep(const AcousticArgs a)
             tiz = a.itz0 + r / w;
             tix = c < a.itx0 ? c : a.itx1 + (c - a.itx0);
         }
-        astep_tile<true>(a, tix, tiz, grp, sR, sU);
+        astep_tile<true>(a, tix, tiz, grp, sR, sU, sO);
     } else {
         const int lin = xcd_remap(b - ne8, gridDim.x - ne8);
         const int grp = lin / nin, tile = lin - grp * nin;
-        astep_tile<false>(a, a.itx0 + tile / nin_z, a.itz0 + tile % nin_z, grp, sR, sU);
+        astep_tile<false>(a, a.itx0 + tile / nin_z, a.itz0 + tile % nin_z, grp, sR, sU, sO);
     }
 }
 
