@@ -917,6 +917,10 @@ struct Engine {
         tune_report.clear();
         const long long steps0 = stat_steps;
         int bes = opt_edge_stream;
+        double t_es2[2][5][3];  // the edge_stream = 2 time of each (tile, spb, spb_edge)
+        for (auto& a : t_es2)
+            for (auto& b : a)
+                for (double& c : b) c = 1e300;
         for (int tx : {16, 32})
             for (int spb : {1, 2, 4, 8, 16}) {
                 if (spb > 1 && spb > ns) continue;
@@ -942,6 +946,10 @@ struct Engine {
                         snprintf(buf, sizeof(buf), "%stile_x=%d spb=%d spb_edge=%d edge_stream=%d: %.1f us",
                                  tune_report.empty() ? "" : "; ", tx, spb, spe, es, us);
                         tune_report += buf;
+                        if (es == 2) {
+                            const int ispb = spb == 1 ? 0 : spb == 2 ? 1 : spb == 4 ? 2 : spb == 8 ? 3 : 4;
+                            t_es2[tx == 32][ispb][spe == 1 ? 0 : spe == 2 ? 1 : 2] = us;
+                        }
                         if (us < best) {
                             best = us;
                             btx = tx;
@@ -953,6 +961,14 @@ struct Engine {
             }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
+        // the forward step cannot tell the edge-stream orders apart (within 1%), but the
+        // graph-replayed gradient can: the core launch issued before the edge bands
+        // (edge_stream 2) took 6.45 s per gradient of 64 shots on 500 x 1700 (nt 4000)
+        // against 6.66 s with the bands first, so that order wins forward-step ties
+        {
+            const int ispb = bspb == 1 ? 0 : bspb == 2 ? 1 : bspb == 4 ? 2 : bspb == 8 ? 3 : 4;
+            if (bes != 2 && t_es2[btx == 32][ispb][bspe == 1 ? 0 : bspe == 2 ? 1 : 2] <= 1.015 * best) bes = 2;
+        }
         set_tile(btx);
         opt_tile_x = btx;
         opt_spb = bspb;
