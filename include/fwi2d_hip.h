@@ -138,8 +138,8 @@ int fwi2d_dot_f64_noalloc(int device, const float* a, const float* b, const floa
    levels; 0: the reference's direct form with three snapshots per launch; the two agree
    to rounding),
    "autotune" (1: at the next forward run, time a few steps of every tile_x,
-   shots-per-workgroup and edge_stream choice and keep the fastest; forward results are
-   unchanged),
+   shots-per-workgroup and edge_stream choice and keep the fastest, edge_stream 2 winning
+   ties within 1.5% (it replays the gradient faster); forward results are unchanged),
    "checkpoint_format" (0 FP32; 1 BF16, 2 FP16: checkpoint levels stored in 16 bits with
    a max-abs scale per level and shot, which halves their memory; the gradient then
    carries the error of the recomputation from rounded levels),
