@@ -47,6 +47,10 @@ struct Acoustic {
     // run's source traces
     DevMem d_ckpt, d_seg, d_rlev, d_fsrc;
     long long snapshot_budget_mib = 131072;  // whole snapshot history up to this size (128 GiB)
+    // 1: the adjoint step kernel images the snapshots; 0 (default): k_agrad before the step,
+    // faster: fused, the step kernel holds 151 VGPRs and 42 KB of LDS, 3 wavefronts per SIMD,
+    // and a 64-shot 500 x 1700 adjoint step with imaging takes 566 us against 211 + 300 us
+    int image_fused = 0;
     bool whole = true;
     int seg_K = 0, nseg = 0, seg_loaded = -1;
     long long stat_recompute_steps = 0;
@@ -253,10 +257,13 @@ struct Acoustic {
     // its buffer holds the neighbours jK - 1 and (j+1)K too)
     float* snapshot(int j, int idx) { return whole ? snap(idx) : slot(j, idx); }
 
-    // one forward step it on levels (p, c, n): the step kernel, then the point sources
-    void step(float* p, float* c, float* n, int it, const InjList& inj, const float* tr)
+    // one step it on levels (p, c, n): the step kernel, then the point sources; with
+    // img_idx >= 0 (adjoint run, the image_fused option) the step kernel also images
+    // snapshot img_idx against c
+    void step(float* p, float* c, float* n, int it, const InjList& inj, const float* tr, int img_idx = -1)
     {
-        AcousticArgs a = base_args();
+        AcousticArgs a = img_idx >= 0 ? image_args(img_idx) : base_args();
+        a.img = img_idx >= 0;
         a.taper_prev = it > 0;
         a.prev = p;
         a.curr = c;
@@ -344,13 +351,15 @@ struct Acoustic {
             }
         };
         for (int it = 0; it < nt - 1; it++) {
+            int img_idx = -1;
             if (!adjoint) {
                 capture(it);
                 if (rcv) launch_aextract(level(ic), nxz, ldz, d_ext.as<PointEntry>(), nr, rcv, it, st);
             } else if ((nt - 1 - it) % sub == 0 && it != 0) {
-                image(level(ic), (nt - 1 - it) / sub);
+                if (image_fused) img_idx = (nt - 1 - it) / sub;
+                else image(level(ic), (nt - 1 - it) / sub);
             }
-            step(level(ip), level(ic), level(in), it, inj, tr);
+            step(level(ip), level(ic), level(in), it, inj, tr, img_idx);
             const int t = ip;
             ip = ic;
             ic = in;
@@ -367,6 +376,14 @@ struct Acoustic {
     // compute_gradients (we_op.cpp:1883-1935) of snapshot idx with the adjoint level adj
     void image(const float* adj, int idx)
     {
+        AcousticArgs a = image_args(idx);
+        a.curr = adj;
+        launch_agrad(a, st);
+    }
+
+    // the imaging arguments of snapshot idx (its segment loaded first)
+    AcousticArgs image_args(int idx)
+    {
         AcousticArgs a = base_args();
         const int last = g.nt / g.sub;
         const double dtg = g.sub * g.dt, t = idx * dtg;
@@ -379,14 +396,13 @@ struct Acoustic {
         a.wr = (float)(idx == 0 ? w * 0.5 * dtg : w * dtg);
         const int j = whole ? 0 : idx / seg_K;
         if (!whole) load_segment(j);
-        a.curr = adj;
         a.f0 = snapshot(j, idx);
         a.fm = a.mode != 1 ? snapshot(j, idx - 1) : nullptr;
         a.fp = a.mode != 2 ? snapshot(j, idx + 1) : nullptr;
         a.fshot_stride = nxz;
         a.grad = d_gslab.as<float>();
         a.gslab_stride = 2 * nxz;
-        launch_agrad(a, st);
+        return a;
     }
 
     // the operator level's per-trace time maps on the device (see trace_matrix.h)
@@ -591,6 +607,9 @@ int fwi2d_acoustic_set_option(fwi2d_acoustic* e, const char* key, long long valu
             if (value < 0) throw std::runtime_error("snapshot_budget must be >= 0 (MiB)");
             e->impl->snapshot_budget_mib = value;
             e->impl->snap_epoch = -1;  // the next forward run plans the snapshots again
+        } else if (k == "image_fused") {
+            if (value != 0 && value != 1) throw std::runtime_error("image_fused must be 0 or 1");
+            e->impl->image_fused = (int)value;
         } else {
             throw std::runtime_error("unknown acoustic engine option: " + k);
         }
@@ -607,6 +626,7 @@ long long fwi2d_acoustic_get_stat(fwi2d_acoustic* e, const char* key)
     if (k == "segment") return a.seg_K;
     if (k == "nseg") return a.nseg;
     if (k == "snapshot_budget") return a.snapshot_budget_mib;
+    if (k == "image_fused") return a.image_fused;
     if (k == "shots_per_block") return a.base_args().spb;
     return -1;
 }

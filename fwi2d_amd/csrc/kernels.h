@@ -181,6 +181,7 @@ struct AcousticArgs {
     float* grad;                 // per shot-group slabs [groups][2][nx][ldz] (groups of spb shots)
     long long gslab_stride;
     int mode;                    // 0 interior snapshot, 1 the first, 2 the last
+    int img;                     // k_astep: image snapshot f0 with the adjoint level curr as well
     float wk;                    // K term weight w / dtg
     float wr;                    // 1/rho term weight w dtg (halved for the first snapshot)
     float rdx, rdz;              // 1 / dx, 1 / dz

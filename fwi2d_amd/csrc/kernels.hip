@@ -2010,9 +2010,12 @@ __device__ __forceinline__ float ac_frame_cell(const AcousticArgs& a, const floa
 // a tile of the frame around it, whose interior cells take the interior rows (weights
 // formed per shot) and whose other cells the closure rows and SAT terms, dealt one cell
 // per thread.
-template <bool EDGE>
+// IMG: the adjoint step also images snapshot a.f0 of every shot against the level it
+// reads (the curr window is the adjoint level compute_gradients takes), as k_agrad does,
+// with the snapshot's window double buffered beside the level's.
+template <bool EDGE, bool IMG>
 __device__ __forceinline__ void astep_tile(const AcousticArgs& a, int tix, int tiz, int grp, float* sR,
-                                           float (*sU)[kAcSWX * kAcWZ], float* sO)
+                                           float (*sU)[kAcSWX * kAcWZ], float* sO, float (*sF)[kAcSWX * kAcWZ])
 {
     const int sx = tix * kAcTX, sz = tiz * kAcTZ;
     const int nx = a.nx, nz = a.nz, ldz = a.ldz;
@@ -2026,6 +2029,7 @@ __device__ __forceinline__ void astep_tile(const AcousticArgs& a, int tix, int t
         st4(&sR[lx * kAcWZ + 4 * lq], ok ? ld4(a.ri + (long long)gx * ldz + gz) : make_float4(0.f, 0.f, 0.f, 0.f));
     }
     ac_window_x(a.curr + (long long)s0 * a.shot_stride, sU[0], wsx, wsz, nx, ldz);
+    if (IMG) ac_window_x(a.f0 + (long long)s0 * a.fshot_stride, sF[0], wsx, wsz, nx, ldz);
     const int q = threadIdx.x;  // quads column-major: a wavefront covers 8 whole columns
     const int col_q = q / (kAcTZ / 4), row_q = q % (kAcTZ / 4);
     const int gx = sx + col_q, z0 = sz + 4 * row_q;
@@ -2058,10 +2062,24 @@ __device__ __forceinline__ void astep_tile(const AcousticArgs& a, int tix, int t
     const int rt = sz < 6 ? max(0, min(6, zr) - sz) : 0, rbs = max(nz - 6, sz + rt), rb = max(0, zr - rbs);
     const int nsc = cl + cr, nsr = rt + rb, nnc = ncx - nsc;
     const int nslow = EDGE ? nsc * nrr + nnc * nsr : 0;
+    // imaging: the group's gradient slab and the K term's weights w / (dtg K^2)
+    float gk[4] = {0.f, 0.f, 0.f, 0.f}, gr[4] = {0.f, 0.f, 0.f, 0.f}, kw[4] = {0.f, 0.f, 0.f, 0.f};
+    float* G = a.grad + (long long)grp * a.gslab_stride + g;
+    if (IMG && active) {
+        float K[4];
+        unpack(ld4(a.K + g), K);
+#pragma unroll
+        for (int k = 0; k < 4; k++) kw[k] = a.wk / (K[k] * K[k]);
+        unpack(ld4(G), gk);
+        unpack(ld4(G + a.shot_stride), gr);
+    }
     for (int sh = s0; sh < s1; sh++) {
         const int b = (sh - s0) & 1;
         const long long so = (long long)sh * a.shot_stride;
-        if (sh + 1 < s1) ac_window_x(a.curr + so + a.shot_stride, sU[b ^ 1], wsx, wsz, nx, ldz);
+        if (sh + 1 < s1) {
+            ac_window_x(a.curr + so + a.shot_stride, sU[b ^ 1], wsx, wsz, nx, ldz);
+            if (IMG) ac_window_x(a.f0 + (long long)(sh + 1) * a.fshot_stride, sF[b ^ 1], wsx, wsz, nx, ldz);
+        }
         if (active) {
             float pv[4], out[4];
             unpack(ld4(a.prev + so + g), pv);
@@ -2098,6 +2116,26 @@ __device__ __forceinline__ void astep_tile(const AcousticArgs& a, int tix, int t
                 }
             }
             if (!EDGE) st4(a.next + so + g, make_float4(out[0], out[1], out[2], out[3]));
+            if (IMG) {
+                // compute_gradients of this shot (we_op.cpp:1883-1935), as k_agrad
+                const long long fo = (long long)sh * a.fshot_stride;
+                float fm[4] = {0.f, 0.f, 0.f, 0.f}, fp[4] = {0.f, 0.f, 0.f, 0.f};
+                if (a.mode != 1) unpack(ld4(a.fm + fo + g), fm);
+                if (a.mode != 2) unpack(ld4(a.fp + fo + g), fp);
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const int gz = z0 + k;
+                    if (gz >= nz) continue;
+                    const float* pa = &U[col + zw + k];
+                    const float* pf = &sF[b][col + zw + k];
+                    const float ax = d1<false>(pa, kAcWZ, gx, nx) * a.rdx, az = d1<false>(pa, 1, gz, nz) * a.rdz;
+                    const float fx = d1<false>(pf, kAcWZ, gx, nx) * a.rdx, fz = d1<false>(pf, 1, gz, nz) * a.rdz;
+                    const float d2 =
+                        a.mode == 0 ? fp[k] - 2.f * pf[0] + fm[k] : (a.mode == 1 ? fp[k] - pf[0] : -pf[0] + fm[k]);
+                    gk[k] -= kw[k] * pa[0] * d2;
+                    gr[k] += a.wr * (ax * fx + az * fz);
+                }
+            }
         }
         if (EDGE) {
 #pragma unroll 1
@@ -2120,17 +2158,23 @@ __device__ __forceinline__ void astep_tile(const AcousticArgs& a, int tix, int t
         }
         __syncthreads();  // the next shot's window is in; this shot's window is read
     }
+    if (IMG && active) {
+        st4(G, make_float4(gk[0], gk[1], gk[2], gk[3]));
+        st4(G + a.shot_stride, make_float4(gr[0], gr[1], gr[2], gr[3]));
+    }
 }
 
 // One launch for all tiles: the frame tiles' workgroups first (their closure rows make
 // them the slowest), then the interior rectangle's, each class spread over all XCDs (the
 // frame's block count rounded up to a multiple of 8, the extra blocks idle) and remapped
 // within its class so that an XCD runs neighbouring tiles.
+template <bool IMG>
 __global__ void __launch_bounds__(kThreads) k_astep(const AcousticArgs a)
 {
     __shared__ __attribute__((aligned(16))) float sR[kAcSWX * kAcWZ];
     __shared__ __attribute__((aligned(16))) float sU[2][kAcSWX * kAcWZ];
     __shared__ __attribute__((aligned(16))) float sO[kAcTX * kAcTZ];  // a frame tile's new level
+    __shared__ __attribute__((aligned(16))) float sF[IMG ? 2 : 1][IMG ? kAcSWX * kAcWZ : 4];  // snapshot windows
     const int nin_x = a.itx1 - a.itx0, nin_z = a.itz1 - a.itz0;
     const int nin = nin_x * nin_z, nedge = a.ntx * a.ntz - nin;
     const int groups = (a.nshots + a.spb - 1) / a.spb;
@@ -2153,11 +2197,12 @@ __global__ void __launch_bounds__(kThreads) k_astep(const AcousticArgs a)
             tiz = a.itz0 + r / w;
             tix = c < a.itx0 ? c : a.itx1 + (c - a.itx0);
         }
-        astep_tile<true>(a, tix, tiz, grp, sR, sU, sO);
+        astep_tile<true, IMG>(a, tix, tiz, grp, sR, sU, sO, reinterpret_cast<float (*)[kAcSWX * kAcWZ]>(&sF[0][0]));
     } else {
         const int lin = xcd_remap(b - ne8, gridDim.x - ne8);
         const int grp = lin / nin, tile = lin - grp * nin;
-        astep_tile<false>(a, a.itx0 + tile / nin_z, a.itz0 + tile % nin_z, grp, sR, sU, sO);
+        astep_tile<false, IMG>(a, a.itx0 + tile / nin_z, a.itz0 + tile % nin_z, grp, sR, sU, sO,
+                               reinterpret_cast<float (*)[kAcSWX * kAcWZ]>(&sF[0][0]));
     }
 }
 
@@ -2268,7 +2313,9 @@ void launch_astep(const AcousticArgs& a, hipStream_t s)
 {
     const int groups = (a.nshots + a.spb - 1) / a.spb;
     const int nin = (a.itx1 - a.itx0) * (a.itz1 - a.itz0), ne = (a.ntx * a.ntz - nin) * groups;
-    hipLaunchKernelGGL(k_astep, dim3(((ne + 7) & ~7) + nin * groups), dim3(kThreads), 0, s, a);
+    const dim3 grid(((ne + 7) & ~7) + nin * groups);
+    if (a.img) hipLaunchKernelGGL(k_astep<true>, grid, dim3(kThreads), 0, s, a);
+    else hipLaunchKernelGGL(k_astep<false>, grid, dim3(kThreads), 0, s, a);
 }
 
 void launch_agrad(const AcousticArgs& a, hipStream_t s)

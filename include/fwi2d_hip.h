@@ -208,7 +208,10 @@ int fwi2d_acoustic_adjoint(fwi2d_acoustic* e, const float* adj, float* grad, int
 
 /* Options (no counterpart in the reference, which keeps every snapshot on the host):
    "snapshot_budget" (MiB, default 131072): the largest snapshot history kept whole; 0
-   always checkpoints.  Statistics: "recompute_steps" (total), "whole_snapshots" (1 or 0,
+   always checkpoints; "image_fused" (1: the adjoint step kernel images the snapshots as
+   it steps; 0, the default and faster: a separate imaging kernel before each imaged step;
+   the gradients are identical).
+   Statistics: "image_fused", "recompute_steps" (total), "whole_snapshots" (1 or 0,
    the last forward run's choice), "segment", "nseg", "snapshot_budget",
    "shots_per_block" (of the step kernel); -1 for an unknown key. */
 int fwi2d_acoustic_set_option(fwi2d_acoustic* e, const char* key, long long value);
