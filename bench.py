@@ -378,6 +378,11 @@ def operator_level(args):
     wav = np.stack([W.ricker(nt, dt, f) for f in freqs])
     src = np.stack([wav * np.cos(1.5708), wav * np.sin(1.5708)]).astype(np.float32)
     prob = W.base_problem(nx, nz, h, h, nt, dt, sxz, rxz, vti=int(vti), dt=dt, sub=1)
+    if args.acoustic is not None:  # nl_we_op_a: (vp, rho), one source and data component
+        prob = W.base_problem(nx, nz, h, h, nt, dt, sxz, rxz, dt=dt, sub=args.acoustic)
+        prob["acoustic"] = 1
+        true, start = true[[0, 2]], start[[0, 2]]
+        src = wav[None].astype(np.float32)
     op = fwi2d_amd.WeOp(prob, src)
     dobs = op.apply_forward(true)
     obj = fwi2d_amd.Nlls(op, dobs, normalize=True)
@@ -389,12 +394,14 @@ def operator_level(args):
         f, g = obj.evaluate(start)
     el = time.perf_counter() - t0
     log("timed evaluations done: %.3f s" % el)
+    kind = "acoustic (nl_we_op_a), sub=%d" % args.acoustic if args.acoustic is not None else None
     print(json.dumps({"metric": "operator-level FWI gradient shots/s (nlls_fwi_eco, host arrays)",
                       "value": round(ns * args.steps / el, 4), "unit": "shots/s", "steps": args.steps,
                       "warmup": args.warmup, "s_per_evaluation": round(el / args.steps, 4),
                       "functional_f64": f, "gradient_finite": bool(np.isfinite(g).all()),
                       "config": {"workload": "%s: %d shots, %dx%d, nt=%d%s" % (args.workload, ns, nz, nx, nt,
-                                                                             ", VTI" if vti else "")}}))
+                                                                             ", VTI" if vti else
+                                                                             (", " + kind if kind else ""))}}))
 
 
 def acoustic_grad_error_small():
@@ -658,7 +665,7 @@ def main():
     if args.step_only is not None:
         step_only(args)
         return
-    if args.operator:
+    if args.operator:  # with --acoustic SUB: the acoustic operator at the operator level
         operator_level(args)
         return
     if args.fwi is not None:
