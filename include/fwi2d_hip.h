@@ -262,20 +262,25 @@ typedef struct {
 
 typedef struct fwi2d_op fwi2d_op;
 
-/* Constructor of nl_we_op_e / nl_we_op_vti (we_op.hpp:58-66): allsrc is the source
-   after analyzeWavelet, [2][ns][nt_data]. */
+/* Constructor of nl_we_op_e / nl_we_op_vti (we_op.hpp:58-66) and nl_we_op_a
+   (we_op.hpp:238-245, desc->nparam = 2): allsrc is the source after analyzeWavelet,
+   [2][ns][nt_data] (point forces), [3][ns][nt_data] (moment tensors, desc->mt) or
+   [1][ns][nt_data] (acoustic). */
 int fwi2d_op_create(const fwi2d_op_desc* desc, const float* allsrc, fwi2d_op** out);
 
 /* nl_we_op::apply_forward (we_op.hpp:48; we_op.cpp:904-1055):
    pdat = add*pdat + F(pmod); pmod [nparam][nx][nz] physical model (km/s, g/cc),
-   pdat [2][nrec_total][nt_data].  Keeps device checkpoints for apply_jacobianT. */
+   pdat [2][nrec_total][nt_data] ([1][nrec_total][nt_data] for the acoustic operator).
+   Keeps device checkpoints for apply_jacobianT. */
 int fwi2d_op_apply_forward(fwi2d_op* op, int add, const float* pmod, float* pdat);
 
 /* nl_we_op::apply_jacobianT (we_op.hpp:49; we_op.cpp:1057-1239):
-   pmod = add*pmod + J(pmod0)^T pdat, gradient w.r.t. (vp, vs, rho[, 0, 0]). */
+   pmod = add*pmod + J(pmod0)^T pdat, gradient w.r.t. (vp, vs, rho[, 0, 0]), or (vp, rho)
+   for the acoustic operator. */
 int fwi2d_op_apply_jacobianT(fwi2d_op* op, int add, float* pmod, const float* pmod0, const float* pdat);
 
-/* Access to the engine of an operator (batch options, statistics). */
+/* Access to the engine of an operator (batch options, statistics); NULL before the
+   operator's first apply_forward / apply_jacobianT and for the acoustic operator. */
 fwi2d_engine* fwi2d_op_engine(fwi2d_op* op);
 
 /* Propagation time axis chosen by analyzeModel for the last call. */
