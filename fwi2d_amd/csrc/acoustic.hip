@@ -191,7 +191,7 @@ struct Acoustic {
         a.ntx = ntx;
         a.ntz = ntz;
         // shots per step workgroup: up to 8 while the launch keeps 2048 or more workgroups
-        // (64 shots of 500 x 1700: 4 shots 242 us, 8 shots 209 us, 16 shots 214 us a step)
+        // (64 shots of 500 x 1700: 8 shots 209 us a step, 16 shots 214 us)
         a.spb = (int)std::max(1LL, std::min(8LL, (long long)ntx * ntz * ns / 2048));
         // the rectangle of tiles whose cells are all interior along both axes (no closure
         // rows, no SAT terms): columns 6..nx-7, rows 6..nz-7
