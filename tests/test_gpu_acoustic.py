@@ -140,14 +140,17 @@ def op_case(seismotype=0, bc=(1, 2, 2, 2), dt_data=0.002, nt_data=201, nx=120, n
     return prob, true.astype(np.float32), start.astype(np.float32), src.astype(np.float32)
 
 
-@pytest.mark.parametrize("seismotype,bc,dt_data,stiffness", [(0, (1, 2, 2, 2), 0.002, 1.0),
-                                                             (1, (2, 3, 2, 1), 0.001, 1.0),
-                                                             (0, (1, 2, 2, 2), 0.003, 2.0)])
-def test_acoustic_operator_matches_oracle(lib, orc64, seismotype, bc, dt_data, stiffness):
-    """WeOp with prob["acoustic"] (nl_we_op_a): apply_forward (resampling, time derivative,
-    dt/dt_data scaling) and apply_jacobianT (H quadrature, chain rule to vp and rho) against
-    the oracle's operator within 1e-4."""
+@pytest.mark.parametrize("seismotype,bc,dt_data,stiffness,resampling", [(0, (1, 2, 2, 2), 0.002, 1.0, 0),
+                                                                        (1, (2, 3, 2, 1), 0.001, 1.0, 0),
+                                                                        (0, (1, 2, 2, 2), 0.003, 2.0, 0),
+                                                                        (1, (1, 2, 2, 2), 0.0025, 1.05, 1)])
+def test_acoustic_operator_matches_oracle(lib, orc64, seismotype, bc, dt_data, stiffness, resampling):
+    """WeOp with prob["acoustic"] (nl_we_op_a): apply_forward (sinc or linear resampling,
+    also at a non-integer ratio of the sampling rates, time derivative, dt/dt_data
+    scaling) and apply_jacobianT (H quadrature, chain rule to vp and rho) against the
+    oracle's operator within 1e-4."""
     prob, true, start, src = op_case(seismotype, bc, dt_data, stiffness=stiffness)
+    prob["resampling"] = resampling
     dobs, _ = orc64.apply_forward(orc64.make_op(prob), true, src)
     d0, full = orc64.apply_forward(orc64.make_op(prob), start, src, keep_full=True)
     op = lib.WeOp(prob, src)
